@@ -1,0 +1,125 @@
+"""ctypes mirror of include/bxstep.h (the C ABI of libbxstep.so).
+
+The structures here must stay byte-identical to the C declarations; tests/test_capi_layout.py
+checks every size and offset against the compiled library's own report.
+"""
+import ctypes as C
+import os
+
+u32 = C.c_uint32
+f32 = C.c_float
+
+BX_OK = 0
+BX_SHAPE_SPHERE, BX_SHAPE_CAPSULE, BX_SHAPE_TRIANGLE, BX_SHAPE_HULL, BX_SHAPE_MESH = 0, 1, 2, 3, 4
+BX_BODY_STATIC, BX_BODY_KINEMATIC, BX_BODY_DYNAMIC = 0, 1, 2
+BX_BODY_AWAKE = 0x1
+BX_BODY_ISLAND = 0x2
+BX_BODY_AUTO_SLEEP = 0x4
+BX_BODY_FIXED_ROT_X, BX_BODY_FIXED_ROT_Y, BX_BODY_FIXED_ROT_Z = 0x8, 0x10, 0x20
+BX_WORLD_SLEEPING = 0x1
+BX_WORLD_NO_WARM_START = 0x2
+BX_WORLD_NO_CLEAR_FORCES = 0x4
+BX_JOINT_REVOLUTE = 4
+BX_NULL_INDEX = 0xFFFFFFFF
+
+
+def _arr(t, n):
+    return t * n
+
+
+class bx_hull_desc(C.Structure):
+    _fields_ = [("vertex_count", u32), ("edge_count", u32), ("face_count", u32),
+                ("vertices", C.POINTER(f32)), ("edges", C.POINTER(u32)), ("faces", C.POINTER(u32)),
+                ("planes", C.POINTER(f32)), ("centroid", _arr(f32, 3))]
+
+
+class bx_mesh_desc(C.Structure):
+    _fields_ = [("vertex_count", u32), ("triangle_count", u32),
+                ("vertices", C.POINTER(f32)), ("triangles", C.POINTER(u32))]
+
+
+class bx_shape_library(C.Structure):
+    _fields_ = [("hull_count", u32), ("hulls", C.POINTER(bx_hull_desc)),
+                ("mesh_count", u32), ("meshes", C.POINTER(bx_mesh_desc))]
+
+
+class bx_body_desc(C.Structure):
+    _fields_ = [("type", u32), ("flags", u32), ("position", _arr(f32, 3)), ("orientation", _arr(f32, 4)),
+                ("linear_velocity", _arr(f32, 3)), ("angular_velocity", _arr(f32, 3)),
+                ("linear_damping", _arr(f32, 3)), ("angular_damping", _arr(f32, 3)),
+                ("gravity_scale", _arr(f32, 3)), ("linear_sleep_tolerance", f32),
+                ("angular_sleep_tolerance", f32)]
+
+
+class bx_fixture_desc(C.Structure):
+    _fields_ = [("body", u32), ("shape_type", u32), ("shape_index", u32), ("is_sensor", u32),
+                ("radius", f32), ("vertex1", _arr(f32, 3)), ("vertex2", _arr(f32, 3)),
+                ("vertex3", _arr(f32, 3)), ("mesh_scale", _arr(f32, 3)),
+                ("density", f32), ("friction", f32), ("restitution", f32)]
+
+
+class bx_joint_desc(C.Structure):
+    _fields_ = [("type", u32), ("body_a", u32), ("body_b", u32), ("collide_connected", u32),
+                ("use_initialize", u32), ("axis", _arr(f32, 3)), ("anchor", _arr(f32, 3)),
+                ("local_anchor_a", _arr(f32, 3)), ("local_rotation_a", _arr(f32, 4)),
+                ("local_anchor_b", _arr(f32, 3)), ("local_rotation_b", _arr(f32, 4)),
+                ("reference_rotation", _arr(f32, 4)), ("enable_limit", u32), ("lower_angle", f32),
+                ("upper_angle", f32), ("enable_motor", u32), ("motor_speed", f32),
+                ("max_motor_torque", f32)]
+
+
+class bx_world_desc(C.Structure):
+    _fields_ = [("gravity", _arr(f32, 3)), ("flags", u32),
+                ("body_count", u32), ("bodies", C.POINTER(bx_body_desc)),
+                ("fixture_count", u32), ("fixtures", C.POINTER(bx_fixture_desc)),
+                ("joint_count", u32), ("joints", C.POINTER(bx_joint_desc))]
+
+
+class bx_body_state(C.Structure):
+    _fields_ = [("position", _arr(f32, 3)), ("orientation", _arr(f32, 4)), ("world_center", _arr(f32, 3)),
+                ("linear_velocity", _arr(f32, 3)), ("angular_velocity", _arr(f32, 3)),
+                ("sleep_time", f32), ("flags", u32), ("island_index", u32)]
+
+
+class bx_manifold_point_state(C.Structure):
+    _fields_ = [("local_normal1", _arr(f32, 3)), ("local_point1", _arr(f32, 3)),
+                ("local_point2", _arr(f32, 3)), ("normal_impulse", f32), ("triangle_key", u32),
+                ("key", _arr(u32, 4)), ("persisting", u32), ("edge_contact", u32)]
+
+
+class bx_manifold_state(C.Structure):
+    _fields_ = [("point_count", u32), ("points", _arr(bx_manifold_point_state, 4)),
+                ("tangent_impulse", _arr(f32, 2)), ("motor_impulse", f32)]
+
+
+class bx_contact_state(C.Structure):
+    _fields_ = [("fixture_a", u32), ("fixture_b", u32), ("touching", u32), ("manifold_count", u32),
+                ("cache", _arr(u32, 4)), ("manifolds", _arr(bx_manifold_state, 3))]
+
+
+class bx_pair(C.Structure):
+    _fields_ = [("proxy1", u32), ("proxy2", u32)]
+
+
+class bx_world_counts(C.Structure):
+    _fields_ = [("body_count", u32), ("fixture_count", u32), ("joint_count", u32), ("contact_count", u32),
+                ("touching_count", u32), ("pair_count", u32), ("island_count", u32), ("awake_count", u32),
+                ("status", u32)]
+
+
+class bx_step_cfg(C.Structure):
+    _fields_ = [("dt", f32), ("velocity_iterations", u32), ("position_iterations", u32)]
+
+
+class bx_batch_opts(C.Structure):
+    _fields_ = [("max_contacts_per_world", u32), ("max_pairs_per_world", u32), ("solver_mode", u32),
+                ("reserved", _arr(u32, 5))]
+
+
+STRUCTS = [bx_hull_desc, bx_mesh_desc, bx_shape_library, bx_body_desc, bx_fixture_desc, bx_joint_desc,
+           bx_world_desc, bx_body_state, bx_manifold_point_state, bx_manifold_state, bx_contact_state,
+           bx_pair, bx_world_counts, bx_step_cfg, bx_batch_opts]
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libbxstep.so")

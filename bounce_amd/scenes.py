@@ -1,0 +1,263 @@
+"""Scene recipes and descriptor building for the batched step.
+
+A SceneSet holds a shape library and a list of worlds as numpy structured arrays laid out exactly
+like the C descriptors of include/bxstep.h, and exposes them as ctypes pointers for
+bx_batch_create (and, in tests, for the oracles, which take the same descriptors).
+
+Workloads (sizes as named by the benchmark configurations):
+  * drop_boxes    -- 1,000 unit boxes (half-extent 0.5, density 1) at seeded uniform positions in a
+                     20 x 50 x 20 m volume over a static 100 x 1 x 100 m ground box.
+  * pyramid       -- a 20-level 2-D box pyramid (row r has 20 - r boxes, 210 boxes) of unit boxes,
+                     friction 0.6, seeded +-1 mm jitter per axis, on the same ground box.
+  * hello_world   -- examples/hello_world/main.cpp:24-112 of the reference.
+Defaults follow b3BodyDef (body.h:49-123), b3FixtureDef (fixture.h:37-70) and b3World
+(world.cpp:41-69: gravity (0, -9.8, 0), warm starting on, sleeping off).
+"""
+import ctypes as C
+import numpy as np
+
+from . import capi
+from .capi import (bx_body_desc, bx_fixture_desc, bx_joint_desc, bx_world_desc, bx_hull_desc,
+                   bx_mesh_desc, bx_shape_library)
+from . import hulls as hull_lib
+
+F32 = np.float32
+BODY_DT = np.dtype(bx_body_desc)
+FIX_DT = np.dtype(bx_fixture_desc)
+JOINT_DT = np.dtype(bx_joint_desc)
+
+# b3BodyDef::angularSleepTolerance = scalar(2) / scalar(180) * B3_PI, evaluated in binary32
+ANGULAR_SLEEP_TOL = F32(F32(2.0) / F32(180.0)) * F32(np.pi)
+
+
+def body_array(n):
+    a = np.zeros(n, dtype=BODY_DT)
+    a["type"] = capi.BX_BODY_STATIC
+    a["flags"] = capi.BX_BODY_AWAKE | capi.BX_BODY_AUTO_SLEEP
+    a["orientation"][:, 3] = 1.0
+    a["gravity_scale"] = 1.0
+    a["linear_sleep_tolerance"] = F32(0.05)
+    a["angular_sleep_tolerance"] = ANGULAR_SLEEP_TOL
+    return a
+
+
+def fixture_array(n):
+    a = np.zeros(n, dtype=FIX_DT)
+    a["friction"] = F32(0.3)
+    a["mesh_scale"] = 1.0
+    return a
+
+
+def joint_array(n):
+    a = np.zeros(n, dtype=JOINT_DT)
+    a["type"] = capi.BX_JOINT_REVOLUTE
+    for k in ("local_rotation_a", "local_rotation_b", "reference_rotation"):
+        a[k][:, 3] = 1.0
+    return a
+
+
+class World:
+    """One world's descriptor arrays (creation order)."""
+
+    def __init__(self, bodies, fixtures, joints=None, gravity=(0.0, -9.8, 0.0), flags=0):
+        self.bodies = bodies
+        self.fixtures = fixtures
+        self.joints = joints if joints is not None else joint_array(0)
+        self.gravity = gravity
+        self.flags = flags
+
+    @property
+    def body_count(self):
+        return len(self.bodies)
+
+    def desc(self):
+        d = bx_world_desc()
+        d.gravity[:] = [float(g) for g in self.gravity]
+        d.flags = self.flags
+        d.body_count = len(self.bodies)
+        d.bodies = self.bodies.ctypes.data_as(C.POINTER(bx_body_desc))
+        d.fixture_count = len(self.fixtures)
+        d.fixtures = self.fixtures.ctypes.data_as(C.POINTER(bx_fixture_desc))
+        d.joint_count = len(self.joints)
+        d.joints = self.joints.ctypes.data_as(C.POINTER(bx_joint_desc))
+        return d
+
+
+class SceneSet:
+    """A shape library plus worlds; keeps every array referenced by the descriptors alive."""
+
+    def __init__(self, name):
+        self.name = name
+        self.hulls = []      # list of hull_lib.HullData
+        self.meshes = []     # list of (vertices f32[V,3], triangles u32[T,3])
+        self.worlds = []
+        self._lib = None
+        self._descs = None
+
+    def add_hull(self, h):
+        self.hulls.append(h)
+        return len(self.hulls) - 1
+
+    def library(self):
+        hd = (bx_hull_desc * max(1, len(self.hulls)))()
+        for i, h in enumerate(self.hulls):
+            hd[i].vertex_count = len(h.vertices)
+            hd[i].edge_count = len(h.edges)
+            hd[i].face_count = len(h.faces)
+            hd[i].vertices = h.vertices.ctypes.data_as(C.POINTER(C.c_float))
+            hd[i].edges = h.edges.ctypes.data_as(C.POINTER(C.c_uint32))
+            hd[i].faces = h.faces.ctypes.data_as(C.POINTER(C.c_uint32))
+            hd[i].planes = h.planes.ctypes.data_as(C.POINTER(C.c_float))
+            hd[i].centroid[:] = [float(x) for x in h.centroid]
+        md = (bx_mesh_desc * max(1, len(self.meshes)))()
+        for i, (v, t) in enumerate(self.meshes):
+            md[i].vertex_count = len(v)
+            md[i].triangle_count = len(t)
+            md[i].vertices = v.ctypes.data_as(C.POINTER(C.c_float))
+            md[i].triangles = t.ctypes.data_as(C.POINTER(C.c_uint32))
+        lib = bx_shape_library()
+        lib.hull_count = len(self.hulls)
+        lib.hulls = hd
+        lib.mesh_count = len(self.meshes)
+        lib.meshes = md
+        self._lib = (lib, hd, md)
+        return lib
+
+    def world_descs(self):
+        arr = (bx_world_desc * len(self.worlds))()
+        for i, w in enumerate(self.worlds):
+            arr[i] = w.desc()
+        self._descs = arr
+        return arr
+
+    @property
+    def body_count(self):
+        return sum(w.body_count for w in self.worlds)
+
+
+def _ground(bodies, fixtures, hull_index, friction=0.3, y=-0.5):
+    bodies[0]["type"] = capi.BX_BODY_STATIC
+    bodies[0]["position"] = (0.0, y, 0.0)
+    fixtures[0]["body"] = 0
+    fixtures[0]["shape_type"] = capi.BX_SHAPE_HULL
+    fixtures[0]["shape_index"] = hull_index
+    fixtures[0]["friction"] = F32(friction)
+
+
+def hello_world():
+    """examples/hello_world/main.cpp: a 2 m box spinning at pi rad/s dropped from y = 10."""
+    s = SceneSet("hello_world")
+    g = s.add_hull(hull_lib.box_hull(10.0, 1.0, 10.0))
+    b = s.add_hull(hull_lib.box_hull(1.0, 1.0, 1.0))
+    bodies = body_array(2)
+    fixtures = fixture_array(2)
+    fixtures[0]["shape_type"] = capi.BX_SHAPE_HULL
+    fixtures[0]["shape_index"] = g
+    bodies[1]["type"] = capi.BX_BODY_DYNAMIC
+    bodies[1]["position"] = (0.0, 10.0, 0.0)
+    bodies[1]["angular_velocity"] = (0.0, F32(np.pi), 0.0)
+    fixtures[1]["body"] = 1
+    fixtures[1]["shape_type"] = capi.BX_SHAPE_HULL
+    fixtures[1]["shape_index"] = b
+    fixtures[1]["density"] = 1.0
+    s.worlds.append(World(bodies, fixtures, gravity=(0.0, F32(-9.8), 0.0)))
+    return s
+
+
+def drop_boxes_world(rng, n_boxes, ground_hull, box_hull, extent=(20.0, 50.0, 20.0)):
+    bodies = body_array(n_boxes + 1)
+    fixtures = fixture_array(n_boxes + 1)
+    _ground(bodies, fixtures, ground_hull)
+    ex, ey, ez = extent
+    pos = np.empty((n_boxes, 3), dtype=F32)
+    pos[:, 0] = rng.uniform(-ex / 2, ex / 2, n_boxes)
+    pos[:, 1] = rng.uniform(1.0, 1.0 + ey, n_boxes)
+    pos[:, 2] = rng.uniform(-ez / 2, ez / 2, n_boxes)
+    bodies["type"][1:] = capi.BX_BODY_DYNAMIC
+    bodies["position"][1:] = pos
+    fixtures["body"][1:] = np.arange(1, n_boxes + 1)
+    fixtures["shape_type"][1:] = capi.BX_SHAPE_HULL
+    fixtures["shape_index"][1:] = box_hull
+    fixtures["density"][1:] = 1.0
+    return World(bodies, fixtures)
+
+
+def drop_boxes(n_boxes=1000, n_worlds=1, seed=1, extent=(20.0, 50.0, 20.0)):
+    s = SceneSet("drop_boxes")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    rng = np.random.default_rng(seed)
+    for _ in range(n_worlds):
+        s.worlds.append(drop_boxes_world(rng, n_boxes, g, b, extent))
+    return s
+
+
+def pyramid_world(rng, levels, ground_hull, box_hull, friction=0.6, jitter=1e-3):
+    n = levels * (levels + 1) // 2
+    bodies = body_array(n + 1)
+    fixtures = fixture_array(n + 1)
+    _ground(bodies, fixtures, ground_hull, friction=friction)
+    pos = []
+    for r in range(levels):
+        count = levels - r
+        for k in range(count):
+            pos.append((-0.5 * (count - 1) + k, 0.5 + r, 0.0))
+    pos = np.asarray(pos, dtype=np.float64)
+    if jitter > 0:
+        pos += rng.uniform(-jitter, jitter, pos.shape)
+    bodies["type"][1:] = capi.BX_BODY_DYNAMIC
+    bodies["position"][1:] = pos.astype(F32)
+    fixtures["body"][1:] = np.arange(1, n + 1)
+    fixtures["shape_type"][1:] = capi.BX_SHAPE_HULL
+    fixtures["shape_index"][1:] = box_hull
+    fixtures["density"][1:] = 1.0
+    fixtures["friction"][1:] = F32(friction)
+    return World(bodies, fixtures)
+
+
+def pyramids(n_worlds=4096, levels=20, seed=2, jitter=1e-3):
+    s = SceneSet("pyramids")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    rng = np.random.default_rng(seed)
+    for _ in range(n_worlds):
+        s.worlds.append(pyramid_world(rng, levels, g, b, jitter=jitter))
+    return s
+
+
+def box_stack(n_boxes=5, seed=3):
+    """A vertical stack of unit boxes with small seeded offsets (testbed box_stack.h recipe)."""
+    s = SceneSet("box_stack")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    rng = np.random.default_rng(seed)
+    bodies = body_array(n_boxes + 1)
+    fixtures = fixture_array(n_boxes + 1)
+    _ground(bodies, fixtures, g)
+    for i in range(n_boxes):
+        bodies[i + 1]["type"] = capi.BX_BODY_DYNAMIC
+        bodies[i + 1]["position"] = (rng.uniform(-0.05, 0.05), 0.5 + 1.0 * i + 0.01 * i, rng.uniform(-0.05, 0.05))
+        fixtures[i + 1]["body"] = i + 1
+        fixtures[i + 1]["shape_type"] = capi.BX_SHAPE_HULL
+        fixtures[i + 1]["shape_index"] = b
+        fixtures[i + 1]["density"] = 1.0
+    s.worlds.append(World(bodies, fixtures))
+    return s
+
+
+def concat(sets, name="mix"):
+    """Merge scene sets into one batch (hull indices are remapped)."""
+    out = SceneSet(name)
+    for s in sets:
+        base = len(out.hulls)
+        out.hulls.extend(s.hulls)
+        mbase = len(out.meshes)
+        out.meshes.extend(s.meshes)
+        for w in s.worlds:
+            f = w.fixtures.copy()
+            hull = f["shape_type"] == capi.BX_SHAPE_HULL
+            f["shape_index"][hull] += base
+            mesh = f["shape_type"] == capi.BX_SHAPE_MESH
+            f["shape_index"][mesh] += mbase
+            out.worlds.append(World(w.bodies.copy(), f, w.joints.copy(), w.gravity, w.flags))
+    return out

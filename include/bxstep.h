@@ -1,0 +1,307 @@
+/*
+ * bxstep.h -- C ABI of the MI355X batched rigid-body step (libbxstep.so).
+ *
+ * This is the one boundary between host code and the HIP step. It replaces the
+ * body of the reference's
+ *     void b3World::Step(scalar dt, uint32 velocityIterations, uint32 positionIterations)
+ * (include/bounce/dynamics/world.h:127, src/bounce/dynamics/world.cpp:187-242)
+ * for a batch of independent worlds. The C++ drop-in API in bounce_amd/include/bounce/
+ * (b3World, b3Body, b3Fixture, b3Shape, b3Hull, b3RevoluteJoint ...) builds these
+ * descriptors and calls bx_batch_step() from b3World::Step.
+ *
+ * Conventions
+ *  - Plain C: POD structs, pointers and sizes, no exceptions, no C++ types.
+ *  - Every entry point returns BX_OK (0) or a negative BX_E_* code; the text of the last
+ *    error on the calling thread is available from bx_last_error().
+ *  - The caller owns every array passed in; the library copies what it keeps during the call.
+ *  - A context owns one device, one HIP stream and all device memory of its batches.
+ *    A context is used by one host thread at a time.
+ *  - Orientation quaternions are stored (x, y, z, w) = (v.x, v.y, v.z, s).
+ *  - Bodies, fixtures and joints are listed in creation order; the reference semantics of
+ *    list order (bodies and fixtures are prepended to their lists, world.cpp:94-110,
+ *    body.cpp:105-136) are applied by the library.
+ */
+#ifndef BXSTEP_H
+#define BXSTEP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Status codes
+ * ------------------------------------------------------------------------------------------- */
+#define BX_OK 0
+#define BX_E_INVALID (-1)      /* bad argument or descriptor */
+#define BX_E_DEVICE (-2)       /* HIP runtime error */
+#define BX_E_CAPACITY (-3)     /* a fixed capacity was exceeded (see bx_batch_opts) */
+#define BX_E_UNSUPPORTED (-4)  /* feature not supported by the device path */
+#define BX_E_NOMEM (-5)        /* device or host allocation failed */
+
+/* ---------------------------------------------------------------------------------------------
+ * Enumerations (values equal the reference's enums)
+ * ------------------------------------------------------------------------------------------- */
+/* b3Shape::Type (include/bounce/collision/shapes/shape.h) */
+#define BX_SHAPE_SPHERE 0u
+#define BX_SHAPE_CAPSULE 1u
+#define BX_SHAPE_TRIANGLE 2u
+#define BX_SHAPE_HULL 3u
+#define BX_SHAPE_MESH 4u
+
+/* b3BodyType (include/bounce/dynamics/body.h) */
+#define BX_BODY_STATIC 0u
+#define BX_BODY_KINEMATIC 1u
+#define BX_BODY_DYNAMIC 2u
+
+/* body flags: b3Body::e_* values (include/bounce/dynamics/body.h:379-384) */
+#define BX_BODY_AWAKE 0x0001u
+#define BX_BODY_ISLAND 0x0002u
+#define BX_BODY_AUTO_SLEEP 0x0004u
+#define BX_BODY_FIXED_ROT_X 0x0008u
+#define BX_BODY_FIXED_ROT_Y 0x0010u
+#define BX_BODY_FIXED_ROT_Z 0x0020u
+
+/* world flags */
+#define BX_WORLD_SLEEPING 0x0001u      /* b3World::SetSleeping(true); default off (world.cpp:46) */
+#define BX_WORLD_NO_WARM_START 0x0002u /* b3World::SetWarmStart(false); default on */
+#define BX_WORLD_NO_CLEAR_FORCES 0x0004u
+
+/* b3JointType (include/bounce/dynamics/joints/joint.h) */
+#define BX_JOINT_REVOLUTE 4u
+
+#define BX_NULL_INDEX 0xFFFFFFFFu
+#define BX_MAX_MANIFOLD_POINTS 4
+#define BX_MAX_MANIFOLDS 3
+
+/* ---------------------------------------------------------------------------------------------
+ * Shape library (shared by all worlds of a batch; geometry is borrowed by index, as b3HullShape
+ * and b3MeshShape borrow their b3Hull / b3Mesh, hull_shape.h:42-43, mesh_shape.h:49-50)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_hull_desc {
+    uint32_t vertex_count;
+    uint32_t edge_count;      /* half-edges; twins are adjacent (2k, 2k+1) */
+    uint32_t face_count;
+    const float* vertices;    /* [vertex_count][3] */
+    const uint32_t* edges;    /* [edge_count][5]: origin, twin, face, prev, next (b3HalfEdge) */
+    const uint32_t* faces;    /* [face_count]: first half-edge of the face (b3Face) */
+    const float* planes;      /* [face_count][4]: normal x y z, offset (b3Plane) */
+    float centroid[3];
+} bx_hull_desc;
+
+typedef struct bx_mesh_desc {
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    const float* vertices;      /* [vertex_count][3] */
+    const uint32_t* triangles;  /* [triangle_count][3] vertex indices (b3MeshTriangle) */
+} bx_mesh_desc;
+
+typedef struct bx_shape_library {
+    uint32_t hull_count;
+    const bx_hull_desc* hulls;
+    uint32_t mesh_count;
+    const bx_mesh_desc* meshes;
+} bx_shape_library;
+
+/* ---------------------------------------------------------------------------------------------
+ * World descriptors (b3BodyDef body.h:49-123, b3FixtureDef fixture.h:37-70,
+ * b3RevoluteJointDef revolute_joint.h:25-78)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_body_desc {
+    uint32_t type;                 /* BX_BODY_* */
+    uint32_t flags;                /* BX_BODY_AWAKE | BX_BODY_AUTO_SLEEP | BX_BODY_FIXED_ROT_* */
+    float position[3];
+    float orientation[4];          /* x y z w */
+    float linear_velocity[3];
+    float angular_velocity[3];
+    float linear_damping[3];
+    float angular_damping[3];
+    float gravity_scale[3];
+    float linear_sleep_tolerance;
+    float angular_sleep_tolerance;
+} bx_body_desc;
+
+typedef struct bx_fixture_desc {
+    uint32_t body;                 /* creation index of the owning body */
+    uint32_t shape_type;           /* BX_SHAPE_* */
+    uint32_t shape_index;          /* hull or mesh index in the library */
+    uint32_t is_sensor;
+    float radius;                  /* sphere / capsule radius; hulls use the hull skin (0) */
+    float vertex1[3];              /* sphere centre, capsule / triangle vertex 1 */
+    float vertex2[3];
+    float vertex3[3];
+    float mesh_scale[3];           /* b3MeshShape::m_scale */
+    float density;
+    float friction;
+    float restitution;
+} bx_fixture_desc;
+
+typedef struct bx_joint_desc {
+    uint32_t type;                 /* BX_JOINT_REVOLUTE */
+    uint32_t body_a;               /* creation indices */
+    uint32_t body_b;
+    uint32_t collide_connected;
+    /* If use_initialize != 0 the local frames are computed as
+       b3RevoluteJointDef::Initialize(bodyA, bodyB, axis, anchor, lower_angle, upper_angle)
+       does (revolute_joint.cpp) from the bodies' transforms at creation; otherwise the
+       local_* and reference_rotation fields are used as given. */
+    uint32_t use_initialize;
+    float axis[3];
+    float anchor[3];
+    float local_anchor_a[3];
+    float local_rotation_a[4];
+    float local_anchor_b[3];
+    float local_rotation_b[4];
+    float reference_rotation[4];
+    uint32_t enable_limit;
+    float lower_angle;
+    float upper_angle;
+    uint32_t enable_motor;
+    float motor_speed;
+    float max_motor_torque;
+} bx_joint_desc;
+
+typedef struct bx_world_desc {
+    float gravity[3];              /* default (0, -9.8, 0), world.cpp:51 */
+    uint32_t flags;                /* BX_WORLD_* */
+    uint32_t body_count;
+    const bx_body_desc* bodies;
+    uint32_t fixture_count;
+    const bx_fixture_desc* fixtures;
+    uint32_t joint_count;
+    const bx_joint_desc* joints;
+} bx_world_desc;
+
+/* ---------------------------------------------------------------------------------------------
+ * State read back after stepping
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_body_state {
+    float position[3];             /* b3Body::GetTransform().translation */
+    float orientation[4];          /* b3Body::GetTransform().rotation, x y z w */
+    float world_center[3];         /* b3Sweep::worldCenter */
+    float linear_velocity[3];
+    float angular_velocity[3];
+    float sleep_time;
+    uint32_t flags;                /* BX_BODY_AWAKE | BX_BODY_ISLAND ... after the step */
+    uint32_t island_index;         /* b3Body::m_islandIndex (position in its last island) */
+} bx_body_state;
+
+typedef struct bx_manifold_point_state {
+    float local_normal1[3];
+    float local_point1[3];
+    float local_point2[3];
+    float normal_impulse;
+    uint32_t triangle_key;         /* b3ContactID::triangleKey */
+    uint32_t key[4];               /* b3FeaturePair: inEdge1 outEdge1 inEdge2 outEdge2 */
+    uint32_t persisting;
+    uint32_t edge_contact;
+} bx_manifold_point_state;
+
+typedef struct bx_manifold_state {
+    uint32_t point_count;
+    bx_manifold_point_state points[BX_MAX_MANIFOLD_POINTS];
+    float tangent_impulse[2];
+    float motor_impulse;
+} bx_manifold_state;
+
+typedef struct bx_contact_state {
+    uint32_t fixture_a;            /* creation index of b3Contact::GetFixtureA() */
+    uint32_t fixture_b;
+    uint32_t touching;
+    uint32_t manifold_count;
+    uint32_t cache[4];             /* b3FeatureCache: cacheType, featureType, index1, index2
+                                      (hull-hull contacts; zero otherwise) */
+    bx_manifold_state manifolds[BX_MAX_MANIFOLDS];
+} bx_contact_state;
+
+typedef struct bx_pair {
+    uint32_t proxy1;               /* b3Pair after the last FindPairs: proxy1 <= proxy2 */
+    uint32_t proxy2;
+} bx_pair;
+
+typedef struct bx_world_counts {
+    uint32_t body_count;
+    uint32_t fixture_count;
+    uint32_t joint_count;
+    uint32_t contact_count;        /* b3World::GetContactCount */
+    uint32_t touching_count;
+    uint32_t pair_count;           /* unique pairs of the last FindPairs */
+    uint32_t island_count;         /* islands solved in the last step */
+    uint32_t awake_count;
+    uint32_t status;               /* BX_WS_* bits */
+} bx_world_counts;
+
+/* per-world status bits */
+#define BX_WS_NONFINITE 0x1u       /* a non-finite body state was produced */
+#define BX_WS_CAPACITY 0x2u        /* a per-world capacity overflowed */
+
+/* ---------------------------------------------------------------------------------------------
+ * Device path
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_context bx_context;
+typedef struct bx_batch bx_batch;
+
+typedef struct bx_step_cfg {
+    float dt;                      /* b3World::Step(dt, velocityIterations, positionIterations) */
+    uint32_t velocity_iterations;
+    uint32_t position_iterations;
+} bx_step_cfg;
+
+typedef struct bx_batch_opts {
+    uint32_t max_contacts_per_world;   /* 0: derived from the descriptors */
+    uint32_t max_pairs_per_world;      /* 0: derived */
+    uint32_t solver_mode;              /* 0: auto, 1: one lane per island, 2: scheduled wavefront */
+    uint32_t reserved[5];
+} bx_batch_opts;
+
+/* Library version and build information ("bxstep gfx950 ..."). */
+const char* bx_version(void);
+
+/* Text of the last error on the calling thread (empty string if none). */
+const char* bx_last_error(void);
+
+/* Create a context on HIP device `device` with its own stream. */
+int bx_context_create(int device, bx_context** out);
+void bx_context_destroy(bx_context* ctx);
+
+/* Upload `world_count` worlds built from the descriptors into device memory.
+   The shape library is copied; descriptors may be freed after the call. */
+int bx_batch_create(bx_context* ctx, const bx_shape_library* library,
+                    const bx_world_desc* worlds, uint32_t world_count,
+                    const bx_batch_opts* opts, bx_batch** out);
+void bx_batch_destroy(bx_batch* batch);
+
+/* Advance every world of the batch by `step_count` steps of b3World::Step(cfg).
+   Asynchronous on the context stream; call bx_batch_sync before reading state. */
+int bx_batch_step(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
+
+/* Advance only the worlds whose indices are listed (serving: a mix of resident worlds). */
+int bx_batch_step_subset(bx_batch* batch, const bx_step_cfg* cfg, const uint32_t* worlds,
+                         const uint32_t* step_counts, uint32_t count);
+
+int bx_batch_sync(bx_batch* batch);
+
+/* Read back one world. Arrays must hold the counts given by bx_batch_get_counts. */
+int bx_batch_get_counts(bx_batch* batch, uint32_t world, bx_world_counts* out);
+int bx_batch_get_bodies(bx_batch* batch, uint32_t world, bx_body_state* out, uint32_t capacity);
+int bx_batch_get_contacts(bx_batch* batch, uint32_t world, bx_contact_state* out, uint32_t capacity);
+int bx_batch_get_pairs(bx_batch* batch, uint32_t world, bx_pair* out, uint32_t capacity);
+/* Contacts in island solve order (indices into the bx_batch_get_contacts list) for the last step. */
+int bx_batch_get_island_contacts(bx_batch* batch, uint32_t world, uint32_t* out, uint32_t capacity,
+                                 uint32_t* count);
+
+/* Overwrite body velocities / forces between steps (b3Body::SetLinearVelocity, ApplyForce ...).
+   `bodies` is in creation order with `count` == body_count of the world. */
+int bx_batch_set_bodies(bx_batch* batch, uint32_t world, const bx_body_state* bodies, uint32_t count);
+
+/* Per-phase GPU time of the last bx_batch_step call, in milliseconds:
+   [0] update contacts, [1] islands, [2] solve, [3] find new pairs, [4] total. */
+int bx_batch_get_timings(bx_batch* batch, float* ms, uint32_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BXSTEP_H */

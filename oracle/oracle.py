@@ -1,0 +1,157 @@
+"""Python binding of the CPU oracles (test infrastructure).
+
+Two CPU implementations of b3World::Step share one C ABI (see oracle/ref_harness.cpp and
+oracle/src/orc_capi.cpp):
+  * "oracle"  -- the restatement in oracle/src, built into oracle/lib/liboracle.so;
+  * "ref32" / "ref64" -- the reference itself, compiled from its own sources into oracle/_ref
+    (FP32 and FP64 builds), present wherever `make -C oracle ref` ran.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline import this module.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from bounce_amd import capi  # noqa: E402
+
+LIBS = {
+    "oracle": (os.path.join(HERE, "lib", "liboracle.so"), "orc_"),
+    "ref32": (os.path.join(HERE, "_ref", "libbxref32.so"), "bxref32_"),
+    "ref64": (os.path.join(HERE, "_ref", "libbxref64.so"), "bxref64_"),
+}
+_loaded = {}
+
+
+def available(kind):
+    return os.path.exists(LIBS[kind][0])
+
+
+def load(kind):
+    if kind in _loaded:
+        return _loaded[kind]
+    path, prefix = LIBS[kind]
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    u = C.c_uint32
+
+    def fn(name, res, *args):
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = list(args)
+        return f
+
+    api = {
+        "create": fn("create", P, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc)),
+        "destroy": fn("destroy", None, P),
+        "step": fn("step", None, P, C.c_float, u, u, u),
+        "get_counts": fn("get_counts", C.c_int, P, C.POINTER(capi.bx_world_counts)),
+        "get_bodies": fn("get_bodies", C.c_int, P, C.POINTER(capi.bx_body_state), u),
+        "get_contacts": fn("get_contacts", C.c_int, P, C.POINTER(capi.bx_contact_state), u),
+        "get_pairs": fn("get_pairs", C.c_int, P, C.POINTER(capi.bx_pair), u),
+        "get_island_contacts": fn("get_island_contacts", C.c_int, P, C.POINTER(u), u, C.POINTER(u)),
+        "get_proxy_ids": fn("get_proxy_ids", C.c_int, P, C.POINTER(u), u),
+        "get_fat_aabbs": fn("get_fat_aabbs", C.c_int, P, C.POINTER(C.c_float), u),
+        "get_mass": fn("get_mass", C.c_int, P, C.POINTER(C.c_float), u),
+        "bench": fn("bench", C.c_double, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc), u,
+                    C.c_float, u, u, u, u, u),
+    }
+    if kind.startswith("ref"):
+        api["get_bodies_f64"] = fn("get_bodies_f64", C.c_int, P, C.POINTER(C.c_double), u)
+        api["box_hull"] = fn("box_hull", None, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
+                             C.POINTER(u), C.POINTER(u), C.POINTER(C.c_float), C.POINTER(C.c_float))
+    _loaded[kind] = api
+    return api
+
+
+BODY_DT = np.dtype(capi.bx_body_state)
+CONTACT_DT = np.dtype(capi.bx_contact_state)
+PAIR_DT = np.dtype(capi.bx_pair)
+
+
+class CpuWorld:
+    """One world stepped by a CPU implementation ("oracle", "ref32" or "ref64")."""
+
+    def __init__(self, kind, scene, world_index=0):
+        self.kind = kind
+        self.api = load(kind)
+        self._lib = scene.library()
+        self._desc = scene.worlds[world_index].desc()
+        self.h = self.api["create"](C.byref(self._lib), C.byref(self._desc))
+        if not self.h:
+            raise RuntimeError(f"{kind}: world creation failed")
+
+    def close(self):
+        if self.h:
+            self.api["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
+        self.api["step"](self.h, C.c_float(dt), vit, pit, n)
+
+    def counts(self):
+        c = capi.bx_world_counts()
+        self.api["get_counts"](self.h, C.byref(c))
+        return c
+
+    def bodies(self):
+        n = self.counts().body_count
+        out = np.zeros(n, dtype=BODY_DT)
+        self.api["get_bodies"](self.h, out.ctypes.data_as(C.POINTER(capi.bx_body_state)), n)
+        return out
+
+    def bodies_f64(self):
+        n = self.counts().body_count
+        out = np.zeros((n, 16), dtype=np.float64)
+        self.api["get_bodies_f64"](self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        return out
+
+    def contacts(self):
+        n = self.counts().contact_count
+        out = np.zeros(n, dtype=CONTACT_DT)
+        self.api["get_contacts"](self.h, out.ctypes.data_as(C.POINTER(capi.bx_contact_state)), n)
+        return out
+
+    def pairs(self):
+        n = self.counts().pair_count
+        out = np.zeros(n, dtype=PAIR_DT)
+        self.api["get_pairs"](self.h, out.ctypes.data_as(C.POINTER(capi.bx_pair)), n)
+        return out
+
+    def island_contacts(self):
+        cap = max(1, self.counts().contact_count)
+        out = np.zeros(cap, dtype=np.uint32)
+        cnt = C.c_uint32(0)
+        self.api["get_island_contacts"](self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)), cap, C.byref(cnt))
+        return out[: cnt.value]
+
+    def proxy_ids(self):
+        n = self.counts().fixture_count
+        out = np.zeros(n, dtype=np.uint32)
+        self.api["get_proxy_ids"](self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+        return out
+
+    def fat_aabbs(self):
+        n = self.counts().fixture_count
+        out = np.zeros((n, 6), dtype=np.float32)
+        self.api["get_fat_aabbs"](self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        return out
+
+    def mass(self):
+        n = self.counts().body_count
+        out = np.zeros((n, 23), dtype=np.float32)
+        self.api["get_mass"](self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        return out
+
+
+def bench(kind, scene, dt=1.0 / 60.0, vit=8, pit=2, warmup=0, steps=10, threads=1):
+    """Seconds of the timed steps, worlds spread one per thread over `threads` threads."""
+    api = load(kind)
+    lib = scene.library()
+    descs = scene.world_descs()
+    return api["bench"](C.byref(lib), descs, len(scene.worlds), C.c_float(dt), vit, pit, warmup, steps, threads)

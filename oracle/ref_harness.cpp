@@ -1,0 +1,527 @@
+// ref_harness.cpp -- test infrastructure, NOT product code.
+//
+// Drives the reference build of Bounce (compiled from the sources under the reference tree
+// into oracle/_ref by oracle/Makefile) through its public C++ API, from the same
+// bx_world_desc descriptors the device library takes, and dumps the state the parity tests
+// compare: bodies, contacts with manifolds and feature ids, the broadphase pair buffer and the
+// island solve order. It is compiled twice: FP32 (scalar = float) and FP64 (B3_USE_DOUBLE).
+//
+// Only tests/, the smoke check and bench.py's CPU baseline load the resulting library.
+// Internal fields (m_flags, m_sleepTime, m_islandIndex, the broadphase pair buffer) are read by
+// compiling this file with the class access keywords opened; the reference objects are built
+// without that, and access specifiers do not change the layout of these classes.
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <new>
+#include <vector>
+#include <algorithm>
+#include <unordered_map>
+#include <thread>
+#include <atomic>
+#include <chrono>
+#include <stdarg.h>
+#include <float.h>
+#include <assert.h>
+#include <stddef.h>
+
+#include "../include/bxstep.h"
+
+#define private public
+#define protected public
+#include <bounce/dynamics/contacts/contact.h>
+#include <bounce/dynamics/joints/joint.h>
+#include <bounce/bounce.h>
+#include <bounce/dynamics/contacts/hull_contact.h>
+#undef private
+#undef protected
+
+#ifdef B3_USE_DOUBLE
+#define BXREF(name) bxref64_##name
+#else
+#define BXREF(name) bxref32_##name
+#endif
+
+namespace {
+
+struct RefHull {
+    std::vector<b3Vec3> vertices;
+    std::vector<b3HalfEdge> edges;
+    std::vector<b3Face> faces;
+    std::vector<b3Plane> planes;
+    b3Hull hull;
+};
+
+struct RefMesh {
+    std::vector<b3Vec3> vertices;
+    std::vector<b3MeshTriangle> triangles;
+    b3Mesh mesh;
+};
+
+struct IslandRecorder : public b3ContactListener {
+    std::vector<b3Contact*> order;
+    void BeginContact(b3Contact*) override {}
+    void EndContact(b3Contact*) override {}
+    void PreSolve(b3Contact*) override {}
+    void PostSolve(b3Contact* c) override { order.push_back(c); }
+};
+
+struct RefWorld {
+    b3World world;
+    std::vector<RefHull*> hulls;
+    std::vector<RefMesh*> meshes;
+    std::vector<b3Body*> bodies;
+    std::vector<b3Fixture*> fixtures;
+    std::vector<b3Joint*> joints;
+    IslandRecorder recorder;
+    std::vector<b3Contact*> lastIslandOrder;
+};
+
+b3Vec3 V3(const float* p) { return b3Vec3(scalar(p[0]), scalar(p[1]), scalar(p[2])); }
+b3Quat Q4(const float* p) { return b3Quat(b3Vec3(scalar(p[0]), scalar(p[1]), scalar(p[2])), scalar(p[3])); }
+
+void BuildHull(RefHull* h, const bx_hull_desc& d) {
+    h->vertices.resize(d.vertex_count);
+    for (uint32 i = 0; i < d.vertex_count; ++i) h->vertices[i] = V3(d.vertices + 3 * i);
+    h->edges.resize(d.edge_count);
+    for (uint32 i = 0; i < d.edge_count; ++i) {
+        const uint32_t* e = d.edges + 5 * i;
+        h->edges[i] = b3MakeEdge(e[0], e[1], e[2], e[3], e[4]);
+    }
+    h->faces.resize(d.face_count);
+    h->planes.resize(d.face_count);
+    for (uint32 i = 0; i < d.face_count; ++i) {
+        h->faces[i].edge = d.faces[i];
+        h->planes[i].normal = V3(d.planes + 4 * i);
+        h->planes[i].offset = scalar(d.planes[4 * i + 3]);
+    }
+    h->hull.centroid = V3(d.centroid);
+    h->hull.vertexCount = d.vertex_count;
+    h->hull.vertices = h->vertices.data();
+    h->hull.edgeCount = d.edge_count;
+    h->hull.edges = h->edges.data();
+    h->hull.faceCount = d.face_count;
+    h->hull.faces = h->faces.data();
+    h->hull.planes = h->planes.data();
+}
+
+void BuildMesh(RefMesh* m, const bx_mesh_desc& d) {
+    m->vertices.resize(d.vertex_count);
+    for (uint32 i = 0; i < d.vertex_count; ++i) m->vertices[i] = V3(d.vertices + 3 * i);
+    m->triangles.resize(d.triangle_count);
+    for (uint32 i = 0; i < d.triangle_count; ++i) {
+        m->triangles[i].v1 = d.triangles[3 * i + 0];
+        m->triangles[i].v2 = d.triangles[3 * i + 1];
+        m->triangles[i].v3 = d.triangles[3 * i + 2];
+    }
+    m->mesh.vertexCount = d.vertex_count;
+    m->mesh.vertices = m->vertices.data();
+    m->mesh.triangleCount = d.triangle_count;
+    m->mesh.triangles = m->triangles.data();
+    m->mesh.BuildTree();
+    m->mesh.BuildAdjacency();
+}
+
+void InitRegistersOnce() {
+    static std::atomic<int> done{0};
+    if (done.load()) return;
+    static std::atomic_flag lock = ATOMIC_FLAG_INIT;
+    while (lock.test_and_set()) {}
+    if (!b3Contact::s_initialized) {
+        b3Contact::InitializeRegisters();
+        b3Contact::s_initialized = true;
+    }
+    done.store(1);
+    lock.clear();
+}
+
+RefWorld* CreateWorld(const bx_shape_library* lib, const bx_world_desc* wd) {
+    InitRegistersOnce();
+    RefWorld* w = new RefWorld();
+    // b3World's constructor writes the process-global b3_convexCache = true (world.cpp:68).
+    w->world.SetGravity(V3(wd->gravity));
+    w->world.SetSleeping((wd->flags & BX_WORLD_SLEEPING) != 0);
+    w->world.SetWarmStarting((wd->flags & BX_WORLD_NO_WARM_START) == 0);
+    w->world.SetAutoClearForces((wd->flags & BX_WORLD_NO_CLEAR_FORCES) == 0);
+    w->world.SetContactListener(&w->recorder);
+    for (uint32 i = 0; lib && i < lib->hull_count; ++i) {
+        RefHull* h = new RefHull();
+        BuildHull(h, lib->hulls[i]);
+        w->hulls.push_back(h);
+    }
+    for (uint32 i = 0; lib && i < lib->mesh_count; ++i) {
+        RefMesh* m = new RefMesh();
+        BuildMesh(m, lib->meshes[i]);
+        w->meshes.push_back(m);
+    }
+    for (uint32 i = 0; i < wd->body_count; ++i) {
+        const bx_body_desc& bd = wd->bodies[i];
+        b3BodyDef def;
+        def.type = b3BodyType(bd.type);
+        def.awake = (bd.flags & BX_BODY_AWAKE) != 0;
+        def.allowSleep = (bd.flags & BX_BODY_AUTO_SLEEP) != 0;
+        def.fixedRotationX = (bd.flags & BX_BODY_FIXED_ROT_X) != 0;
+        def.fixedRotationY = (bd.flags & BX_BODY_FIXED_ROT_Y) != 0;
+        def.fixedRotationZ = (bd.flags & BX_BODY_FIXED_ROT_Z) != 0;
+        def.userData = (void*)(uintptr_t)i;
+        def.position = V3(bd.position);
+        def.orientation = Q4(bd.orientation);
+        def.linearVelocity = V3(bd.linear_velocity);
+        def.angularVelocity = V3(bd.angular_velocity);
+        def.linearDamping = V3(bd.linear_damping);
+        def.angularDamping = V3(bd.angular_damping);
+        def.gravityScale = V3(bd.gravity_scale);
+        def.linearSleepTolerance = scalar(bd.linear_sleep_tolerance);
+        def.angularSleepTolerance = scalar(bd.angular_sleep_tolerance);
+        w->bodies.push_back(w->world.CreateBody(def));
+    }
+    for (uint32 i = 0; i < wd->fixture_count; ++i) {
+        const bx_fixture_desc& fd = wd->fixtures[i];
+        b3FixtureDef def;
+        def.userData = (void*)(uintptr_t)i;
+        def.isSensor = fd.is_sensor != 0;
+        def.density = scalar(fd.density);
+        def.friction = scalar(fd.friction);
+        def.restitution = scalar(fd.restitution);
+        b3SphereShape sphere;
+        b3CapsuleShape capsule;
+        b3TriangleShape triangle;
+        b3HullShape hull;
+        b3MeshShape mesh;
+        switch (fd.shape_type) {
+        case BX_SHAPE_SPHERE:
+            sphere.m_center = V3(fd.vertex1);
+            sphere.m_radius = scalar(fd.radius);
+            def.shape = &sphere;
+            break;
+        case BX_SHAPE_CAPSULE:
+            capsule.m_vertex1 = V3(fd.vertex1);
+            capsule.m_vertex2 = V3(fd.vertex2);
+            capsule.m_radius = scalar(fd.radius);
+            def.shape = &capsule;
+            break;
+        case BX_SHAPE_TRIANGLE:
+            triangle.Set(V3(fd.vertex1), V3(fd.vertex2), V3(fd.vertex3));
+            def.shape = &triangle;
+            break;
+        case BX_SHAPE_HULL:
+            hull.m_hull = &w->hulls[fd.shape_index]->hull;
+            def.shape = &hull;
+            break;
+        case BX_SHAPE_MESH:
+            mesh.m_mesh = &w->meshes[fd.shape_index]->mesh;
+            mesh.m_scale = V3(fd.mesh_scale);
+            def.shape = &mesh;
+            break;
+        default:
+            delete w;
+            return nullptr;
+        }
+        w->fixtures.push_back(w->bodies[fd.body]->CreateFixture(def));
+    }
+    for (uint32 i = 0; i < wd->joint_count; ++i) {
+        const bx_joint_desc& jd = wd->joints[i];
+        if (jd.type != BX_JOINT_REVOLUTE) { delete w; return nullptr; }
+        b3RevoluteJointDef def;
+        b3Body* a = w->bodies[jd.body_a];
+        b3Body* b = w->bodies[jd.body_b];
+        if (jd.use_initialize) {
+            def.Initialize(a, b, V3(jd.axis), V3(jd.anchor), scalar(jd.lower_angle), scalar(jd.upper_angle));
+        } else {
+            def.bodyA = a;
+            def.bodyB = b;
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localRotationA = Q4(jd.local_rotation_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.localRotationB = Q4(jd.local_rotation_b);
+            def.referenceRotation = Q4(jd.reference_rotation);
+            def.lowerAngle = scalar(jd.lower_angle);
+            def.upperAngle = scalar(jd.upper_angle);
+        }
+        def.collideConnected = jd.collide_connected != 0;
+        def.enableLimit = jd.enable_limit != 0;
+        def.enableMotor = jd.enable_motor != 0;
+        def.motorSpeed = scalar(jd.motor_speed);
+        def.maxMotorTorque = scalar(jd.max_motor_torque);
+        w->joints.push_back(w->world.CreateJoint(def));
+    }
+    return w;
+}
+
+void Step(RefWorld* w, scalar dt, uint32 vit, uint32 pit) {
+    w->recorder.order.clear();
+    w->world.Step(dt, vit, pit);
+    w->lastIslandOrder = w->recorder.order;
+}
+
+uint32 FixtureIndex(const b3Fixture* f) { return uint32(uintptr_t(f->m_userData)); }
+
+void FillContact(const b3Contact* c, bx_contact_state* out) {
+    memset(out, 0, sizeof(*out));
+    out->fixture_a = FixtureIndex(c->m_fixtureA);
+    out->fixture_b = FixtureIndex(c->m_fixtureB);
+    out->touching = c->IsTouching() ? 1u : 0u;
+    out->manifold_count = c->m_manifoldCount;
+    if (c->m_fixtureA->GetType() == b3Shape::e_hull && c->m_fixtureB->GetType() == b3Shape::e_hull) {
+        const b3HullContact* hc = static_cast<const b3HullContact*>(c);
+        out->cache[0] = uint32(hc->m_cache.cacheType);
+        out->cache[1] = uint32(hc->m_cache.featureType);
+        out->cache[2] = hc->m_cache.index1;
+        out->cache[3] = hc->m_cache.index2;
+    }
+    for (uint32 i = 0; i < c->m_manifoldCount && i < BX_MAX_MANIFOLDS; ++i) {
+        const b3Manifold& m = c->m_manifolds[i];
+        bx_manifold_state& ms = out->manifolds[i];
+        ms.point_count = m.pointCount;
+        ms.tangent_impulse[0] = float(m.tangentImpulse.x);
+        ms.tangent_impulse[1] = float(m.tangentImpulse.y);
+        ms.motor_impulse = float(m.motorImpulse);
+        for (uint32 k = 0; k < m.pointCount; ++k) {
+            const b3ManifoldPoint& p = m.points[k];
+            bx_manifold_point_state& ps = ms.points[k];
+            for (int a = 0; a < 3; ++a) {
+                ps.local_normal1[a] = float(p.localNormal1[a]);
+                ps.local_point1[a] = float(p.localPoint1[a]);
+                ps.local_point2[a] = float(p.localPoint2[a]);
+            }
+            ps.normal_impulse = float(p.normalImpulse);
+            ps.triangle_key = p.id.triangleKey;
+            ps.key[0] = p.id.key.pair.inEdge1;
+            ps.key[1] = p.id.key.pair.outEdge1;
+            ps.key[2] = p.id.key.pair.inEdge2;
+            ps.key[3] = p.id.key.pair.outEdge2;
+            ps.persisting = p.persisting ? 1u : 0u;
+            ps.edge_contact = p.edgeContact ? 1u : 0u;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* BXREF(create)(const bx_shape_library* lib, const bx_world_desc* wd) {
+    return CreateWorld(lib, wd);
+}
+
+void BXREF(destroy)(void* h) {
+    RefWorld* w = (RefWorld*)h;
+    if (!w) return;
+    std::vector<RefHull*> hulls = w->hulls;
+    std::vector<RefMesh*> meshes = w->meshes;
+    delete w;
+    for (RefHull* p : hulls) delete p;
+    for (RefMesh* p : meshes) delete p;
+}
+
+void BXREF(step)(void* h, float dt, uint32_t vit, uint32_t pit, uint32_t n) {
+    RefWorld* w = (RefWorld*)h;
+    for (uint32_t i = 0; i < n; ++i) Step(w, scalar(dt), vit, pit);
+}
+
+int BXREF(get_counts)(void* h, bx_world_counts* out) {
+    RefWorld* w = (RefWorld*)h;
+    memset(out, 0, sizeof(*out));
+    out->body_count = uint32_t(w->bodies.size());
+    out->fixture_count = uint32_t(w->fixtures.size());
+    out->joint_count = uint32_t(w->joints.size());
+    out->contact_count = w->world.m_contactManager.m_contactCount;
+    for (b3Contact* c = w->world.m_contactManager.m_contactList; c; c = c->m_next)
+        out->touching_count += c->IsTouching() ? 1u : 0u;
+    const b3BroadPhase& bp = w->world.m_contactManager.m_broadPhase;
+    uint32_t unique = 0;
+    for (uint32_t i = 0; i < bp.m_pairCount; ++i) {
+        if (i == 0 || bp.m_pairs[i].proxy1 != bp.m_pairs[i - 1].proxy1 ||
+            bp.m_pairs[i].proxy2 != bp.m_pairs[i - 1].proxy2)
+            ++unique;
+    }
+    out->pair_count = unique;
+    for (b3Body* b : w->bodies) out->awake_count += b->IsAwake() ? 1u : 0u;
+    return 0;
+}
+
+int BXREF(get_bodies)(void* h, bx_body_state* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = std::min<uint32_t>(cap, uint32_t(w->bodies.size()));
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3Body* b = w->bodies[i];
+        bx_body_state& s = out[i];
+        memset(&s, 0, sizeof(s));
+        for (int a = 0; a < 3; ++a) {
+            s.position[a] = float(b->m_xf.translation[a]);
+            s.world_center[a] = float(b->m_sweep.worldCenter[a]);
+            s.linear_velocity[a] = float(b->m_linearVelocity[a]);
+            s.angular_velocity[a] = float(b->m_angularVelocity[a]);
+        }
+        s.orientation[0] = float(b->m_xf.rotation.v.x);
+        s.orientation[1] = float(b->m_xf.rotation.v.y);
+        s.orientation[2] = float(b->m_xf.rotation.v.z);
+        s.orientation[3] = float(b->m_xf.rotation.s);
+        s.sleep_time = float(b->m_sleepTime);
+        s.flags = b->m_flags;
+        s.island_index = b->m_islandIndex;
+    }
+    return int(n);
+}
+
+// 16 doubles per body: position 3, orientation 4, world centre 3, linear velocity 3, angular 3.
+int BXREF(get_bodies_f64)(void* h, double* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = std::min<uint32_t>(cap, uint32_t(w->bodies.size()));
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3Body* b = w->bodies[i];
+        double* o = out + 16 * i;
+        for (int a = 0; a < 3; ++a) o[a] = double(b->m_xf.translation[a]);
+        o[3] = double(b->m_xf.rotation.v.x);
+        o[4] = double(b->m_xf.rotation.v.y);
+        o[5] = double(b->m_xf.rotation.v.z);
+        o[6] = double(b->m_xf.rotation.s);
+        for (int a = 0; a < 3; ++a) o[7 + a] = double(b->m_sweep.worldCenter[a]);
+        for (int a = 0; a < 3; ++a) o[10 + a] = double(b->m_linearVelocity[a]);
+        for (int a = 0; a < 3; ++a) o[13 + a] = double(b->m_angularVelocity[a]);
+    }
+    return int(n);
+}
+
+int BXREF(get_contacts)(void* h, bx_contact_state* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = 0;
+    for (b3Contact* c = w->world.m_contactManager.m_contactList; c && n < cap; c = c->m_next)
+        FillContact(c, out + n++);
+    return int(n);
+}
+
+int BXREF(get_pairs)(void* h, bx_pair* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    const b3BroadPhase& bp = w->world.m_contactManager.m_broadPhase;
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < bp.m_pairCount; ++i) {
+        if (i > 0 && bp.m_pairs[i].proxy1 == bp.m_pairs[i - 1].proxy1 &&
+            bp.m_pairs[i].proxy2 == bp.m_pairs[i - 1].proxy2)
+            continue;
+        if (n < cap) {
+            out[n].proxy1 = bp.m_pairs[i].proxy1;
+            out[n].proxy2 = bp.m_pairs[i].proxy2;
+        }
+        ++n;
+    }
+    return int(n);
+}
+
+// Proxy id of each fixture (creation order).
+int BXREF(get_proxy_ids)(void* h, uint32_t* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = std::min<uint32_t>(cap, uint32_t(w->fixtures.size()));
+    for (uint32_t i = 0; i < n; ++i) out[i] = w->fixtures[i]->m_proxyId;
+    return int(n);
+}
+
+// Fat AABBs of each fixture's proxy (creation order), 6 floats each.
+int BXREF(get_fat_aabbs)(void* h, float* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = std::min<uint32_t>(cap, uint32_t(w->fixtures.size()));
+    const b3BroadPhase& bp = w->world.m_contactManager.m_broadPhase;
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3AABB& a = bp.GetFatAABB(w->fixtures[i]->m_proxyId);
+        for (int k = 0; k < 3; ++k) {
+            out[6 * i + k] = float(a.lowerBound[k]);
+            out[6 * i + 3 + k] = float(a.upperBound[k]);
+        }
+    }
+    return int(n);
+}
+
+// Island solve order of the last step: indices into the contact list of get_contacts.
+int BXREF(get_island_contacts)(void* h, uint32_t* out, uint32_t cap, uint32_t* count) {
+    RefWorld* w = (RefWorld*)h;
+    std::unordered_map<const b3Contact*, uint32_t> index;
+    uint32_t k = 0;
+    for (b3Contact* c = w->world.m_contactManager.m_contactList; c; c = c->m_next) index[c] = k++;
+    uint32_t n = 0;
+    for (const b3Contact* c : w->lastIslandOrder) {
+        auto it = index.find(c);
+        if (n < cap) out[n] = it == index.end() ? BX_NULL_INDEX : it->second;
+        ++n;
+    }
+    *count = n;
+    return 0;
+}
+
+// Mass properties after creation (creation order): mass, invMass, localCenter 3, I 9, invI 9.
+int BXREF(get_mass)(void* h, float* out, uint32_t cap) {
+    RefWorld* w = (RefWorld*)h;
+    uint32_t n = std::min<uint32_t>(cap, uint32_t(w->bodies.size()));
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3Body* b = w->bodies[i];
+        float* o = out + 23 * i;
+        o[0] = float(b->m_mass);
+        o[1] = float(b->m_invMass);
+        for (int a = 0; a < 3; ++a) o[2 + a] = float(b->m_sweep.localCenter[a]);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                o[5 + 3 * c + r] = float(b->m_I[c][r]);
+                o[14 + 3 * c + r] = float(b->m_invI[c][r]);
+            }
+    }
+    return int(n);
+}
+
+// b3BoxHull(ex, ey, ez) data in bx_hull_desc layout: 8 vertices, 24 edges, 6 faces.
+void BXREF(box_hull)(float ex, float ey, float ez, float* vertices, uint32_t* edges,
+                     uint32_t* faces, float* planes, float* centroid) {
+    b3BoxHull box{scalar(ex), scalar(ey), scalar(ez)};
+    for (uint32 i = 0; i < 8; ++i)
+        for (int a = 0; a < 3; ++a) vertices[3 * i + a] = float(box.vertices[i][a]);
+    for (uint32 i = 0; i < 24; ++i) {
+        const b3HalfEdge& e = box.edges[i];
+        edges[5 * i + 0] = e.origin;
+        edges[5 * i + 1] = e.twin;
+        edges[5 * i + 2] = e.face;
+        edges[5 * i + 3] = e.prev;
+        edges[5 * i + 4] = e.next;
+    }
+    for (uint32 i = 0; i < 6; ++i) {
+        faces[i] = box.faces[i].edge;
+        for (int a = 0; a < 3; ++a) planes[4 * i + a] = float(box.planes[i].normal[a]);
+        planes[4 * i + 3] = float(box.planes[i].offset);
+    }
+    for (int a = 0; a < 3; ++a) centroid[a] = float(box.centroid[a]);
+}
+
+// CPU baseline: step `world_count` worlds, one world at a time per thread, `threads` threads.
+// Returns wall seconds of the timed steps (after `warmup` untimed steps per world).
+double BXREF(bench)(const bx_shape_library* lib, const bx_world_desc* worlds, uint32_t world_count,
+                    float dt, uint32_t vit, uint32_t pit, uint32_t warmup, uint32_t steps,
+                    uint32_t threads) {
+    InitRegistersOnce();
+    if (threads == 0) threads = 1;
+    std::atomic<uint32_t> next{0};
+    std::vector<double> busy(threads, 0.0);
+    auto worker = [&](uint32_t t) {
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= world_count) break;
+            RefWorld* w = CreateWorld(lib, worlds + i);
+            w->world.SetContactListener(nullptr);
+            for (uint32_t s = 0; s < warmup; ++s) w->world.Step(scalar(dt), vit, pit);
+            auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t s = 0; s < steps; ++s) w->world.Step(scalar(dt), vit, pit);
+            auto t1 = std::chrono::steady_clock::now();
+            busy[t] += std::chrono::duration<double>(t1 - t0).count();
+            BXREF(destroy)(w);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; ++t) pool.emplace_back(worker, t);
+    for (auto& th : pool) th.join();
+    // Aggregate throughput over threads: timed work runs concurrently, so the wall time of
+    // the timed regions is the busy time of the most loaded thread.
+    double wall = 0.0;
+    for (double b : busy) wall = std::max(wall, b);
+    return wall;
+}
+
+}  // extern "C"

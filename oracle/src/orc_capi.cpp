@@ -1,0 +1,306 @@
+// orc_capi.cpp -- oracle (test infrastructure): C ABI of the CPU restatement, with the same
+// signatures as the reference harness (oracle/ref_harness.cpp), so tests drive both alike.
+// World construction restates b3World::CreateBody (world.cpp:94-110, body.cpp:25-103),
+// b3Body::CreateFixture (body.cpp:105-136, fixture.cpp:43-56) and b3World::CreateJoint.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include "orc_world.h"
+#include "../../include/bxstep.h"
+
+namespace orc {
+void CreateJoint(World& w, const bx_joint_desc& jd);
+bool BuildMesh(Mesh& m, const bx_mesh_desc& d);
+void InitTriangleShape(Shape& s, const Vec3& a, const Vec3& b, const Vec3& c);
+}
+
+using namespace orc;
+
+static Vec3 V3(const float* p) { return Vec3(p[0], p[1], p[2]); }
+static Quat Q4(const float* p) { return Quat(Vec3(p[0], p[1], p[2]), p[3]); }
+
+static bool BuildHull(Hull& h, const bx_hull_desc& d) {
+    if (d.vertex_count < 4 || d.edge_count < 6 || d.face_count < 4) return false;
+    h.vertices.resize(d.vertex_count);
+    for (u32 i = 0; i < d.vertex_count; ++i) h.vertices[i] = V3(d.vertices + 3 * i);
+    h.edges.resize(d.edge_count);
+    for (u32 i = 0; i < d.edge_count; ++i) {
+        const uint32_t* e = d.edges + 5 * i;
+        h.edges[i] = HalfEdge{e[0], e[1], e[2], e[3], e[4]};
+    }
+    h.faces.assign(d.faces, d.faces + d.face_count);
+    h.planes.resize(d.face_count);
+    for (u32 i = 0; i < d.face_count; ++i) h.planes[i] = Plane(V3(d.planes + 4 * i), d.planes[4 * i + 3]);
+    h.centroid = V3(d.centroid);
+    return true;
+}
+
+static World* Create(const bx_shape_library* lib, const bx_world_desc* wd) {
+    World* w = new World();
+    w->gravity = V3(wd->gravity);
+    w->sleeping = (wd->flags & BX_WORLD_SLEEPING) != 0;
+    w->warmStarting = (wd->flags & BX_WORLD_NO_WARM_START) == 0;
+    w->clearForces = (wd->flags & BX_WORLD_NO_CLEAR_FORCES) == 0;
+    if (lib) {
+        w->hulls.resize(lib->hull_count);
+        for (u32 i = 0; i < lib->hull_count; ++i)
+            if (!BuildHull(w->hulls[i], lib->hulls[i])) { delete w; return nullptr; }
+        w->meshes.resize(lib->mesh_count);
+        for (u32 i = 0; i < lib->mesh_count; ++i)
+            if (!BuildMesh(w->meshes[i], lib->meshes[i])) { delete w; return nullptr; }
+    }
+    w->bodies.resize(wd->body_count);
+    for (u32 i = 0; i < wd->body_count; ++i) {
+        const bx_body_desc& d = wd->bodies[i];
+        Body& b = w->bodies[i];
+        b.type = d.type;
+        b.flags = 0;
+        if (d.flags & BX_BODY_AWAKE) b.flags |= e_awakeFlag;
+        if (d.flags & BX_BODY_AUTO_SLEEP) b.flags |= e_autoSleepFlag;
+        if (b.type == e_dynamicBody) {
+            b.mass = 1; b.invMass = 1;
+            b.flags |= d.flags & (BX_BODY_FIXED_ROT_X | BX_BODY_FIXED_ROT_Y | BX_BODY_FIXED_ROT_Z);
+        } else {
+            b.mass = 0; b.invMass = 0;
+        }
+        b.I.SetZero(); b.invI.SetZero(); b.worldInvI.SetZero();
+        b.force.SetZero(); b.torque.SetZero();
+        b.linearVelocity = V3(d.linear_velocity);
+        b.angularVelocity = V3(d.angular_velocity);
+        b.sweep.localCenter.SetZero();
+        b.sweep.worldCenter = V3(d.position);
+        b.sweep.orientation = Q4(d.orientation);
+        b.sweep.worldCenter0 = V3(d.position);
+        b.sweep.orientation0 = Q4(d.orientation);
+        b.sweep.t0 = 0;
+        b.xf.translation = b.sweep.worldCenter;
+        b.xf.rotation = b.sweep.orientation;
+        b.linearDamping = V3(d.linear_damping);
+        b.angularDamping = V3(d.angular_damping);
+        b.gravityScale = V3(d.gravity_scale);
+        b.linearSleepTolerance = d.linear_sleep_tolerance;
+        b.angularSleepTolerance = d.angular_sleep_tolerance;
+        b.sleepTime = 0;
+    }
+    w->fixtures.resize(wd->fixture_count);
+    for (u32 i = 0; i < wd->fixture_count; ++i) {
+        const bx_fixture_desc& d = wd->fixtures[i];
+        Fixture& f = w->fixtures[i];
+        if (d.body >= wd->body_count) { delete w; return nullptr; }
+        f.body = d.body;
+        f.friction = d.friction;
+        f.restitution = d.restitution;
+        f.isSensor = d.is_sensor != 0;
+        f.density = d.density;
+        Shape& s = f.shape;
+        s.type = d.shape_type;
+        switch (d.shape_type) {
+        case BX_SHAPE_SPHERE: s.v1 = V3(d.vertex1); s.radius = d.radius; break;
+        case BX_SHAPE_CAPSULE: s.v1 = V3(d.vertex1); s.v2 = V3(d.vertex2); s.radius = d.radius; break;
+        case BX_SHAPE_TRIANGLE: InitTriangleShape(s, V3(d.vertex1), V3(d.vertex2), V3(d.vertex3)); break;
+        case BX_SHAPE_HULL:
+            if (d.shape_index >= w->hulls.size()) { delete w; return nullptr; }
+            s.hull = &w->hulls[d.shape_index];
+            s.radius = kHullRadius;
+            break;
+        case BX_SHAPE_MESH:
+            if (d.shape_index >= w->meshes.size()) { delete w; return nullptr; }
+            s.mesh = &w->meshes[d.shape_index];
+            s.scale = V3(d.mesh_scale);
+            s.radius = kHullRadius;
+            break;
+        default: delete w; return nullptr;
+        }
+        Body& b = w->bodies[d.body];
+        AABB aabb;
+        ComputeAABB(s, b.xf, &aabb);
+        f.proxyId = w->CreateProxy(aabb, i);
+        b.fixtures.insert(b.fixtures.begin(), i);
+        if (f.density > scalar(0)) w->ResetMass(d.body);
+        w->newContacts = true;
+    }
+    for (u32 i = 0; i < wd->joint_count; ++i) CreateJoint(*w, wd->joints[i]);
+    return w;
+}
+
+static void FillContact(const World& w, const Contact* c, bx_contact_state* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->fixture_a = c->fixtureA;
+    out->fixture_b = c->fixtureB;
+    out->touching = (c->flags & 2u) ? 1u : 0u;
+    out->manifold_count = c->manifoldCount;
+    if (w.fixtures[c->fixtureA].shape.type == e_hull && w.fixtures[c->fixtureB].shape.type == e_hull) {
+        out->cache[0] = c->cache.cacheType;
+        out->cache[1] = c->cache.featureType;
+        out->cache[2] = c->cache.index1;
+        out->cache[3] = c->cache.index2;
+    }
+    for (u32 i = 0; i < c->manifoldCount && i < kMaxManifolds; ++i) {
+        const Manifold& m = c->manifolds[i];
+        bx_manifold_state& ms = out->manifolds[i];
+        ms.point_count = m.pointCount;
+        ms.tangent_impulse[0] = m.tangentImpulse.x;
+        ms.tangent_impulse[1] = m.tangentImpulse.y;
+        ms.motor_impulse = m.motorImpulse;
+        for (u32 k = 0; k < m.pointCount; ++k) {
+            const ManifoldPoint& p = m.points[k];
+            bx_manifold_point_state& ps = ms.points[k];
+            for (int a = 0; a < 3; ++a) {
+                ps.local_normal1[a] = p.localNormal1[a];
+                ps.local_point1[a] = p.localPoint1[a];
+                ps.local_point2[a] = p.localPoint2[a];
+            }
+            ps.normal_impulse = p.normalImpulse;
+            ps.triangle_key = p.id.triangleKey;
+            ps.key[0] = p.id.pair.inEdge1;
+            ps.key[1] = p.id.pair.outEdge1;
+            ps.key[2] = p.id.pair.inEdge2;
+            ps.key[3] = p.id.pair.outEdge2;
+            ps.persisting = p.persisting ? 1u : 0u;
+            ps.edge_contact = p.edgeContact ? 1u : 0u;
+        }
+    }
+}
+
+extern "C" {
+
+void* orc_create(const bx_shape_library* lib, const bx_world_desc* wd) { return Create(lib, wd); }
+void orc_destroy(void* h) { delete (World*)h; }
+
+void orc_step(void* h, float dt, uint32_t vit, uint32_t pit, uint32_t n) {
+    World* w = (World*)h;
+    for (uint32_t i = 0; i < n; ++i) w->Step(dt, vit, pit);
+}
+
+int orc_get_counts(void* h, bx_world_counts* out) {
+    World* w = (World*)h;
+    std::memset(out, 0, sizeof(*out));
+    out->body_count = u32(w->bodies.size());
+    out->fixture_count = u32(w->fixtures.size());
+    out->joint_count = u32(w->joints.size());
+    out->contact_count = u32(w->contactList.size());
+    for (Contact* c : w->contactList) out->touching_count += (c->flags & 2u) ? 1u : 0u;
+    out->pair_count = u32(w->lastPairs.size());
+    out->island_count = w->lastIslandCount;
+    for (const Body& b : w->bodies) out->awake_count += b.IsAwake() ? 1u : 0u;
+    return 0;
+}
+
+int orc_get_bodies(void* h, bx_body_state* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = std::min<u32>(cap, u32(w->bodies.size()));
+    for (u32 i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        bx_body_state& s = out[i];
+        std::memset(&s, 0, sizeof(s));
+        for (int a = 0; a < 3; ++a) {
+            s.position[a] = b.xf.translation[a];
+            s.world_center[a] = b.sweep.worldCenter[a];
+            s.linear_velocity[a] = b.linearVelocity[a];
+            s.angular_velocity[a] = b.angularVelocity[a];
+        }
+        s.orientation[0] = b.xf.rotation.v.x;
+        s.orientation[1] = b.xf.rotation.v.y;
+        s.orientation[2] = b.xf.rotation.v.z;
+        s.orientation[3] = b.xf.rotation.s;
+        s.sleep_time = b.sleepTime;
+        s.flags = b.flags;
+        s.island_index = b.islandIndex;
+    }
+    return int(n);
+}
+
+int orc_get_contacts(void* h, bx_contact_state* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = 0;
+    for (Contact* c : w->contactList) {
+        if (n >= cap) break;
+        FillContact(*w, c, out + n++);
+    }
+    return int(n);
+}
+
+int orc_get_pairs(void* h, bx_pair* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = 0;
+    for (auto& p : w->lastPairs) {
+        if (n < cap) { out[n].proxy1 = p.first; out[n].proxy2 = p.second; }
+        ++n;
+    }
+    return int(n);
+}
+
+int orc_get_island_contacts(void* h, uint32_t* out, uint32_t cap, uint32_t* count) {
+    World* w = (World*)h;
+    u32 n = 0;
+    for (Contact* c : w->lastIslandOrder) {
+        auto it = std::find(w->contactList.begin(), w->contactList.end(), c);
+        if (n < cap) out[n] = it == w->contactList.end() ? BX_NULL_INDEX : u32(it - w->contactList.begin());
+        ++n;
+    }
+    *count = n;
+    return 0;
+}
+
+int orc_get_proxy_ids(void* h, uint32_t* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = std::min<u32>(cap, u32(w->fixtures.size()));
+    for (u32 i = 0; i < n; ++i) out[i] = w->fixtures[i].proxyId;
+    return int(n);
+}
+
+int orc_get_fat_aabbs(void* h, float* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = std::min<u32>(cap, u32(w->fixtures.size()));
+    for (u32 i = 0; i < n; ++i) {
+        const AABB& a = w->fatAABB[w->fixtures[i].proxyId];
+        for (int k = 0; k < 3; ++k) { out[6 * i + k] = a.lowerBound[k]; out[6 * i + 3 + k] = a.upperBound[k]; }
+    }
+    return int(n);
+}
+
+int orc_get_mass(void* h, float* out, uint32_t cap) {
+    World* w = (World*)h;
+    u32 n = std::min<u32>(cap, u32(w->bodies.size()));
+    for (u32 i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        float* o = out + 23 * i;
+        o[0] = b.mass; o[1] = b.invMass;
+        for (int a = 0; a < 3; ++a) o[2 + a] = b.sweep.localCenter[a];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) { o[5 + 3 * c + r] = b.I[c][r]; o[14 + 3 * c + r] = b.invI[c][r]; }
+    }
+    return int(n);
+}
+
+// CPU baseline with the restatement: same contract as the reference harness's bench entry.
+double orc_bench(const bx_shape_library* lib, const bx_world_desc* worlds, uint32_t world_count, float dt,
+                 uint32_t vit, uint32_t pit, uint32_t warmup, uint32_t steps, uint32_t threads) {
+    if (threads == 0) threads = 1;
+    std::atomic<uint32_t> next{0};
+    std::vector<double> busy(threads, 0.0);
+    auto worker = [&](uint32_t t) {
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= world_count) break;
+            World* w = Create(lib, worlds + i);
+            if (!w) continue;
+            for (uint32_t s = 0; s < warmup; ++s) w->Step(dt, vit, pit);
+            auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t s = 0; s < steps; ++s) w->Step(dt, vit, pit);
+            auto t1 = std::chrono::steady_clock::now();
+            busy[t] += std::chrono::duration<double>(t1 - t0).count();
+            delete w;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; ++t) pool.emplace_back(worker, t);
+    for (auto& th : pool) th.join();
+    double wall = 0.0;
+    for (double b : busy) wall = std::max(wall, b);
+    return wall;
+}
+
+}  // extern "C"

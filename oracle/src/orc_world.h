@@ -1,0 +1,156 @@
+// orc_world.h -- oracle (test infrastructure): world state, restating the reference's dynamics
+// data model (include/bounce/dynamics/{world,body,fixture}.h, contacts/contact.h,
+// joints/joint.h, joints/revolute_joint.h) with the same list-order semantics.
+#pragma once
+#include <vector>
+#include <cstdint>
+#include "orc_geom.h"
+
+namespace orc {
+
+enum BodyType : u32 { e_staticBody = 0, e_kinematicBody = 1, e_dynamicBody = 2 };
+enum BodyFlags : u32 {
+    e_awakeFlag = 0x0001, e_islandFlag = 0x0002, e_autoSleepFlag = 0x0004,
+    e_fixedRotationX = 0x0008, e_fixedRotationY = 0x0010, e_fixedRotationZ = 0x0020
+};
+
+struct Contact;
+struct Joint;
+
+struct JointEdge { u32 joint; u32 other; };
+
+struct Body {
+    u32 type = e_staticBody;
+    u32 flags = 0;
+    u32 islandIndex = 0;
+    scalar mass = 0, invMass = 0;
+    Mat33 I, invI, worldInvI;
+    Vec3 force, torque;
+    Vec3 linearVelocity, angularVelocity;
+    Sweep sweep;
+    Transform xf;
+    Vec3 linearDamping, angularDamping, gravityScale;
+    scalar linearSleepTolerance = 0, angularSleepTolerance = 0, sleepTime = 0;
+    std::vector<u32> fixtures;       // list order: front = last created (body.cpp:120-122)
+    std::vector<JointEdge> joints;   // list order: front = last created (joint_manager.cpp)
+    bool IsAwake() const { return (flags & e_awakeFlag) != 0; }
+    void SetAwake(bool flag) {       // body.h:588-607
+        if (flag) {
+            if (!IsAwake()) { flags |= e_awakeFlag; sleepTime = 0; }
+        } else {
+            flags &= ~u32(e_awakeFlag);
+            sleepTime = 0;
+            force.SetZero(); torque.SetZero();
+            linearVelocity.SetZero(); angularVelocity.SetZero();
+        }
+    }
+    void SynchronizeTransform() {    // body.h:847-851
+        xf.rotation = sweep.orientation;
+        xf.translation = sweep.worldCenter - Mul(xf.rotation, sweep.localCenter);
+    }
+};
+
+struct Fixture {
+    u32 body = 0;
+    Shape shape;
+    scalar density = 0, friction = 0, restitution = 0;
+    bool isSensor = false;
+    u32 proxyId = 0;
+    std::vector<Contact*> contacts;  // contact edge list, front = last created (contact_manager.cpp:118-139)
+};
+
+struct Contact {
+    u32 fixtureA = 0, fixtureB = 0;
+    u32 flags = 0;                   // e_islandFlag 1, e_touchingFlag 2 (contact.h)
+    u32 manifoldCapacity = 1;
+    u32 manifoldCount = 0;
+    Manifold manifolds[kMaxManifolds];
+    FeatureCache cache;
+    bool isMesh = false;
+    // mesh contact state (mesh_contact.h)
+    AABB meshAABB;
+    bool meshAABBMoved = false;
+    std::vector<u32> triangles;
+};
+
+struct RevoluteState;
+struct Joint {
+    u32 type = 0;
+    u32 bodyA = 0, bodyB = 0;
+    bool collideConnected = false;
+    bool islandFlag = false;
+    // revolute joint definition and solver state (revolute_joint.h/.cpp)
+    Vec3 localAnchorA, localAnchorB;
+    Quat localRotationA, localRotationB, referenceRotation;
+    bool enableLimit = false, enableMotor = false;
+    scalar lowerAngle = 0, upperAngle = 0, motorSpeed = 0, maxMotorTorque = 0;
+    scalar motorImpulse = 0, lowerImpulse = 0, upperImpulse = 0;
+    Vec3 linearImpulse;
+    Vec2 angularImpulse;
+    // per-solve temporaries
+    u32 indexA = 0, indexB = 0;
+    scalar mA = 0, mB = 0;
+    Mat33 iA, iB, localInvIA, localInvIB;
+    Vec3 localCenterA, localCenterB;
+    Vec3 rA, rB;
+    Mat33 linearMass;
+    Vec3 motorAxis;  // world hinge axis
+    scalar motorMass = 0, axialMass = 0;
+    Vec3 angularAxisA1, angularAxisA2;  // unused placeholders for alternative formulations
+    Mat22 angularMass;
+    Vec3 u2x, u2y;   // hinge perpendicular axes for the 2-dof angular row
+    scalar angle = 0;
+};
+
+struct Position { Vec3 x; Quat q; Mat33 I; };
+struct Velocity { Vec3 v, w; };
+
+struct TimeStep {
+    scalar dt, inv_dt, dtRatio;
+    u32 velocityIterations, positionIterations;
+    bool warmStarting;
+};
+
+struct World {
+    Vec3 gravity;
+    bool sleeping = false, warmStarting = true, clearForces = true, newContacts = false;
+    scalar inv_dt0 = 0;
+    std::vector<Hull> hulls;
+    std::vector<Mesh> meshes;
+    std::vector<Body> bodies;        // creation order; body list order is reverse creation
+    std::vector<Fixture> fixtures;   // creation order
+    std::vector<Joint> joints;       // creation order; joint list order is reverse creation
+    // broadphase (broad_phase.cpp, dynamic_tree.cpp)
+    std::vector<AABB> fatAABB;       // indexed by proxy id
+    std::vector<u32> proxyFixture;   // proxy id -> fixture, kNullU32 for internal nodes
+    std::vector<u32> moveBuffer;
+    std::vector<u32> freeStack;      // freed node ids (LIFO), then sequential ids from nextNode
+    u32 nextNode = 0;
+    u32 leafCount = 0;
+    std::vector<std::pair<u32, u32>> lastPairs;  // unique sorted pairs of the last FindPairs
+    // contacts: world contact list, front = last created (contact_manager.cpp:105-113)
+    std::vector<Contact*> contactList;
+    std::vector<Contact*> lastIslandOrder;
+    u32 lastIslandCount = 0;
+
+    ~World();
+    u32 AllocateNode();
+    u32 CreateProxy(const AABB& aabb, u32 fixture);
+    void MoveProxy(u32 proxy, const AABB& aabb, const Vec3& displacement);
+    void SynchronizeFixture(u32 f, const Transform& xf1, const Transform& xf2);
+    void SynchronizeFixtures(u32 b);
+    void ResetMass(u32 b);
+    bool ShouldCollide(u32 a, u32 b) const;
+    void FindNewContacts();
+    void AddPair(u32 fA, u32 fB);
+    void UpdateContacts();
+    void DestroyContact(Contact* c);
+    void UpdateContact(Contact* c);
+    void Collide(Contact* c);
+    void Solve(const TimeStep& step);
+    void SolveIsland(const TimeStep& step, std::vector<u32>& bodies, std::vector<Contact*>& contacts,
+                     std::vector<u32>& joints);
+    void Step(scalar dt, u32 vit, u32 pit);
+};
+
+}  // namespace orc
