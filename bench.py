@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched b3World::Step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds 4096] [--levels 20]
+
+Workload: the batched-stacking configuration, 4,096 independent worlds, each a 20-level box
+pyramid (210 unit boxes, friction 0.6, seeded +-1 mm jitter) on a static ground box, stepped
+together at dt = 1/60 s with 8 velocity / 2 position iterations in FP32 (sleeping off). A "step"
+is one b3World::Step of every world of the batch; inputs are resident in HBM before timing.
+
+Printed (one JSON line): body-steps/s of the batched path (value), the plain per-world GPU path
+(one world per launch sequence, measured on a sample of worlds), their ratio, p50/p99 step
+latency, the roofline of the dominant kernel (the island solver) from HIP events on the
+library's stream, and the reference CPU Step timed on this host's cores (one world per thread)
+on a bounded sample of the same worlds.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of the island-solver kernel (the step's dominant kernel), per unit of work:
+#  * awake non-static body: 256 B of persistent state read (pose, velocity, force, mass and
+#    inertia, damping, sleep data) and 176 B written (pose, previous pose, velocity, world
+#    inverse inertia, sleep data, cleared force) -> 432 B;
+#  * island contact: its manifold (4 points x 68 B + 28 B) read and written -> 600 B.
+BODY_BYTES = 432
+CONTACT_BYTES = 600
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=60)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--worlds", type=int, default=4096)
+    p.add_argument("--levels", type=int, default=20)
+    p.add_argument("--plain-worlds", type=int, default=16)
+    p.add_argument("--cpu-worlds", type=int, default=256)
+    p.add_argument("--cpu-steps", type=int, default=60)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(args, scene_factory, body_count_per_world):
+    """Reference Step (oracle/_ref build) on host cores, one world per thread, bounded sample."""
+    try:
+        from oracle import oracle as O
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "error": str(e)}
+    kind = "reference" if O.available("ref32") else "port"
+    lib = "ref32" if kind == "reference" else "oracle"
+    threads = min(os.cpu_count() or 1, 16)
+    sample = scene_factory(args.cpu_worlds)
+    secs = O.bench(lib, sample, warmup=0, steps=args.cpu_steps, threads=threads)
+    value = args.cpu_worlds * body_count_per_world * args.cpu_steps / secs if secs > 0 else None
+    return {"value": value, "unit": "body-steps/s", "cores": threads, "kind": kind,
+            "sample": f"{args.cpu_worlds} worlds x {args.cpu_steps} steps from step 0, one world per thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    from bounce_amd import device, scenes
+
+    def factory(n):
+        return scenes.pyramids(n, levels=args.levels, seed=2)
+
+    scene = factory(args.worlds)
+    bodies_per_world = scene.worlds[0].body_count
+    total_bodies = scene.body_count
+    ctx = device.Context(0)
+    batch = device.Batch(ctx, scene)
+    # warmup
+    for _ in range(args.warmup):
+        batch.step()
+    batch.sync()
+    # timed region: K steps, synchronised on both sides
+    t0 = time.perf_counter()
+    batch.step(n=args.steps)
+    batch.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = total_bodies * args.steps / wall
+    # per-step latency (GPU events around one step) and per-phase times of the dominant kernel
+    lat = []
+    phases = []
+    for _ in range(10):
+        batch.step()
+        t = batch.timings()
+        lat.append(t[4])
+        phases.append(t)
+    phases = np.array(phases)
+    # counts for the algorithmic bytes of the solver kernel (sampled over worlds)
+    sample_w = list(range(0, args.worlds, max(1, args.worlds // 16)))
+    awake = np.mean([batch.counts(w).awake_count - 1 for w in sample_w])
+    island_contacts = np.mean([len(batch.island_contacts(w)) for w in sample_w])
+    status = int(np.bitwise_or.reduce([batch.counts(w).status for w in sample_w]))
+    solve_ms = float(np.median(phases[:, 2]))
+    solve_bytes = args.worlds * (awake * BODY_BYTES + island_contacts * CONTACT_BYTES)
+    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    batch.close()
+    # plain per-world path: the same kernels, one world per launch sequence
+    plain_scene = factory(args.plain_worlds)
+    pb = device.Batch(ctx, plain_scene)
+    worlds = np.arange(args.plain_worlds, dtype=np.uint32)
+    ones = np.ones(args.plain_worlds, dtype=np.uint32)
+    for _ in range(2):
+        pb.step_subset(worlds, ones)
+    pb.sync()
+    p0 = time.perf_counter()
+    plain_steps = 5
+    for _ in range(plain_steps):
+        pb.step_subset(worlds, ones)
+    pb.sync()
+    p1 = time.perf_counter()
+    plain_value = plain_scene.body_count * plain_steps / (p1 - p0)
+    pb.close()
+    ctx.close()
+    cpu = None if args.no_cpu else cpu_baseline(args, factory, bodies_per_world)
+    out = {
+        "metric": "body-steps/sec on 1 MI355X, batched vs plain per-world; p99 step latency",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded pyramids)",
+        "config": {"workload": f"batched stacking: {args.worlds} worlds x {args.levels}-level box pyramid "
+                               f"({bodies_per_world} bodies incl. ground), dt=1/60, 8/2 iterations, sleep off",
+                   "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
+        "plain_per_world": {"value": plain_value, "unit": "body-steps/s",
+                            "sample": f"{args.plain_worlds} worlds x {plain_steps} steps, one world per launch"},
+        "batched_over_plain": value / plain_value,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "phase_ms": {"update_contacts": float(np.median(phases[:, 0])), "islands": float(np.median(phases[:, 1])),
+                     "solve": solve_ms, "sync_and_pairs": float(np.median(phases[:, 3]))},
+        "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "status": status,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,51 @@
+"""Builds libbxstep.so (HIP for gfx950) in-tree: bounce_amd/lib/libbxstep.so.
+
+Floating-point flags are part of the product's contract (bit-exact binary32 against the
+reference): no FMA contraction, IEEE division and square root, no denormal flushing, and packed
+FP32 instructions disabled (-packed-fp32-ops).
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB_DIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIB_DIR, "libbxstep.so")
+SOURCES = ["bx_step.hip", "bx_api.hip"]
+HEADERS = ["bx_math.h", "bx_types.h", "bx_collide.h", "bx_solver.h", "bx_shapes.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("BX_OFFLOAD_ARCH", "gfx950")
+
+FP_FLAGS = [
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def command(out=LIB, extra=()):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    return ([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-I", os.path.join(REPO, "include"), "-Wno-unused-result"] + FP_FLAGS + list(extra)
+            + ["-o", out] + srcs)
+
+
+def build(verbose=True):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

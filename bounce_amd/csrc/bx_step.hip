@@ -1,0 +1,717 @@
+// bx_step.hip -- the batched b3World::Step on the GPU (gfx950).
+//
+// One step is a fixed sequence of kernels, each launched over all worlds of the batch
+// (blockIdx.x = world, one workgroup per world):
+//
+//   k_find_pairs (only when new fixtures were added)   b3ContactManager::FindNewContacts
+//   k_update_contacts   narrowphase + stable compaction   b3ContactManager::UpdateContacts
+//   k_islands           fixture adjacency + island DFS    b3World::Solve (world.cpp:253-422)
+//   k_solve             per-island integration + solver   b3Island::Solve (island.cpp:81-307)
+//   k_sync              fixture AABBs / fat-AABB update   b3Body::SynchronizeFixtures, MoveProxy
+//   k_find_pairs        pairs + contact creation          b3BroadPhase::FindPairs + AddPair
+//
+// Exactness: every floating-point formula is the reference's (see bx_math.h); every ordering
+// decision the reference takes implicitly through list order is reproduced explicitly:
+// contact order lists, fixture edge lists rebuilt in reference order, DFS over bodies in body-
+// list order, Gauss-Seidel in island contact order. Work is parallel only where the reference's
+// result is order-independent (per contact narrowphase, per fixture AABBs, per pair filtering,
+// independent islands).
+#include "bx_collide.h"
+#include "bx_solver.h"
+
+namespace bx {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ HullView GetHull(const BatchView& B, u32 index) {
+    HullInfo hi = B.hulls[index];
+    HullView h;
+    h.vertices = B.hullVertices + hi.vBase;
+    h.edges = B.hullEdges + hi.eBase;
+    h.faces = B.hullFaces + hi.fBase;
+    h.planes = B.hullPlanes + hi.fBase;
+    h.centroid = Vec3(hi.cx, hi.cy, hi.cz);
+    h.vertexCount = hi.vCount;
+    h.edgeCount = hi.eCount;
+    h.faceCount = hi.fCount;
+    return h;
+}
+
+// Block-wide exclusive scan of one u32 per thread; returns the prefix and writes the total.
+__device__ u32 BlockExclusiveScan(u32 value, u32* total) {
+    __shared__ u32 warpSums[kBlock / 64 + 1];
+    const u32 lane = threadIdx.x & 63u;
+    const u32 warp = threadIdx.x >> 6;
+    u32 x = value;
+    for (u32 d = 1; d < 64; d <<= 1) {
+        u32 y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) warpSums[warp] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 run = 0;
+        for (u32 i = 0; i < blockDim.x / 64; ++i) {
+            u32 s = warpSums[i];
+            warpSums[i] = run;
+            run += s;
+        }
+        warpSums[kBlock / 64] = run;
+    }
+    __syncthreads();
+    u32 result = x - value + warpSums[warp];
+    *total = warpSums[kBlock / 64];
+    __syncthreads();
+    return result;
+}
+
+__device__ __forceinline__ bool BodiesShouldCollide(const BatchView& B, const WorldInfo& wi, u32 a, u32 b) {
+    // b3Body::ShouldCollide (body.cpp:240-263)
+    const Body& A = B.bodies[wi.bodyBase + a];
+    const Body& Bb = B.bodies[wi.bodyBase + b];
+    if (A.type != kDynamic && Bb.type != kDynamic) return false;
+    for (u32 k = 0; k < A.jointCount; ++k) {
+        JointEdge je = B.jointEdges[wi.jointEdgeBase + A.jointBegin + k];
+        if (je.other == b && B.joints[wi.jointBase + je.joint].collideConnected == 0) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void WakeBody(Body& b) {  // b3Body::SetAwake(true)
+    if ((b.flags & kAwake) == 0) {
+        atomicOr(&b.flags, kAwake);
+        b.sleepTime = 0;
+    }
+}
+
+// ================================================================== update contacts
+// b3ContactManager::UpdateContacts (contact_manager.cpp:165-227) + b3Contact::Update
+// (contact.cpp:183-285) for one contact. Returns false if the contact is destroyed.
+__device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot, bool applyWakes) {
+    Contact& c = B.contacts[wi.contactBase + slot];
+    const Fixture fA = B.fixtures[wi.fixtureBase + c.fixtureA];
+    const Fixture fB = B.fixtures[wi.fixtureBase + c.fixtureB];
+    Body& bA = B.bodies[wi.bodyBase + fA.body];
+    Body& bB = B.bodies[wi.bodyBase + fB.body];
+    bool sensor = fA.isSensor || fB.isSensor;
+    auto destroy = [&]() {
+        if (applyWakes && !sensor && c.manifoldCount > 0 && c.manifold.pointCount > 0) {
+            WakeBody(bA);
+            WakeBody(bB);
+        }
+        return false;
+    };
+    if (!BodiesShouldCollide(B, wi, fA.body, fB.body)) return destroy();
+    bool activeA = (bA.flags & kAwake) && bA.type != kStatic;
+    bool activeB = (bB.flags & kAwake) && bB.type != kStatic;
+    if (!activeA && !activeB) return true;
+    if (!TestOverlap(B.fatAABB[wi.fixtureBase + c.fixtureA], B.fatAABB[wi.fixtureBase + c.fixtureB])) return destroy();
+
+    bool wasTouching = (c.flags & 2u) != 0;
+    bool touching = false;
+    if (sensor) {
+        atomicOr(&ws.status, 0x100u);  // sensor overlap tests are not supported on the device path
+        c.manifoldCount = 0;
+    } else {
+        const u32 oldCount = c.manifoldCount;
+        Manifold old = c.manifold;
+        Manifold m;
+        // b3Manifold::Initialize (collision.cpp:23-38); point geometry and ids keep their old
+        // contents exactly as the reference's reused manifold storage does.
+        m = old;
+        m.pointCount = 0;
+        m.tangentImpulse.SetZero();
+        m.motorImpulse = 0;
+        m.motorSpeed = 0;
+        m.tangentSpeed1 = 0;
+        m.tangentSpeed2 = 0;
+        for (u32 i = 0; i < kMaxManifoldPoints; ++i) {
+            m.points[i].normalImpulse = 0;
+            m.points[i].persisting = 0;
+            m.points[i].edgeContact = 0;
+        }
+        Transform xfA(bA.xfT, bA.q), xfB(bB.xfT, bB.q);
+        if (fA.shapeType == 3u && fB.shapeType == 3u) {
+            // b3HullContact::Evaluate (hull_contact.cpp:41-47)
+            Transform xf0A = SweepTransform(bA.x0, bA.q0, bA.x, bA.q, bA.localCenter, scalar(0));
+            Transform xf0B = SweepTransform(bB.x0, bB.q0, bB.x, bB.q, bB.localCenter, scalar(0));
+            HullView hA = GetHull(B, fA.shapeIndex), hB = GetHull(B, fB.shapeIndex);
+            FaceScratch scratch;
+            u32 overflow = 0;
+            FeatureCache cache = c.cache;
+            CollideHulls(m, xfA, hA, fA.radius, xfB, hB, fB.radius, cache, xf0A, xf0B, scratch, overflow);
+            c.cache = cache;
+            if (overflow) atomicOr(&ws.status, 0x2u);
+        } else {
+            atomicOr(&ws.status, 0x200u);  // shape pair not supported on the device path yet
+        }
+        c.manifoldCount = 1;
+        if (wi.flags & 0x2u) {
+            // warm starting disabled: nothing to match
+        } else if (oldCount > 0) {
+            // b3Manifold::Initialize(const b3Manifold&) (collision.cpp:40-61)
+            m.tangentImpulse = old.tangentImpulse;
+            m.motorImpulse = old.motorImpulse;
+            for (u32 i = 0; i < m.pointCount; ++i) {
+                ManifoldPoint& p1 = m.points[i];
+                for (u32 j = 0; j < old.pointCount; ++j) {
+                    const ManifoldPoint& p2 = old.points[j];
+                    if (p2.triangleKey == p1.triangleKey && p2.key[0] == p1.key[0] && p2.key[1] == p1.key[1] &&
+                        p2.key[2] == p1.key[2] && p2.key[3] == p1.key[3]) {
+                        p1.normalImpulse = p2.normalImpulse;
+                        p1.persisting = 1;
+                        break;
+                    }
+                }
+            }
+        }
+        c.manifold = m;
+        touching = m.pointCount > 0;
+        if (touching != wasTouching && applyWakes) {
+            WakeBody(bA);
+            WakeBody(bB);
+        }
+    }
+    if (touching) c.flags |= 2u;
+    else c.flags &= ~2u;
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_update_contacts(BatchView B) {
+    const u32 w = blockIdx.x;
+    const WorldInfo wi = B.worlds[w];
+    WorldState& ws = B.wstate[w];
+    const u32 n = ws.contactCount;
+    u32* order = B.contactOrder + wi.contactBase;
+    u32* alive = B.contactAlive + wi.contactBase;
+    // The reference visits contacts newest-first and a contact can wake a sleeping body that a
+    // later contact tests for activity; with every non-static body awake the visits commute.
+    int asleep = 0;
+    for (u32 i = threadIdx.x; i < wi.bodyCount; i += blockDim.x) {
+        const Body& b = B.bodies[wi.bodyBase + i];
+        if (b.type != kStatic && (b.flags & kAwake) == 0) asleep = 1;
+    }
+    asleep = __syncthreads_or(asleep);
+    if (!asleep) {
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) alive[i] = UpdateOneContact(B, wi, ws, order[i], true) ? 1u : 0u;
+    } else if (threadIdx.x == 0) {
+        for (u32 k = n; k-- > 0;) alive[k] = UpdateOneContact(B, wi, ws, order[k], true) ? 1u : 0u;
+    }
+    __syncthreads();
+    // stable compaction of the order list; destroyed slots go back on the free stack
+    u32 kept = 0, freed = 0;
+    u32 freeBase = ws.freeCount;
+    __syncthreads();
+    for (u32 base = 0; base < n; base += blockDim.x) {
+        u32 i = base + threadIdx.x;
+        u32 a = i < n ? alive[i] : 0u;
+        u32 d = (i < n && !a) ? 1u : 0u;
+        u32 slot = i < n ? order[i] : 0u;
+        u32 totalA, totalD;
+        u32 pa = BlockExclusiveScan(a, &totalA);
+        u32 pd = BlockExclusiveScan(d, &totalD);
+        __syncthreads();
+        if (a) order[kept + pa] = slot;
+        if (d) B.freeSlots[wi.contactBase + freeBase + freed + pd] = slot;
+        kept += totalA;
+        freed += totalD;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ws.contactCount = kept;
+        ws.freeCount = freeBase + freed;
+    }
+}
+
+// ================================================================== adjacency + islands
+// Builds, for every fixture, its contact edge list in the reference's order (newest contact
+// first: contact_manager.cpp:118-139 prepends) as CSR over order-list positions.
+__device__ void BuildAdjacency(const BatchView& B, const WorldInfo& wi, const WorldState& ws, u32* off, u32* cursor) {
+    const u32 nf = wi.fixtureCount;
+    const u32 n = ws.contactCount;
+    const u32* order = B.contactOrder + wi.contactBase;
+    for (u32 f = threadIdx.x; f <= nf; f += blockDim.x) { off[f] = 0; }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+        const Contact& c = B.contacts[wi.contactBase + order[i]];
+        atomicAdd(&off[c.fixtureA], 1u);
+        atomicAdd(&off[c.fixtureB], 1u);
+    }
+    __syncthreads();
+    u32 carry = 0;
+    for (u32 base = 0; base <= nf; base += blockDim.x) {
+        u32 f = base + threadIdx.x;
+        u32 v = f <= nf ? off[f] : 0u;
+        u32 total;
+        u32 p = BlockExclusiveScan(v, &total);
+        if (f <= nf) { off[f] = carry + p; cursor[f] = carry + p; }
+        carry += total;
+        __syncthreads();
+    }
+    // newest first: walk the order list backwards (sequential; keeps the exact reference order)
+    if (threadIdx.x == 0) {
+        u32* edges = B.edgeSlots + 2 * wi.contactBase;
+        for (u32 k = n; k-- > 0;) {
+            const Contact& c = B.contacts[wi.contactBase + order[k]];
+            edges[cursor[c.fixtureA]++] = k;
+            edges[cursor[c.fixtureB]++] = k;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
+    const u32 w = blockIdx.x;
+    const WorldInfo wi = B.worlds[w];
+    WorldState& ws = B.wstate[w];
+    u32* off = B.edgeOffset + wi.fixtureBase + w;
+    u32* cursor = B.pairCounts + wi.fixtureBase + w;
+    BuildAdjacency(B, wi, ws, off, cursor);
+    const u32 n = ws.contactCount;
+    const u32* order = B.contactOrder + wi.contactBase;
+    // clear island flags (world.cpp:265-277)
+    for (u32 i = threadIdx.x; i < wi.bodyCount; i += blockDim.x) B.bodies[wi.bodyBase + i].flags &= ~kIsland;
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) B.contacts[wi.contactBase + order[i]].flags &= ~1u;
+    for (u32 i = threadIdx.x; i < wi.jointCount; i += blockDim.x) B.joints[wi.jointBase + i].islandFlag = 0;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+
+    const u32* edges = B.edgeSlots + 2 * wi.contactBase;
+    u32* stack = B.dfsStack + wi.bodyBase;
+    IslandInfo* islands = B.islands + wi.bodyBase;
+    u32* ibodies = B.islandBodies + wi.islandBodyBase;
+    uint4* icontacts = B.islandContacts + wi.contactBase;
+    u32* ijoints = B.islandJoints + wi.jointBase;
+    Body* bodies = B.bodies + wi.bodyBase;
+    u32 islandCount = 0, nb = 0, nc = 0, nj = 0;
+    u32 overflow = 0;
+    for (u32 s = wi.bodyCount; s-- > 0;) {  // body list order: newest body first
+        Body& seed = bodies[s];
+        if (seed.flags & kIsland) continue;
+        if ((seed.flags & kAwake) == 0) continue;
+        if (seed.type == kStatic) continue;
+        IslandInfo info;
+        info.bodyStart = nb;
+        info.contactStart = nc;
+        info.jointStart = nj;
+        u32 bodyCount = 0;
+        u32 stackCount = 0;
+        stack[stackCount++] = s;
+        seed.flags |= kIsland;
+        while (stackCount > 0) {
+            u32 bi = stack[--stackCount];
+            Body& b = bodies[bi];
+            b.islandIndex = bodyCount;
+            if (nb < wi.islandBodyCap) ibodies[nb] = bi;
+            else overflow = 1;
+            ++nb;
+            ++bodyCount;
+            if (b.type == kStatic) continue;
+            b.flags |= kAwake;
+            for (u32 fi = 0; fi < b.fixtureCount; ++fi) {
+                u32 f = B.fixtureList[wi.fixtureListBase + b.fixtureBegin + fi];
+                for (u32 e = off[f]; e < off[f + 1]; ++e) {
+                    u32 k = edges[e];
+                    Contact& c = B.contacts[wi.contactBase + order[k]];
+                    if (c.flags & 1u) continue;
+                    if ((c.flags & 2u) == 0) continue;
+                    if (B.fixtures[wi.fixtureBase + c.fixtureA].isSensor ||
+                        B.fixtures[wi.fixtureBase + c.fixtureB].isSensor)
+                        continue;
+                    icontacts[nc++] = make_uint4(order[k], 0u, 0u, 0u);
+                    c.flags |= 1u;
+                    u32 otherF = c.fixtureA == f ? c.fixtureB : c.fixtureA;
+                    u32 other = B.fixtures[wi.fixtureBase + otherF].body;
+                    if (bodies[other].flags & kIsland) continue;
+                    stack[stackCount++] = other;
+                    bodies[other].flags |= kIsland;
+                }
+            }
+            for (u32 ji = 0; ji < b.jointCount; ++ji) {
+                JointEdge je = B.jointEdges[wi.jointEdgeBase + b.jointBegin + ji];
+                Joint& j = B.joints[wi.jointBase + je.joint];
+                if (j.islandFlag) continue;
+                ijoints[nj++] = je.joint;
+                j.islandFlag = 1;
+                if (bodies[je.other].flags & kIsland) continue;
+                stack[stackCount++] = je.other;
+                bodies[je.other].flags |= kIsland;
+            }
+        }
+        info.bodyCount = bodyCount;
+        info.contactCount = nc - info.contactStart;
+        info.jointCount = nj - info.jointStart;
+        // Island-local body indices of each contact: static bodies join several islands and keep
+        // only their index in the last one (b3Island::Add), so resolve them now.
+        for (u32 k = info.contactStart; k < nc; ++k) {
+            const Contact& c = B.contacts[wi.contactBase + icontacts[k].x];
+            u32 a = B.fixtures[wi.fixtureBase + c.fixtureA].body;
+            u32 bb = B.fixtures[wi.fixtureBase + c.fixtureB].body;
+            icontacts[k].y = bodies[a].islandIndex;
+            icontacts[k].z = bodies[bb].islandIndex;
+        }
+        islands[islandCount++] = info;
+        for (u32 k = 0; k < bodyCount; ++k) {  // world.cpp:411-419
+            u32 bi = ibodies[info.bodyStart + k];
+            if (bodies[bi].type == kStatic) bodies[bi].flags &= ~kIsland;
+        }
+    }
+    ws.islandCount = islandCount;
+    ws.islandBodyCount = nb;
+    ws.islandContactCount = nc;
+    ws.islandJointCount = nj;
+    if (overflow) ws.status |= 0x2u;
+}
+
+// ================================================================== island solve
+struct StepParams {
+    scalar dt, inv_dt;
+    u32 velocityIterations, positionIterations;
+};
+
+__device__ void SolveIsland(const BatchView& B, const WorldInfo& wi, WorldState& ws, const IslandInfo& isl,
+                            const StepParams& sp) {
+    const scalar h = sp.dt;
+    const scalar dtRatio = ws.inv_dt0 * sp.dt;
+    const bool warm = (wi.flags & 0x2u) == 0;
+    Body* bodies = B.bodies + wi.bodyBase;
+    const u32* ib = B.islandBodies + wi.islandBodyBase + isl.bodyStart;
+    const uint4* ic = B.islandContacts + wi.contactBase + isl.contactStart;
+    SolverBody* sb = B.solverBodies + wi.islandBodyBase + isl.bodyStart;
+    ContactConstraint* cc = B.constraints + wi.contactBase + isl.contactStart;
+    // 1. integrate velocities
+    for (u32 i = 0; i < isl.bodyCount; ++i) {
+        Body& b = bodies[ib[i]];
+        sb[i] = IntegrateVelocity(b, wi.gravity, h);
+        b.x0 = b.x;
+        b.q0 = b.q;
+    }
+    // 2. contact constraints
+    for (u32 k = 0; k < isl.contactCount; ++k) {
+        const Contact& c = B.contacts[wi.contactBase + ic[k].x];
+        const Fixture& fA = B.fixtures[wi.fixtureBase + c.fixtureA];
+        const Fixture& fB = B.fixtures[wi.fixtureBase + c.fixtureB];
+        InitContactConstraint(cc[k], c, fA, fB, bodies[fA.body], bodies[fB.body], ic[k].y, ic[k].z, sb, dtRatio, warm);
+    }
+    if (warm)
+        for (u32 k = 0; k < isl.contactCount; ++k) WarmStartContact(cc[k], sb);
+    // 3. velocity iterations
+    for (u32 it = 0; it < sp.velocityIterations; ++it) {
+        for (u32 k = 0; k < isl.contactCount; ++k) {
+            ContactConstraint& con = cc[k];
+            Vec3 vA = sb[con.indexA].v, wA = sb[con.indexA].w, vB = sb[con.indexB].v, wB = sb[con.indexB].w;
+            SolveContactVelocity(con, vA, wA, vB, wB);
+            sb[con.indexA].v = vA; sb[con.indexA].w = wA;
+            sb[con.indexB].v = vB; sb[con.indexB].w = wB;
+        }
+    }
+    if (warm) {
+        for (u32 k = 0; k < isl.contactCount; ++k) {
+            Contact& c = B.contacts[wi.contactBase + ic[k].x];
+            if (c.manifoldCount == 0) continue;
+            c.manifold.tangentImpulse = cc[k].tangentImpulse;
+            c.manifold.motorImpulse = cc[k].motorImpulse;
+            for (u32 p = 0; p < c.manifold.pointCount; ++p) c.manifold.points[p].normalImpulse = cc[k].points[p].normalImpulse;
+        }
+    }
+    // 4. integrate positions
+    for (u32 i = 0; i < isl.bodyCount; ++i) IntegratePosition(sb[i], bodies[ib[i]], h);
+    // 5. position iterations
+    bool positionsSolved = false;
+    for (u32 it = 0; it < sp.positionIterations; ++it) {
+        scalar minSeparation = 0;
+        for (u32 k = 0; k < isl.contactCount; ++k) {
+            const ContactConstraint& con = cc[k];
+            const Contact& c = B.contacts[wi.contactBase + ic[k].x];
+            SolverBody& A = sb[con.indexA];
+            SolverBody& Bb = sb[con.indexB];
+            Vec3 cA = A.x, cB = Bb.x;
+            Quat qA = A.q, qB = Bb.q;
+            Mat33 iA = A.I, iB = Bb.I;
+            minSeparation = SolveContactPosition(con, c.manifold, cA, qA, iA, cB, qB, iB, minSeparation);
+            A.x = cA; A.q = qA; A.I = iA;
+            Bb.x = cB; Bb.q = qB; Bb.I = iB;
+        }
+        bool contactsSolved = minSeparation >= scalar(-3) * kLinearSlop;
+        if (contactsSolved) { positionsSolved = true; break; }
+    }
+    // 6. copy back
+    for (u32 i = 0; i < isl.bodyCount; ++i) {
+        Body& b = bodies[ib[i]];
+        b.x = sb[i].x;
+        b.q = sb[i].q;
+        b.worldInvI = sb[i].I;
+        b.v = sb[i].v;
+        b.w = sb[i].w;
+        b.xfT = b.x - Mul(b.q, b.localCenter);
+    }
+    // 7. sleep
+    if (wi.flags & 0x1u) {
+        scalar minSleepTime = kMaxScalar;
+        for (u32 i = 0; i < isl.bodyCount; ++i) {
+            Body& b = bodies[ib[i]];
+            if (b.type == kStatic) continue;
+            scalar sqrLin = Dot(b.v, b.v);
+            scalar sqrAng = Dot(b.w, b.w);
+            if ((b.flags & kAutoSleep) == 0 || sqrLin > b.linTol * b.linTol || sqrAng > b.angTol * b.angTol)
+                b.sleepTime = 0;
+            else
+                b.sleepTime += h;
+            minSleepTime = Min(minSleepTime, b.sleepTime);
+        }
+        if (minSleepTime >= kTimeToSleep && positionsSolved) {
+            for (u32 i = 0; i < isl.bodyCount; ++i) {
+                Body& b = bodies[ib[i]];
+                b.flags &= ~kAwake;
+                b.sleepTime = 0;
+                b.force.SetZero();
+                b.torque.SetZero();
+                b.v.SetZero();
+                b.w.SetZero();
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_solve(BatchView B, StepParams sp) {
+    const u32 w = blockIdx.x;
+    const WorldInfo wi = B.worlds[w];
+    WorldState& ws = B.wstate[w];
+    const u32 islands = ws.islandCount;
+    if (wi.serialIslands) {
+        if (threadIdx.x == 0)
+            for (u32 i = 0; i < islands; ++i) SolveIsland(B, wi, ws, B.islands[wi.bodyBase + i], sp);
+    } else {
+        for (u32 i = threadIdx.x; i < islands; i += blockDim.x) SolveIsland(B, wi, ws, B.islands[wi.bodyBase + i], sp);
+    }
+}
+
+// ================================================================== fixture sync + clear forces
+__device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transform& xf, AABB& out) {
+    if (f.shapeType == 3u) {  // hull_shape.cpp:160-164, aabb.h:53-62
+        HullView h = GetHull(B, f.shapeIndex);
+        out.lowerBound = out.upperBound = Mul(xf, h.vertices[0]);
+        for (u32 i = 1; i < h.vertexCount; ++i) {
+            Vec3 v = Mul(xf, h.vertices[i]);
+            out.lowerBound = Min(out.lowerBound, v);
+            out.upperBound = Max(out.upperBound, v);
+        }
+        out.Extend(f.radius);
+    } else if (f.shapeType == 0u) {  // sphere_shape.cpp:51-57
+        Vec3 c = Mul(xf, f.v1);
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = c - r;
+        out.upperBound = c + r;
+    } else if (f.shapeType == 1u) {  // capsule_shape.cpp:124-131
+        Vec3 c1 = Mul(xf, f.v1), c2 = Mul(xf, f.v2);
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = Min(c1, c2) - r;
+        out.upperBound = Max(c1, c2) + r;
+    } else {
+        out.lowerBound.SetZero();
+        out.upperBound.SetZero();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
+    const u32 w = blockIdx.x;
+    const WorldInfo wi = B.worlds[w];
+    const bool solved = sp.dt > scalar(0);
+    if (threadIdx.x == 0 && solved) B.wstate[w].inv_dt0 = sp.inv_dt;  // world.cpp:233-236
+    for (u32 i = threadIdx.x; i < wi.bodyCount; i += blockDim.x) {
+        Body& b = B.bodies[wi.bodyBase + i];
+        if (solved && (b.flags & kIsland) && b.type != kStatic) {
+            // b3Body::SynchronizeFixtures (body.cpp:216-238)
+            Transform xf2(b.xfT, b.q);
+            Transform xf1 = xf2;
+            if (b.flags & kAwake) xf1 = Transform(b.x0 - Mul(b.q0, b.localCenter), b.q0);
+            for (u32 k = 0; k < b.fixtureCount; ++k) {
+                u32 f = B.fixtureList[wi.fixtureListBase + b.fixtureBegin + k];
+                const Fixture& fx = B.fixtures[wi.fixtureBase + f];
+                AABB a1, a2;
+                ComputeAABB(B, fx, xf1, a1);
+                ComputeAABB(B, fx, xf2, a2);
+                AABB aabb = Combine(a1, a2);
+                Vec3 d = kAabbMultiplier * (a2.GetCenter() - a1.GetCenter());
+                // b3DynamicTree::MoveProxy (dynamic_tree.cpp:139-207)
+                AABB fat = aabb;
+                fat.Extend(kAabbExtension);
+                if (d.x < scalar(0)) fat.lowerBound.x += d.x; else fat.upperBound.x += d.x;
+                if (d.y < scalar(0)) fat.lowerBound.y += d.y; else fat.upperBound.y += d.y;
+                if (d.z < scalar(0)) fat.lowerBound.z += d.z; else fat.upperBound.z += d.z;
+                AABB& tree = B.fatAABB[wi.fixtureBase + f];
+                bool keep = false;
+                if (tree.Contains(aabb)) {
+                    AABB huge = fat;
+                    huge.Extend(scalar(4) * kAabbExtension);
+                    keep = huge.Contains(tree);
+                }
+                if (!keep) {
+                    tree = fat;
+                    B.moved[wi.fixtureBase + f] = 1u;
+                }
+            }
+        }
+        if ((wi.flags & 0x4u) == 0) {  // b3World::ClearForces
+            b.force.SetZero();
+            b.torque.SetZero();
+        }
+    }
+}
+
+// ================================================================== pairs + contact creation
+// contact.cpp:58-72: registered shape pairs; +1 primary order, -1 swapped, 0 none.
+__device__ __forceinline__ int RegisterLookup(u32 t1, u32 t2) {
+    // bit (4*t1 + t2) set when (t1, t2) is a primary registered pair
+    const u32 primary = (1u << (0 * 5 + 0)) | (1u << (1 * 5 + 0)) | (1u << (1 * 5 + 1)) | (1u << (2 * 5 + 0)) |
+                        (1u << (2 * 5 + 1)) | (1u << (3 * 5 + 0)) | (1u << (3 * 5 + 1)) | (1u << (3 * 5 + 2)) |
+                        (1u << (3 * 5 + 3)) | (1u << (3 * 5 + 4)) | (1u << (4 * 5 + 0)) | (1u << (4 * 5 + 1));
+    if (primary & (1u << (t1 * 5 + t2))) return 1;
+    if (primary & (1u << (t2 * 5 + t1))) return -1;
+    return 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildAdjacency) {
+    const u32 w = blockIdx.x;
+    const WorldInfo wi = B.worlds[w];
+    WorldState& ws = B.wstate[w];
+    if (rebuildAdjacency) {
+        BuildAdjacency(B, wi, ws, B.edgeOffset + wi.fixtureBase + w, B.pairCounts + wi.fixtureBase + w);
+    }
+    const u32 nf = wi.fixtureCount;
+    const u32* porder = B.proxyOrder + wi.fixtureBase;
+    const u32* moved = B.moved + wi.fixtureBase;
+    const AABB* fat = B.fatAABB + wi.fixtureBase;
+    uint2* pairs = B.pairs + wi.pairBase;
+    // b3BroadPhase::FindPairs: every overlap of a moved proxy with any other proxy, as sorted
+    // unique (min, max) proxy-id pairs. Enumerated directly in sorted order: thread k owns the
+    // pairs whose smaller proxy is the k-th proxy.
+    u32 carry = 0;
+    for (u32 base = 0; base < nf; base += blockDim.x) {
+        u32 k = base + threadIdx.x;
+        u32 cnt = 0;
+        if (k < nf) {
+            u32 p = porder[k];
+            AABB ap = fat[p];
+            u32 mp = moved[p];
+            for (u32 l = k + 1; l < nf; ++l) {
+                u32 q = porder[l];
+                if ((mp | moved[q]) && TestOverlap(ap, fat[q])) ++cnt;
+            }
+        }
+        u32 total;
+        u32 pos = BlockExclusiveScan(cnt, &total);
+        if (k < nf && cnt) {
+            u32 p = porder[k];
+            AABB ap = fat[p];
+            u32 mp = moved[p];
+            u32 o = carry + pos;
+            for (u32 l = k + 1; l < nf; ++l) {
+                u32 q = porder[l];
+                if ((mp | moved[q]) && TestOverlap(ap, fat[q])) {
+                    if (o < wi.pairCap) pairs[o] = make_uint2(p, q);
+                    ++o;
+                }
+            }
+        }
+        carry += total;
+    }
+    __syncthreads();
+    u32 npairs = carry;
+    if (npairs > wi.pairCap) {
+        if (threadIdx.x == 0) ws.status |= 0x2u;
+        npairs = wi.pairCap;
+    }
+    for (u32 f = threadIdx.x; f < nf; f += blockDim.x) B.moved[wi.fixtureBase + f] = 0u;
+    // b3ContactManager::AddPair filters (contact_manager.cpp:38-103), then creation in pair order.
+    const u32* off = B.edgeOffset + wi.fixtureBase + w;
+    const u32* edges = B.edgeSlots + 2 * wi.contactBase;
+    const u32* order = B.contactOrder + wi.contactBase;
+    u32 created = 0;
+    const u32 n0 = ws.contactCount;
+    const u32 free0 = ws.freeCount;
+    __syncthreads();
+    for (u32 base = 0; base < npairs; base += blockDim.x) {
+        u32 i = base + threadIdx.x;
+        u32 keep = 0;
+        u32 fa = 0, fb = 0;
+        if (i < npairs) {
+            uint2 pr = pairs[i];
+            fa = pr.x;
+            fb = pr.y;
+            const Fixture& FA = B.fixtures[wi.fixtureBase + fa];
+            const Fixture& FB = B.fixtures[wi.fixtureBase + fb];
+            keep = 1;
+            if (FA.body == FB.body) keep = 0;
+            if (keep) {
+                for (u32 e = off[fb]; e < off[fb + 1]; ++e) {
+                    const Contact& c = B.contacts[wi.contactBase + order[edges[e]]];
+                    if ((c.fixtureA == fa && c.fixtureB == fb) || (c.fixtureA == fb && c.fixtureB == fa)) { keep = 0; break; }
+                }
+            }
+            if (keep && !BodiesShouldCollide(B, wi, FA.body, FB.body)) keep = 0;
+            if (keep) {
+                int reg = RegisterLookup(FA.shapeType, FB.shapeType);
+                if (reg == 0) keep = 0;
+                else if (reg < 0) { u32 t = fa; fa = fb; fb = t; }
+            }
+        }
+        u32 total;
+        u32 rank = BlockExclusiveScan(keep, &total);
+        if (keep) {
+            u32 r = created + rank;
+            if (r < free0) {
+                u32 slot = B.freeSlots[wi.contactBase + free0 - 1 - r];
+                Contact& c = B.contacts[wi.contactBase + slot];
+                c.fixtureA = fa;
+                c.fixtureB = fb;
+                c.flags = 0;
+                c.manifoldCount = 0;
+                c.cache.cacheType = 0;
+                c.cache.featureType = 0;
+                c.cache.index1 = 0;
+                c.cache.index2 = 0;
+                B.contactOrder[wi.contactBase + n0 + r] = slot;
+            }
+        }
+        created += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (created > free0) {
+            ws.status |= 0x2u;
+            created = free0;
+        }
+        ws.contactCount = n0 + created;
+        ws.freeCount = free0 - created;
+        ws.pairCount = npairs;
+    }
+}
+
+// ================================================================== host launch
+void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const StepParams& sp, bool findFirst,
+                hipStream_t stream, hipEvent_t* ev) {
+    BatchView B = view;
+    B.worlds += firstWorld;
+    B.wstate += firstWorld;
+    B.worldCount = worldCount;
+    dim3 grid(worldCount);
+    if (ev) hipEventRecord(ev[0], stream);
+    if (findFirst) hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 1u);
+    hipLaunchKernelGGL(k_update_contacts, grid, dim3(kBlock), 0, stream, B);
+    if (ev) hipEventRecord(ev[1], stream);
+    if (sp.dt > scalar(0)) {
+        hipLaunchKernelGGL(k_islands, grid, dim3(kBlock), 0, stream, B);
+        if (ev) hipEventRecord(ev[2], stream);
+        hipLaunchKernelGGL(k_solve, grid, dim3(64), 0, stream, B, sp);
+        if (ev) hipEventRecord(ev[3], stream);
+        hipLaunchKernelGGL(k_sync, grid, dim3(kBlock), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 0u);
+    } else {
+        if (ev) { hipEventRecord(ev[2], stream); hipEventRecord(ev[3], stream); }
+        hipLaunchKernelGGL(k_sync, grid, dim3(kBlock), 0, stream, B, sp);
+    }
+    if (ev) hipEventRecord(ev[4], stream);
+}
+
+}  // namespace bx

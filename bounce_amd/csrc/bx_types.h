@@ -1,0 +1,241 @@
+// bx_types.h -- device data layout of a batch of worlds.
+//
+// All worlds of a batch live in flat arrays; each world owns contiguous ranges (WorldInfo).
+// Indices stored inside a world (bodies, fixtures, contact slots) are world-local.
+//
+// Contacts: each world has a pool of contact slots and an order list holding the live slots in
+// creation order. The reference keeps contacts in linked lists with prepend insertion
+// (contact_manager.cpp:105-139), so its world list and every fixture's edge list run from the
+// newest contact to the oldest: the reverse of the order list. Creation order inside one
+// FindNewContacts call is the sorted proxy-pair order (broad_phase.h:184-206).
+#pragma once
+#include "bx_math.h"
+
+namespace bx {
+
+struct HalfEdge { u32 origin, twin, face, prev, next; };
+
+struct HullInfo {
+    u32 vBase, vCount;
+    u32 eBase, eCount;
+    u32 fBase, fCount;
+    scalar cx, cy, cz;  // centroid
+};
+
+// Read-only view of one hull in the shape library (b3Hull, hull.h:37-152).
+struct HullView {
+    const Vec3* vertices;
+    const HalfEdge* edges;
+    const u32* faces;
+    const Plane* planes;
+    Vec3 centroid;
+    u32 vertexCount, edgeCount, faceCount;
+
+    BX_HD u32 GetSupportVertex(const Vec3& d) const {
+        u32 maxIndex = 0;
+        scalar maxProjection = Dot(d, vertices[0]);
+        for (u32 i = 1; i < vertexCount; ++i) {
+            scalar p = Dot(d, vertices[i]);
+            if (p > maxProjection) { maxIndex = i; maxProjection = p; }
+        }
+        return maxIndex;
+    }
+    BX_HD u32 GetSupportFace(const Vec3& d) const {
+        u32 maxIndex = 0;
+        scalar maxProjection = Dot(d, planes[0].normal);
+        for (u32 i = 1; i < faceCount; ++i) {
+            scalar p = Dot(d, planes[i].normal);
+            if (p > maxProjection) { maxIndex = i; maxProjection = p; }
+        }
+        return maxIndex;
+    }
+    BX_HD Plane GetEdgeSidePlane(u32 index) const {
+        const HalfEdge e = edges[index];
+        const HalfEdge t = edges[e.twin];
+        Vec3 P = vertices[e.origin];
+        Vec3 Q = vertices[t.origin];
+        Vec3 D = Cross(Q - P, planes[e.face].normal);
+        Plane p;
+        p.normal = Normalize(D);
+        p.offset = Dot(p.normal, P);
+        return p;
+    }
+};
+
+// Body flags: b3Body::e_* (body.h:379-384).
+constexpr u32 kAwake = 0x1, kIsland = 0x2, kAutoSleep = 0x4;
+constexpr u32 kFixedX = 0x8, kFixedY = 0x10, kFixedZ = 0x20;
+constexpr u32 kStatic = 0, kKinematic = 1, kDynamic = 2;
+
+struct Body {
+    Vec3 x;            // sweep.worldCenter
+    Quat q;            // sweep.orientation == xf.rotation
+    Vec3 x0;           // sweep.worldCenter0
+    Quat q0;           // sweep.orientation0
+    Vec3 localCenter;
+    Vec3 xfT;          // xf.translation
+    Vec3 v, w;
+    Vec3 force, torque;
+    scalar invMass, mass;
+    Mat33 I, invI, worldInvI;
+    Vec3 linearDamping, angularDamping, gravityScale;
+    scalar linTol, angTol, sleepTime;
+    u32 type, flags, islandIndex;
+    u32 fixtureBegin, fixtureCount;   // into WorldInfo fixture-list range, list order
+    u32 jointBegin, jointCount;       // into the joint-edge range, list order
+};
+
+struct Fixture {
+    u32 body;
+    u32 shapeType;
+    u32 shapeIndex;
+    u32 proxyId;
+    scalar radius;
+    Vec3 v1, v2, v3;
+    scalar density, friction, restitution;
+    u32 isSensor;
+};
+
+struct ManifoldPoint {
+    Vec3 localNormal1, localPoint1, localPoint2;
+    scalar normalImpulse;
+    u32 triangleKey;
+    u32 key[4];        // b3FeaturePair: inEdge1 outEdge1 inEdge2 outEdge2
+    u32 persisting, edgeContact;
+};
+
+struct Manifold {
+    ManifoldPoint points[kMaxManifoldPoints];
+    u32 pointCount;
+    Vec2 tangentImpulse;
+    scalar motorImpulse, motorSpeed, tangentSpeed1, tangentSpeed2;
+};
+
+struct FeatureCache { u32 cacheType, featureType, index1, index2; };
+
+struct Contact {
+    u32 fixtureA, fixtureB;
+    u32 flags;          // 1 island, 2 touching (contact.h)
+    u32 manifoldCount;
+    FeatureCache cache;
+    Manifold manifold;  // convex contacts carry one manifold (convex_contact.cpp:23-30)
+};
+
+struct JointEdge { u32 joint, other; };
+
+// Revolute joint (revolute_joint.h / revolute_joint.cpp).
+struct Joint {
+    u32 bodyA, bodyB;
+    u32 collideConnected, islandFlag;
+    Vec3 localAnchorA, localAnchorB;
+    Quat localRotationA, localRotationB, referenceRotation;
+    u32 enableLimit, enableMotor;
+    scalar lowerAngle, upperAngle, motorSpeed, maxMotorTorque;
+    // warm-start state
+    scalar motorImpulse, lowerImpulse, upperImpulse;
+    Vec3 linearImpulse;
+    Vec2 angularImpulse;
+    u32 limitState;
+    scalar pad;
+};
+
+struct WorldInfo {
+    u32 bodyBase, bodyCount;
+    u32 fixtureBase, fixtureCount;     // fixtures in creation order (world-local index)
+    u32 fixtureListBase;               // per-body fixture lists (list order), fixtureCount entries
+    u32 jointBase, jointCount;
+    u32 jointEdgeBase;                 // 2 * jointCount entries
+    u32 contactBase, contactCap;       // slot pool, order list, free stack, island contact list
+    u32 pairBase, pairCap;
+    u32 islandBodyBase, islandBodyCap; // island body entries (statics may repeat across islands)
+    u32 flags;                         // BX_WORLD_* flags
+    u32 serialIslands;                 // islands must be solved in order (see bx_step.hip)
+    Vec3 gravity;
+};
+
+// Per-world mutable counters.
+struct WorldState {
+    u32 contactCount;     // live contacts (order list length)
+    u32 freeCount;        // free slot stack length
+    u32 pairCount;        // unique pairs of the last FindPairs
+    u32 islandCount;
+    u32 islandBodyCount;
+    u32 islandContactCount;
+    u32 islandJointCount;
+    u32 newContacts;
+    u32 status;
+    scalar inv_dt0;
+    u32 touchingCount;
+    u32 anyAsleep;
+};
+
+struct IslandInfo {
+    u32 bodyStart, bodyCount;
+    u32 contactStart, contactCount;
+    u32 jointStart, jointCount;
+};
+
+// Solver scratch per island contact (contact_solver.h:31-138), indexed like the island contact list.
+struct VCPoint { Vec3 rA, rB, normal; scalar normalMass, normalImpulse, velocityBias; };
+struct ContactConstraint {
+    u32 indexA, indexB;
+    scalar invMassA, invMassB;
+    Mat33 invIA, invIB;
+    scalar friction, restitution;
+    scalar radiusA, radiusB;
+    Mat33 localInvIA, localInvIB;
+    Vec3 localCenterA, localCenterB;
+    // manifold
+    u32 pointCount;
+    Vec3 rA, rB, normal, tangent1, tangent2;
+    scalar tangentSpeed1, tangentSpeed2;
+    Mat22 tangentMass;
+    Vec2 tangentImpulse;
+    scalar motorImpulse, motorMass, motorSpeed;
+    VCPoint points[kMaxManifoldPoints];
+};
+
+struct SolverBody { Vec3 x; Quat q; Mat33 I; Vec3 v, w; };
+
+// Everything a step kernel needs; all pointers are device pointers.
+struct BatchView {
+    u32 worldCount;
+    const WorldInfo* worlds;
+    WorldState* wstate;
+    // shape library
+    const HullInfo* hulls;
+    const Vec3* hullVertices;
+    const HalfEdge* hullEdges;
+    const u32* hullFaces;
+    const Plane* hullPlanes;
+    // bodies / fixtures / joints
+    Body* bodies;
+    Fixture* fixtures;
+    const u32* fixtureList;   // per world: bodies' fixture lists in list order (world-local fixture ids)
+    const u32* proxyOrder;    // per world: fixtures sorted by proxy id
+    AABB* fatAABB;            // per fixture
+    u32* moved;               // per fixture: proxy is in the move buffer
+    Joint* joints;
+    const JointEdge* jointEdges;
+    // contacts
+    Contact* contacts;        // slot pool
+    u32* contactOrder;        // per world: live slots, creation order
+    u32* freeSlots;           // per world: free slot stack
+    u32* contactAlive;        // per slot: scratch flag for the update pass
+    // adjacency (rebuilt each step)
+    u32* edgeOffset;          // per fixture + 1 per world (fixtureCount + 1 entries, base fixtureBase + world)
+    u32* edgeSlots;           // per world: 2 * contactCap
+    // pairs
+    uint2* pairs;             // per world: pairCap
+    u32* pairCounts;          // per fixture (world-local proxy order) scratch
+    // islands
+    IslandInfo* islands;      // per world: up to bodyCount islands (base = bodyBase)
+    u32* islandBodies;        // per world: islandBodyCap
+    uint4* islandContacts;    // per world: contactCap entries {slot, localA, localB, 0}
+    u32* islandJoints;        // per world: jointCount entries
+    u32* dfsStack;            // per world: bodyCount
+    SolverBody* solverBodies; // per world: islandBodyCap
+    ContactConstraint* constraints;  // per world: contactCap
+};
+
+}  // namespace bx

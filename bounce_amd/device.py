@@ -1,0 +1,177 @@
+"""Python binding of libbxstep.so (the HIP step), used by tests, the bench and the smoke check.
+
+The library is loaded from bounce_amd/lib; if it is missing, loading raises: there is no
+fallback path.
+"""
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+from . import capi
+
+_lib = None
+
+
+class BxError(RuntimeError):
+    pass
+
+
+def load(path=capi.LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BxError(f"libbxstep.so not built at {path}; run bounce_amd/build.py")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    u = C.c_uint32
+    sig = {
+        "bx_version": (C.c_char_p, []),
+        "bx_last_error": (C.c_char_p, []),
+        "bx_context_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+        "bx_context_destroy": (None, [P]),
+        "bx_batch_create": (C.c_int, [P, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc), u,
+                                      C.POINTER(capi.bx_batch_opts), C.POINTER(P)]),
+        "bx_batch_destroy": (None, [P]),
+        "bx_batch_step": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
+        "bx_batch_step_subset": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), C.POINTER(u), u]),
+        "bx_batch_sync": (C.c_int, [P]),
+        "bx_batch_get_counts": (C.c_int, [P, u, C.POINTER(capi.bx_world_counts)]),
+        "bx_batch_get_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
+        "bx_batch_get_contacts": (C.c_int, [P, u, C.POINTER(capi.bx_contact_state), u]),
+        "bx_batch_get_pairs": (C.c_int, [P, u, C.POINTER(capi.bx_pair), u]),
+        "bx_batch_get_island_contacts": (C.c_int, [P, u, C.POINTER(u), u, C.POINTER(u)]),
+        "bx_batch_set_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
+        "bx_batch_get_timings": (C.c_int, [P, C.POINTER(C.c_float), u]),
+        "bx_struct_size": (u, [C.c_char_p]),
+        "bx_batch_device_bytes": (C.c_uint64, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise BxError(f"bxstep error {rc}: {_lib.bx_last_error().decode()}")
+    return rc
+
+
+class Context:
+    def __init__(self, device=0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.bx_context_create(device, C.byref(h)))
+        self.h = h
+        self._batches = weakref.WeakSet()
+
+    def close(self):
+        """Destroy the context; batches still alive on it are destroyed first."""
+        if self.h:
+            for b in list(self._batches):
+                b.close()
+            _lib.bx_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BODY_DT = np.dtype(capi.bx_body_state)
+CONTACT_DT = np.dtype(capi.bx_contact_state)
+PAIR_DT = np.dtype(capi.bx_pair)
+
+
+class Batch:
+    """A batch of worlds resident on the device."""
+
+    def __init__(self, ctx, scene, max_contacts=0, max_pairs=0, solver_mode=0):
+        self.ctx = ctx
+        self.scene = scene
+        self._lib = scene.library()
+        self._descs = scene.world_descs()
+        opts = capi.bx_batch_opts()
+        opts.max_contacts_per_world = max_contacts
+        opts.max_pairs_per_world = max_pairs
+        opts.solver_mode = solver_mode
+        h = C.c_void_p()
+        _check(_lib.bx_batch_create(ctx.h, C.byref(self._lib), self._descs, len(scene.worlds), C.byref(opts),
+                                    C.byref(h)))
+        self.h = h
+        self.world_count = len(scene.worlds)
+        ctx._batches.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            _lib.bx_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def status(self, world=0):
+        return self.counts(world).status
+
+    def step(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
+        cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
+        _check(_lib.bx_batch_step(self.h, C.byref(cfg), n))
+
+    def step_subset(self, worlds, counts, dt=1.0 / 60.0, vit=8, pit=2):
+        cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
+        w = np.ascontiguousarray(worlds, dtype=np.uint32)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        _check(_lib.bx_batch_step_subset(self.h, C.byref(cfg), w.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         c.ctypes.data_as(C.POINTER(C.c_uint32)), len(w)))
+
+    def sync(self):
+        _check(_lib.bx_batch_sync(self.h))
+
+    def counts(self, world=0):
+        c = capi.bx_world_counts()
+        _check(_lib.bx_batch_get_counts(self.h, world, C.byref(c)))
+        return c
+
+    def bodies(self, world=0):
+        n = self.scene.worlds[world].body_count
+        out = np.zeros(n, dtype=BODY_DT)
+        _check(_lib.bx_batch_get_bodies(self.h, world, out.ctypes.data_as(C.POINTER(capi.bx_body_state)), n))
+        return out
+
+    def contacts(self, world=0):
+        n = self.counts(world).contact_count
+        out = np.zeros(n, dtype=CONTACT_DT)
+        _check(_lib.bx_batch_get_contacts(self.h, world, out.ctypes.data_as(C.POINTER(capi.bx_contact_state)), n))
+        return out
+
+    def pairs(self, world=0):
+        n = self.counts(world).pair_count
+        out = np.zeros(n, dtype=PAIR_DT)
+        _check(_lib.bx_batch_get_pairs(self.h, world, out.ctypes.data_as(C.POINTER(capi.bx_pair)), n))
+        return out
+
+    def island_contacts(self, world=0):
+        cap = max(1, self.counts(world).contact_count)
+        out = np.zeros(cap, dtype=np.uint32)
+        cnt = C.c_uint32(0)
+        _check(_lib.bx_batch_get_island_contacts(self.h, world, out.ctypes.data_as(C.POINTER(C.c_uint32)), cap,
+                                                 C.byref(cnt)))
+        return out[: cnt.value]
+
+    def timings(self):
+        t = (C.c_float * 5)()
+        _check(_lib.bx_batch_get_timings(self.h, t, 5))
+        return list(t)
+
+    def device_bytes(self):
+        return int(_lib.bx_batch_device_bytes(self.h))

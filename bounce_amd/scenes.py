@@ -245,6 +245,13 @@ def box_stack(n_boxes=5, seed=3):
     return s
 
 
+def with_flags(scene, flags):
+    """Set the BX_WORLD_* flags (e.g. BX_WORLD_SLEEPING) of every world of a scene set."""
+    for w in scene.worlds:
+        w.flags = flags
+    return scene
+
+
 def concat(sets, name="mix"):
     """Merge scene sets into one batch (hull indices are remapped)."""
     out = SceneSet(name)
