@@ -19,7 +19,7 @@
 namespace bx {
 struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIterations; };
 void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const StepParams& sp, bool findFirst,
-                hipStream_t stream, hipEvent_t* ev);
+                u32 maxIslands, hipStream_t stream, hipEvent_t* ev);
 }
 
 using namespace bx;
@@ -262,9 +262,11 @@ static int StepRange(bx_batch* b, const bx_step_cfg* cfg, u32 first, u32 count, 
     sp.inv_dt = cfg->dt > scalar(0) ? scalar(1) / cfg->dt : scalar(0);
     sp.velocityIterations = cfg->velocity_iterations;
     sp.positionIterations = cfg->position_iterations;
+    u32 maxIslands = 0;
+    for (u32 i = 0; i < count; ++i) maxIslands += b->worlds[first + i].bodyCount;
     for (u32 s = 0; s < steps; ++s) {
         bool last = timed && s + 1 == steps;
-        LaunchStep(b->view, first, count, sp, b->newContacts, b->ctx->stream, last ? b->ctx->ev : nullptr);
+        LaunchStep(b->view, first, count, sp, b->newContacts, maxIslands, b->ctx->stream, last ? b->ctx->ev : nullptr);
         b->newContacts = false;
     }
     HIP_CHECK(hipGetLastError());
@@ -542,8 +544,32 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     ALLOC(V.islandContacts, contactTotal, (uint4*)nullptr);
     ALLOC(V.islandJoints, joints.size(), (u32*)nullptr);
     ALLOC(V.dfsStack, nb, (u32*)nullptr);
-    ALLOC(V.solverBodies, islandBodyTotal, (SolverBody*)nullptr);
-    ALLOC(V.constraints, contactTotal, (ContactConstraint*)nullptr);
+    ALLOC(V.velBodies, islandBodyTotal, (VelBody*)nullptr);
+    ALLOC(V.posBodies, islandBodyTotal, (PosBody*)nullptr);
+    ALLOC(V.velCons, contactTotal, (VelCon*)nullptr);
+    ALLOC(V.posCons, contactTotal, (PosCon*)nullptr);
+    ALLOC(V.islandList, nb, (uint2*)nullptr);
+    ALLOC(V.islandListCount, 1, (u32*)nullptr);
+    {
+        // island builder: LDS when a world's scratch fits, otherwise a global area
+        const u32 kLdsWords = 64 * 1024 / 4;
+        u32 need = 0, scratch = 0;
+        for (u32 w = 0; w < world_count; ++w) {
+            WorldInfo& wi = b->worlds[w];
+            u32 words = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+            wi.islandScratchWords = words;
+            if (words <= kLdsWords) {
+                wi.islandScratchBase = kNull;
+                need = std::max(need, words);
+            } else {
+                wi.islandScratchBase = scratch;
+                scratch += words;
+            }
+        }
+        V.islandLdsWords = need;
+        ALLOC(V.islandScratch, scratch, (u32*)nullptr);
+        HIP_CHECK(hipMemcpy((void*)V.worlds, b->worlds.data(), world_count * sizeof(WorldInfo), hipMemcpyHostToDevice));
+    }
 #undef ALLOC
     *out = b;
     return BX_OK;

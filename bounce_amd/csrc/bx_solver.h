@@ -18,8 +18,8 @@ BX_HD Vec3 SolveGyro(const Quat& q, const Mat33& Ib, const Vec3& w1, scalar h) {
     return Mul(q, w1b);
 }
 
-// island.cpp:86-135: velocity integration of one body; returns the solver body.
-BX_HD SolverBody IntegrateVelocity(const Body& b, const Vec3& gravity, scalar h) {
+// island.cpp:86-135: velocity integration of one body.
+BX_HD void IntegrateVelocity(const Body& b, const Vec3& gravity, scalar h, VelBody& vb, PosBody& pb) {
     Vec3 v = b.v, w = b.w;
     if (b.type == kDynamic) {
         v += h * (Mul(b.gravityScale, gravity) + b.invMass * b.force);
@@ -34,94 +34,102 @@ BX_HD SolverBody IntegrateVelocity(const Body& b, const Vec3& gravity, scalar h)
         w.y *= scalar(1) / (scalar(1) + h * b.angularDamping.y);
         w.z *= scalar(1) / (scalar(1) + h * b.angularDamping.z);
     }
-    SolverBody s;
-    s.x = b.x;
-    s.q = b.q;
-    s.I = b.worldInvI;
-    s.v = v;
-    s.w = w;
-    return s;
+    vb.v = v;
+    vb.w = w;
+    pb.x = b.x;
+    pb.q = b.q;
+    pb.I = b.worldInvI;
 }
 
 // island.cpp:190-229: position integration with translation / rotation clamps.
-BX_HD void IntegratePosition(SolverBody& s, const Body& b, scalar h) {
+BX_HD void IntegratePosition(VelBody& vb, PosBody& pb, const Body& b, scalar h) {
     if (b.type == kStatic) return;
-    Vec3 translation = h * s.v;
+    Vec3 translation = h * vb.v;
     if (Dot(translation, translation) > kMaxTranslationSq) {
         scalar ratio = kMaxTranslation / Length(translation);
-        s.v *= ratio;
+        vb.v *= ratio;
     }
-    Vec3 rotation = h * s.w;
+    Vec3 rotation = h * vb.w;
     if (Dot(rotation, rotation) > kMaxRotationSq) {
         scalar ratio = kMaxRotation / Length(rotation);
-        s.w *= ratio;
+        vb.w *= ratio;
     }
-    s.x += h * s.v;
-    s.q = Integrate(s.q, s.w, h);
-    s.I = RotateToFrame(b.invI, s.q);
+    pb.x += h * vb.v;
+    pb.q = Integrate(pb.q, vb.w, h);
+    pb.I = RotateToFrame(b.invI, pb.q);
 }
 
-// contact_solver.cpp:28-150 (constructor) + 152-270 (InitializeVelocityConstraints) for one
-// convex contact. `pos` is read at the island's post-velocity-integration state.
-BX_HD void InitContactConstraint(ContactConstraint& cc, const Contact& c, const Fixture& fA, const Fixture& fB,
-                                 const Body& bA, const Body& bB, u32 indexA, u32 indexB, const SolverBody* sb,
-                                 scalar dtRatio, bool warmStarting) {
-    cc.indexA = indexA;
-    cc.indexB = indexB;
-    cc.invMassA = bA.invMass;
-    cc.invMassB = bB.invMass;
-    cc.localInvIA = bA.invI;
-    cc.localInvIB = bB.invI;
-    cc.localCenterA = bA.localCenter;
-    cc.localCenterB = bB.localCenter;
-    cc.radiusA = fA.radius;
-    cc.radiusB = fB.radius;
-    cc.invIA = sb[indexA].I;
-    cc.invIB = sb[indexB].I;
-    cc.friction = Sqrt(fA.friction * fB.friction);         // b3MixFriction
-    cc.restitution = Max(fA.restitution, fB.restitution);  // b3MixRestitution
+// contact_solver.cpp:28-150 (constructor) and 152-270 (InitializeVelocityConstraints) for one
+// convex contact, reading the island state right after velocity integration.
+BX_HD void InitContact(VelCon& vc, PosCon& pc, const Contact& c, const Fixture& fA, const Fixture& fB,
+                       const Body& bA, const Body& bB, u32 indexA, u32 indexB, const VelBody* vel,
+                       const PosBody* pos, scalar dtRatio, bool warmStarting) {
     const Manifold& m = c.manifold;
-    u32 pointCount = c.manifoldCount > 0 ? m.pointCount : 0;
-    cc.pointCount = pointCount;
-    cc.motorSpeed = m.motorSpeed;
-    cc.tangentSpeed1 = m.tangentSpeed1;
-    cc.tangentSpeed2 = m.tangentSpeed2;
-    if (warmStarting) {
-        cc.tangentImpulse = dtRatio * m.tangentImpulse;
-        cc.motorImpulse = dtRatio * m.motorImpulse;
-    } else {
-        cc.tangentImpulse.SetZero();
-        cc.motorImpulse = 0;
+    const u32 pointCount = c.manifoldCount > 0 ? m.pointCount : 0;
+    pc.indexA = indexA;
+    pc.indexB = indexB;
+    pc.pointCount = pointCount;
+    pc.invMassA = bA.invMass;
+    pc.invMassB = bB.invMass;
+    pc.radiusA = fA.radius;
+    pc.radiusB = fB.radius;
+    pc.localCenterA = bA.localCenter;
+    pc.localCenterB = bB.localCenter;
+    pc.localInvIA = bA.invI;
+    pc.localInvIB = bB.invI;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= pointCount) break;
+        pc.localNormalA[k] = m.points[k].localNormal1;
+        pc.localPointA[k] = m.points[k].localPoint1;
+        pc.localPointB[k] = m.points[k].localPoint2;
     }
-    cc.tangentMass.SetZero();
-    cc.motorMass = 0;
-    cc.rA.SetZero(); cc.rB.SetZero(); cc.normal.SetZero(); cc.tangent1.SetZero(); cc.tangent2.SetZero();
-    for (u32 k = 0; k < pointCount; ++k)
-        cc.points[k].normalImpulse = warmStarting ? dtRatio * m.points[k].normalImpulse : scalar(0);
-
-    if (c.manifoldCount == 0) return;
-    scalar mA = cc.invMassA, mB = cc.invMassB;
-    Mat33 iA = cc.invIA, iB = cc.invIB;
-    Vec3 xA = sb[indexA].x, xB = sb[indexB].x;
-    Quat qA = sb[indexA].q, qB = sb[indexB].q;
-    Vec3 vA = sb[indexA].v, wA = sb[indexA].w, vB = sb[indexB].v, wB = sb[indexB].w;
-    Transform xfA(xA - Mul(qA, cc.localCenterA), qA);
-    Transform xfB(xB - Mul(qB, cc.localCenterB), qB);
-    // b3WorldManifold::Initialize (collision.cpp:76-101)
+    vc.indexA = indexA;
+    vc.indexB = indexB;
+    vc.pointCount = pointCount;
+    vc.invMassA = bA.invMass;
+    vc.invMassB = bB.invMass;
+    vc.invIA = pos[indexA].I;
+    vc.invIB = pos[indexB].I;
+    vc.friction = Sqrt(fA.friction * fB.friction);         // b3MixFriction
+    vc.restitution = Max(fA.restitution, fB.restitution);  // b3MixRestitution
+    vc.motorSpeed = m.motorSpeed;
+    vc.tangentSpeed1 = m.tangentSpeed1;
+    vc.tangentSpeed2 = m.tangentSpeed2;
+    if (warmStarting) {
+        vc.tangentImpulse = dtRatio * m.tangentImpulse;
+        vc.motorImpulse = dtRatio * m.motorImpulse;
+    } else {
+        vc.tangentImpulse.SetZero();
+        vc.motorImpulse = 0;
+    }
+    vc.tangentMass.SetZero();
+    vc.motorMass = 0;
+    vc.rA.SetZero(); vc.rB.SetZero(); vc.normal.SetZero(); vc.tangent1.SetZero(); vc.tangent2.SetZero();
+    if (pointCount == 0) return;
+    const scalar mA = vc.invMassA, mB = vc.invMassB;
+    const Mat33 iA = vc.invIA, iB = vc.invIB;
+    const Vec3 xA = pos[indexA].x, xB = pos[indexB].x;
+    const Quat qA = pos[indexA].q, qB = pos[indexB].q;
+    const Vec3 vA = vel[indexA].v, wA = vel[indexA].w, vB = vel[indexB].v, wB = vel[indexB].w;
+    Transform xfA(xA - Mul(qA, pc.localCenterA), qA);
+    Transform xfB(xB - Mul(qB, pc.localCenterB), qB);
+    // b3WorldManifold::Initialize (collision.cpp:76-101) fused with the per-point constraint setup
     Vec3 center, normal;
     center.SetZero();
     normal.SetZero();
-    for (u32 k = 0; k < pointCount; ++k) {
-        const ManifoldPoint& p = m.points[k];
-        Vec3 nA = Mul(xfA.rotation, p.localNormal1);
-        Vec3 cA = Mul(xfA, p.localPoint1);
-        Vec3 cB = Mul(xfB, p.localPoint2);
-        Vec3 pA = cA + cc.radiusA * nA;
-        Vec3 pB = cB - cc.radiusB * nA;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= pointCount) break;
+        Vec3 nA = Mul(xfA.rotation, pc.localNormalA[k]);
+        Vec3 cA = Mul(xfA, pc.localPointA[k]);
+        Vec3 cB = Mul(xfB, pc.localPointB[k]);
+        Vec3 pA = cA + pc.radiusA * nA;
+        Vec3 pB = cB - pc.radiusB * nA;
         Vec3 point = scalar(0.5) * (pA + pB);
         center += point;
         normal += nA;
-        VCPoint& vcp = cc.points[k];
+        VCPoint& vcp = vc.points[k];
         Vec3 rA = point - xA;
         Vec3 rB = point - xB;
         vcp.rA = rA;
@@ -131,69 +139,67 @@ BX_HD void InitContactConstraint(ContactConstraint& cc, const Contact& c, const 
         Vec3 rnB = Cross(rB, nA);
         scalar K = mA + mB + Dot(iA * rnA, rnA) + Dot(iB * rnB, rnB);
         vcp.normalMass = K > scalar(0) ? scalar(1) / K : scalar(0);
+        vcp.normalImpulse = warmStarting ? dtRatio * m.points[k].normalImpulse : scalar(0);
         Vec3 dv = vB + Cross(wB, rB) - vA - Cross(wA, rA);
         scalar vn = Dot(nA, dv);
         vcp.velocityBias = 0;
-        if (vn < -kVelocityThreshold) vcp.velocityBias = -cc.restitution * vn;
+        if (vn < -kVelocityThreshold) vcp.velocityBias = -vc.restitution * vn;
     }
-    Vec3 t1, t2;
-    if (pointCount > 0) {
-        center /= scalar(pointCount);
-        normal.Normalize();
-        t1 = Perp(normal);
-        t2 = Cross(t1, normal);
-        cc.normal = normal;
-        cc.tangent1 = t1;
-        cc.tangent2 = t2;
-        Vec3 rA = center - xA;
-        Vec3 rB = center - xB;
-        cc.rA = rA;
-        cc.rB = rB;
-        Vec3 rn1A = Cross(rA, t1), rn1B = Cross(rB, t1);
-        Vec3 rn2A = Cross(rA, t2), rn2B = Cross(rB, t2);
-        scalar k11 = mA + mB + Dot(iA * rn1A, rn1A) + Dot(iB * rn1B, rn1B);
-        scalar k12 = Dot(iA * rn1A, rn2A) + Dot(iB * rn1B, rn2B);
-        scalar k22 = mA + mB + Dot(iA * rn2A, rn2A) + Dot(iB * rn2B, rn2B);
-        Mat22 K(Vec2(k11, k12), Vec2(k12, k22));
-        cc.tangentMass = Inverse(K);
-        scalar mass = Dot(cc.normal, (iA + iB) * cc.normal);
-        cc.motorMass = mass > scalar(0) ? scalar(1) / mass : scalar(0);
-    }
+    center /= scalar(pointCount);
+    normal.Normalize();
+    Vec3 t1 = Perp(normal);
+    Vec3 t2 = Cross(t1, normal);
+    vc.normal = normal;
+    vc.tangent1 = t1;
+    vc.tangent2 = t2;
+    Vec3 rA = center - xA;
+    Vec3 rB = center - xB;
+    vc.rA = rA;
+    vc.rB = rB;
+    Vec3 rn1A = Cross(rA, t1), rn1B = Cross(rB, t1);
+    Vec3 rn2A = Cross(rA, t2), rn2B = Cross(rB, t2);
+    scalar k11 = mA + mB + Dot(iA * rn1A, rn1A) + Dot(iB * rn1B, rn1B);
+    scalar k12 = Dot(iA * rn1A, rn2A) + Dot(iB * rn1B, rn2B);
+    scalar k22 = mA + mB + Dot(iA * rn2A, rn2A) + Dot(iB * rn2B, rn2B);
+    vc.tangentMass = Inverse(Mat22(Vec2(k11, k12), Vec2(k12, k22)));
+    scalar mass = Dot(vc.normal, (iA + iB) * vc.normal);
+    vc.motorMass = mass > scalar(0) ? scalar(1) / mass : scalar(0);
 }
 
 // contact_solver.cpp:272-326
-BX_HD void WarmStartContact(const ContactConstraint& cc, SolverBody* sb) {
-    scalar mA = cc.invMassA, mB = cc.invMassB;
-    Mat33 iA = cc.invIA, iB = cc.invIB;
-    Vec3 vA = sb[cc.indexA].v, wA = sb[cc.indexA].w, vB = sb[cc.indexB].v, wB = sb[cc.indexB].w;
-    for (u32 k = 0; k < cc.pointCount; ++k) {
-        const VCPoint& vcp = cc.points[k];
+BX_HD void WarmStartContact(const VelCon& vc, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
+    const scalar mA = vc.invMassA, mB = vc.invMassB;
+    const Mat33 iA = vc.invIA, iB = vc.invIB;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= vc.pointCount) break;
+        const VCPoint& vcp = vc.points[k];
         Vec3 P = vcp.normalImpulse * vcp.normal;
         vA -= mA * P;
         wA -= iA * Cross(vcp.rA, P);
         vB += mB * P;
         wB += iB * Cross(vcp.rB, P);
     }
-    if (cc.pointCount > 0) {
-        Vec3 P1 = cc.tangentImpulse.x * cc.tangent1;
-        Vec3 P2 = cc.tangentImpulse.y * cc.tangent2;
-        Vec3 P3 = cc.motorImpulse * cc.normal;
+    if (vc.pointCount > 0) {
+        Vec3 P1 = vc.tangentImpulse.x * vc.tangent1;
+        Vec3 P2 = vc.tangentImpulse.y * vc.tangent2;
+        Vec3 P3 = vc.motorImpulse * vc.normal;
         vA -= mA * (P1 + P2);
-        wA -= iA * (Cross(cc.rA, P1 + P2) + P3);
+        wA -= iA * (Cross(vc.rA, P1 + P2) + P3);
         vB += mB * (P1 + P2);
-        wB += iB * (Cross(cc.rB, P1 + P2) + P3);
+        wB += iB * (Cross(vc.rB, P1 + P2) + P3);
     }
-    sb[cc.indexA].v = vA; sb[cc.indexA].w = wA;
-    sb[cc.indexB].v = vB; sb[cc.indexB].w = wB;
 }
 
 // contact_solver.cpp:328-486: one contact of one PGS sweep.
-BX_HD void SolveContactVelocity(ContactConstraint& cc, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
-    scalar mA = cc.invMassA, mB = cc.invMassB;
-    const Mat33 iA = cc.invIA, iB = cc.invIB;
+BX_HD void SolveContactVelocity(VelCon& vc, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
+    const scalar mA = vc.invMassA, mB = vc.invMassB;
+    const Mat33 iA = vc.invIA, iB = vc.invIB;
     scalar normalImpulse = 0;
-    for (u32 k = 0; k < cc.pointCount; ++k) {
-        VCPoint& vcp = cc.points[k];
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= vc.pointCount) break;
+        VCPoint& vcp = vc.points[k];
         Vec3 dv = vB + Cross(wB, vcp.rB) - vA - Cross(wA, vcp.rA);
         scalar Cdot = Dot(vcp.normal, dv);
         scalar impulse = -vcp.normalMass * (Cdot - vcp.velocityBias);
@@ -207,56 +213,57 @@ BX_HD void SolveContactVelocity(ContactConstraint& cc, Vec3& vA, Vec3& wA, Vec3&
         wB += iB * Cross(vcp.rB, P);
         normalImpulse += vcp.normalImpulse;
     }
-    if (cc.pointCount > 0) {
+    if (vc.pointCount > 0) {
         {
-            Vec3 dv = vB + Cross(wB, cc.rB) - vA - Cross(wA, cc.rA);
+            Vec3 dv = vB + Cross(wB, vc.rB) - vA - Cross(wA, vc.rA);
             Vec2 Cdot;
-            Cdot.x = Dot(dv, cc.tangent1) - cc.tangentSpeed1;
-            Cdot.y = Dot(dv, cc.tangent2) - cc.tangentSpeed2;
-            Vec2 impulse = cc.tangentMass * -Cdot;
-            Vec2 oldImpulse = cc.tangentImpulse;
-            cc.tangentImpulse += impulse;
-            scalar maxImpulse = cc.friction * normalImpulse;
-            if (Dot(cc.tangentImpulse, cc.tangentImpulse) > maxImpulse * maxImpulse) {
-                cc.tangentImpulse.Normalize();
-                cc.tangentImpulse *= maxImpulse;
+            Cdot.x = Dot(dv, vc.tangent1) - vc.tangentSpeed1;
+            Cdot.y = Dot(dv, vc.tangent2) - vc.tangentSpeed2;
+            Vec2 impulse = vc.tangentMass * -Cdot;
+            Vec2 oldImpulse = vc.tangentImpulse;
+            vc.tangentImpulse += impulse;
+            scalar maxImpulse = vc.friction * normalImpulse;
+            if (Dot(vc.tangentImpulse, vc.tangentImpulse) > maxImpulse * maxImpulse) {
+                vc.tangentImpulse.Normalize();
+                vc.tangentImpulse *= maxImpulse;
             }
-            impulse = cc.tangentImpulse - oldImpulse;
-            Vec3 P1 = impulse.x * cc.tangent1;
-            Vec3 P2 = impulse.y * cc.tangent2;
+            impulse = vc.tangentImpulse - oldImpulse;
+            Vec3 P1 = impulse.x * vc.tangent1;
+            Vec3 P2 = impulse.y * vc.tangent2;
             Vec3 P = P1 + P2;
             vA -= mA * P;
-            wA -= iA * Cross(cc.rA, P);
+            wA -= iA * Cross(vc.rA, P);
             vB += mB * P;
-            wB += iB * Cross(cc.rB, P);
+            wB += iB * Cross(vc.rB, P);
         }
         {
-            scalar Cdot = Dot(cc.normal, wB - wA) - cc.motorSpeed;
-            scalar impulse = -cc.motorMass * Cdot;
-            scalar oldImpulse = cc.motorImpulse;
-            scalar maxImpulse = cc.friction * normalImpulse;
-            cc.motorImpulse = Clamp(cc.motorImpulse + impulse, -maxImpulse, maxImpulse);
-            impulse = cc.motorImpulse - oldImpulse;
-            Vec3 P = impulse * cc.normal;
+            scalar Cdot = Dot(vc.normal, wB - wA) - vc.motorSpeed;
+            scalar impulse = -vc.motorMass * Cdot;
+            scalar oldImpulse = vc.motorImpulse;
+            scalar maxImpulse = vc.friction * normalImpulse;
+            vc.motorImpulse = Clamp(vc.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = vc.motorImpulse - oldImpulse;
+            Vec3 P = impulse * vc.normal;
             wA -= iA * P;
             wB += iB * P;
         }
     }
 }
 
-// contact_solver.cpp:517-624: one contact of one NGS sweep; returns its minimum separation (<= 0 start).
-BX_HD scalar SolveContactPosition(const ContactConstraint& cc, const Manifold& m, Vec3& cA, Quat& qA, Mat33& iA,
-                                  Vec3& cB, Quat& qB, Mat33& iB, scalar minSeparation) {
-    scalar mA = cc.invMassA, mB = cc.invMassB;
-    for (u32 k = 0; k < cc.pointCount; ++k) {
-        const ManifoldPoint& p = m.points[k];
-        Transform xfA(cA - Mul(qA, cc.localCenterA), qA);
-        Transform xfB(cB - Mul(qB, cc.localCenterB), qB);
-        Vec3 normal = Mul(xfA.rotation, p.localNormal1);
-        Vec3 c1 = Mul(xfA, p.localPoint1);
-        Vec3 c2 = Mul(xfB, p.localPoint2);
+// contact_solver.cpp:517-624: one contact of one NGS sweep; returns the running minimum separation.
+BX_HD scalar SolveContactPosition(const PosCon& pc, Vec3& cA, Quat& qA, Mat33& iA, Vec3& cB, Quat& qB, Mat33& iB,
+                                  scalar minSeparation) {
+    const scalar mA = pc.invMassA, mB = pc.invMassB;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= pc.pointCount) break;
+        Transform xfA(cA - Mul(qA, pc.localCenterA), qA);
+        Transform xfB(cB - Mul(qB, pc.localCenterB), qB);
+        Vec3 normal = Mul(xfA.rotation, pc.localNormalA[k]);
+        Vec3 c1 = Mul(xfA, pc.localPointA[k]);
+        Vec3 c2 = Mul(xfB, pc.localPointB[k]);
         Vec3 point = c2;
-        scalar separation = Dot(c2 - c1, normal) - cc.radiusA - cc.radiusB;
+        scalar separation = Dot(c2 - c1, normal) - pc.radiusA - pc.radiusB;
         minSeparation = Min(minSeparation, separation);
         scalar C = Clamp(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, scalar(0));
         Vec3 rA = point - cA;
@@ -269,11 +276,11 @@ BX_HD scalar SolveContactPosition(const ContactConstraint& cc, const Manifold& m
         cA -= mA * P;
         qA -= Derivative(qA, iA * Cross(rA, P));
         qA.Normalize();
-        iA = RotateToFrame(cc.localInvIA, qA);
+        iA = RotateToFrame(pc.localInvIA, qA);
         cB += mB * P;
         qB += Derivative(qB, iB * Cross(rB, P));
         qB.Normalize();
-        iB = RotateToFrame(cc.localInvIB, qB);
+        iB = RotateToFrame(pc.localInvIB, qB);
     }
     return minSeparation;
 }

@@ -16,6 +16,7 @@
 // list order, Gauss-Seidel in island contact order. Work is parallel only where the reference's
 // result is order-independent (per contact narrowphase, per fixture AABBs, per pair filtering,
 // independent islands).
+#include <algorithm>
 #include "bx_collide.h"
 #include "bx_solver.h"
 
@@ -184,6 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_update_contacts(BatchView B) {
     const u32 n = ws.contactCount;
     u32* order = B.contactOrder + wi.contactBase;
     u32* alive = B.contactAlive + wi.contactBase;
+    if (w == 0 && threadIdx.x == 0) *B.islandListCount = 0;  // the island builder refills it
     // The reference visits contacts newest-first and a contact can wake a sleeping body that a
     // later contact tests for activity; with every non-static body awake the visits commute.
     int asleep = 0;
@@ -223,268 +225,8 @@ __global__ void __launch_bounds__(kBlock) k_update_contacts(BatchView B) {
     }
 }
 
-// ================================================================== adjacency + islands
-// Builds, for every fixture, its contact edge list in the reference's order (newest contact
-// first: contact_manager.cpp:118-139 prepends) as CSR over order-list positions.
-__device__ void BuildAdjacency(const BatchView& B, const WorldInfo& wi, const WorldState& ws, u32* off, u32* cursor) {
-    const u32 nf = wi.fixtureCount;
-    const u32 n = ws.contactCount;
-    const u32* order = B.contactOrder + wi.contactBase;
-    for (u32 f = threadIdx.x; f <= nf; f += blockDim.x) { off[f] = 0; }
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
-        const Contact& c = B.contacts[wi.contactBase + order[i]];
-        atomicAdd(&off[c.fixtureA], 1u);
-        atomicAdd(&off[c.fixtureB], 1u);
-    }
-    __syncthreads();
-    u32 carry = 0;
-    for (u32 base = 0; base <= nf; base += blockDim.x) {
-        u32 f = base + threadIdx.x;
-        u32 v = f <= nf ? off[f] : 0u;
-        u32 total;
-        u32 p = BlockExclusiveScan(v, &total);
-        if (f <= nf) { off[f] = carry + p; cursor[f] = carry + p; }
-        carry += total;
-        __syncthreads();
-    }
-    // newest first: walk the order list backwards (sequential; keeps the exact reference order)
-    if (threadIdx.x == 0) {
-        u32* edges = B.edgeSlots + 2 * wi.contactBase;
-        for (u32 k = n; k-- > 0;) {
-            const Contact& c = B.contacts[wi.contactBase + order[k]];
-            edges[cursor[c.fixtureA]++] = k;
-            edges[cursor[c.fixtureB]++] = k;
-        }
-    }
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
-    const u32 w = blockIdx.x;
-    const WorldInfo wi = B.worlds[w];
-    WorldState& ws = B.wstate[w];
-    u32* off = B.edgeOffset + wi.fixtureBase + w;
-    u32* cursor = B.pairCounts + wi.fixtureBase + w;
-    BuildAdjacency(B, wi, ws, off, cursor);
-    const u32 n = ws.contactCount;
-    const u32* order = B.contactOrder + wi.contactBase;
-    // clear island flags (world.cpp:265-277)
-    for (u32 i = threadIdx.x; i < wi.bodyCount; i += blockDim.x) B.bodies[wi.bodyBase + i].flags &= ~kIsland;
-    for (u32 i = threadIdx.x; i < n; i += blockDim.x) B.contacts[wi.contactBase + order[i]].flags &= ~1u;
-    for (u32 i = threadIdx.x; i < wi.jointCount; i += blockDim.x) B.joints[wi.jointBase + i].islandFlag = 0;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-
-    const u32* edges = B.edgeSlots + 2 * wi.contactBase;
-    u32* stack = B.dfsStack + wi.bodyBase;
-    IslandInfo* islands = B.islands + wi.bodyBase;
-    u32* ibodies = B.islandBodies + wi.islandBodyBase;
-    uint4* icontacts = B.islandContacts + wi.contactBase;
-    u32* ijoints = B.islandJoints + wi.jointBase;
-    Body* bodies = B.bodies + wi.bodyBase;
-    u32 islandCount = 0, nb = 0, nc = 0, nj = 0;
-    u32 overflow = 0;
-    for (u32 s = wi.bodyCount; s-- > 0;) {  // body list order: newest body first
-        Body& seed = bodies[s];
-        if (seed.flags & kIsland) continue;
-        if ((seed.flags & kAwake) == 0) continue;
-        if (seed.type == kStatic) continue;
-        IslandInfo info;
-        info.bodyStart = nb;
-        info.contactStart = nc;
-        info.jointStart = nj;
-        u32 bodyCount = 0;
-        u32 stackCount = 0;
-        stack[stackCount++] = s;
-        seed.flags |= kIsland;
-        while (stackCount > 0) {
-            u32 bi = stack[--stackCount];
-            Body& b = bodies[bi];
-            b.islandIndex = bodyCount;
-            if (nb < wi.islandBodyCap) ibodies[nb] = bi;
-            else overflow = 1;
-            ++nb;
-            ++bodyCount;
-            if (b.type == kStatic) continue;
-            b.flags |= kAwake;
-            for (u32 fi = 0; fi < b.fixtureCount; ++fi) {
-                u32 f = B.fixtureList[wi.fixtureListBase + b.fixtureBegin + fi];
-                for (u32 e = off[f]; e < off[f + 1]; ++e) {
-                    u32 k = edges[e];
-                    Contact& c = B.contacts[wi.contactBase + order[k]];
-                    if (c.flags & 1u) continue;
-                    if ((c.flags & 2u) == 0) continue;
-                    if (B.fixtures[wi.fixtureBase + c.fixtureA].isSensor ||
-                        B.fixtures[wi.fixtureBase + c.fixtureB].isSensor)
-                        continue;
-                    icontacts[nc++] = make_uint4(order[k], 0u, 0u, 0u);
-                    c.flags |= 1u;
-                    u32 otherF = c.fixtureA == f ? c.fixtureB : c.fixtureA;
-                    u32 other = B.fixtures[wi.fixtureBase + otherF].body;
-                    if (bodies[other].flags & kIsland) continue;
-                    stack[stackCount++] = other;
-                    bodies[other].flags |= kIsland;
-                }
-            }
-            for (u32 ji = 0; ji < b.jointCount; ++ji) {
-                JointEdge je = B.jointEdges[wi.jointEdgeBase + b.jointBegin + ji];
-                Joint& j = B.joints[wi.jointBase + je.joint];
-                if (j.islandFlag) continue;
-                ijoints[nj++] = je.joint;
-                j.islandFlag = 1;
-                if (bodies[je.other].flags & kIsland) continue;
-                stack[stackCount++] = je.other;
-                bodies[je.other].flags |= kIsland;
-            }
-        }
-        info.bodyCount = bodyCount;
-        info.contactCount = nc - info.contactStart;
-        info.jointCount = nj - info.jointStart;
-        // Island-local body indices of each contact: static bodies join several islands and keep
-        // only their index in the last one (b3Island::Add), so resolve them now.
-        for (u32 k = info.contactStart; k < nc; ++k) {
-            const Contact& c = B.contacts[wi.contactBase + icontacts[k].x];
-            u32 a = B.fixtures[wi.fixtureBase + c.fixtureA].body;
-            u32 bb = B.fixtures[wi.fixtureBase + c.fixtureB].body;
-            icontacts[k].y = bodies[a].islandIndex;
-            icontacts[k].z = bodies[bb].islandIndex;
-        }
-        islands[islandCount++] = info;
-        for (u32 k = 0; k < bodyCount; ++k) {  // world.cpp:411-419
-            u32 bi = ibodies[info.bodyStart + k];
-            if (bodies[bi].type == kStatic) bodies[bi].flags &= ~kIsland;
-        }
-    }
-    ws.islandCount = islandCount;
-    ws.islandBodyCount = nb;
-    ws.islandContactCount = nc;
-    ws.islandJointCount = nj;
-    if (overflow) ws.status |= 0x2u;
-}
-
-// ================================================================== island solve
-struct StepParams {
-    scalar dt, inv_dt;
-    u32 velocityIterations, positionIterations;
-};
-
-__device__ void SolveIsland(const BatchView& B, const WorldInfo& wi, WorldState& ws, const IslandInfo& isl,
-                            const StepParams& sp) {
-    const scalar h = sp.dt;
-    const scalar dtRatio = ws.inv_dt0 * sp.dt;
-    const bool warm = (wi.flags & 0x2u) == 0;
-    Body* bodies = B.bodies + wi.bodyBase;
-    const u32* ib = B.islandBodies + wi.islandBodyBase + isl.bodyStart;
-    const uint4* ic = B.islandContacts + wi.contactBase + isl.contactStart;
-    SolverBody* sb = B.solverBodies + wi.islandBodyBase + isl.bodyStart;
-    ContactConstraint* cc = B.constraints + wi.contactBase + isl.contactStart;
-    // 1. integrate velocities
-    for (u32 i = 0; i < isl.bodyCount; ++i) {
-        Body& b = bodies[ib[i]];
-        sb[i] = IntegrateVelocity(b, wi.gravity, h);
-        b.x0 = b.x;
-        b.q0 = b.q;
-    }
-    // 2. contact constraints
-    for (u32 k = 0; k < isl.contactCount; ++k) {
-        const Contact& c = B.contacts[wi.contactBase + ic[k].x];
-        const Fixture& fA = B.fixtures[wi.fixtureBase + c.fixtureA];
-        const Fixture& fB = B.fixtures[wi.fixtureBase + c.fixtureB];
-        InitContactConstraint(cc[k], c, fA, fB, bodies[fA.body], bodies[fB.body], ic[k].y, ic[k].z, sb, dtRatio, warm);
-    }
-    if (warm)
-        for (u32 k = 0; k < isl.contactCount; ++k) WarmStartContact(cc[k], sb);
-    // 3. velocity iterations
-    for (u32 it = 0; it < sp.velocityIterations; ++it) {
-        for (u32 k = 0; k < isl.contactCount; ++k) {
-            ContactConstraint& con = cc[k];
-            Vec3 vA = sb[con.indexA].v, wA = sb[con.indexA].w, vB = sb[con.indexB].v, wB = sb[con.indexB].w;
-            SolveContactVelocity(con, vA, wA, vB, wB);
-            sb[con.indexA].v = vA; sb[con.indexA].w = wA;
-            sb[con.indexB].v = vB; sb[con.indexB].w = wB;
-        }
-    }
-    if (warm) {
-        for (u32 k = 0; k < isl.contactCount; ++k) {
-            Contact& c = B.contacts[wi.contactBase + ic[k].x];
-            if (c.manifoldCount == 0) continue;
-            c.manifold.tangentImpulse = cc[k].tangentImpulse;
-            c.manifold.motorImpulse = cc[k].motorImpulse;
-            for (u32 p = 0; p < c.manifold.pointCount; ++p) c.manifold.points[p].normalImpulse = cc[k].points[p].normalImpulse;
-        }
-    }
-    // 4. integrate positions
-    for (u32 i = 0; i < isl.bodyCount; ++i) IntegratePosition(sb[i], bodies[ib[i]], h);
-    // 5. position iterations
-    bool positionsSolved = false;
-    for (u32 it = 0; it < sp.positionIterations; ++it) {
-        scalar minSeparation = 0;
-        for (u32 k = 0; k < isl.contactCount; ++k) {
-            const ContactConstraint& con = cc[k];
-            const Contact& c = B.contacts[wi.contactBase + ic[k].x];
-            SolverBody& A = sb[con.indexA];
-            SolverBody& Bb = sb[con.indexB];
-            Vec3 cA = A.x, cB = Bb.x;
-            Quat qA = A.q, qB = Bb.q;
-            Mat33 iA = A.I, iB = Bb.I;
-            minSeparation = SolveContactPosition(con, c.manifold, cA, qA, iA, cB, qB, iB, minSeparation);
-            A.x = cA; A.q = qA; A.I = iA;
-            Bb.x = cB; Bb.q = qB; Bb.I = iB;
-        }
-        bool contactsSolved = minSeparation >= scalar(-3) * kLinearSlop;
-        if (contactsSolved) { positionsSolved = true; break; }
-    }
-    // 6. copy back
-    for (u32 i = 0; i < isl.bodyCount; ++i) {
-        Body& b = bodies[ib[i]];
-        b.x = sb[i].x;
-        b.q = sb[i].q;
-        b.worldInvI = sb[i].I;
-        b.v = sb[i].v;
-        b.w = sb[i].w;
-        b.xfT = b.x - Mul(b.q, b.localCenter);
-    }
-    // 7. sleep
-    if (wi.flags & 0x1u) {
-        scalar minSleepTime = kMaxScalar;
-        for (u32 i = 0; i < isl.bodyCount; ++i) {
-            Body& b = bodies[ib[i]];
-            if (b.type == kStatic) continue;
-            scalar sqrLin = Dot(b.v, b.v);
-            scalar sqrAng = Dot(b.w, b.w);
-            if ((b.flags & kAutoSleep) == 0 || sqrLin > b.linTol * b.linTol || sqrAng > b.angTol * b.angTol)
-                b.sleepTime = 0;
-            else
-                b.sleepTime += h;
-            minSleepTime = Min(minSleepTime, b.sleepTime);
-        }
-        if (minSleepTime >= kTimeToSleep && positionsSolved) {
-            for (u32 i = 0; i < isl.bodyCount; ++i) {
-                Body& b = bodies[ib[i]];
-                b.flags &= ~kAwake;
-                b.sleepTime = 0;
-                b.force.SetZero();
-                b.torque.SetZero();
-                b.v.SetZero();
-                b.w.SetZero();
-            }
-        }
-    }
-}
-
-__global__ void __launch_bounds__(64) k_solve(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
-    const WorldInfo wi = B.worlds[w];
-    WorldState& ws = B.wstate[w];
-    const u32 islands = ws.islandCount;
-    if (wi.serialIslands) {
-        if (threadIdx.x == 0)
-            for (u32 i = 0; i < islands; ++i) SolveIsland(B, wi, ws, B.islands[wi.bodyBase + i], sp);
-    } else {
-        for (u32 i = threadIdx.x; i < islands; i += blockDim.x) SolveIsland(B, wi, ws, B.islands[wi.bodyBase + i], sp);
-    }
-}
+#include "bx_islands.h"
+#include "bx_solve_kernels.h"
 
 // ================================================================== fixture sync + clear forces
 __device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transform& xf, AABB& out) {
@@ -690,7 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
 
 // ================================================================== host launch
 void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const StepParams& sp, bool findFirst,
-                hipStream_t stream, hipEvent_t* ev) {
+                u32 maxIslands, hipStream_t stream, hipEvent_t* ev) {
     BatchView B = view;
     B.worlds += firstWorld;
     B.wstate += firstWorld;
@@ -701,9 +443,17 @@ void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const Ste
     hipLaunchKernelGGL(k_update_contacts, grid, dim3(kBlock), 0, stream, B);
     if (ev) hipEventRecord(ev[1], stream);
     if (sp.dt > scalar(0)) {
-        hipLaunchKernelGGL(k_islands, grid, dim3(kBlock), 0, stream, B);
+        hipLaunchKernelGGL(k_islands, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
         if (ev) hipEventRecord(ev[2], stream);
-        hipLaunchKernelGGL(k_solve, grid, dim3(64), 0, stream, B, sp);
+        // one lane per island; islands <= bodies of the launched worlds
+        u32 vblocks = std::max(1u, std::min((maxIslands + kSolveLanes - 1) / kSolveLanes, 16384u));
+        u32 pblocks = std::max(1u, std::min((maxIslands + 63u) / 64u, 16384u));
+        hipLaunchKernelGGL(k_integrate_vel, grid, dim3(kBlock), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_init_contacts, grid, dim3(kBlock), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_solve_velocity, dim3(vblocks), dim3(64), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_finish, grid, dim3(kBlock), 0, stream, B, sp);
         if (ev) hipEventRecord(ev[3], stream);
         hipLaunchKernelGGL(k_sync, grid, dim3(kBlock), 0, stream, B, sp);
         hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 0u);

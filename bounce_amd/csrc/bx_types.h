@@ -151,6 +151,8 @@ struct WorldInfo {
     u32 flags;                         // BX_WORLD_* flags
     u32 serialIslands;                 // islands must be solved in order (see bx_step.hip)
     Vec3 gravity;
+    u32 islandScratchBase;             // words in BatchView::islandScratch, kNull when LDS suffices
+    u32 islandScratchWords;            // words the island builder needs for this world
 };
 
 // Per-world mutable counters.
@@ -169,33 +171,48 @@ struct WorldState {
     u32 anyAsleep;
 };
 
+// Words of island-builder scratch a world needs: per body flags, fixture range, joint range,
+// island index and DFS stack; per fixture its list entry and owner, CSR offsets and cursors; per
+// contact slot its two fixtures, flags and two edge entries.
+BX_HD u32 IslandScratchWords(u32 nb, u32 nf, u32 cap) { return 5 * nb + 4 * nf + 2 + 5 * cap; }
+
 struct IslandInfo {
     u32 bodyStart, bodyCount;
     u32 contactStart, contactCount;
     u32 jointStart, jointCount;
+    u32 positionsSolved, pad;
 };
 
-// Solver scratch per island contact (contact_solver.h:31-138), indexed like the island contact list.
+// Solver state of one island body (b3Velocity / b3Position, time_step.h), split so that the
+// velocity iterations touch only 24 bytes per body.
+struct VelBody { Vec3 v, w; };
+struct PosBody { Vec3 x; Quat q; Mat33 I; };
+
+// Velocity constraint of one convex contact (b3ContactVelocityConstraint with its single
+// manifold, contact_solver.h:57-111). Impulses are the only fields the iterations write.
 struct VCPoint { Vec3 rA, rB, normal; scalar normalMass, normalImpulse, velocityBias; };
-struct ContactConstraint {
-    u32 indexA, indexB;
-    scalar invMassA, invMassB;
+struct VelCon {
+    u32 indexA, indexB, pointCount, pad;
+    scalar invMassA, invMassB, friction, restitution;
     Mat33 invIA, invIB;
-    scalar friction, restitution;
-    scalar radiusA, radiusB;
-    Mat33 localInvIA, localInvIB;
-    Vec3 localCenterA, localCenterB;
-    // manifold
-    u32 pointCount;
     Vec3 rA, rB, normal, tangent1, tangent2;
-    scalar tangentSpeed1, tangentSpeed2;
+    scalar tangentSpeed1, tangentSpeed2, motorSpeed, motorMass;
     Mat22 tangentMass;
     Vec2 tangentImpulse;
-    scalar motorImpulse, motorMass, motorSpeed;
+    scalar motorImpulse;
     VCPoint points[kMaxManifoldPoints];
 };
 
-struct SolverBody { Vec3 x; Quat q; Mat33 I; Vec3 v, w; };
+// Position constraint of one convex contact (b3ContactPositionConstraint, contact_solver.h:31-55).
+struct PosCon {
+    u32 indexA, indexB, pointCount, pad;
+    scalar invMassA, invMassB, radiusA, radiusB;
+    Vec3 localCenterA, localCenterB;
+    Mat33 localInvIA, localInvIB;
+    Vec3 localNormalA[kMaxManifoldPoints];
+    Vec3 localPointA[kMaxManifoldPoints];
+    Vec3 localPointB[kMaxManifoldPoints];
+};
 
 // Everything a step kernel needs; all pointers are device pointers.
 struct BatchView {
@@ -234,8 +251,14 @@ struct BatchView {
     uint4* islandContacts;    // per world: contactCap entries {slot, localA, localB, 0}
     u32* islandJoints;        // per world: jointCount entries
     u32* dfsStack;            // per world: bodyCount
-    SolverBody* solverBodies; // per world: islandBodyCap
-    ContactConstraint* constraints;  // per world: contactCap
+    VelBody* velBodies;       // per world: islandBodyCap
+    PosBody* posBodies;       // per world: islandBodyCap
+    VelCon* velCons;          // per world: contactCap
+    PosCon* posCons;          // per world: contactCap
+    uint2* islandList;        // islands of the launched worlds: {world, island} (island = kNull: all, in order)
+    u32* islandListCount;
+    u32* islandScratch;       // island-builder scratch for worlds too large for LDS
+    u32 islandLdsWords;       // LDS words available to the island builder per world
 };
 
 }  // namespace bx

@@ -32,6 +32,9 @@ SCENES = {
     "box_stack5": (lambda: scenes.box_stack(5), 120),
     "pyramid10": (lambda: scenes.pyramids(1, levels=10), 120),
     "drop100": (lambda: scenes.drop_boxes(100, extent=(8.0, 20.0, 8.0)), 150),
+    "pyramid10_sleep": (lambda: scenes.with_flags(scenes.pyramids(1, levels=10), 1), 120),
+    "drop100_sleep": (lambda: scenes.with_flags(scenes.drop_boxes(100, extent=(8.0, 20.0, 8.0)), 1), 360),
+    "drop300": (lambda: scenes.drop_boxes(300, extent=(10.0, 30.0, 10.0), seed=5), 120),
 }
 
 
