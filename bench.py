@@ -26,13 +26,20 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of the island-solver kernel (the step's dominant kernel), per unit of work:
-#  * awake non-static body: 256 B of persistent state read (pose, velocity, force, mass and
-#    inertia, damping, sleep data) and 176 B written (pose, previous pose, velocity, world
-#    inverse inertia, sleep data, cleared force) -> 432 B;
-#  * island contact: its manifold (4 points x 68 B + 28 B) read and written -> 600 B.
-BODY_BYTES = 432
-CONTACT_BYTES = 600
+# Algorithmic (compulsory) bytes of one launch of each heavy kernel, per unit of work, from the
+# data it must read and write once (bounce_amd/csrc/bx_types.h layouts):
+#  * k_solve_velocity: per island contact the velocity constraint (400 B) read and its impulses
+#    (28 B) written; per island body its velocity (24 B) read and written;
+#  * k_solve_position: per island contact the position constraint (272 B) read; per island body
+#    its pose and world inverse inertia (64 B) read and written;
+#  * k_update_contacts: per contact its record with manifold and feature cache (332 B) read and
+#    written, and for each of its two fixtures the fixture (72 B), fat AABB (24 B) and the body's
+#    pose, sweep start, local centre, type and flags (88 B) read.
+KERNEL_BYTES = {
+    "k_solve_velocity": lambda ic, ib, c: 428 * ic + 48 * ib,
+    "k_solve_position": lambda ic, ib, c: 272 * ic + 128 * ib,
+    "k_update_contacts": lambda ic, ib, c: (2 * 332 + 2 * (72 + 24 + 88)) * c,
+}
 
 
 def parse():
@@ -45,6 +52,7 @@ def parse():
     p.add_argument("--plain-worlds", type=int, default=16)
     p.add_argument("--cpu-worlds", type=int, default=256)
     p.add_argument("--cpu-steps", type=int, default=60)
+    p.add_argument("--latency-steps", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
 
@@ -92,40 +100,56 @@ def main():
     t1 = time.perf_counter()
     wall = t1 - t0
     value = total_bodies * args.steps / wall
-    # per-step latency (GPU events around one step) and per-phase times of the dominant kernel
-    lat = []
-    phases = []
-    for _ in range(10):
-        batch.step()
-        t = batch.timings()
-        lat.append(t[4])
-        phases.append(t)
-    phases = np.array(phases)
-    # counts for the algorithmic bytes of the solver kernel (sampled over worlds)
+    # per-kernel and per-phase times averaged over the K timed steps (HIP events recorded between
+    # the kernels on the library's stream)
+    kernel_ms = batch.kernel_timings()
+    phase_ms = batch.timings()
+    dominant = max(KERNEL_BYTES, key=lambda k: kernel_ms.get(k, 0.0))
+    # units of work of one launch at the end of the timed region (a sample of worlds, times the
+    # world count)
     sample_w = list(range(0, args.worlds, max(1, args.worlds // 16)))
-    awake = np.mean([batch.counts(w).awake_count - 1 for w in sample_w])
-    island_contacts = np.mean([len(batch.island_contacts(w)) for w in sample_w])
-    status = int(np.bitwise_or.reduce([batch.counts(w).status for w in sample_w]))
-    solve_ms = float(np.median(phases[:, 2]))
-    solve_bytes = args.worlds * (awake * BODY_BYTES + island_contacts * CONTACT_BYTES)
-    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    counts = [batch.counts(w) for w in sample_w]
+    island_bodies = np.mean([c.awake_count for c in counts]) * args.worlds
+    island_contacts = np.mean([len(batch.island_contacts(w)) for w in sample_w]) * args.worlds
+    contacts = np.mean([c.contact_count for c in counts]) * args.worlds
+    status = int(np.bitwise_or.reduce([c.status for c in counts]))
+    device_bytes = batch.device_bytes()
+    dom_bytes = KERNEL_BYTES[dominant](island_contacts, island_bodies, contacts)
+    achieved = dom_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
+    # step latency: single steps, each bracketed by events on the stream
+    lat = []
+    for _ in range(args.latency_steps):
+        batch.step()
+        lat.append(batch.timings()[4])
+    # HBM bytes per launch of the dominant kernel, from the committed FETCH_SIZE / WRITE_SIZE profile
+    # of this workload (bounce_amd/profile_summary.py), when one exists for this configuration
+    traffic = None
+    traffic_file = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tj = json.load(f)
+        if tj.get("worlds") == args.worlds and tj.get("levels") == args.levels:
+            traffic = tj["bytes_per_launch"].get(dominant)
     batch.close()
     # plain per-world path: the same kernels, one world per launch sequence
-    plain_scene = factory(args.plain_worlds)
-    pb = device.Batch(ctx, plain_scene)
-    worlds = np.arange(args.plain_worlds, dtype=np.uint32)
-    ones = np.ones(args.plain_worlds, dtype=np.uint32)
-    for _ in range(2):
-        pb.step_subset(worlds, ones)
-    pb.sync()
-    p0 = time.perf_counter()
-    plain_steps = 5
-    for _ in range(plain_steps):
-        pb.step_subset(worlds, ones)
-    pb.sync()
-    p1 = time.perf_counter()
-    plain_value = plain_scene.body_count * plain_steps / (p1 - p0)
-    pb.close()
+    plain = None
+    if args.plain_worlds > 0:
+        plain_scene = factory(args.plain_worlds)
+        pb = device.Batch(ctx, plain_scene)
+        worlds = np.arange(args.plain_worlds, dtype=np.uint32)
+        ones = np.ones(args.plain_worlds, dtype=np.uint32)
+        for _ in range(2):
+            pb.step_subset(worlds, ones)
+        pb.sync()
+        p0 = time.perf_counter()
+        plain_steps = 5
+        for _ in range(plain_steps):
+            pb.step_subset(worlds, ones)
+        pb.sync()
+        p1 = time.perf_counter()
+        plain = {"value": plain_scene.body_count * plain_steps / (p1 - p0), "unit": "body-steps/s",
+                 "sample": f"{args.plain_worlds} worlds x {plain_steps} steps, one world per launch"}
+        pb.close()
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline(args, factory, bodies_per_world)
     out = {
@@ -144,15 +168,18 @@ def main():
         "config": {"workload": f"batched stacking: {args.worlds} worlds x {args.levels}-level box pyramid "
                                f"({bodies_per_world} bodies incl. ground), dt=1/60, 8/2 iterations, sleep off",
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
-        "plain_per_world": {"value": plain_value, "unit": "body-steps/s",
-                            "sample": f"{args.plain_worlds} worlds x {plain_steps} steps, one world per launch"},
-        "batched_over_plain": value / plain_value,
-        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "phase_ms": {"update_contacts": float(np.median(phases[:, 0])), "islands": float(np.median(phases[:, 1])),
-                     "solve": solve_ms, "sync_and_pairs": float(np.median(phases[:, 3]))},
-        "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "plain_per_world": plain,
+        "batched_over_plain": value / plain["value"] if plain else None,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                            "samples": len(lat)},
+        "phase_ms": {"update_contacts": phase_ms[0], "islands": phase_ms[1], "solve": phase_ms[2],
+                     "sync_and_pairs": phase_ms[3], "step": phase_ms[4]},
+        "kernel_ms": kernel_ms,
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": dom_bytes, "kernel_ms": kernel_ms[dominant]},
         "cpu_baseline": cpu,
+        "device_bytes": device_bytes,
         "status": status,
     }
     print(json.dumps(out))

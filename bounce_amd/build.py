@@ -35,14 +35,20 @@ def command(out=LIB, extra=()):
             + ["-o", out] + srcs)
 
 
+CAL_LIB = os.path.join(LIB_DIR, "libbxcal.so")
+
+
 def build(verbose=True):
     os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = command(tmp)
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    jobs = [(LIB, command(LIB + ".tmp")),
+            # measurement support: plain copy kernel for calibrating the profiler's byte counters
+            (CAL_LIB, [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared"] + FP_FLAGS
+             + ["-o", CAL_LIB + ".tmp", os.path.join(CSRC, "bx_calibrate.hip")])]
+    for out, cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
     return LIB
 
 

@@ -10,6 +10,7 @@ u32 = C.c_uint32
 f32 = C.c_float
 
 BX_OK = 0
+BX_E_INVALID, BX_E_DEVICE, BX_E_CAPACITY, BX_E_UNSUPPORTED, BX_E_NOMEM = -1, -2, -3, -4, -5
 BX_SHAPE_SPHERE, BX_SHAPE_CAPSULE, BX_SHAPE_TRIANGLE, BX_SHAPE_HULL, BX_SHAPE_MESH = 0, 1, 2, 3, 4
 BX_BODY_STATIC, BX_BODY_KINEMATIC, BX_BODY_DYNAMIC = 0, 1, 2
 BX_BODY_AWAKE = 0x1

@@ -37,11 +37,26 @@ static int Fail(int code, const std::string& msg) {
         if (_e != hipSuccess) return Fail(BX_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+// Steps of one bx_batch_step call whose kernels are timed with HIP events (12 events per step,
+// see LaunchStep); later steps of a longer call run untimed.
+static const u32 kMaxTimedSteps = 1024;
+
 struct bx_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> ev;  // 12 per timed step of the last bx_batch_step call
+    u32 timedSteps = 0;
 };
+
+static int EnsureEvents(bx_context* ctx, u32 steps) {
+    size_t need = size_t(12) * steps;
+    while (ctx->ev.size() < need) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        ctx->ev.push_back(e);
+    }
+    return BX_OK;
+}
 
 template <class T>
 struct DevArray {
@@ -264,9 +279,15 @@ static int StepRange(bx_batch* b, const bx_step_cfg* cfg, u32 first, u32 count, 
     sp.positionIterations = cfg->position_iterations;
     u32 maxIslands = 0;
     for (u32 i = 0; i < count; ++i) maxIslands += b->worlds[first + i].bodyCount;
+    u32 timedSteps = timed ? std::min(steps, kMaxTimedSteps) : 0u;
+    if (timedSteps) {
+        int rc = EnsureEvents(b->ctx, timedSteps);
+        if (rc) return rc;
+        b->ctx->timedSteps = timedSteps;
+    }
     for (u32 s = 0; s < steps; ++s) {
-        bool last = timed && s + 1 == steps;
-        LaunchStep(b->view, first, count, sp, b->newContacts, maxIslands, b->ctx->stream, last ? b->ctx->ev : nullptr);
+        hipEvent_t* ev = s < timedSteps ? b->ctx->ev.data() + size_t(12) * s : nullptr;
+        LaunchStep(b->view, first, count, sp, b->newContacts, maxIslands, b->ctx->stream, ev);
         b->newContacts = false;
     }
     HIP_CHECK(hipGetLastError());
@@ -287,7 +308,12 @@ int bx_context_create(int device, bx_context** out) {
     bx_context* ctx = new bx_context();
     ctx->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    for (auto& e : ctx->ev) HIP_CHECK(hipEventCreate(&e));
+    int rc = EnsureEvents(ctx, 1);
+    if (rc) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return BX_OK;
 }
@@ -295,7 +321,7 @@ int bx_context_create(int device, bx_context** out) {
 void bx_context_destroy(bx_context* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
-    for (auto& e : ctx->ev) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -772,16 +798,50 @@ int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, ui
 
 int bx_batch_get_timings(bx_batch* b, float* ms, uint32_t count) {
     if (!b || !ms) return Fail(BX_E_INVALID, "bad argument");
-    HIP_CHECK(hipSetDevice(b->ctx->device));
-    HIP_CHECK(hipEventSynchronize(b->ctx->ev[4]));
-    float t[5] = {};
-    HIP_CHECK(hipEventElapsedTime(&t[0], b->ctx->ev[0], b->ctx->ev[1]));
-    HIP_CHECK(hipEventElapsedTime(&t[1], b->ctx->ev[1], b->ctx->ev[2]));
-    HIP_CHECK(hipEventElapsedTime(&t[2], b->ctx->ev[2], b->ctx->ev[3]));
-    HIP_CHECK(hipEventElapsedTime(&t[3], b->ctx->ev[3], b->ctx->ev[4]));
-    HIP_CHECK(hipEventElapsedTime(&t[4], b->ctx->ev[0], b->ctx->ev[4]));
-    for (u32 i = 0; i < count && i < 5; ++i) ms[i] = t[i];
+    bx_context* ctx = b->ctx;
+    if (ctx->timedSteps == 0) return Fail(BX_E_INVALID, "no timed step yet");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    HIP_CHECK(hipEventSynchronize(ctx->ev[size_t(12) * (ctx->timedSteps - 1) + 11]));
+    // phase boundaries (event indices): update contacts (+ initial pair search) 0-2, islands 2-3,
+    // island solve kernels 3-9, fixture sync + pair search 9-11, whole step 0-11
+    static const int kFrom[5] = {0, 2, 3, 9, 0}, kTo[5] = {2, 3, 9, 11, 11};
+    double sum[5] = {};
+    for (u32 s = 0; s < ctx->timedSteps; ++s) {
+        hipEvent_t* e = ctx->ev.data() + size_t(12) * s;
+        for (int i = 0; i < 5; ++i) {
+            float t = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&t, e[kFrom[i]], e[kTo[i]]));
+            sum[i] += t;
+        }
+    }
+    for (u32 i = 0; i < count && i < 5; ++i) ms[i] = float(sum[i] / ctx->timedSteps);
     return BX_OK;
+}
+
+static const char* kKernelNames[11] = {"k_find_pairs(initial)", "k_update_contacts", "k_islands", "k_integrate_vel",
+                                       "k_init_contacts", "k_solve_velocity", "k_mid", "k_solve_position",
+                                       "k_finish", "k_sync", "k_find_pairs"};
+
+const char* bx_kernel_name(uint32_t index) { return index < 11 ? kKernelNames[index] : ""; }
+
+int bx_batch_get_kernel_timings(bx_batch* b, float* ms, uint32_t count) {
+    if (!b || !ms) return Fail(BX_E_INVALID, "bad argument");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    bx_context* ctx = b->ctx;
+    if (ctx->timedSteps == 0) return Fail(BX_E_INVALID, "no timed step yet");
+    HIP_CHECK(hipEventSynchronize(ctx->ev[size_t(12) * (ctx->timedSteps - 1) + 11]));
+    u32 n = std::min<u32>(count, 11);
+    for (u32 i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (u32 s = 0; s < ctx->timedSteps; ++s) {
+            hipEvent_t* e = ctx->ev.data() + size_t(12) * s;
+            float t = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&t, e[i], e[i + 1]));
+            sum += t;
+        }
+        ms[i] = float(sum / ctx->timedSteps);
+    }
+    return int(n);
 }
 
 // Size in bytes of each ABI structure, for layout checks by bindings.

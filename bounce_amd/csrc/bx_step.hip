@@ -438,30 +438,44 @@ void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const Ste
     B.wstate += firstWorld;
     B.worldCount = worldCount;
     dim3 grid(worldCount);
-    if (ev) hipEventRecord(ev[0], stream);
+    // ev (when given) holds kStepEvents events: one before the first kernel and one after each
+    // kernel slot, so the host can report per-kernel times measured on this stream.
+    auto mark = [&](int i) {
+        if (ev) hipEventRecord(ev[i], stream);
+    };
+    mark(0);
     if (findFirst) hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 1u);
+    mark(1);
     hipLaunchKernelGGL(k_update_contacts, grid, dim3(kBlock), 0, stream, B);
-    if (ev) hipEventRecord(ev[1], stream);
+    mark(2);
     if (sp.dt > scalar(0)) {
         hipLaunchKernelGGL(k_islands, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
-        if (ev) hipEventRecord(ev[2], stream);
+        mark(3);
         // one lane per island; islands <= bodies of the launched worlds
         u32 vblocks = std::max(1u, std::min((maxIslands + kSolveLanes - 1) / kSolveLanes, 16384u));
         u32 pblocks = std::max(1u, std::min((maxIslands + 63u) / 64u, 16384u));
         hipLaunchKernelGGL(k_integrate_vel, grid, dim3(kBlock), 0, stream, B, sp);
+        mark(4);
         hipLaunchKernelGGL(k_init_contacts, grid, dim3(kBlock), 0, stream, B, sp);
+        mark(5);
         hipLaunchKernelGGL(k_solve_velocity, dim3(vblocks), dim3(64), 0, stream, B, sp);
+        mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
+        mark(7);
         hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), 0, stream, B, sp);
+        mark(8);
         hipLaunchKernelGGL(k_finish, grid, dim3(kBlock), 0, stream, B, sp);
-        if (ev) hipEventRecord(ev[3], stream);
+        mark(9);
         hipLaunchKernelGGL(k_sync, grid, dim3(kBlock), 0, stream, B, sp);
+        mark(10);
         hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 0u);
+        mark(11);
     } else {
-        if (ev) { hipEventRecord(ev[2], stream); hipEventRecord(ev[3], stream); }
+        for (int i = 3; i <= 9; ++i) mark(i);
         hipLaunchKernelGGL(k_sync, grid, dim3(kBlock), 0, stream, B, sp);
+        mark(10);
+        mark(11);
     }
-    if (ev) hipEventRecord(ev[4], stream);
 }
 
 }  // namespace bx

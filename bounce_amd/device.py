@@ -45,6 +45,8 @@ def load(path=capi.LIB_PATH):
         "bx_batch_get_island_contacts": (C.c_int, [P, u, C.POINTER(u), u, C.POINTER(u)]),
         "bx_batch_set_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
         "bx_batch_get_timings": (C.c_int, [P, C.POINTER(C.c_float), u]),
+        "bx_batch_get_kernel_timings": (C.c_int, [P, C.POINTER(C.c_float), u]),
+        "bx_kernel_name": (C.c_char_p, [u]),
         "bx_struct_size": (u, [C.c_char_p]),
         "bx_batch_device_bytes": (C.c_uint64, [P]),
     }
@@ -172,6 +174,12 @@ class Batch:
         t = (C.c_float * 5)()
         _check(_lib.bx_batch_get_timings(self.h, t, 5))
         return list(t)
+
+    def kernel_timings(self):
+        """{kernel name: ms} of the last step of the last step() call (HIP events on the stream)."""
+        t = (C.c_float * 11)()
+        n = _check(_lib.bx_batch_get_kernel_timings(self.h, t, 11))
+        return {_lib.bx_kernel_name(i).decode(): float(t[i]) for i in range(n)}
 
     def device_bytes(self):
         return int(_lib.bx_batch_device_bytes(self.h))

@@ -296,9 +296,22 @@ int bx_batch_get_island_contacts(bx_batch* batch, uint32_t world, uint32_t* out,
    `bodies` is in creation order with `count` == body_count of the world. */
 int bx_batch_set_bodies(bx_batch* batch, uint32_t world, const bx_body_state* bodies, uint32_t count);
 
-/* Per-phase GPU time of the last bx_batch_step call, in milliseconds:
-   [0] update contacts, [1] islands, [2] solve, [3] find new pairs, [4] total. */
+/* Per-phase GPU time per step, in milliseconds, averaged over the steps of the last bx_batch_step
+   call on the batch's context (its first 1024 steps), from HIP events recorded on the context
+   stream between the kernels: [0] update contacts, [1] islands, [2] island solve, [3] fixture
+   sync and new pairs, [4] whole step. Waits for those steps to finish. */
 int bx_batch_get_timings(bx_batch* batch, float* ms, uint32_t count);
+
+/* Per-kernel GPU time (ms) per step over the same steps, in launch order; names from
+   bx_kernel_name(i). Returns the number of entries written (at most 11). */
+int bx_batch_get_kernel_timings(bx_batch* batch, float* ms, uint32_t count);
+const char* bx_kernel_name(uint32_t index);
+
+/* Size in bytes of an ABI structure by name (layout checks for bindings); 0 if unknown. */
+uint32_t bx_struct_size(const char* name);
+
+/* Device memory held by a batch, in bytes. */
+uint64_t bx_batch_device_bytes(bx_batch* batch);
 
 #ifdef __cplusplus
 }
