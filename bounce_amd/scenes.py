@@ -268,3 +268,63 @@ def concat(sets, name="mix"):
             f["shape_index"][mesh] += mbase
             out.worlds.append(World(w.bodies.copy(), f, w.joints.copy(), w.gravity, w.flags))
     return out
+
+
+def random_orientations(rng, n):
+    """Uniform random unit quaternions (x, y, z, w), rounded to binary32."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(F32)
+
+
+def mixed_drop_world(rng, n_bodies, ground_hull, box_hull=None, kinds=("sphere", "capsule", "box"),
+                     extent=(8.0, 16.0, 8.0), sensors=0.0, hull_choices=None):
+    """A static ground box plus `n_bodies` dynamic bodies of the given kinds at seeded positions and
+    orientations: spheres r ~ U[0.2, 0.6], capsules with segment length ~ U[0.5, 2] along local x and
+    radius 0.25, unit boxes, or hulls drawn from `hull_choices` (library indices). A fraction
+    `sensors` of the fixtures are sensors."""
+    bodies = body_array(n_bodies + 1)
+    fixtures = fixture_array(n_bodies + 1)
+    _ground(bodies, fixtures, ground_hull)
+    ex, ey, ez = extent
+    bodies["type"][1:] = capi.BX_BODY_DYNAMIC
+    bodies["position"][1:, 0] = rng.uniform(-ex / 2, ex / 2, n_bodies)
+    bodies["position"][1:, 1] = rng.uniform(1.0, 1.0 + ey, n_bodies)
+    bodies["position"][1:, 2] = rng.uniform(-ez / 2, ez / 2, n_bodies)
+    bodies["orientation"][1:] = random_orientations(rng, n_bodies)
+    fixtures["body"][1:] = np.arange(1, n_bodies + 1)
+    fixtures["density"][1:] = 1.0
+    kind = rng.integers(0, len(kinds), n_bodies)
+    for i in range(n_bodies):
+        f = fixtures[i + 1]
+        k = kinds[kind[i]]
+        if k == "sphere":
+            f["shape_type"] = capi.BX_SHAPE_SPHERE
+            f["radius"] = F32(rng.uniform(0.2, 0.6))
+        elif k == "capsule":
+            half = F32(0.5 * rng.uniform(0.5, 2.0))
+            f["shape_type"] = capi.BX_SHAPE_CAPSULE
+            f["vertex1"] = (-half, 0.0, 0.0)
+            f["vertex2"] = (half, 0.0, 0.0)
+            f["radius"] = F32(0.25)
+        elif k == "box":
+            f["shape_type"] = capi.BX_SHAPE_HULL
+            f["shape_index"] = box_hull
+        else:
+            f["shape_type"] = capi.BX_SHAPE_HULL
+            f["shape_index"] = hull_choices[rng.integers(0, len(hull_choices))]
+    if sensors > 0:
+        fixtures["is_sensor"][1:] = (rng.uniform(size=n_bodies) < sensors).astype(np.uint32)
+    return World(bodies, fixtures)
+
+
+def mixed_drop(n_bodies=60, n_worlds=1, seed=4, kinds=("sphere", "capsule", "box"), extent=(8.0, 16.0, 8.0),
+               sensors=0.0):
+    """Spheres, capsules and boxes dropped on a ground box (the convex narrowphase mix)."""
+    s = SceneSet("mixed_drop")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    rng = np.random.default_rng(seed)
+    for _ in range(n_worlds):
+        s.worlds.append(mixed_drop_world(rng, n_bodies, g, b, kinds, extent, sensors))
+    return s

@@ -92,6 +92,10 @@ inline ContactID MakeID(u32 k1, u32 k2) {  // b3MakeID(uint64 key1, uint64 key2)
     ContactID id; id.triangleKey = kNullU32; id.pair = MakePair(k1, 0, k2, 0); return id;
 }
 
+// b3ClipVertex / b3ClipPlane (clip.h:47-61)
+struct ClipVertex { Vec3 position; FeaturePair pair; };
+struct ClipPlane { Plane plane; u32 edge; };
+
 struct ManifoldPoint {
     Vec3 localNormal1, localPoint1, localPoint2;
     scalar normalImpulse;
@@ -146,6 +150,20 @@ struct FeatureCache {
 void ComputeAABB(const Shape& s, const Transform& xf, AABB* out);
 struct MassData { Vec3 center; scalar mass; Mat33 I; };
 void ComputeMass(const Shape& s, scalar density, MassData* out);
+void SphereMass(const Vec3& center, scalar radius, scalar density, MassData* out);
+void CapsuleMass(const Vec3& A, const Vec3& B, scalar radius, scalar density, MassData* out);
+
+// GJK (gjk.cpp:37-925, gjk.h): closest points between two convex vertex clouds with radii
+struct GJKProxy {
+    const Vec3* vertices = nullptr;
+    u32 count = 0;
+    scalar radius = 0;
+    Vec3 buffer[3];
+    void Set(const Shape& s);
+    u32 GetSupportIndex(const Vec3& d) const;
+};
+struct GJKOutput { Vec3 point1, point2; scalar distance; u32 iterations; };
+GJKOutput GJK(const Transform& xf1, const GJKProxy& p1, const Transform& xf2, const GJKProxy& p2, bool applyRadius);
 
 // Narrowphase (collide/*.cpp)
 void CollideHulls(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2,

@@ -247,9 +247,6 @@ static u32 ReadState(FeatureCache& c, const Transform& xf1, const Hull& h1, cons
 }
 
 // ------------------------------------------------------------------ clipping (clip.cpp)
-struct ClipVertex { Vec3 position; FeaturePair pair; };
-struct ClipPlane { Plane plane; u32 edge; };
-
 typedef std::vector<ClipVertex> ClipPolygon;
 
 static void BuildPolygon(ClipPolygon& out, const Transform& xf, u32 index, const Hull& h) {
@@ -617,6 +614,8 @@ void ComputeAABB(const Shape& s, const Transform& xf, AABB* out) {
 void ComputeMass(const Shape& s, scalar density, MassData* out) {
     switch (s.type) {
     case e_hull: HullMass(*s.hull, density, out); break;
+    case e_sphere: SphereMass(s.v1, s.radius, density, out); break;
+    case e_capsule: CapsuleMass(s.v1, s.v2, s.radius, density, out); break;
     default:
         out->mass = 0; out->center.SetZero(); out->I.SetZero();
         break;

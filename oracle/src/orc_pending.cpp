@@ -19,12 +19,6 @@ bool SolveJointsPosition(World&, std::vector<u32>& j, Position*) { if (!j.empty(
 void MeshContactFindPairs(World&, Contact*) { Missing("mesh contacts"); }
 void MeshContactSynchronize(World&, Contact*) { Missing("mesh contacts"); }
 void MeshContactCollide(World&, Contact*) { Missing("mesh contacts"); }
-void CollideSpheres(Manifold&, const Transform&, const Shape&, const Transform&, const Shape&) { Missing("sphere contacts"); }
-void CollideCapsuleAndSphere(Manifold&, const Transform&, const Shape&, const Transform&, const Shape&) { Missing("capsule contacts"); }
-void CollideCapsules(Manifold&, const Transform&, const Shape&, const Transform&, const Shape&) { Missing("capsule contacts"); }
-void CollideHullAndSphere(Manifold&, const Transform&, const Shape&, const Transform&, const Shape&) { Missing("hull-sphere contacts"); }
-void CollideHullAndCapsule(Manifold&, const Transform&, const Shape&, const Transform&, const Shape&) { Missing("hull-capsule contacts"); }
-bool ConvexTestOverlap(const Transform&, const Shape&, const Transform&, const Shape&) { Missing("sensor overlap"); }
 
 }  // namespace orc
 

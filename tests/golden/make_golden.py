@@ -35,6 +35,10 @@ SCENES = {
     "drop1000": (lambda: scenes.drop_boxes(1000), 60, (60,)),
     "pyramid10_sleep": (lambda: scenes.with_flags(scenes.pyramids(1, levels=10), 1), 240, (120, 240)),
     "drop100_sleep": (lambda: scenes.with_flags(scenes.drop_boxes(100, extent=(8.0, 20.0, 8.0)), 1), 360, (180, 360)),
+    "mixed60": (lambda: scenes.mixed_drop(60, seed=0), 240, (120, 240)),
+    "mixed60_sensors": (lambda: scenes.mixed_drop(60, seed=1, sensors=0.3), 240, (240,)),
+    "mixed150_dense_sleep": (lambda: scenes.with_flags(scenes.mixed_drop(150, seed=2, extent=(4.0, 20.0, 4.0)), 1),
+                             300, (150, 300)),
 }
 
 
