@@ -498,7 +498,7 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
             if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
         }
         // capacities
-        u32 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : 8 * d.fixture_count + 64;
+        u32 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : 12 * d.fixture_count + 64;
         u32 pcap = opts && opts->max_pairs_per_world ? opts->max_pairs_per_world : 2 * cap;
         wi.contactBase = contactTotal;
         wi.contactCap = cap;

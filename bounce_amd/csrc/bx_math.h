@@ -109,6 +109,7 @@ BX_HD Vec3 Normalize(const Vec3& v) {
     return v;
 }
 BX_HD scalar DistanceSquared(const Vec3& a, const Vec3& b) { return LengthSquared(a - b); }
+BX_HD scalar Distance(const Vec3& a, const Vec3& b) { return Length(a - b); }
 BX_HD Vec3 Min(const Vec3& a, const Vec3& b) { return Vec3(Min(a.x, b.x), Min(a.y, b.y), Min(a.z, b.z)); }
 BX_HD Vec3 Max(const Vec3& a, const Vec3& b) { return Vec3(Max(a.x, b.x), Max(a.y, b.y), Max(a.z, b.z)); }
 BX_HD Vec3 Mul(const Vec3& a, const Vec3& b) { return Vec3(a.x * b.x, a.y * b.y, a.z * b.z); }

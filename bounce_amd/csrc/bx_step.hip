@@ -18,6 +18,7 @@
 // independent islands).
 #include <algorithm>
 #include "bx_collide.h"
+#include "bx_convex.h"
 #include "bx_solver.h"
 
 namespace bx {
@@ -111,7 +112,12 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     bool wasTouching = (c.flags & 2u) != 0;
     bool touching = false;
     if (sensor) {
-        atomicOr(&ws.status, 0x100u);  // sensor overlap tests are not supported on the device path
+        // b3ConvexContact::TestOverlap (convex_contact.cpp:34-43)
+        Transform xfA(bA.xfT, bA.q), xfB(bB.xfT, bB.q);
+        HullView hA, hB;
+        if (fA.shapeType == 3u) hA = GetHull(B, fA.shapeIndex);
+        if (fB.shapeType == 3u) hB = GetHull(B, fB.shapeIndex);
+        touching = ConvexTestOverlap(xfA, MakeProxy(fA, &hA), xfB, MakeProxy(fB, &hB));
         c.manifoldCount = 0;
     } else {
         const u32 oldCount = c.manifoldCount;
@@ -143,8 +149,20 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
             CollideHulls(m, xfA, hA, fA.radius, xfB, hB, fB.radius, cache, xf0A, xf0B, scratch, overflow);
             c.cache = cache;
             if (overflow) atomicOr(&ws.status, 0x2u);
+        } else if (fA.shapeType == 0u && fB.shapeType == 0u) {  // sphere_contact.cpp:42-45
+            CollideSpheres(m, xfA, fA, xfB, fB);
+        } else if (fA.shapeType == 1u && fB.shapeType == 0u) {  // capsule_sphere_contact.cpp:42-45
+            CollideCapsuleAndSphere(m, xfA, fA, xfB, fB);
+        } else if (fA.shapeType == 1u && fB.shapeType == 1u) {  // capsule_contact.cpp:41-44
+            CollideCapsules(m, xfA, fA, xfB, fB);
+        } else if (fA.shapeType == 3u && fB.shapeType == 0u) {  // hull_sphere_contact.cpp:42-45
+            HullView hA = GetHull(B, fA.shapeIndex);
+            CollideHullAndSphere(m, xfA, hA, fA, xfB, fB);
+        } else if (fA.shapeType == 3u && fB.shapeType == 1u) {  // hull_capsule_contact.cpp:42-45
+            HullView hA = GetHull(B, fA.shapeIndex);
+            CollideHullAndCapsule(m, xfA, hA, fA, xfB, fB);
         } else {
-            atomicOr(&ws.status, 0x200u);  // shape pair not supported on the device path yet
+            atomicOr(&ws.status, 0x200u);  // shape pair without a device narrowphase (meshes, triangles)
         }
         c.manifoldCount = 1;
         if (wi.flags & 0x2u) {

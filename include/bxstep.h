@@ -250,8 +250,8 @@ typedef struct bx_step_cfg {
 } bx_step_cfg;
 
 typedef struct bx_batch_opts {
-    uint32_t max_contacts_per_world;   /* 0: derived from the descriptors */
-    uint32_t max_pairs_per_world;      /* 0: derived */
+    uint32_t max_contacts_per_world;   /* 0: 12 x fixtures + 64 */
+    uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t solver_mode;              /* 0: auto, 1: one lane per island, 2: scheduled wavefront */
     uint32_t reserved[5];
 } bx_batch_opts;
