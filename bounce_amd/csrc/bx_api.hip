@@ -15,6 +15,7 @@
 #include <vector>
 #include "../../include/bxstep.h"
 #include "bx_shapes.h"
+#include "bx_joint.h"
 
 namespace bx {
 struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIterations; };
@@ -369,10 +370,6 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
         wi.flags = d.flags;
         wi.gravity = V3(d.gravity);
         wi.serialIslands = 0;
-        if (d.joint_count) {
-            delete b;
-            return Fail(BX_E_UNSUPPORTED, "joints are not supported by the device path yet");
-        }
         // bodies (b3Body::b3Body, body.cpp:25-103)
         for (u32 i = 0; i < d.body_count; ++i) {
             const bx_body_desc& bd = d.bodies[i];
@@ -482,6 +479,63 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
                 x.fixtureCount = fc;
             }
         }
+        // revolute joints (b3World::CreateJoint after the bodies' fixtures; joint_manager.cpp:30-72
+        // prepends each joint's edges to both bodies' joint lists)
+        std::vector<std::vector<JointEdge>> edgesOf(d.body_count);
+        for (u32 i = 0; i < d.joint_count; ++i) {
+            const bx_joint_desc& jd = d.joints[i];
+            if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= d.body_count || jd.body_b >= d.body_count ||
+                jd.body_a == jd.body_b) {
+                delete b;
+                return Fail(BX_E_INVALID, "bad joint descriptor");
+            }
+            if (jd.enable_limit) {
+                delete b;
+                return Fail(BX_E_UNSUPPORTED, "revolute joint limits are not supported by the device path");
+            }
+            Joint j;
+            std::memset(&j, 0, sizeof(j));
+            j.bodyA = jd.body_a;
+            j.bodyB = jd.body_b;
+            j.collideConnected = jd.collide_connected ? 1u : 0u;
+            if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
+                Mat33 rotation;
+                rotation.z = V3(jd.axis);
+                ComputeBasis(rotation.z, rotation.x, rotation.y);
+                Quat q = Mat33Quat(rotation);
+                const Body& A = bodies[wi.bodyBase + jd.body_a];
+                const Body& Bd = bodies[wi.bodyBase + jd.body_b];
+                Vec3 anchor = V3(jd.anchor);
+                j.localAnchorA = MulT(Transform(A.xfT, A.q), anchor);
+                j.localRotationA = MulC(A.q, q);
+                j.localAnchorB = MulT(Transform(Bd.xfT, Bd.q), anchor);
+                j.localRotationB = MulC(Bd.q, q);
+                j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
+            } else {
+                j.localAnchorA = V3(jd.local_anchor_a);
+                j.localRotationA = Q4(jd.local_rotation_a);
+                j.localAnchorB = V3(jd.local_anchor_b);
+                j.localRotationB = Q4(jd.local_rotation_b);
+                j.referenceRotation = Q4(jd.reference_rotation);
+            }
+            j.enableMotor = jd.enable_motor ? 1u : 0u;
+            j.enableLimit = 0;
+            j.lowerAngle = jd.lower_angle;
+            j.upperAngle = jd.upper_angle;
+            j.motorSpeed = jd.motor_speed;
+            j.maxMotorTorque = jd.max_motor_torque;
+            joints.push_back(j);
+            edgesOf[jd.body_a].insert(edgesOf[jd.body_a].begin(), JointEdge{i, jd.body_b});
+            edgesOf[jd.body_b].insert(edgesOf[jd.body_b].begin(), JointEdge{i, jd.body_a});
+        }
+        u32 ecursor = 0;
+        for (u32 i = 0; i < d.body_count; ++i) {
+            Body& x = bodies[wi.bodyBase + i];
+            x.jointBegin = ecursor;
+            x.jointCount = u32(edgesOf[i].size());
+            for (const JointEdge& e : edgesOf[i]) jointEdges.push_back(e);
+            ecursor += x.jointCount;
+        }
         std::vector<u32> po(d.fixture_count);
         for (u32 i = 0; i < d.fixture_count; ++i) po[i] = i;
         std::sort(po.begin(), po.end(), [&](u32 a, u32 c) { return pid[a] < pid[c]; });
@@ -568,7 +622,8 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     ALLOC(V.islands, nb, (IslandInfo*)nullptr);
     ALLOC(V.islandBodies, islandBodyTotal, (u32*)nullptr);
     ALLOC(V.islandContacts, contactTotal, (uint4*)nullptr);
-    ALLOC(V.islandJoints, joints.size(), (u32*)nullptr);
+    ALLOC(V.islandJoints, joints.size(), (uint4*)nullptr);
+    ALLOC(V.jointCons, joints.size(), (JointCon*)nullptr);
     ALLOC(V.dfsStack, nb, (u32*)nullptr);
     ALLOC(V.velBodies, islandBodyTotal, (VelBody*)nullptr);
     ALLOC(V.posBodies, islandBodyTotal, (PosBody*)nullptr);

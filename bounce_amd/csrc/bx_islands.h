@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
         IslandInfo* islands = B.islands + wi.bodyBase;
         u32* ibodies = B.islandBodies + wi.islandBodyBase;
         uint4* icontacts = B.islandContacts + wi.contactBase;
-        u32* ijoints = B.islandJoints + wi.jointBase;
+        uint4* ijoints = B.islandJoints + wi.jointBase;
         u32 islandCount = 0, nbo = 0, nc = 0, nj = 0, overflow = 0;
         for (u32 s = nb; s-- > 0;) {
             u32 sf = bflags[s];
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
                     JointEdge je = B.jointEdges[wi.jointEdgeBase + jb + ji];
                     Joint& j = B.joints[wi.jointBase + je.joint];
                     if (j.islandFlag) continue;
-                    ijoints[nj++] = je.joint;
+                    ijoints[nj++] = make_uint4(je.joint, j.bodyA, j.bodyB, 0u);
                     j.islandFlag = 1;
                     if (bflags[je.other] & kIsland) continue;
                     stack[stackCount++] = je.other;
@@ -196,6 +196,13 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
                 e.z = bidx[e.z & 0x7FFFFFFFu];
                 e.w = info.bodyStart;
                 icontacts[k] = e;
+            }
+            for (u32 k = info.jointStart; k < nj; ++k) {  // same for the island's joints
+                uint4 e = ijoints[k];
+                e.y = bidx[e.y];
+                e.z = bidx[e.z];
+                e.w = info.bodyStart;
+                ijoints[k] = e;
             }
             islands[islandCount++] = info;
             for (u32 k = 0; k < bodyCount; ++k) {  // world.cpp:411-419

@@ -1,0 +1,295 @@
+// bx_joint.h -- the revolute joint's solver rows for the device island sweeps.
+//
+// Same binary32 expression trees as revolute_joint.cpp:78-531 (InitializeVelocityConstraints,
+// WarmStart, SolveVelocityConstraints, SolvePositionConstraints). Joint limits need the joint
+// angle through atan2 (revolute_joint.cpp:126, 437), whose last bit the device library does not
+// share with the host's; worlds with limited joints are rejected at upload, so the limit rows are
+// kept here only for completeness of the state machine (enableLimit is always 0 on the device).
+#pragma once
+#include "bx_types.h"
+
+namespace bx {
+
+// b3ComputeBasis (mat33.h:310-328)
+BX_HD void ComputeBasis(const Vec3& a, Vec3& b, Vec3& c) {
+    if (Abs(a.x) >= kSqrtInv3) b = Vec3(a.y, -a.x, scalar(0));
+    else b = Vec3(scalar(0), a.z, -a.y);
+    b.Normalize();
+    c = Cross(a, b);
+}
+
+// b3Mat33Quat (quat.h:283-345); columns x, y, z are m[0], m[1], m[2].
+BX_HD Quat Mat33Quat(const Mat33& m) {
+    auto at = [&](u32 c, u32 r) -> scalar {
+        const Vec3& col = c == 0 ? m.x : (c == 1 ? m.y : m.z);
+        return r == 0 ? col.x : (r == 1 ? col.y : col.z);
+    };
+    scalar trace = at(0, 0) + at(1, 1) + at(2, 2);
+    if (trace > scalar(0)) {
+        Quat r;
+        scalar s = Sqrt(trace + scalar(1));
+        r.s = scalar(0.5) * s;
+        scalar t = scalar(0.5) / s;
+        r.v.x = t * (at(1, 2) - at(2, 1));
+        r.v.y = t * (at(2, 0) - at(0, 2));
+        r.v.z = t * (at(0, 1) - at(1, 0));
+        return r;
+    }
+    u32 i = 0;
+    if (at(1, 1) > at(0, 0)) i = 1;
+    if (at(2, 2) > at(i, i)) i = 2;
+    u32 j = i == 0 ? 1u : (i == 1 ? 2u : 0u);
+    u32 k = j == 0 ? 1u : (j == 1 ? 2u : 0u);
+    scalar s = Sqrt((at(i, i) - (at(j, j) + at(k, k))) + scalar(1));
+    scalar q[4];
+    q[i] = s * scalar(0.5);
+    scalar t = s != scalar(0) ? scalar(0.5) / s : s;
+    q[3] = t * (at(j, k) - at(k, j));
+    q[j] = t * (at(i, j) + at(j, i));
+    q[k] = t * (at(i, k) + at(k, i));
+    Quat r;
+    r.v.x = q[0];
+    r.v.y = q[1];
+    r.v.z = q[2];
+    r.s = q[3];
+    return r;
+}
+
+// revolute_joint.cpp:78-231. `pos` is the island's position array, `indexA/B` the joint bodies'
+// island-local indices in this island.
+BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
+                        const PosBody* pos, scalar dtRatio, bool warm) {
+    jc.indexA = indexA;
+    jc.indexB = indexB;
+    jc.mA = A.invMass;
+    jc.mB = B.invMass;
+    jc.localCenterA = A.localCenter;
+    jc.localCenterB = B.localCenter;
+    jc.localInvIA = A.invI;
+    jc.localInvIB = B.invI;
+    jc.iA = pos[indexA].I;
+    jc.iB = pos[indexB].I;
+    jc.localAnchorA = j.localAnchorA;
+    jc.localAnchorB = j.localAnchorB;
+    jc.localRotationA = j.localRotationA;
+    jc.localRotationB = j.localRotationB;
+    jc.lowerAngle = j.lowerAngle;
+    jc.upperAngle = j.upperAngle;
+    jc.enableMotor = j.enableMotor;
+    jc.enableLimit = 0;
+    jc.motorSpeed = j.motorSpeed;
+    jc.maxMotorTorque = j.maxMotorTorque;
+    Quat qA = pos[indexA].q;
+    Quat qB = pos[indexB].q;
+    scalar mA = jc.mA;
+    Mat33 iA = jc.iA;
+    scalar mB = jc.mB;
+    Mat33 iB = jc.iB;
+    Quat fA = qA * j.localRotationA;
+    Quat fB = qB * j.localRotationB;
+    Mat33 RfA = fA.GetRotationMatrix();
+    Mat33 RfB = fB.GetRotationMatrix();
+    Vec3 w1 = RfA.z;
+    Vec3 u2 = RfB.x, v2 = RfB.y;
+    scalar motorImpulse = j.motorImpulse;
+    if (j.enableMotor) {
+        jc.motorAxis = w1;
+        jc.motorAxis.Normalize();
+        scalar K = Dot((iA + iB) * jc.motorAxis, jc.motorAxis);
+        jc.motorMass = K > scalar(0) ? scalar(1) / K : scalar(0);
+    } else {
+        jc.motorAxis.SetZero();
+        jc.motorMass = scalar(0);
+        motorImpulse = scalar(0);
+    }
+    // without limits the limit state is inactive and its impulse zero (revolute_joint.cpp:163-167)
+    j.limitState = 0;
+    jc.limitState = 0;
+    scalar limitImpulse = scalar(0);
+    {
+        jc.rA = Mul(qA, j.localAnchorA - jc.localCenterA);
+        jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
+        Mat33 RA = Skew(jc.rA);
+        Mat33 RAT = Transpose(RA);
+        Mat33 RB = Skew(jc.rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 M = Diagonal(mA + mB, mA + mB, mA + mB);
+        jc.linearMass = M + RA * iA * RAT + RB * iB * RBT;
+    }
+    {
+        Vec3 a1 = Cross(u2, w1);
+        Vec3 a2 = Cross(v2, w1);
+        scalar k11 = Dot(a1, (iA + iB) * a1);
+        scalar k12 = Dot(a1, iA * a2) + Dot(a1, iB * a2);
+        scalar k22 = Dot(a2, (iA + iB) * a2);
+        jc.a1 = a1;
+        jc.a2 = a2;
+        jc.angularMass.x.x = k11;
+        jc.angularMass.x.y = k12;
+        jc.angularMass.y.x = k12;
+        jc.angularMass.y.y = k22;
+    }
+    Vec3 linearImpulse = j.linearImpulse;
+    Vec2 angularImpulse = j.angularImpulse;
+    if (warm) {
+        motorImpulse *= dtRatio;
+        limitImpulse *= dtRatio;
+        linearImpulse *= dtRatio;
+        angularImpulse *= dtRatio;
+    } else {
+        motorImpulse = scalar(0);
+        limitImpulse = scalar(0);
+        linearImpulse.SetZero();
+        angularImpulse.SetZero();
+    }
+    jc.motorImpulse = motorImpulse;
+    jc.limitImpulse = limitImpulse;
+    jc.linearImpulse = linearImpulse;
+    jc.angularImpulse = angularImpulse;
+}
+
+// revolute_joint.cpp:233-277
+template <class V>
+BX_HD void WarmStartJoint(const JointCon& jc, V* vel) {
+    Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
+    Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
+    if (jc.enableMotor) {  // limitState is never e_equalLimits without limits
+        Vec3 P = jc.motorAxis * jc.motorImpulse;
+        wA -= jc.iA * P;
+        wB += jc.iB * P;
+    }
+    {
+        vA -= jc.mA * jc.linearImpulse;
+        wA -= jc.iA * Cross(jc.rA, jc.linearImpulse);
+        vB += jc.mB * jc.linearImpulse;
+        wB += jc.iB * Cross(jc.rB, jc.linearImpulse);
+    }
+    {
+        Vec3 L = jc.angularImpulse.x * jc.a1 + jc.angularImpulse.y * jc.a2;
+        wA -= jc.iA * L;
+        wB += jc.iB * L;
+    }
+    vel[jc.indexA].v = vA;
+    vel[jc.indexA].w = wA;
+    vel[jc.indexB].v = vB;
+    vel[jc.indexB].w = wB;
+}
+
+// revolute_joint.cpp:279-367
+template <class V>
+BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
+    Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
+    Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
+    Mat33 iA = jc.iA, iB = jc.iB;
+    if (jc.enableMotor) {
+        scalar dw = Dot(wB - wA, jc.motorAxis);
+        scalar Cdot = dw - jc.motorSpeed;
+        scalar impulse = -jc.motorMass * Cdot;
+        scalar oldImpulse = jc.motorImpulse;
+        scalar maxImpulse = dt * jc.maxMotorTorque;
+        jc.motorImpulse = Clamp(jc.motorImpulse + impulse, -maxImpulse, maxImpulse);
+        impulse = jc.motorImpulse - oldImpulse;
+        Vec3 P = jc.motorAxis * impulse;
+        wA -= iA * P;
+        wB += iB * P;
+    }
+    {
+        Vec3 Cdot = vB + Cross(wB, jc.rB) - vA - Cross(wA, jc.rA);
+        Vec3 impulse = jc.linearMass.Solve(-Cdot);
+        jc.linearImpulse += impulse;
+        vA -= jc.mA * impulse;
+        wA -= jc.iA * Cross(jc.rA, impulse);
+        vB += jc.mB * impulse;
+        wB += jc.iB * Cross(jc.rB, impulse);
+    }
+    {
+        Vec3 dw = wB - wA;
+        Vec2 Cdot;
+        Cdot.x = Dot(dw, jc.a1);
+        Cdot.y = Dot(dw, jc.a2);
+        Vec2 impulse = Solve22(jc.angularMass, -Cdot);
+        jc.angularImpulse += impulse;
+        Vec3 L = impulse.x * jc.a1 + impulse.y * jc.a2;
+        wA -= jc.iA * L;
+        wB += jc.iB * L;
+    }
+    vel[jc.indexA].v = vA;
+    vel[jc.indexA].w = wA;
+    vel[jc.indexB].v = vB;
+    vel[jc.indexB].w = wB;
+}
+
+// revolute_joint.cpp:369-531 without the limit row. `P` is an array of PosBody.
+template <class P>
+BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
+    Vec3 xA = pos[jc.indexA].x;
+    Quat qA = pos[jc.indexA].q;
+    Vec3 xB = pos[jc.indexB].x;
+    Quat qB = pos[jc.indexB].q;
+    Mat33 iA = pos[jc.indexA].I;
+    Mat33 iB = pos[jc.indexB].I;
+    scalar mA = jc.mA, mB = jc.mB;
+    scalar linearError = scalar(0);
+    {
+        Vec3 rA = Mul(qA, jc.localAnchorA - jc.localCenterA);
+        Vec3 rB = Mul(qB, jc.localAnchorB - jc.localCenterB);
+        Vec3 C = xB + rB - xA - rA;
+        linearError += Length(C);
+        Mat33 M = Diagonal(mA + mB, mA + mB, mA + mB);
+        Mat33 RA = Skew(rA);
+        Mat33 RAT = Transpose(RA);
+        Mat33 RB = Skew(rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 mass = M + RA * iA * RAT + RB * iB * RBT;
+        Vec3 impulse = mass.Solve(-C);
+        xA -= mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        qA.Normalize();
+        iA = RotateToFrame(jc.localInvIA, qA);
+        xB += mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        qB.Normalize();
+        iB = RotateToFrame(jc.localInvIB, qB);
+    }
+    scalar angularError = scalar(0);
+    const scalar kCosSlop = scalar(0.002);
+    {
+        Quat fA = qA * jc.localRotationA;
+        Quat fB = qB * jc.localRotationB;
+        Mat33 RfA = fA.GetRotationMatrix();
+        Mat33 RfB = fB.GetRotationMatrix();
+        Vec3 w1 = RfA.z;
+        Vec3 u2 = RfB.x, v2 = RfB.y;
+        Vec3 a1 = Cross(u2, w1);
+        Vec3 a2 = Cross(v2, w1);
+        scalar k11 = Dot(a1, (iA + iB) * a1);
+        scalar k12 = Dot(a1, iA * a2) + Dot(a1, iB * a2);
+        scalar k22 = Dot(a2, (iA + iB) * a2);
+        Mat22 K;
+        K.x.x = k11;
+        K.x.y = k12;
+        K.y.x = k12;
+        K.y.y = k22;
+        Vec2 C;
+        C.x = Dot(u2, w1);
+        C.y = Dot(v2, w1);
+        Vec2 impulse = Solve22(K, -C);
+        Vec3 Pa = impulse.x * a1 + impulse.y * a2;
+        qA -= Derivative(qA, iA * Pa);
+        qA.Normalize();
+        iA = RotateToFrame(jc.localInvIA, qA);
+        qB += Derivative(qB, iB * Pa);
+        qB.Normalize();
+        iB = RotateToFrame(jc.localInvIB, qB);
+        angularError = Abs(C.x) + Abs(C.y);
+    }
+    pos[jc.indexA].x = xA;
+    pos[jc.indexA].q = qA;
+    pos[jc.indexB].x = xB;
+    pos[jc.indexB].q = qB;
+    pos[jc.indexA].I = iA;
+    pos[jc.indexB].I = iB;
+    return linearError <= kLinearSlop && angularError <= kCosSlop;  // limitError is 0 without limits
+}
+
+}  // namespace bx

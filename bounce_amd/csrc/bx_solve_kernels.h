@@ -55,15 +55,28 @@ __global__ void __launch_bounds__(kBlock) k_init_contacts(BatchView B, StepParam
         B.velCons[wi.contactBase + k] = vc;
         B.posCons[wi.contactBase + k] = pc;
     }
+    // b3JointSolver::InitializeVelocityConstraints: independent per joint (revolute_joint.cpp:78-231)
+    const uint4* ij = B.islandJoints + wi.jointBase;
+    for (u32 k = threadIdx.x; k < ws.islandJointCount; k += blockDim.x) {
+        const uint4 e = ij[k];
+        Joint& j = B.joints[wi.jointBase + e.x];
+        JointCon jc;
+        InitJointCon(jc, j, bodies[j.bodyA], bodies[j.bodyB], e.y, e.z, B.posBodies + wi.islandBodyBase + e.w,
+                     dtRatio, warm);
+        B.jointCons[wi.jointBase + k] = jc;
+    }
 }
 
 // Warm start and velocity iterations of one island on one lane. kInLds selects at compile time
 // whether the velocities live in the lane's LDS slab (LDS instructions, which do not share the
 // in-order vector-memory counter with the constraint prefetch) or stay in global memory.
 template <bool kInLds>
-__device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, VelBody* vel, u32 iterations, bool warm) {
+__device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jcs, u32 nj, VelBody* vel,
+                                               u32 iterations, bool warm, scalar dt) {
     // Each contact's body velocities are read first and only then the next contact's constraint is
     // requested (clamped index, no branch), so waiting for the bodies never waits for the prefetch.
+    // Order of island.cpp:154-181: contact warm start, joint warm start, then per iteration all
+    // joints followed by all contacts.
     if (warm && nc > 0) {
         VelCon next = vcs[0];
         for (u32 k = 0; k < nc; ++k) {
@@ -75,7 +88,11 @@ __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, VelBody* vel
             vel[vc.indexB].v = vB; vel[vc.indexB].w = wB;
         }
     }
-    for (u32 it = 0; it < iterations && nc > 0; ++it) {
+    if (warm)
+        for (u32 k = 0; k < nj; ++k) WarmStartJoint(jcs[k], vel);
+    for (u32 it = 0; it < iterations && (nc > 0 || nj > 0); ++it) {
+        for (u32 k = 0; k < nj; ++k) SolveJointVelocity(jcs[k], vel, dt);
+        if (nc == 0) continue;
         VelCon next = vcs[0];
         for (u32 k = 0; k < nc; ++k) {
             VelCon vc = next;
@@ -109,13 +126,16 @@ __global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams s
             const IslandInfo isl = B.islands[wi.bodyBase + k];
             VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;
             VelCon* vcs = B.velCons + wi.contactBase + isl.contactStart;
+            JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
             if (isl.bodyCount <= kSlabBodies) {
                 VelBody* slab = slabs[threadIdx.x];
                 for (u32 b = 0; b < isl.bodyCount; ++b) slab[b] = gvel[b];
-                VelocitySweeps<true>(vcs, isl.contactCount, slab, sp.velocityIterations, warm);
+                VelocitySweeps<true>(vcs, isl.contactCount, jcs, isl.jointCount, slab, sp.velocityIterations, warm,
+                                     sp.dt);
                 for (u32 b = 0; b < isl.bodyCount; ++b) gvel[b] = slab[b];
             } else {
-                VelocitySweeps<false>(vcs, isl.contactCount, gvel, sp.velocityIterations, warm);
+                VelocitySweeps<false>(vcs, isl.contactCount, jcs, isl.jointCount, gvel, sp.velocityIterations, warm,
+                                      sp.dt);
             }
         }
     }
@@ -136,6 +156,16 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
             for (u32 p = 0; p < c.manifold.pointCount; ++p) c.manifold.points[p].normalImpulse = vc.points[p].normalImpulse;
         }
     }
+    // the joints keep their impulses for the next step's warm start (revolute_joint.h members)
+    const uint4* ij = B.islandJoints + wi.jointBase;
+    for (u32 k = threadIdx.x; k < ws.islandJointCount; k += blockDim.x) {
+        Joint& j = B.joints[wi.jointBase + ij[k].x];
+        const JointCon& jc = B.jointCons[wi.jointBase + k];
+        j.motorImpulse = jc.motorImpulse;
+        j.limitImpulse = jc.limitImpulse;
+        j.linearImpulse = jc.linearImpulse;
+        j.angularImpulse = jc.angularImpulse;
+    }
     const u32 n = Min(ws.islandBodyCount, wi.islandBodyCap);
     const u32* ib = B.islandBodies + wi.islandBodyBase;
     for (u32 i = threadIdx.x; i < n; i += blockDim.x)
@@ -143,7 +173,8 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
                           B.bodies[wi.bodyBase + ib[i]], sp.dt);
 }
 
-__device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, PosBody* pos, u32 iterations) {
+__device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, const JointCon* jcs, u32 nj, PosBody* pos,
+                                               u32 iterations) {
     for (u32 it = 0; it < iterations; ++it) {
         scalar minSeparation = 0;
         PosCon next;
@@ -157,7 +188,13 @@ __device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, PosBod
             pos[pc.indexA] = A;
             pos[pc.indexB] = Bb;
         }
-        if (minSeparation >= scalar(-3) * kLinearSlop) return true;  // joints: none on this path
+        bool contactsSolved = minSeparation >= scalar(-3) * kLinearSlop;
+        bool jointsSolved = true;  // joint_solver.cpp:55-64
+        for (u32 k = 0; k < nj; ++k) {
+            bool solved = SolveJointPosition(jcs[k], pos);
+            jointsSolved = jointsSolved && solved;
+        }
+        if (contactsSolved && jointsSolved) return true;
     }
     return false;
 }
@@ -173,6 +210,7 @@ __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams s
         for (u32 k = first; k < last; ++k) {
             IslandInfo& isl = B.islands[wi.bodyBase + k];
             isl.positionsSolved = PositionSweeps(B.posCons + wi.contactBase + isl.contactStart, isl.contactCount,
+                                                 B.jointCons + wi.jointBase + isl.jointStart, isl.jointCount,
                                                  B.posBodies + wi.islandBodyBase + isl.bodyStart,
                                                  sp.positionIterations) ? 1u : 0u;
         }

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include "bx_collide.h"
 #include "bx_convex.h"
+#include "bx_joint.h"
 #include "bx_solver.h"
 
 namespace bx {

@@ -123,7 +123,8 @@ struct Contact {
 
 struct JointEdge { u32 joint, other; };
 
-// Revolute joint (revolute_joint.h / revolute_joint.cpp).
+// Revolute joint (revolute_joint.h / revolute_joint.cpp): definition and the warm-start state that
+// persists between steps.
 struct Joint {
     u32 bodyA, bodyB;
     u32 collideConnected, islandFlag;
@@ -131,13 +132,37 @@ struct Joint {
     Quat localRotationA, localRotationB, referenceRotation;
     u32 enableLimit, enableMotor;
     scalar lowerAngle, upperAngle, motorSpeed, maxMotorTorque;
-    // warm-start state
-    scalar motorImpulse, lowerImpulse, upperImpulse;
+    scalar motorImpulse, limitImpulse;
     Vec3 linearImpulse;
     Vec2 angularImpulse;
     u32 limitState;
-    scalar pad;
+    u32 pad;
 };
+
+// Solver constants of one island joint (b3RevoluteJoint's per-solve members), built in parallel
+// before the island sweeps; the sweeps update its impulses.
+struct JointCon {
+    u32 indexA, indexB;
+    u32 enableMotor, enableLimit, limitState;
+    scalar mA, mB;
+    Mat33 iA, iB;
+    Vec3 motorAxis;
+    scalar motorMass;
+    Vec3 rA, rB;
+    Mat33 linearMass;
+    Vec3 a1, a2;
+    Mat22 angularMass;
+    scalar motorSpeed, maxMotorTorque;
+    scalar motorImpulse, limitImpulse;
+    Vec3 linearImpulse;
+    Vec2 angularImpulse;
+    // position solver inputs
+    Vec3 localAnchorA, localAnchorB, localCenterA, localCenterB;
+    Mat33 localInvIA, localInvIB;
+    Quat localRotationA, localRotationB;
+    scalar lowerAngle, upperAngle;
+};
+
 
 struct WorldInfo {
     u32 bodyBase, bodyCount;
@@ -249,7 +274,8 @@ struct BatchView {
     IslandInfo* islands;      // per world: up to bodyCount islands (base = bodyBase)
     u32* islandBodies;        // per world: islandBodyCap
     uint4* islandContacts;    // per world: contactCap entries {slot, localA, localB, 0}
-    u32* islandJoints;        // per world: jointCount entries
+    uint4* islandJoints;      // per world: jointCount entries {joint, localA, localB, island bodyStart}
+    JointCon* jointCons;      // per world: jointCount entries, island order
     u32* dfsStack;            // per world: bodyCount
     VelBody* velBodies;       // per world: islandBodyCap
     PosBody* posBodies;       // per world: islandBodyCap

@@ -328,3 +328,58 @@ def mixed_drop(n_bodies=60, n_worlds=1, seed=4, kinds=("sphere", "capsule", "box
     for _ in range(n_worlds):
         s.worlds.append(mixed_drop_world(rng, n_bodies, g, b, kinds, extent, sensors))
     return s
+
+
+def chain_world(rng, links=10, ground_hull=0, link_hull=1, motor=True, limit=False, extra=0, origin=(0.0, 12.0, 0.0),
+                motor_speed=0.5 * np.pi, max_torque=50.0, lower=-0.25 * np.pi, upper=0.25 * np.pi):
+    """A chain of `links` boxes hanging from the static ground body by revolute joints about z,
+    built with b3RevoluteJointDef::Initialize (anchors between consecutive links), plus `extra`
+    dynamic boxes dropped next to it."""
+    nb = 1 + links + extra
+    bodies = body_array(nb)
+    fixtures = fixture_array(nb)
+    _ground(bodies, fixtures, ground_hull)
+    ox, oy, oz = origin
+    for i in range(links):
+        b = bodies[1 + i]
+        b["type"] = capi.BX_BODY_DYNAMIC
+        b["position"] = (ox + 1.0 * i + 0.5, oy, oz)
+        f = fixtures[1 + i]
+        f["body"] = 1 + i
+        f["shape_type"] = capi.BX_SHAPE_HULL
+        f["shape_index"] = link_hull
+        f["density"] = 1.0
+    for k in range(extra):
+        i = 1 + links + k
+        bodies[i]["type"] = capi.BX_BODY_DYNAMIC
+        bodies[i]["position"] = (ox + rng.uniform(-3, 3), oy - 6.0 + rng.uniform(-1, 1), oz + rng.uniform(-3, 3))
+        fixtures[i]["body"] = i
+        fixtures[i]["shape_type"] = capi.BX_SHAPE_HULL
+        fixtures[i]["shape_index"] = link_hull
+        fixtures[i]["density"] = 1.0
+    joints = joint_array(links)
+    for i in range(links):
+        j = joints[i]
+        j["body_a"] = 0 if i == 0 else i
+        j["body_b"] = 1 + i
+        j["use_initialize"] = 1
+        j["axis"] = (0.0, 0.0, 1.0)
+        j["anchor"] = (ox + 1.0 * i, oy, oz)
+        j["enable_motor"] = 1 if motor else 0
+        j["motor_speed"] = F32(motor_speed)
+        j["max_motor_torque"] = F32(max_torque)
+        j["enable_limit"] = 1 if limit else 0
+        j["lower_angle"] = F32(lower)
+        j["upper_angle"] = F32(upper)
+    return World(bodies, fixtures, joints)
+
+
+def chains(n_worlds=1, links=10, seed=6, motor=True, limit=False, extra=0):
+    """Revolute chains of boxes (half-extents 0.5 x 0.125 x 0.125) with motors, one per world."""
+    s = SceneSet("chains")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.125, 0.125))
+    rng = np.random.default_rng(seed)
+    for _ in range(n_worlds):
+        s.worlds.append(chain_world(rng, links, g, b, motor, limit, extra))
+    return s

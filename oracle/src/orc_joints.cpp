@@ -1,0 +1,404 @@
+// orc_joints.cpp -- oracle (test infrastructure): the revolute joint and the joint solver, restated.
+//   joints/revolute_joint.cpp:22-48     b3RevoluteJointDef::Initialize
+//   joints/revolute_joint.cpp:50-76     b3RevoluteJoint constructor (warm-start state)
+//   joints/revolute_joint.cpp:78-231    InitializeVelocityConstraints
+//   joints/revolute_joint.cpp:233-277   WarmStart
+//   joints/revolute_joint.cpp:279-367   SolveVelocityConstraints
+//   joints/revolute_joint.cpp:369-531   SolvePositionConstraints
+//   joints/joint_solver.cpp:31-64       the joint solver loops (island order)
+//   joint_manager.cpp:30-72             joint edges prepended to both bodies' joint lists
+#include <cmath>
+#include "orc_world.h"
+#include "../../include/bxstep.h"
+
+namespace orc {
+
+static void ComputeBasis(const Vec3& a, Vec3& b, Vec3& c) {  // mat33.h:310-328
+    static const scalar sqrt_inv3 = Sqrt(scalar(1) / scalar(3));
+    if (Abs(a.x) >= sqrt_inv3) b.Set(a.y, -a.x, scalar(0));
+    else b.Set(scalar(0), a.z, -a.y);
+    b.Normalize();
+    c = Cross(a, b);
+}
+
+static Quat Mat33Quat(const Mat33& m) {  // quat.h:283-345
+    scalar trace = m[0][0] + m[1][1] + m[2][2];
+    if (trace > scalar(0)) {
+        Quat r;
+        scalar s = Sqrt(trace + scalar(1));
+        r.s = scalar(0.5) * s;
+        scalar t = scalar(0.5) / s;
+        r.v.x = t * (m[1][2] - m[2][1]);
+        r.v.y = t * (m[2][0] - m[0][2]);
+        r.v.z = t * (m[0][1] - m[1][0]);
+        return r;
+    }
+    const u32 next[3] = {1, 2, 0};
+    u32 i = 0;
+    if (m[1][1] > m[0][0]) i = 1;
+    if (m[2][2] > m[i][i]) i = 2;
+    u32 j = next[i];
+    u32 k = next[j];
+    scalar s = Sqrt((m[i][i] - (m[j][j] + m[k][k])) + scalar(1));
+    scalar q[4];
+    q[i] = s * scalar(0.5);
+    scalar t = s != scalar(0) ? scalar(0.5) / s : s;
+    q[3] = t * (m[j][k] - m[k][j]);
+    q[j] = t * (m[i][j] + m[j][i]);
+    q[k] = t * (m[i][k] + m[k][i]);
+    Quat r;
+    r.v.x = q[0];
+    r.v.y = q[1];
+    r.v.z = q[2];
+    r.s = q[3];
+    return r;
+}
+
+static Vec3 V3(const float* p) { return Vec3(p[0], p[1], p[2]); }
+static Quat Q4(const float* p) { return Quat(Vec3(p[0], p[1], p[2]), p[3]); }
+
+void CreateJoint(World& w, const bx_joint_desc& d) {
+    Joint j;
+    j.type = d.type;
+    j.bodyA = d.body_a;
+    j.bodyB = d.body_b;
+    j.collideConnected = d.collide_connected != 0;
+    if (d.use_initialize) {  // revolute_joint.cpp:22-48
+        Mat33 rotation;
+        rotation.z = V3(d.axis);
+        ComputeBasis(rotation.z, rotation.x, rotation.y);
+        Quat q = Mat33Quat(rotation);
+        const Body& A = w.bodies[j.bodyA];
+        const Body& B = w.bodies[j.bodyB];
+        Vec3 anchor = V3(d.anchor);
+        j.localAnchorA = MulT(A.xf, anchor);
+        j.localRotationA = MulC(A.sweep.orientation, q);
+        j.localAnchorB = MulT(B.xf, anchor);
+        j.localRotationB = MulC(B.sweep.orientation, q);
+        j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
+    } else {
+        j.localAnchorA = V3(d.local_anchor_a);
+        j.localRotationA = Q4(d.local_rotation_a);
+        j.localAnchorB = V3(d.local_anchor_b);
+        j.localRotationB = Q4(d.local_rotation_b);
+        j.referenceRotation = Q4(d.reference_rotation);
+    }
+    j.lowerAngle = d.lower_angle;
+    j.upperAngle = d.upper_angle;
+    j.enableLimit = d.enable_limit != 0;
+    j.enableMotor = d.enable_motor != 0;
+    j.motorSpeed = d.motor_speed;
+    j.maxMotorTorque = d.max_motor_torque;
+    j.motorImpulse = 0;
+    j.limitState = e_inactiveLimit;
+    j.limitImpulse = 0;
+    j.linearImpulse.SetZero();
+    j.angularImpulse.SetZero();
+    u32 index = u32(w.joints.size());
+    w.joints.push_back(j);
+    Body& A = w.bodies[j.bodyA];
+    Body& B = w.bodies[j.bodyB];
+    A.joints.insert(A.joints.begin(), JointEdge{index, j.bodyB});
+    B.joints.insert(B.joints.begin(), JointEdge{index, j.bodyA});
+}
+
+static void InitJoint(World& w, const TimeStep& step, Joint& j, Position* pos) {  // revolute_joint.cpp:78-231
+    const Body& A = w.bodies[j.bodyA];
+    const Body& B = w.bodies[j.bodyB];
+    j.indexA = A.islandIndex;
+    j.indexB = B.islandIndex;
+    j.mA = A.invMass;
+    j.mB = B.invMass;
+    j.localCenterA = A.sweep.localCenter;
+    j.localCenterB = B.sweep.localCenter;
+    j.localInvIA = A.invI;
+    j.localInvIB = B.invI;
+    j.iA = pos[j.indexA].I;
+    j.iB = pos[j.indexB].I;
+    Quat qA = pos[j.indexA].q;
+    Quat qB = pos[j.indexB].q;
+    scalar mA = j.mA;
+    Mat33 iA = j.iA;
+    scalar mB = j.mB;
+    Mat33 iB = j.iB;
+    Quat fA = qA * j.localRotationA;
+    Quat fB = qB * j.localRotationB;
+    Mat33 RfA = fA.GetRotationMatrix();
+    Mat33 RfB = fB.GetRotationMatrix();
+    Vec3 u1 = RfA.x, v1 = RfA.y, w1 = RfA.z;
+    Vec3 u2 = RfB.x, v2 = RfB.y;
+    if (j.enableMotor || j.enableLimit) {
+        j.motorAxis = w1;
+        j.motorAxis.Normalize();
+        scalar K = Dot((iA + iB) * j.motorAxis, j.motorAxis);
+        j.motorMass = K > scalar(0) ? scalar(1) / K : scalar(0);
+    }
+    if (!j.enableMotor) j.motorImpulse = scalar(0);
+    if (j.enableLimit) {
+        scalar c = Dot(u2, u1);
+        scalar s = Dot(u2, v1);
+        scalar angle = scalar(::atan2(double(s), double(c)));  // the reference calls ::atan2(double, double)
+        if (Abs(j.upperAngle - j.lowerAngle) < scalar(2) * kAngularSlop) {
+            if (j.limitState != e_equalLimits) { j.limitState = e_equalLimits; j.limitImpulse = scalar(0); }
+        } else if (angle <= j.lowerAngle) {
+            if (j.limitState != e_atLowerLimit) { j.limitState = e_atLowerLimit; j.limitImpulse = scalar(0); }
+        } else if (angle >= j.upperAngle) {
+            if (j.limitState != e_atUpperLimit) { j.limitState = e_atUpperLimit; j.limitImpulse = scalar(0); }
+        } else {
+            j.limitState = e_inactiveLimit;
+            j.limitImpulse = scalar(0);
+        }
+    } else {
+        j.limitState = e_inactiveLimit;
+        j.limitImpulse = scalar(0);
+    }
+    {
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Mat33 RA = Skew(j.rA);
+        Mat33 RAT = Transpose(RA);
+        Mat33 RB = Skew(j.rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 M = Diagonal(mA + mB);
+        j.linearMass = M + RA * iA * RAT + RB * iB * RBT;
+    }
+    {
+        Vec3 a1 = Cross(u2, w1);
+        Vec3 a2 = Cross(v2, w1);
+        scalar k11 = Dot(a1, (iA + iB) * a1);
+        scalar k12 = Dot(a1, iA * a2) + Dot(a1, iB * a2);
+        scalar k22 = Dot(a2, (iA + iB) * a2);
+        Mat22 K;
+        K.x.x = k11;
+        K.x.y = k12;
+        K.y.x = k12;
+        K.y.y = k22;
+        j.a1 = a1;
+        j.a2 = a2;
+        j.angularMass = K;
+    }
+    if (step.warmStarting) {
+        j.motorImpulse *= step.dtRatio;
+        j.limitImpulse *= step.dtRatio;
+        j.linearImpulse *= step.dtRatio;
+        j.angularImpulse *= step.dtRatio;
+    } else {
+        j.motorImpulse = scalar(0);
+        j.limitImpulse = scalar(0);
+        j.linearImpulse.SetZero();
+        j.angularImpulse.SetZero();
+    }
+}
+
+static void WarmStartJoint(Joint& j, Velocity* vel) {  // revolute_joint.cpp:233-277
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w;
+    Vec3 vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    if (j.enableMotor && j.limitState != e_equalLimits) {
+        Vec3 P = j.motorAxis * j.motorImpulse;
+        wA -= j.iA * P;
+        wB += j.iB * P;
+    }
+    if (j.enableLimit && j.limitState != e_inactiveLimit) {
+        Vec3 P = j.motorAxis * j.limitImpulse;
+        wA -= j.iA * P;
+        wB += j.iB * P;
+    }
+    {
+        vA -= j.mA * j.linearImpulse;
+        wA -= j.iA * Cross(j.rA, j.linearImpulse);
+        vB += j.mB * j.linearImpulse;
+        wB += j.iB * Cross(j.rB, j.linearImpulse);
+    }
+    {
+        Vec3 L = j.angularImpulse.x * j.a1 + j.angularImpulse.y * j.a2;
+        wA -= j.iA * L;
+        wB += j.iB * L;
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+static void SolveJointVelocity(const TimeStep& step, Joint& j, Velocity* vel) {  // revolute_joint.cpp:279-367
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w;
+    Vec3 vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    Mat33 iA = j.iA, iB = j.iB;
+    if (j.enableMotor && j.limitState != e_equalLimits) {
+        scalar dw = Dot(wB - wA, j.motorAxis);
+        scalar Cdot = dw - j.motorSpeed;
+        scalar impulse = -j.motorMass * Cdot;
+        scalar oldImpulse = j.motorImpulse;
+        scalar maxImpulse = step.dt * j.maxMotorTorque;
+        j.motorImpulse = Clamp(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+        impulse = j.motorImpulse - oldImpulse;
+        Vec3 P = j.motorAxis * impulse;
+        wA -= iA * P;
+        wB += iB * P;
+    }
+    if (j.enableLimit && j.limitState != e_inactiveLimit) {
+        scalar Cdot = Dot(wB - wA, j.motorAxis);
+        scalar impulse = -j.motorMass * Cdot;
+        if (j.limitState == e_equalLimits) {
+            j.limitImpulse += impulse;
+        } else if (j.limitState == e_atLowerLimit) {
+            scalar oldImpulse = j.limitImpulse;
+            j.limitImpulse = Max(j.limitImpulse + impulse, scalar(0));
+            impulse = j.limitImpulse - oldImpulse;
+        } else if (j.limitState == e_atUpperLimit) {
+            scalar oldImpulse = j.limitImpulse;
+            j.limitImpulse = Min(j.limitImpulse + impulse, scalar(0));
+            impulse = j.limitImpulse - oldImpulse;
+        }
+        Vec3 P = impulse * j.motorAxis;
+        wA -= iA * P;
+        wB += iB * P;
+    }
+    {
+        Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+        Vec3 impulse = j.linearMass.Solve(-Cdot);
+        j.linearImpulse += impulse;
+        vA -= j.mA * impulse;
+        wA -= j.iA * Cross(j.rA, impulse);
+        vB += j.mB * impulse;
+        wB += j.iB * Cross(j.rB, impulse);
+    }
+    {
+        Vec3 dw = wB - wA;
+        Vec2 Cdot;
+        Cdot.x = Dot(dw, j.a1);
+        Cdot.y = Dot(dw, j.a2);
+        Vec2 impulse = Solve(j.angularMass, -Cdot);
+        j.angularImpulse += impulse;
+        Vec3 L = impulse.x * j.a1 + impulse.y * j.a2;
+        wA -= j.iA * L;
+        wB += j.iB * L;
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+static bool SolveJointPosition(Joint& j, Position* pos) {  // revolute_joint.cpp:369-531
+    Vec3 xA = pos[j.indexA].x;
+    Quat qA = pos[j.indexA].q;
+    Vec3 xB = pos[j.indexB].x;
+    Quat qB = pos[j.indexB].q;
+    Mat33 iA = pos[j.indexA].I;
+    Mat33 iB = pos[j.indexB].I;
+    scalar mA = j.mA, mB = j.mB;
+    scalar linearError = scalar(0);
+    {
+        Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Vec3 C = xB + rB - xA - rA;
+        linearError += Length(C);
+        Mat33 M = Diagonal(mA + mB);
+        Mat33 RA = Skew(rA);
+        Mat33 RAT = Transpose(RA);
+        Mat33 RB = Skew(rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 mass = M + RA * iA * RAT + RB * iB * RBT;
+        Vec3 impulse = mass.Solve(-C);
+        xA -= mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        qA.Normalize();
+        iA = RotateToFrame(j.localInvIA, qA);
+        xB += mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        qB.Normalize();
+        iB = RotateToFrame(j.localInvIB, qB);
+    }
+    scalar limitError = scalar(0);
+    if (j.enableLimit) {
+        Quat fA = qA * j.localRotationA;
+        Quat fB = qB * j.localRotationB;
+        Mat33 RfA = fA.GetRotationMatrix();
+        Mat33 RfB = fB.GetRotationMatrix();
+        Vec3 u1 = RfA.x, v1 = RfA.y, w1 = RfA.z;
+        Vec3 u2 = RfB.x;
+        Vec3 limitAxis = w1;
+        scalar K = Dot((iA + iB) * limitAxis, limitAxis);
+        scalar limitMass = K > scalar(0) ? scalar(1) / K : scalar(0);
+        scalar c = Dot(u2, u1);
+        scalar s = Dot(u2, v1);
+        scalar angle = scalar(::atan2(double(s), double(c)));  // the reference calls ::atan2(double, double)
+        scalar C = scalar(0);
+        if (Abs(j.upperAngle - j.lowerAngle) < scalar(2) * kAngularSlop)
+            C = Clamp(angle - j.lowerAngle, -kMaxAngularCorrection, kMaxAngularCorrection);
+        else if (angle <= j.lowerAngle)
+            C = Clamp(angle - j.lowerAngle + kAngularSlop, -kMaxAngularCorrection, scalar(0));
+        else if (angle >= j.upperAngle)
+            C = Clamp(angle - j.upperAngle - kAngularSlop, scalar(0), kMaxAngularCorrection);
+        limitError = Abs(C);
+        scalar limitImpulse = -C * limitMass;
+        Vec3 P = limitImpulse * limitAxis;
+        qA -= Derivative(qA, iA * P);
+        qA.Normalize();
+        iA = RotateToFrame(j.localInvIA, qA);
+        qB += Derivative(qB, iB * P);
+        qB.Normalize();
+        iB = RotateToFrame(j.localInvIB, qB);
+    }
+    scalar angularError = scalar(0);
+    const scalar kCosSlop = scalar(0.002);
+    {
+        Quat fA = qA * j.localRotationA;
+        Quat fB = qB * j.localRotationB;
+        Mat33 RfA = fA.GetRotationMatrix();
+        Mat33 RfB = fB.GetRotationMatrix();
+        Vec3 w1 = RfA.z;
+        Vec3 u2 = RfB.x, v2 = RfB.y;
+        Vec3 a1 = Cross(u2, w1);
+        Vec3 a2 = Cross(v2, w1);
+        scalar k11 = Dot(a1, (iA + iB) * a1);
+        scalar k12 = Dot(a1, iA * a2) + Dot(a1, iB * a2);
+        scalar k22 = Dot(a2, (iA + iB) * a2);
+        Mat22 K;
+        K.x.x = k11;
+        K.x.y = k12;
+        K.y.x = k12;
+        K.y.y = k22;
+        Vec2 C;
+        C.x = Dot(u2, w1);
+        C.y = Dot(v2, w1);
+        Vec2 impulse = Solve(K, -C);
+        Vec3 P = impulse.x * a1 + impulse.y * a2;
+        qA -= Derivative(qA, iA * P);
+        qA.Normalize();
+        iA = RotateToFrame(j.localInvIA, qA);
+        qB += Derivative(qB, iB * P);
+        qB.Normalize();
+        iB = RotateToFrame(j.localInvIB, qB);
+        angularError = Abs(C.x) + Abs(C.y);
+    }
+    pos[j.indexA].x = xA;
+    pos[j.indexA].q = qA;
+    pos[j.indexB].x = xB;
+    pos[j.indexB].q = qB;
+    pos[j.indexA].I = iA;
+    pos[j.indexB].I = iB;
+    return linearError <= kLinearSlop && limitError <= kAngularSlop && angularError <= kCosSlop;
+}
+
+// b3JointSolver (joint_solver.cpp:31-64): the island's joints in island order.
+void SolveJointsInit(World& w, const TimeStep& step, std::vector<u32>& joints, Position* pos, Velocity*) {
+    for (u32 i : joints) InitJoint(w, step, w.joints[i], pos);
+}
+void SolveJointsWarmStart(World& w, std::vector<u32>& joints, Velocity* vel) {
+    for (u32 i : joints) WarmStartJoint(w.joints[i], vel);
+}
+void SolveJointsVelocity(World& w, const TimeStep& step, std::vector<u32>& joints, Velocity* vel) {
+    for (u32 i : joints) SolveJointVelocity(step, w.joints[i], vel);
+}
+bool SolveJointsPosition(World& w, std::vector<u32>& joints, Position* pos) {
+    bool solved = true;
+    for (u32 i : joints) {
+        bool s = SolveJointPosition(w.joints[i], pos);
+        solved = solved && s;
+    }
+    return solved;
+}
+
+}  // namespace orc

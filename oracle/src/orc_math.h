@@ -132,6 +132,16 @@ struct Mat22 {
     void SetZero() { x.SetZero(); y.SetZero(); }
 };
 inline Vec2 operator*(const Mat22& A, const Vec2& v) { return v.x * A.x + v.y * A.y; }
+inline Vec2 Solve(const Mat22& A, const Vec2& b) {  // b3Mat22::Solve, Cramer (math.cpp:70-86)
+    scalar a11 = A.x.x, a12 = A.y.x;
+    scalar a21 = A.x.y, a22 = A.y.y;
+    scalar det = a11 * a22 - a12 * a21;
+    if (det != scalar(0)) det = scalar(1) / det;
+    Vec2 r;
+    r.x = det * (a22 * b.x - a12 * b.y);
+    r.y = det * (a11 * b.y - a21 * b.x);
+    return r;
+}
 inline Mat22 operator*(scalar s, const Mat22& A) { return Mat22(s * A.x, s * A.y); }
 inline Mat22 Adjugate(const Mat22& A) { return Mat22(Vec2(A.y.y, -A.x.y), Vec2(-A.y.x, A.x.x)); }
 inline Mat22 Inverse(const Mat22& A) {

@@ -12,10 +12,6 @@ namespace orc {
     std::abort();
 }
 
-void SolveJointsInit(World&, const TimeStep&, std::vector<u32>& j, Position*, Velocity*) { if (!j.empty()) Missing("joints"); }
-void SolveJointsWarmStart(World&, std::vector<u32>& j, Velocity*) { if (!j.empty()) Missing("joints"); }
-void SolveJointsVelocity(World&, const TimeStep&, std::vector<u32>& j, Velocity*) { if (!j.empty()) Missing("joints"); }
-bool SolveJointsPosition(World&, std::vector<u32>& j, Position*) { if (!j.empty()) Missing("joints"); return true; }
 void MeshContactFindPairs(World&, Contact*) { Missing("mesh contacts"); }
 void MeshContactSynchronize(World&, Contact*) { Missing("mesh contacts"); }
 void MeshContactCollide(World&, Contact*) { Missing("mesh contacts"); }
@@ -24,7 +20,6 @@ void MeshContactCollide(World&, Contact*) { Missing("mesh contacts"); }
 
 #include "../../include/bxstep.h"
 namespace orc {
-void CreateJoint(World&, const bx_joint_desc&) { Missing("joints"); }
 bool BuildMesh(Mesh&, const bx_mesh_desc&) { Missing("meshes"); }
 void InitTriangleShape(Shape&, const Vec3&, const Vec3&, const Vec3&) { Missing("triangle shapes"); }
 }  // namespace orc

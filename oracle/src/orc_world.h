@@ -73,33 +73,33 @@ struct Contact {
     std::vector<u32> triangles;
 };
 
-struct RevoluteState;
+// b3RevoluteJoint (revolute_joint.h/.cpp): definition, warm-start state and solver temporaries.
+enum LimitState : u32 { e_inactiveLimit = 0, e_atLowerLimit = 1, e_atUpperLimit = 2, e_equalLimits = 3 };
 struct Joint {
     u32 type = 0;
     u32 bodyA = 0, bodyB = 0;
     bool collideConnected = false;
     bool islandFlag = false;
-    // revolute joint definition and solver state (revolute_joint.h/.cpp)
     Vec3 localAnchorA, localAnchorB;
     Quat localRotationA, localRotationB, referenceRotation;
     bool enableLimit = false, enableMotor = false;
     scalar lowerAngle = 0, upperAngle = 0, motorSpeed = 0, maxMotorTorque = 0;
-    scalar motorImpulse = 0, lowerImpulse = 0, upperImpulse = 0;
+    // warm-start state
+    scalar motorImpulse = 0, limitImpulse = 0;
+    u32 limitState = e_inactiveLimit;
     Vec3 linearImpulse;
     Vec2 angularImpulse;
     // per-solve temporaries
     u32 indexA = 0, indexB = 0;
     scalar mA = 0, mB = 0;
-    Mat33 iA, iB, localInvIA, localInvIB;
     Vec3 localCenterA, localCenterB;
+    Mat33 localInvIA, localInvIB, iA, iB;
+    Vec3 motorAxis;
+    scalar motorMass = 0;
     Vec3 rA, rB;
     Mat33 linearMass;
-    Vec3 motorAxis;  // world hinge axis
-    scalar motorMass = 0, axialMass = 0;
-    Vec3 angularAxisA1, angularAxisA2;  // unused placeholders for alternative formulations
+    Vec3 a1, a2;
     Mat22 angularMass;
-    Vec3 u2x, u2y;   // hinge perpendicular axes for the 2-dof angular row
-    scalar angle = 0;
 };
 
 struct Position { Vec3 x; Quat q; Mat33 I; };

@@ -39,6 +39,9 @@ SCENES = {
     "mixed60_sensors": (lambda: scenes.mixed_drop(60, seed=1, sensors=0.3), 240, (240,)),
     "mixed150_dense_sleep": (lambda: scenes.with_flags(scenes.mixed_drop(150, seed=2, extent=(4.0, 20.0, 4.0)), 1),
                              300, (150, 300)),
+    "chain10_motor": (lambda: scenes.chains(1, 10), 240, (120, 240)),
+    "chain10_limit": (lambda: scenes.chains(1, 10, motor=False, limit=True), 240, (240,)),
+    "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300, (300,)),
 }
 
 

@@ -38,6 +38,9 @@ SCENES = {
     "mixed60": (lambda: scenes.mixed_drop(60, seed=0), 240),
     "mixed60_sensors": (lambda: scenes.mixed_drop(60, seed=1, sensors=0.3), 240),
     "spheres_capsules": (lambda: scenes.mixed_drop(80, seed=7, kinds=("sphere", "capsule")), 200),
+    "chain10_motor": (lambda: scenes.chains(1, 10), 240),
+    "chain10_free": (lambda: scenes.chains(1, 10, motor=False), 150),
+    "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300),
 }
 
 
