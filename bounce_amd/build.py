@@ -37,6 +37,9 @@ def command(out=LIB, extra=()):
 
 
 CAL_LIB = os.path.join(LIB_DIR, "libbxcal.so")
+FACADE_LIB = os.path.join(LIB_DIR, "libbounce_amd.so")
+HELLO_BIN = os.path.join(LIB_DIR, "bx_hello_world")
+LIB_DIR_RPATH = "$ORIGIN"
 
 
 def build(verbose=True):
@@ -45,6 +48,13 @@ def build(verbose=True):
             # measurement support: plain copy kernel for calibrating the profiler's byte counters
             (CAL_LIB, [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared"] + FP_FLAGS
              + ["-o", CAL_LIB + ".tmp", os.path.join(CSRC, "bx_calibrate.hip")])]
+    # the drop-in C++ API (bounce/bounce.h) over the C ABI, and a program built against it
+    host = ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+            "-I", os.path.join(PKG, "include"), "-I", os.path.join(REPO, "include")]
+    jobs.append((FACADE_LIB, host + ["-shared", "-o", FACADE_LIB + ".tmp", os.path.join(PKG, "facade", "world.cpp"),
+                                     "-L", LIB_DIR, "-lbxstep", "-Wl,-rpath,$ORIGIN"]))
+    jobs.append((HELLO_BIN, host + ["-o", HELLO_BIN + ".tmp", os.path.join(REPO, "tests", "drop_in", "hello_world.cpp"),
+                                    "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
     for out, cmd in jobs:
         if verbose:
             print(" ".join(cmd), flush=True)

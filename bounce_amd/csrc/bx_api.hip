@@ -364,6 +364,7 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
         wi.fixtureBase = u32(fixtures.size());
         wi.fixtureCount = d.fixture_count;
         wi.fixtureListBase = u32(fixtureList.size());
+        wi.edgeBase = wi.fixtureBase + w;
         wi.jointBase = u32(joints.size());
         wi.jointCount = d.joint_count;
         wi.jointEdgeBase = u32(jointEdges.size());

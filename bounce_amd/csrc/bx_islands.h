@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
     }
     __syncthreads();
     // the pair search of this step reuses the adjacency
-    for (u32 f = threadIdx.x; f <= nf; f += blockDim.x) B.edgeOffset[wi.fixtureBase + w + f] = off[f];
+    for (u32 f = threadIdx.x; f <= nf; f += blockDim.x) B.edgeOffset[wi.edgeBase + f] = off[f];
     for (u32 e = threadIdx.x; e < 2 * n; e += blockDim.x) B.edgeSlots[2 * wi.contactBase + e] = edges[e];
 
     // 3. depth-first search (one lane)

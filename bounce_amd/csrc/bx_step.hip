@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
     const WorldInfo wi = B.worlds[w];
     WorldState& ws = B.wstate[w];
     if (rebuildAdjacency) {
-        BuildAdjacency(B, wi, ws, B.edgeOffset + wi.fixtureBase + w, B.pairCounts + wi.fixtureBase + w);
+        BuildAdjacency(B, wi, ws, B.edgeOffset + wi.edgeBase, B.pairCounts + wi.edgeBase);
     }
     const u32 nf = wi.fixtureCount;
     const u32* porder = B.proxyOrder + wi.fixtureBase;
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
     }
     for (u32 f = threadIdx.x; f < nf; f += blockDim.x) B.moved[wi.fixtureBase + f] = 0u;
     // b3ContactManager::AddPair filters (contact_manager.cpp:38-103), then creation in pair order.
-    const u32* off = B.edgeOffset + wi.fixtureBase + w;
+    const u32* off = B.edgeOffset + wi.edgeBase;
     const u32* edges = B.edgeSlots + 2 * wi.contactBase;
     const u32* order = B.contactOrder + wi.contactBase;
     u32 created = 0;

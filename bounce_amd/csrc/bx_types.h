@@ -178,6 +178,7 @@ struct WorldInfo {
     Vec3 gravity;
     u32 islandScratchBase;             // words in BatchView::islandScratch, kNull when LDS suffices
     u32 islandScratchWords;            // words the island builder needs for this world
+    u32 edgeBase;                      // fixtureBase + world index: CSR offsets (fixtureCount + 1 entries)
 };
 
 // Per-world mutable counters.
@@ -265,7 +266,7 @@ struct BatchView {
     u32* freeSlots;           // per world: free slot stack
     u32* contactAlive;        // per slot: scratch flag for the update pass
     // adjacency (rebuilt each step)
-    u32* edgeOffset;          // per fixture + 1 per world (fixtureCount + 1 entries, base fixtureBase + world)
+    u32* edgeOffset;          // per fixture + 1 per world (fixtureCount + 1 entries from WorldInfo::edgeBase)
     u32* edgeSlots;           // per world: 2 * contactCap
     // pairs
     uint2* pairs;             // per world: pairCap

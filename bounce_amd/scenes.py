@@ -383,3 +383,81 @@ def chains(n_worlds=1, links=10, seed=6, motor=True, limit=False, extra=0):
     for _ in range(n_worlds):
         s.worlds.append(chain_world(rng, links, g, b, motor, limit, extra))
     return s
+
+
+def mixed_world(rng, n_bodies, ground_hull, link_hull, hull_indices, links=10, sleeping=False):
+    """One world of the mixed-shapes workload: a static ground box; `n_bodies` dynamic bodies,
+    each with equal probability a sphere (r ~ U[0.2, 0.6]), a capsule (segment length ~ U[0.5, 2]
+    along local x, radius 0.25) or a hull drawn from `hull_indices`, at seeded uniform positions
+    and orientations in a drop box of side L = 1.5 n^(1/3) m (height 2L); and a `links`-link
+    revolute chain of boxes with motors (speed pi/2 rad/s, max torque 50 N m, no limits) hanging
+    from the ground body beside the drop box."""
+    nb = 1 + n_bodies + links
+    bodies = body_array(nb)
+    fixtures = fixture_array(nb)
+    _ground(bodies, fixtures, ground_hull)
+    L = 1.5 * n_bodies ** (1.0 / 3.0)
+    sl = slice(1, 1 + n_bodies)
+    bodies["type"][sl] = capi.BX_BODY_DYNAMIC
+    bodies["position"][sl, 0] = rng.uniform(-L / 2, L / 2, n_bodies)
+    bodies["position"][sl, 1] = rng.uniform(1.0, 1.0 + 2 * L, n_bodies)
+    bodies["position"][sl, 2] = rng.uniform(-L / 2, L / 2, n_bodies)
+    bodies["orientation"][sl] = random_orientations(rng, n_bodies)
+    fixtures["body"][sl] = np.arange(1, 1 + n_bodies)
+    fixtures["density"][sl] = 1.0
+    kind = rng.integers(0, 3, n_bodies)
+    radius = rng.uniform(0.2, 0.6, n_bodies).astype(F32)
+    half = (0.5 * rng.uniform(0.5, 2.0, n_bodies)).astype(F32)
+    hull = np.asarray(hull_indices, dtype=np.uint32)[rng.integers(0, len(hull_indices), n_bodies)]
+    f = fixtures[sl]
+    sph, cap, hul = kind == 0, kind == 1, kind == 2
+    f["shape_type"][sph] = capi.BX_SHAPE_SPHERE
+    f["radius"][sph] = radius[sph]
+    f["shape_type"][cap] = capi.BX_SHAPE_CAPSULE
+    f["radius"][cap] = F32(0.25)
+    f["vertex1"][cap, 0] = -half[cap]
+    f["vertex2"][cap, 0] = half[cap]
+    f["shape_type"][hul] = capi.BX_SHAPE_HULL
+    f["shape_index"][hul] = hull[hul]
+    fixtures[sl] = f
+    # the chain
+    ox, oy = L / 2 + 2.0, 2 * L + 2.0
+    cl = slice(1 + n_bodies, nb)
+    bodies["type"][cl] = capi.BX_BODY_DYNAMIC
+    bodies["position"][cl, 0] = ox + np.arange(links) + 0.5
+    bodies["position"][cl, 1] = oy
+    fixtures["body"][cl] = np.arange(1 + n_bodies, nb)
+    fixtures["shape_type"][cl] = capi.BX_SHAPE_HULL
+    fixtures["shape_index"][cl] = link_hull
+    fixtures["density"][cl] = 1.0
+    joints = joint_array(links)
+    joints["body_a"][0] = 0
+    joints["body_a"][1:] = np.arange(1 + n_bodies, nb - 1)
+    joints["body_b"] = np.arange(1 + n_bodies, nb)
+    joints["use_initialize"] = 1
+    joints["axis"] = (0.0, 0.0, 1.0)
+    joints["anchor"][:, 0] = ox + np.arange(links)
+    joints["anchor"][:, 1] = oy
+    joints["enable_motor"] = 1
+    joints["motor_speed"] = F32(0.5 * np.pi)
+    joints["max_motor_torque"] = F32(50.0)
+    return World(bodies, fixtures, joints, flags=capi.BX_WORLD_SLEEPING if sleeping else 0)
+
+
+def mixed_worlds(n_worlds=16384, seed=8, median=200, sigma=0.8, clip=(20, 2000), links=10, hull_count=1024,
+                 sleeping=False, sizes=None):
+    """The batched mixed-shapes workload: per-world body counts round(LogNormal(ln median, sigma))
+    clipped to `clip` (or the given `sizes`), shapes from a seeded library of `hull_count` hulls of
+    8-16 points in the unit ball, one motor chain per world."""
+    s = SceneSet("mixed_worlds")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    link = s.add_hull(hull_lib.box_hull(0.5, 0.125, 0.125))
+    base = len(s.hulls)
+    s.hulls.extend(hull_lib.random_hull_library(hull_count, seed=seed + 1))
+    hull_indices = np.arange(base, base + hull_count)
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = np.clip(np.round(rng.lognormal(np.log(median), sigma, n_worlds)), clip[0], clip[1]).astype(int)
+    for n in sizes:
+        s.worlds.append(mixed_world(rng, int(n), g, link, hull_indices, links, sleeping))
+    return s

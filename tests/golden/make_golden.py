@@ -42,6 +42,7 @@ SCENES = {
     "chain10_motor": (lambda: scenes.chains(1, 10), 240, (120, 240)),
     "chain10_limit": (lambda: scenes.chains(1, 10, motor=False, limit=True), 240, (240,)),
     "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300, (300,)),
+    "mixed_world200": (lambda: scenes.mixed_worlds(1, sizes=[200], hull_count=64), 180, (90, 180)),
 }
 
 

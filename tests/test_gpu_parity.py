@@ -112,3 +112,24 @@ def test_batched_mixed_shapes_match_single(gpu_ctx):
         diffs = compare_worlds(DeviceWorld(batch, w), r)
         assert not diffs, f"world {w}: {diffs}"
         assert batch.counts(w).status == 0
+
+
+def test_batched_mixed_worlds_match_single(gpu_ctx):
+    """Twenty-four worlds of the mixed-shapes workload (spheres, capsules, random hulls, a motor
+    chain each; lognormal sizes capped at 400 bodies) in one launch equal the oracle world by world."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    rng = np.random.default_rng(21)
+    sizes = np.clip(np.round(rng.lognormal(np.log(200), 0.8, 24)), 20, 400).astype(int)
+    scene = scenes.mixed_worlds(24, sizes=sizes, hull_count=128)
+    print(f"[parity] batched mixed worlds x24 (sizes {sizes.min()}..{sizes.max()}): 90 steps", flush=True)
+    batch = device.Batch(gpu_ctx, scene)
+    refs = [CpuWorld("oracle", scene, w) for w in range(24)]
+    for _ in range(90):
+        batch.step()
+        for r in refs:
+            r.step()
+    for w, r in enumerate(refs):
+        diffs = compare_worlds(DeviceWorld(batch, w), r)
+        assert not diffs, f"world {w}: {diffs}"
+        assert batch.counts(w).status == 0
