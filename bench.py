@@ -11,8 +11,9 @@ is one b3World::Step of every world of the batch; inputs are resident in HBM bef
 Printed (one JSON line): body-steps/s of the batched path (value), the plain per-world GPU path
 (one world per launch sequence, measured on a sample of worlds), their ratio, p50/p99 step
 latency, the roofline of the dominant kernel (the island solver) from HIP events on the
-library's stream, and the reference CPU Step timed on this host's cores (one world per thread)
-on a bounded sample of the same worlds.
+library's stream, the reference CPU Step timed on this host's cores (one world per thread)
+on a bounded sample of the same worlds, and the serving configuration at one arrival rate
+(request and tick latency p50/p99, body-steps/s; bounce_amd/serve_bench.py sweeps rates).
 """
 import argparse
 import json
@@ -54,6 +55,10 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=60)
     p.add_argument("--latency-steps", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--serve-worlds", type=int, default=512)
+    p.add_argument("--serve-rate", type=float, default=1000.0)
+    p.add_argument("--serve-seconds", type=float, default=5.0)
+    p.add_argument("--serve-budget-gb", type=float, default=1.0)
     return p.parse_args()
 
 
@@ -71,6 +76,28 @@ def cpu_baseline(args, scene_factory, body_count_per_world):
     value = args.cpu_worlds * body_count_per_world * args.cpu_steps / secs if secs > 0 else None
     return {"value": value, "unit": "body-steps/s", "cores": threads, "kind": kind,
             "sample": f"{args.cpu_worlds} worlds x {args.cpu_steps} steps from step 0, one world per thread"}
+
+
+def serve(args):
+    """The serving configuration at one arrival rate, in a child process of its own (its JSON line
+    is copied into this benchmark's line)."""
+    import subprocess
+    cmd = [sys.executable, "-m", "bounce_amd.serve_bench", "--worlds", str(args.serve_worlds),
+           "--budget-gb", str(args.serve_budget_gb), "--rates", str(args.serve_rate),
+           "--seconds", str(args.serve_seconds)]
+    try:
+        r = subprocess.run(cmd, cwd=REPO, capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out"}
+    sys.stderr.write(r.stderr[-2000:])
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}"}
+    rep = json.loads(lines[-1])
+    rep["workload"] = (f"Poisson requests at {args.serve_rate}/s for {args.serve_seconds} s, each 1-16 steps of "
+                       f"one of {args.serve_worlds} mixed-shape worlds (50-5000 bodies), "
+                       f"pool budget {args.serve_budget_gb} GiB")
+    return rep
 
 
 def main():
@@ -136,15 +163,15 @@ def main():
     if args.plain_worlds > 0:
         plain_scene = factory(args.plain_worlds)
         pb = device.Batch(ctx, plain_scene)
-        worlds = np.arange(args.plain_worlds, dtype=np.uint32)
-        ones = np.ones(args.plain_worlds, dtype=np.uint32)
         for _ in range(2):
-            pb.step_subset(worlds, ones)
+            for w in range(args.plain_worlds):
+                pb.step_list([w])
         pb.sync()
         p0 = time.perf_counter()
         plain_steps = 5
         for _ in range(plain_steps):
-            pb.step_subset(worlds, ones)
+            for w in range(args.plain_worlds):
+                pb.step_list([w])
         pb.sync()
         p1 = time.perf_counter()
         plain = {"value": plain_scene.body_count * plain_steps / (p1 - p0), "unit": "body-steps/s",
@@ -152,6 +179,7 @@ def main():
         pb.close()
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline(args, factory, bodies_per_world)
+    serving = None if args.serve_seconds <= 0 else serve(args)
     out = {
         "metric": "body-steps/sec on 1 MI355X, batched vs plain per-world; p99 step latency",
         "value": value,
@@ -179,6 +207,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": dom_bytes, "kernel_ms": kernel_ms[dominant]},
         "cpu_baseline": cpu,
+        "serving": serving,
         "device_bytes": device_bytes,
         "status": status,
     }

@@ -117,9 +117,19 @@ class bx_batch_opts(C.Structure):
                 ("reserved", _arr(u32, 5))]
 
 
+class bx_pool_opts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_worlds", u32), ("max_contacts_per_world", u32),
+                ("max_pairs_per_world", u32), ("joints_per_1000_bodies", u32), ("reserved", _arr(u32, 4))]
+
+
+class bx_pool_stats(C.Structure):
+    _fields_ = [("capacity_bytes", C.c_uint64), ("used_bytes", C.c_uint64), ("resident_worlds", u32),
+                ("free_slots", u32), ("world_slots", u32), ("reserved", _arr(u32, 5))]
+
+
 STRUCTS = [bx_hull_desc, bx_mesh_desc, bx_shape_library, bx_body_desc, bx_fixture_desc, bx_joint_desc,
            bx_world_desc, bx_body_state, bx_manifold_point_state, bx_manifold_state, bx_contact_state,
-           bx_pair, bx_world_counts, bx_step_cfg, bx_batch_opts]
+           bx_pair, bx_world_counts, bx_step_cfg, bx_batch_opts, bx_pool_opts, bx_pool_stats]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)

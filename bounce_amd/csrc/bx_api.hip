@@ -9,7 +9,9 @@
 // takes one more for the new parent; moving a proxy frees and retakes the same parent, so ids
 // are fixed at creation (dynamic_tree.cpp:44-106).
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
+#include <map>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,11 +21,14 @@
 
 namespace bx {
 struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIterations; };
-void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const StepParams& sp, bool findFirst,
-                u32 maxIslands, hipStream_t stream, hipEvent_t* ev);
+void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
+                bool findFirst, u32 maxIslands, hipStream_t stream, hipEvent_t* ev);
+void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
 }
 
 using namespace bx;
+typedef uint8_t u8;
+typedef uint64_t u64;
 
 static thread_local std::string g_error;
 
@@ -59,24 +64,166 @@ static int EnsureEvents(bx_context* ctx, u32 steps) {
     return BX_OK;
 }
 
-template <class T>
-struct DevArray {
-    T* ptr = nullptr;
-    size_t count = 0;
+// The per-world arrays of a batch live in seven index spaces. A world owns one contiguous range of
+// units in each space, and every array of a space is indexed from the world's base in that space,
+// so a world's whole state is relocatable: every index stored inside the arrays is world-local.
+enum Space : u32 {
+    kSpaceBody,        // bodyBase: bodies, islands, DFS stack
+    kSpaceFixture,     // fixtureBase (= fixtureListBase = edgeBase): fixtureCount + 1 units
+    kSpaceJoint,       // jointBase (jointEdgeBase = 2 x jointBase)
+    kSpaceContact,     // contactBase: contactCap units
+    kSpacePair,        // pairBase: pairCap units
+    kSpaceIslandBody,  // islandBodyBase: islandBodyCap units
+    kSpaceScratch,     // islandScratchBase: island-builder words at full capacity
+    kSpaces
 };
+
+static void WorldUnits(const WorldInfo& wi, u32 u[kSpaces]) {
+    u[kSpaceBody] = wi.bodyCount;
+    u[kSpaceFixture] = wi.fixtureCount + 1;  // the CSR offsets hold fixtureCount + 1 entries
+    u[kSpaceJoint] = wi.jointCount;
+    u[kSpaceContact] = wi.contactCap;
+    u[kSpacePair] = wi.pairCap;
+    u[kSpaceIslandBody] = wi.islandBodyCap;
+    u[kSpaceScratch] = wi.islandScratchWords;
+}
+
+static void SetBases(WorldInfo& wi, const u32 base[kSpaces]) {
+    wi.bodyBase = base[kSpaceBody];
+    wi.fixtureBase = wi.fixtureListBase = wi.edgeBase = base[kSpaceFixture];
+    wi.jointBase = base[kSpaceJoint];
+    wi.jointEdgeBase = 2 * base[kSpaceJoint];
+    wi.contactBase = base[kSpaceContact];
+    wi.pairBase = base[kSpacePair];
+    wi.islandBodyBase = base[kSpaceIslandBody];
+    wi.islandScratchBase = base[kSpaceScratch];
+}
+
+static void GetBases(const WorldInfo& wi, u32 base[kSpaces]) {
+    base[kSpaceBody] = wi.bodyBase;
+    base[kSpaceFixture] = wi.fixtureBase;
+    base[kSpaceJoint] = wi.jointBase;
+    base[kSpaceContact] = wi.contactBase;
+    base[kSpacePair] = wi.pairBase;
+    base[kSpaceIslandBody] = wi.islandBodyBase;
+    base[kSpaceScratch] = wi.islandScratchBase;
+}
+
+// Every per-world device array: its BatchView member, element size, space and elements per unit.
+struct ArraySpec {
+    size_t member;
+    u32 elemSize, space, perUnit;
+};
+#define BX_ARRAY(m, space, per) {offsetof(BatchView, m), u32(sizeof(*static_cast<BatchView*>(nullptr)->m)), space, per}
+static const ArraySpec kArrays[] = {
+    BX_ARRAY(bodies, kSpaceBody, 1), BX_ARRAY(islands, kSpaceBody, 1), BX_ARRAY(dfsStack, kSpaceBody, 1),
+    BX_ARRAY(fixtures, kSpaceFixture, 1), BX_ARRAY(fixtureList, kSpaceFixture, 1),
+    BX_ARRAY(proxyOrder, kSpaceFixture, 1), BX_ARRAY(fatAABB, kSpaceFixture, 1), BX_ARRAY(moved, kSpaceFixture, 1),
+    BX_ARRAY(edgeOffset, kSpaceFixture, 1), BX_ARRAY(pairCounts, kSpaceFixture, 1),
+    BX_ARRAY(joints, kSpaceJoint, 1), BX_ARRAY(islandJoints, kSpaceJoint, 1), BX_ARRAY(jointCons, kSpaceJoint, 1),
+    BX_ARRAY(jointEdges, kSpaceJoint, 2),
+    BX_ARRAY(contacts, kSpaceContact, 1), BX_ARRAY(contactOrder, kSpaceContact, 1),
+    BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1),
+    BX_ARRAY(islandContacts, kSpaceContact, 1), BX_ARRAY(velCons, kSpaceContact, 1),
+    BX_ARRAY(posCons, kSpaceContact, 1), BX_ARRAY(edgeSlots, kSpaceContact, 2),
+    BX_ARRAY(pairs, kSpacePair, 1),
+    BX_ARRAY(islandBodies, kSpaceIslandBody, 1), BX_ARRAY(velBodies, kSpaceIslandBody, 1),
+    BX_ARRAY(posBodies, kSpaceIslandBody, 1),
+    BX_ARRAY(islandScratch, kSpaceScratch, 1),
+};
+#undef BX_ARRAY
+static const u32 kArrayCount = u32(sizeof(kArrays) / sizeof(kArrays[0]));
+
+static char* ArrayPtr(const BatchView& V, const ArraySpec& a) {
+    char* p;
+    std::memcpy(&p, reinterpret_cast<const char*>(&V) + a.member, sizeof(p));
+    return p;
+}
+static void SetArrayPtr(BatchView& V, const ArraySpec& a, void* p) {
+    std::memcpy(reinterpret_cast<char*>(&V) + a.member, &p, sizeof(p));
+}
+
+// Device bytes one unit of each space costs (the island work list is one entry per body unit).
+static void UnitBytes(u64 out[kSpaces]) {
+    for (u32 s = 0; s < kSpaces; ++s) out[s] = 0;
+    for (const ArraySpec& a : kArrays) out[a.space] += u64(a.elemSize) * a.perUnit;
+    out[kSpaceBody] += sizeof(uint2);
+}
+
+// First-fit allocator of unit ranges in one space, coalescing on free.
+struct RangeAllocator {
+    u64 capacity = 0;
+    std::map<u64, u64> free;  // start -> length
+    void Reset(u64 cap) {
+        capacity = cap;
+        free.clear();
+        if (cap) free[0] = cap;
+    }
+    bool Alloc(u64 n, u64& start) {
+        if (n == 0) { start = 0; return true; }
+        for (auto it = free.begin(); it != free.end(); ++it) {
+            if (it->second < n) continue;
+            start = it->first;
+            u64 rest = it->second - n;
+            free.erase(it);
+            if (rest) free[start + n] = rest;
+            return true;
+        }
+        return false;
+    }
+    void Free(u64 start, u64 n) {
+        if (n == 0) return;
+        auto next = free.lower_bound(start);
+        if (next != free.end() && start + n == next->first) {
+            n += next->second;
+            next = free.erase(next);
+        }
+        if (next != free.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                prev->second += n;
+                return;
+            }
+        }
+        free[start] = n;
+    }
+    u64 FreeUnits() const {
+        u64 t = 0;
+        for (auto& r : free) t += r.second;
+        return t;
+    }
+};
+
+// Device-side world lists of list launches: a ring of regions reused once the launches that read
+// them have finished (one event per region).
+static const u32 kListRing = 4;
 
 struct bx_batch {
     bx_context* ctx = nullptr;
-    u32 worldCount = 0;
+    u32 worldCount = 0;  // world slots
     BatchView view{};
     std::vector<void*> allocations;
     std::vector<WorldInfo> worlds;
     std::vector<std::vector<u32>> proxyIds;  // per world: proxy id of each fixture
-    std::vector<u32> hullRadius;
-    bool newContacts = true;
-    float timings[5] = {};
+    std::vector<u8> resident;                // per slot: holds a world
+    std::vector<u8> needsInitialPairs;       // per slot: creation-time pair search still to run
+    bool newContacts = true;                 // some resident world still needs its initial pair search
     size_t deviceBytes = 0;
+    // pools
+    bool pool = false;
+    RangeAllocator spaces[kSpaces];
+    std::vector<u32> freeWorldSlots;
+    bx_batch_opts worldOpts{};
+    struct HostLibrary* library = nullptr;
+    // world lists
+    u32* listHost = nullptr;  // pinned, kListRing regions of 2 x worldCount
+    u32* listDev = nullptr;
+    hipEvent_t listEv[kListRing] = {};
+    bool listUsed[kListRing] = {};
+    u32 listNext = 0;
 };
+
+static bool IsResident(const bx_batch* b, u32 world) { return world < b->worldCount && b->resident[world]; }
 
 template <class T>
 static int DevAlloc(bx_batch* b, T** out, size_t count, const T* init = nullptr) {
@@ -265,31 +412,447 @@ static int Download(bx_batch* b, std::vector<T>& out, const T* src, size_t count
 }
 
 static int GetState(bx_batch* b, u32 world, WorldState& ws) {
-    if (!b || world >= b->worldCount) return Fail(BX_E_INVALID, "bad batch or world");
+    if (!b || !IsResident(b, world)) return Fail(BX_E_INVALID, "bad batch or world not resident");
     HIP_CHECK(hipSetDevice(b->ctx->device));
     HIP_CHECK(hipMemcpyAsync(&ws, b->view.wstate + world, sizeof(ws), hipMemcpyDeviceToHost, b->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
     return BX_OK;
 }
 
-static int StepRange(bx_batch* b, const bx_step_cfg* cfg, u32 first, u32 count, u32 steps, bool timed) {
+// One world built on the host from its descriptor. Every index in these arrays is world-local;
+// `info` holds sizes, capacities and flags (its bases are set when the world is placed).
+struct HostWorld {
+    WorldInfo info;
+    WorldState state;
+    std::vector<Body> bodies;
+    std::vector<Fixture> fixtures;
+    std::vector<u32> fixtureList, proxyOrder, proxyIds, freeSlots;
+    std::vector<AABB> fat;
+    std::vector<Joint> joints;
+    std::vector<JointEdge> jointEdges;
+};
+
+static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_batch_opts* opts, HostWorld& hw) {
+    WorldInfo& wi = hw.info;
+    std::memset(&wi, 0, sizeof(wi));
+    wi.bodyCount = d.body_count;
+    wi.fixtureCount = d.fixture_count;
+    wi.jointCount = d.joint_count;
+    wi.flags = d.flags;
+    wi.gravity = V3(d.gravity);
+    wi.serialIslands = 0;
+    if ((d.body_count && !d.bodies) || (d.fixture_count && !d.fixtures) || (d.joint_count && !d.joints))
+        return Fail(BX_E_INVALID, "null descriptor array");
+    if (d.fixture_count > 0xFFFFu || d.body_count > 0xFFFFFFu || d.joint_count > 0xFFFFu)
+        return Fail(BX_E_INVALID, "world too large");
+    // bodies (b3Body::b3Body, body.cpp:25-103)
+    std::vector<Body>& bodies = hw.bodies;
+    bodies.reserve(d.body_count);
+    for (u32 i = 0; i < d.body_count; ++i) {
+        const bx_body_desc& bd = d.bodies[i];
+        if (bd.type > 2) return Fail(BX_E_INVALID, "bad body type");
+        Body x;
+        std::memset(&x, 0, sizeof(x));
+        x.type = bd.type;
+        x.flags = 0;
+        if (bd.flags & BX_BODY_AWAKE) x.flags |= kAwake;
+        if (bd.flags & BX_BODY_AUTO_SLEEP) x.flags |= kAutoSleep;
+        if (x.type == kDynamic) {
+            x.mass = 1;
+            x.invMass = 1;
+            x.flags |= bd.flags & (kFixedX | kFixedY | kFixedZ);
+        }
+        x.v = V3(bd.linear_velocity);
+        x.w = V3(bd.angular_velocity);
+        x.localCenter.SetZero();
+        x.x = V3(bd.position);
+        x.q = Q4(bd.orientation);
+        x.x0 = x.x;
+        x.q0 = x.q;
+        x.xfT = x.x;
+        x.linearDamping = V3(bd.linear_damping);
+        x.angularDamping = V3(bd.angular_damping);
+        x.gravityScale = V3(bd.gravity_scale);
+        x.linTol = bd.linear_sleep_tolerance;
+        x.angTol = bd.angular_sleep_tolerance;
+        x.islandIndex = 0;
+        bodies.push_back(x);
+    }
+    // fixtures, grouped per body in list order (newest first)
+    std::vector<Fixture>& fixtures = hw.fixtures;
+    std::vector<std::vector<u32>> perBody(d.body_count);
+    for (u32 i = 0; i < d.fixture_count; ++i) {
+        const bx_fixture_desc& fd = d.fixtures[i];
+        if (fd.body >= d.body_count) return Fail(BX_E_INVALID, "fixture body out of range");
+        Fixture f;
+        std::memset(&f, 0, sizeof(f));
+        f.body = fd.body;
+        f.shapeType = fd.shape_type;
+        f.shapeIndex = fd.shape_index;
+        f.isSensor = fd.is_sensor ? 1u : 0u;
+        f.density = fd.density;
+        f.friction = fd.friction;
+        f.restitution = fd.restitution;
+        f.v1 = V3(fd.vertex1);
+        f.v2 = V3(fd.vertex2);
+        f.v3 = V3(fd.vertex3);
+        switch (fd.shape_type) {
+        case BX_SHAPE_HULL:
+            if (fd.shape_index >= L.info.size()) return Fail(BX_E_INVALID, "hull index out of range");
+            f.radius = kHullRadius;
+            break;
+        case BX_SHAPE_SPHERE:
+        case BX_SHAPE_CAPSULE:
+            f.radius = fd.radius;
+            break;
+        default:
+            return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");
+        }
+        fixtures.push_back(f);
+        perBody[fd.body].insert(perBody[fd.body].begin(), i);
+    }
+    u32 cursor = 0;
+    for (u32 i = 0; i < d.body_count; ++i) {
+        Body& x = bodies[i];
+        x.fixtureBegin = cursor;
+        x.fixtureCount = u32(perBody[i].size());
+        for (u32 f : perBody[i]) hw.fixtureList.push_back(f);
+        cursor += x.fixtureCount;
+    }
+    // proxies, fat AABBs and mass, in fixture creation order (b3Body::CreateFixture)
+    std::vector<u32>& pid = hw.proxyIds;
+    pid.resize(d.fixture_count);
+    u32 nextNode = 0, leaves = 0;
+    const std::vector<Fixture> local(fixtures);
+    for (u32 i = 0; i < d.fixture_count; ++i) {
+        Fixture& f = fixtures[i];
+        Body& x = bodies[f.body];
+        AABB aabb;
+        HostAABB(f, Transform(x.xfT, x.q), L, aabb);
+        Vec3 r(kAabbExtension, kAabbExtension, kAabbExtension);
+        AABB fa;
+        fa.lowerBound = aabb.lowerBound - r;
+        fa.upperBound = aabb.upperBound + r;
+        hw.fat.push_back(fa);
+        pid[i] = nextNode++;
+        if (leaves > 0) ++nextNode;  // InsertLeaf allocates the new parent
+        ++leaves;
+        f.proxyId = pid[i];
+        if (f.density > scalar(0)) {
+            // ResetMass sees the fixtures created so far: restrict the list to them.
+            Body tmp = x;
+            std::vector<u32> visible;
+            for (u32 k = 0; k < x.fixtureCount; ++k) {
+                u32 fi = hw.fixtureList[x.fixtureBegin + k];
+                if (fi <= i) visible.push_back(fi);
+            }
+            tmp.fixtureBegin = 0;
+            tmp.fixtureCount = u32(visible.size());
+            ResetMass(tmp, local, visible, 0, L);
+            u32 fb = x.fixtureBegin, fc = x.fixtureCount;
+            x = tmp;
+            x.fixtureBegin = fb;
+            x.fixtureCount = fc;
+        }
+    }
+    // revolute joints (b3World::CreateJoint after the bodies' fixtures; joint_manager.cpp:30-72
+    // prepends each joint's edges to both bodies' joint lists)
+    std::vector<std::vector<JointEdge>> edgesOf(d.body_count);
+    for (u32 i = 0; i < d.joint_count; ++i) {
+        const bx_joint_desc& jd = d.joints[i];
+        if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= d.body_count || jd.body_b >= d.body_count ||
+            jd.body_a == jd.body_b)
+            return Fail(BX_E_INVALID, "bad joint descriptor");
+        if (jd.enable_limit)
+            return Fail(BX_E_UNSUPPORTED, "revolute joint limits are not supported by the device path");
+        Joint j;
+        std::memset(&j, 0, sizeof(j));
+        j.bodyA = jd.body_a;
+        j.bodyB = jd.body_b;
+        j.collideConnected = jd.collide_connected ? 1u : 0u;
+        if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
+            Mat33 rotation;
+            rotation.z = V3(jd.axis);
+            ComputeBasis(rotation.z, rotation.x, rotation.y);
+            Quat q = Mat33Quat(rotation);
+            const Body& A = bodies[jd.body_a];
+            const Body& Bd = bodies[jd.body_b];
+            Vec3 anchor = V3(jd.anchor);
+            j.localAnchorA = MulT(Transform(A.xfT, A.q), anchor);
+            j.localRotationA = MulC(A.q, q);
+            j.localAnchorB = MulT(Transform(Bd.xfT, Bd.q), anchor);
+            j.localRotationB = MulC(Bd.q, q);
+            j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
+        } else {
+            j.localAnchorA = V3(jd.local_anchor_a);
+            j.localRotationA = Q4(jd.local_rotation_a);
+            j.localAnchorB = V3(jd.local_anchor_b);
+            j.localRotationB = Q4(jd.local_rotation_b);
+            j.referenceRotation = Q4(jd.reference_rotation);
+        }
+        j.enableMotor = jd.enable_motor ? 1u : 0u;
+        j.enableLimit = 0;
+        j.lowerAngle = jd.lower_angle;
+        j.upperAngle = jd.upper_angle;
+        j.motorSpeed = jd.motor_speed;
+        j.maxMotorTorque = jd.max_motor_torque;
+        hw.joints.push_back(j);
+        edgesOf[jd.body_a].insert(edgesOf[jd.body_a].begin(), JointEdge{i, jd.body_b});
+        edgesOf[jd.body_b].insert(edgesOf[jd.body_b].begin(), JointEdge{i, jd.body_a});
+    }
+    u32 ecursor = 0;
+    for (u32 i = 0; i < d.body_count; ++i) {
+        Body& x = bodies[i];
+        x.jointBegin = ecursor;
+        x.jointCount = u32(edgesOf[i].size());
+        for (const JointEdge& e : edgesOf[i]) hw.jointEdges.push_back(e);
+        ecursor += x.jointCount;
+    }
+    hw.proxyOrder.resize(d.fixture_count);
+    for (u32 i = 0; i < d.fixture_count; ++i) hw.proxyOrder[i] = i;
+    std::sort(hw.proxyOrder.begin(), hw.proxyOrder.end(), [&](u32 a, u32 c) { return pid[a] < pid[c]; });
+    // Static bodies join every island they touch; the position solver renormalizes their
+    // orientation once per contact point. If that is not a fixed point of the binary32
+    // normalization the outcome depends on island order, so such worlds solve islands in order.
+    for (u32 i = 0; i < d.body_count; ++i) {
+        const Body& x = bodies[i];
+        if (x.type != kStatic) continue;
+        Quat q = x.q;
+        Quat n1 = q;
+        n1.Normalize();
+        if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
+    }
+    // capacities
+    u64 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : 12ull * d.fixture_count + 64;
+    u64 pcap = opts && opts->max_pairs_per_world ? opts->max_pairs_per_world : 2 * cap;
+    if (cap > (1u << 24) || pcap > (1u << 25)) return Fail(BX_E_INVALID, "contact capacity too large");
+    wi.contactCap = u32(cap);
+    wi.pairCap = u32(pcap);
+    wi.islandBodyCap = d.body_count + wi.contactCap;
+    wi.islandScratchWords = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+    hw.freeSlots.resize(cap);
+    for (u32 k = 0; k < cap; ++k) hw.freeSlots[k] = u32(cap) - 1 - k;
+    std::memset(&hw.state, 0, sizeof(hw.state));
+    hw.state.freeCount = u32(cap);
+    hw.state.newContacts = 1;
+    return BX_OK;
+}
+
+// Island-builder LDS words for a launch over worlds with the given full-capacity needs: worlds
+// whose working arrays fit this budget at the current contact count use LDS, the rest their
+// global area; the budget is small enough to keep several worlds per CU in flight.
+static const u32 kIslandLdsBudgetWords = 6144;
+
+// Allocates every per-world array for the given units per space, zeroed, plus the world tables
+// and the world-list ring.
+static int AllocArrays(bx_batch* b, const u64 cap[kSpaces], u32 worldSlots) {
+    BatchView& V = b->view;
+    for (const ArraySpec& a : kArrays) {
+        u64 count = cap[a.space] * a.perUnit;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<u64>(1, count) * a.elemSize);
+        if (e != hipSuccess) return Fail(BX_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        b->allocations.push_back(p);
+        b->deviceBytes += std::max<u64>(1, count) * a.elemSize;
+        HIP_CHECK(hipMemset(p, 0, std::max<u64>(1, count) * a.elemSize));
+        SetArrayPtr(V, a, p);
+    }
+    int rc;
+    WorldInfo* dw;
+    if ((rc = DevAlloc(b, &dw, worldSlots))) return rc;
+    V.worlds = dw;
+    if ((rc = DevAlloc(b, &V.wstate, worldSlots))) return rc;
+    if ((rc = DevAlloc(b, &V.islandList, cap[kSpaceBody]))) return rc;
+    if ((rc = DevAlloc(b, &V.islandListCount, 1))) return rc;
+    if ((rc = DevAlloc(b, &b->listDev, size_t(kListRing) * 2 * std::max(1u, worldSlots)))) return rc;
+    HIP_CHECK(hipHostMalloc((void**)&b->listHost, sizeof(u32) * kListRing * 2 * std::max(1u, worldSlots)));
+    for (u32 r = 0; r < kListRing; ++r) HIP_CHECK(hipEventCreateWithFlags(&b->listEv[r], hipEventDisableTiming));
+    V.worldCount = worldSlots;
+    return BX_OK;
+}
+
+static int UploadLibrary(bx_batch* b, const HostLibrary& L) {
+    BatchView& V = b->view;
+    int rc;
+    HullInfo* dh;
+    if ((rc = DevAlloc(b, &dh, L.info.size(), L.info.data()))) return rc;
+    V.hulls = dh;
+    Vec3* dv;
+    if ((rc = DevAlloc(b, &dv, L.vertices.size(), L.vertices.data()))) return rc;
+    V.hullVertices = dv;
+    HalfEdge* de;
+    if ((rc = DevAlloc(b, &de, L.edges.size(), L.edges.data()))) return rc;
+    V.hullEdges = de;
+    u32* dfc;
+    if ((rc = DevAlloc(b, &dfc, L.faces.size(), L.faces.data()))) return rc;
+    V.hullFaces = dfc;
+    Plane* dp;
+    if ((rc = DevAlloc(b, &dp, L.planes.size(), L.planes.data()))) return rc;
+    V.hullPlanes = dp;
+    return BX_OK;
+}
+
+// Copies a slice of host elements to element `at` of a device array, on the batch stream.
+template <class T>
+static int PutAsync(bx_batch* b, T* dev, u64 at, const std::vector<T>& host) {
+    if (host.empty()) return BX_OK;
+    HIP_CHECK(hipMemcpyAsync(dev + at, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, b->ctx->stream));
+    return BX_OK;
+}
+
+// Writes a built world into its ranges (bases already in hw.info) and its slot; every other
+// per-world array of the world is cleared first, since a pool reuses ranges.
+static int PlaceWorld(bx_batch* b, u32 slot, const HostWorld& hw) {
+    const WorldInfo& wi = hw.info;
+    BatchView& V = b->view;
+    hipStream_t st = b->ctx->stream;
+    u32 units[kSpaces], base[kSpaces];
+    WorldUnits(wi, units);
+    GetBases(wi, base);
+    for (const ArraySpec& a : kArrays) {
+        u64 bytes = u64(units[a.space]) * a.perUnit * a.elemSize;
+        if (!bytes) continue;
+        HIP_CHECK(hipMemsetAsync(ArrayPtr(V, a) + u64(base[a.space]) * a.perUnit * a.elemSize, 0, bytes, st));
+    }
+    int rc;
+    if ((rc = PutAsync(b, V.bodies, wi.bodyBase, hw.bodies))) return rc;
+    if ((rc = PutAsync(b, V.fixtures, wi.fixtureBase, hw.fixtures))) return rc;
+    if ((rc = PutAsync(b, const_cast<u32*>(V.fixtureList), wi.fixtureListBase, hw.fixtureList))) return rc;
+    if ((rc = PutAsync(b, const_cast<u32*>(V.proxyOrder), wi.fixtureBase, hw.proxyOrder))) return rc;
+    if ((rc = PutAsync(b, V.fatAABB, wi.fixtureBase, hw.fat))) return rc;
+    std::vector<u32> moved(wi.fixtureCount, 1u);  // CreateProxy buffers every new proxy (broad_phase.cpp:78-84)
+    if ((rc = PutAsync(b, V.moved, wi.fixtureBase, moved))) return rc;
+    if ((rc = PutAsync(b, V.joints, wi.jointBase, hw.joints))) return rc;
+    if ((rc = PutAsync(b, const_cast<JointEdge*>(V.jointEdges), wi.jointEdgeBase, hw.jointEdges))) return rc;
+    if ((rc = PutAsync(b, V.freeSlots, wi.contactBase, hw.freeSlots))) return rc;
+    HIP_CHECK(hipMemcpyAsync((void*)(V.worlds + slot), &wi, sizeof(wi), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(V.wstate + slot, &hw.state, sizeof(hw.state), hipMemcpyHostToDevice, st));
+    // the host sources are temporaries of the caller
+    HIP_CHECK(hipStreamSynchronize(st));
+    b->worlds[slot] = wi;
+    b->proxyIds[slot] = hw.proxyIds;
+    b->resident[slot] = 1;
+    b->needsInitialPairs[slot] = 1;
+    b->newContacts = true;
+    return BX_OK;
+}
+
+static void InitSlots(bx_batch* b, u32 slots) {
+    b->worldCount = slots;
+    b->worlds.assign(slots, WorldInfo{});
+    b->proxyIds.assign(slots, {});
+    b->resident.assign(slots, 0);
+    b->needsInitialPairs.assign(slots, 0);
+}
+
+
+static StepParams MakeParams(const bx_step_cfg* cfg) {
     StepParams sp;
     sp.dt = cfg->dt;
     sp.inv_dt = cfg->dt > scalar(0) ? scalar(1) / cfg->dt : scalar(0);
     sp.velocityIterations = cfg->velocity_iterations;
     sp.positionIterations = cfg->position_iterations;
-    u32 maxIslands = 0;
-    for (u32 i = 0; i < count; ++i) maxIslands += b->worlds[first + i].bodyCount;
-    u32 timedSteps = timed ? std::min(steps, kMaxTimedSteps) : 0u;
-    if (timedSteps) {
-        int rc = EnsureEvents(b->ctx, timedSteps);
-        if (rc) return rc;
-        b->ctx->timedSteps = timedSteps;
+    return sp;
+}
+
+// Takes the next region of the world-list ring (waiting for the launches that last read it).
+static int NextListRegion(bx_batch* b, u32** host, u32** dev, u32* r) {
+    *r = b->listNext;
+    b->listNext = (b->listNext + 1) % kListRing;
+    if (b->listUsed[*r]) HIP_CHECK(hipEventSynchronize(b->listEv[*r]));
+    size_t off = size_t(*r) * 2 * std::max(1u, b->worldCount);
+    *host = b->listHost + off;
+    *dev = b->listDev + off;
+    return BX_OK;
+}
+
+// The creation-time pair search of every listed world that has not had it yet.
+static int RunInitialPairs(bx_batch* b, const u32* worlds, u32 count) {
+    if (!b->newContacts) return BX_OK;
+    u32 *h, *d, r;
+    int rc = NextListRegion(b, &h, &d, &r);
+    if (rc) return rc;
+    u32 n = 0;
+    for (u32 i = 0; i < count; ++i)
+        if (b->needsInitialPairs[worlds[i]]) {
+            h[n++] = worlds[i];
+            b->needsInitialPairs[worlds[i]] = 0;
+        }
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
+        LaunchInitialPairs(b->view, d, n, b->ctx->stream);
+        HIP_CHECK(hipGetLastError());
     }
+    HIP_CHECK(hipEventRecord(b->listEv[r], b->ctx->stream));
+    b->listUsed[r] = true;
+    bool any = false;
+    for (u32 w = 0; w < b->worldCount && !any; ++w) any = b->needsInitialPairs[w] != 0;
+    b->newContacts = any;
+    return BX_OK;
+}
+
+// One step of the listed worlds in one launch sequence (continuous batching: any subset of the
+// resident worlds, each block of the one-workgroup-per-world kernels stepping one listed world).
+static int StepList(bx_batch* b, const bx_step_cfg* cfg, const u32* worlds, u32 count) {
+    if (count == 0) return BX_OK;
+    if (count > b->worldCount) return Fail(BX_E_INVALID, "more worlds listed than the batch holds");
+    std::vector<u8> seen(b->worldCount, 0);
+    for (u32 i = 0; i < count; ++i) {
+        if (!IsResident(b, worlds[i])) return Fail(BX_E_INVALID, "listed world is not resident");
+        if (seen[worlds[i]]) return Fail(BX_E_INVALID, "world listed twice");
+        seen[worlds[i]] = 1;
+    }
+    int rc = RunInitialPairs(b, worlds, count);
+    if (rc) return rc;
+    u32 *h, *d, r;
+    if ((rc = NextListRegion(b, &h, &d, &r))) return rc;
+    u32 maxIslands = 0, lds = 0;
+    for (u32 i = 0; i < count; ++i) {
+        const WorldInfo& wi = b->worlds[worlds[i]];
+        h[i] = worlds[i];
+        maxIslands += wi.bodyCount;
+        lds = std::max(lds, std::min(wi.islandScratchWords, kIslandLdsBudgetWords));
+    }
+    HIP_CHECK(hipMemcpyAsync(d, h, count * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
+    BatchView v = b->view;
+    v.islandLdsWords = lds;
+    LaunchStep(v, d, 0, count, MakeParams(cfg), false, maxIslands, b->ctx->stream, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(b->listEv[r], b->ctx->stream));
+    b->listUsed[r] = true;
+    return BX_OK;
+}
+
+// `steps` steps of every world of a (non-pool) batch, the first kMaxTimedSteps timed per kernel.
+static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
+    const StepParams sp = MakeParams(cfg);
+    u32 maxIslands = 0;
+    bool allNew = true;
+    for (u32 i = 0; i < b->worldCount; ++i) {
+        maxIslands += b->worlds[i].bodyCount;
+        allNew = allNew && b->needsInitialPairs[i];
+    }
+    bool findFirst = false;
+    if (b->newContacts) {
+        if (allNew) {  // the usual case: the initial search runs inside the first (timed) step
+            findFirst = true;
+            std::fill(b->needsInitialPairs.begin(), b->needsInitialPairs.end(), 0);
+            b->newContacts = false;
+        } else {
+            std::vector<u32> all(b->worldCount);
+            for (u32 i = 0; i < b->worldCount; ++i) all[i] = i;
+            int rc = RunInitialPairs(b, all.data(), b->worldCount);
+            if (rc) return rc;
+        }
+    }
+    u32 timedSteps = std::min(steps, kMaxTimedSteps);
+    int rc = EnsureEvents(b->ctx, timedSteps);
+    if (rc) return rc;
+    b->ctx->timedSteps = timedSteps;
     for (u32 s = 0; s < steps; ++s) {
         hipEvent_t* ev = s < timedSteps ? b->ctx->ev.data() + size_t(12) * s : nullptr;
-        LaunchStep(b->view, first, count, sp, b->newContacts, maxIslands, b->ctx->stream, ev);
-        b->newContacts = false;
+        LaunchStep(b->view, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+        findFirst = false;
     }
     HIP_CHECK(hipGetLastError());
     return BX_OK;
@@ -332,6 +895,10 @@ void bx_batch_destroy(bx_batch* b) {
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     for (void* p : b->allocations) hipFree(p);
+    for (hipEvent_t e : b->listEv)
+        if (e) hipEventDestroy(e);
+    if (b->listHost) hipHostFree(b->listHost);
+    delete b->library;
     delete b;
 }
 
@@ -339,350 +906,369 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
                     uint32_t world_count, const bx_batch_opts* opts, bx_batch** out) {
     if (!ctx || !out || (!worlds && world_count)) return Fail(BX_E_INVALID, "null argument");
     HIP_CHECK(hipSetDevice(ctx->device));
-    HostLibrary L;
-    int rc = BuildLibrary(library, L);
-    if (rc) return rc;
     bx_batch* b = new bx_batch();
     b->ctx = ctx;
-    b->worldCount = world_count;
-    std::vector<WorldState> wstate(world_count);
+    b->library = new HostLibrary();
+    int rc = BuildLibrary(library, *b->library);
+    if (rc) { bx_batch_destroy(b); return rc; }
+    if (opts) b->worldOpts = *opts;
+    InitSlots(b, world_count);
+    // the initialized arrays, concatenated in world order (one unit of padding per world in the
+    // fixture space, whose ranges hold fixtureCount + 1 units)
     std::vector<Body> bodies;
     std::vector<Fixture> fixtures;
-    std::vector<u32> fixtureList, proxyOrder;
+    std::vector<u32> fixtureList, proxyOrder, moved, freeSlots;
     std::vector<AABB> fat;
     std::vector<Joint> joints;
     std::vector<JointEdge> jointEdges;
-    std::vector<u32> freeSlots;
-    u32 contactTotal = 0, pairTotal = 0, islandBodyTotal = 0;
-    b->worlds.resize(world_count);
-    b->proxyIds.resize(world_count);
+    std::vector<WorldState> wstate(world_count);
+    u64 next[kSpaces] = {};
+    u32 lds = 0;
     for (u32 w = 0; w < world_count; ++w) {
-        const bx_world_desc& d = worlds[w];
-        WorldInfo& wi = b->worlds[w];
-        wi.bodyBase = u32(bodies.size());
-        wi.bodyCount = d.body_count;
-        wi.fixtureBase = u32(fixtures.size());
-        wi.fixtureCount = d.fixture_count;
-        wi.fixtureListBase = u32(fixtureList.size());
-        wi.edgeBase = wi.fixtureBase + w;
-        wi.jointBase = u32(joints.size());
-        wi.jointCount = d.joint_count;
-        wi.jointEdgeBase = u32(jointEdges.size());
-        wi.flags = d.flags;
-        wi.gravity = V3(d.gravity);
-        wi.serialIslands = 0;
-        // bodies (b3Body::b3Body, body.cpp:25-103)
-        for (u32 i = 0; i < d.body_count; ++i) {
-            const bx_body_desc& bd = d.bodies[i];
-            if (bd.type > 2) { delete b; return Fail(BX_E_INVALID, "bad body type"); }
-            Body x;
-            std::memset(&x, 0, sizeof(x));
-            x.type = bd.type;
-            x.flags = 0;
-            if (bd.flags & BX_BODY_AWAKE) x.flags |= kAwake;
-            if (bd.flags & BX_BODY_AUTO_SLEEP) x.flags |= kAutoSleep;
-            if (x.type == kDynamic) {
-                x.mass = 1;
-                x.invMass = 1;
-                x.flags |= bd.flags & (kFixedX | kFixedY | kFixedZ);
-            }
-            x.v = V3(bd.linear_velocity);
-            x.w = V3(bd.angular_velocity);
-            x.localCenter.SetZero();
-            x.x = V3(bd.position);
-            x.q = Q4(bd.orientation);
-            x.x0 = x.x;
-            x.q0 = x.q;
-            x.xfT = x.x;
-            x.linearDamping = V3(bd.linear_damping);
-            x.angularDamping = V3(bd.angular_damping);
-            x.gravityScale = V3(bd.gravity_scale);
-            x.linTol = bd.linear_sleep_tolerance;
-            x.angTol = bd.angular_sleep_tolerance;
-            x.islandIndex = 0;
-            bodies.push_back(x);
+        HostWorld hw;
+        rc = BuildWorld(worlds[w], *b->library, opts, hw);
+        if (rc) { bx_batch_destroy(b); return rc; }
+        u32 units[kSpaces], base[kSpaces];
+        WorldUnits(hw.info, units);
+        for (u32 s = 0; s < kSpaces; ++s) {
+            base[s] = u32(next[s]);
+            next[s] += units[s];
+            if (next[s] > 0x7FFFFFFFull) { bx_batch_destroy(b); return Fail(BX_E_CAPACITY, "batch too large for 32-bit indices"); }
         }
-        // fixtures, grouped per body in list order (newest first)
-        std::vector<std::vector<u32>> perBody(d.body_count);
-        for (u32 i = 0; i < d.fixture_count; ++i) {
-            const bx_fixture_desc& fd = d.fixtures[i];
-            if (fd.body >= d.body_count) { delete b; return Fail(BX_E_INVALID, "fixture body out of range"); }
-            Fixture f;
-            std::memset(&f, 0, sizeof(f));
-            f.body = fd.body;
-            f.shapeType = fd.shape_type;
-            f.shapeIndex = fd.shape_index;
-            f.isSensor = fd.is_sensor ? 1u : 0u;
-            f.density = fd.density;
-            f.friction = fd.friction;
-            f.restitution = fd.restitution;
-            f.v1 = V3(fd.vertex1);
-            f.v2 = V3(fd.vertex2);
-            f.v3 = V3(fd.vertex3);
-            switch (fd.shape_type) {
-            case BX_SHAPE_HULL:
-                if (fd.shape_index >= L.info.size()) { delete b; return Fail(BX_E_INVALID, "hull index out of range"); }
-                f.radius = kHullRadius;
-                break;
-            case BX_SHAPE_SPHERE:
-            case BX_SHAPE_CAPSULE:
-                f.radius = fd.radius;
-                break;
-            default:
-                delete b;
-                return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");
-            }
-            fixtures.push_back(f);
-            perBody[fd.body].insert(perBody[fd.body].begin(), i);
-        }
-        u32 cursor = 0;
-        for (u32 i = 0; i < d.body_count; ++i) {
-            Body& x = bodies[wi.bodyBase + i];
-            x.fixtureBegin = cursor;
-            x.fixtureCount = u32(perBody[i].size());
-            for (u32 f : perBody[i]) fixtureList.push_back(f);
-            cursor += x.fixtureCount;
-        }
-        // proxies, fat AABBs and mass, in fixture creation order (b3Body::CreateFixture)
-        std::vector<u32>& pid = b->proxyIds[w];
-        pid.resize(d.fixture_count);
-        u32 nextNode = 0, leaves = 0;
-        std::vector<Fixture> local(fixtures.begin() + wi.fixtureBase, fixtures.end());
-        std::vector<u32> localList(fixtureList.begin() + wi.fixtureListBase, fixtureList.end());
-        for (u32 i = 0; i < d.fixture_count; ++i) {
-            Fixture& f = fixtures[wi.fixtureBase + i];
-            Body& x = bodies[wi.bodyBase + f.body];
-            AABB aabb;
-            HostAABB(f, Transform(x.xfT, x.q), L, aabb);
-            Vec3 r(kAabbExtension, kAabbExtension, kAabbExtension);
-            AABB fa;
-            fa.lowerBound = aabb.lowerBound - r;
-            fa.upperBound = aabb.upperBound + r;
-            fat.push_back(fa);
-            pid[i] = nextNode++;
-            if (leaves > 0) ++nextNode;  // InsertLeaf allocates the new parent
-            ++leaves;
-            f.proxyId = pid[i];
-            if (f.density > scalar(0)) {
-                // ResetMass sees the fixtures created so far: restrict the list to them.
-                Body tmp = x;
-                std::vector<u32> visible;
-                for (u32 k = 0; k < x.fixtureCount; ++k) {
-                    u32 fi = localList[x.fixtureBegin + k];
-                    if (fi <= i) visible.push_back(fi);
-                }
-                tmp.fixtureBegin = 0;
-                tmp.fixtureCount = u32(visible.size());
-                ResetMass(tmp, local, visible, 0, L);
-                u32 fb = x.fixtureBegin, fc = x.fixtureCount;
-                x = tmp;
-                x.fixtureBegin = fb;
-                x.fixtureCount = fc;
-            }
-        }
-        // revolute joints (b3World::CreateJoint after the bodies' fixtures; joint_manager.cpp:30-72
-        // prepends each joint's edges to both bodies' joint lists)
-        std::vector<std::vector<JointEdge>> edgesOf(d.body_count);
-        for (u32 i = 0; i < d.joint_count; ++i) {
-            const bx_joint_desc& jd = d.joints[i];
-            if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= d.body_count || jd.body_b >= d.body_count ||
-                jd.body_a == jd.body_b) {
-                delete b;
-                return Fail(BX_E_INVALID, "bad joint descriptor");
-            }
-            if (jd.enable_limit) {
-                delete b;
-                return Fail(BX_E_UNSUPPORTED, "revolute joint limits are not supported by the device path");
-            }
-            Joint j;
-            std::memset(&j, 0, sizeof(j));
-            j.bodyA = jd.body_a;
-            j.bodyB = jd.body_b;
-            j.collideConnected = jd.collide_connected ? 1u : 0u;
-            if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
-                Mat33 rotation;
-                rotation.z = V3(jd.axis);
-                ComputeBasis(rotation.z, rotation.x, rotation.y);
-                Quat q = Mat33Quat(rotation);
-                const Body& A = bodies[wi.bodyBase + jd.body_a];
-                const Body& Bd = bodies[wi.bodyBase + jd.body_b];
-                Vec3 anchor = V3(jd.anchor);
-                j.localAnchorA = MulT(Transform(A.xfT, A.q), anchor);
-                j.localRotationA = MulC(A.q, q);
-                j.localAnchorB = MulT(Transform(Bd.xfT, Bd.q), anchor);
-                j.localRotationB = MulC(Bd.q, q);
-                j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
-            } else {
-                j.localAnchorA = V3(jd.local_anchor_a);
-                j.localRotationA = Q4(jd.local_rotation_a);
-                j.localAnchorB = V3(jd.local_anchor_b);
-                j.localRotationB = Q4(jd.local_rotation_b);
-                j.referenceRotation = Q4(jd.reference_rotation);
-            }
-            j.enableMotor = jd.enable_motor ? 1u : 0u;
-            j.enableLimit = 0;
-            j.lowerAngle = jd.lower_angle;
-            j.upperAngle = jd.upper_angle;
-            j.motorSpeed = jd.motor_speed;
-            j.maxMotorTorque = jd.max_motor_torque;
-            joints.push_back(j);
-            edgesOf[jd.body_a].insert(edgesOf[jd.body_a].begin(), JointEdge{i, jd.body_b});
-            edgesOf[jd.body_b].insert(edgesOf[jd.body_b].begin(), JointEdge{i, jd.body_a});
-        }
-        u32 ecursor = 0;
-        for (u32 i = 0; i < d.body_count; ++i) {
-            Body& x = bodies[wi.bodyBase + i];
-            x.jointBegin = ecursor;
-            x.jointCount = u32(edgesOf[i].size());
-            for (const JointEdge& e : edgesOf[i]) jointEdges.push_back(e);
-            ecursor += x.jointCount;
-        }
-        std::vector<u32> po(d.fixture_count);
-        for (u32 i = 0; i < d.fixture_count; ++i) po[i] = i;
-        std::sort(po.begin(), po.end(), [&](u32 a, u32 c) { return pid[a] < pid[c]; });
-        proxyOrder.insert(proxyOrder.end(), po.begin(), po.end());
-        // Static bodies join every island they touch; the position solver renormalizes their
-        // orientation once per contact point. If that is not a fixed point of the binary32
-        // normalization the outcome depends on island order, so such worlds solve islands in order.
-        for (u32 i = 0; i < d.body_count; ++i) {
-            const Body& x = bodies[wi.bodyBase + i];
-            if (x.type != kStatic) continue;
-            Quat q = x.q;
-            Quat n1 = q;
-            n1.Normalize();
-            if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
-        }
-        // capacities
-        u32 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : 12 * d.fixture_count + 64;
-        u32 pcap = opts && opts->max_pairs_per_world ? opts->max_pairs_per_world : 2 * cap;
-        wi.contactBase = contactTotal;
-        wi.contactCap = cap;
-        wi.pairBase = pairTotal;
-        wi.pairCap = pcap;
-        wi.islandBodyBase = islandBodyTotal;
-        wi.islandBodyCap = d.body_count + cap;
-        contactTotal += cap;
-        pairTotal += pcap;
-        islandBodyTotal += wi.islandBodyCap;
-        for (u32 k = 0; k < cap; ++k) freeSlots.push_back(cap - 1 - k);
-        WorldState& ws = wstate[w];
-        std::memset(&ws, 0, sizeof(ws));
-        ws.freeCount = cap;
-        ws.newContacts = 1;
+        SetBases(hw.info, base);
+        bodies.insert(bodies.end(), hw.bodies.begin(), hw.bodies.end());
+        fixtures.insert(fixtures.end(), hw.fixtures.begin(), hw.fixtures.end());
+        fixtures.push_back(Fixture{});
+        fixtureList.insert(fixtureList.end(), hw.fixtureList.begin(), hw.fixtureList.end());
+        fixtureList.push_back(0);
+        proxyOrder.insert(proxyOrder.end(), hw.proxyOrder.begin(), hw.proxyOrder.end());
+        proxyOrder.push_back(0);
+        fat.insert(fat.end(), hw.fat.begin(), hw.fat.end());
+        fat.push_back(AABB{});
+        moved.insert(moved.end(), hw.info.fixtureCount, 1u);  // CreateProxy buffers every new proxy
+        moved.push_back(0);
+        joints.insert(joints.end(), hw.joints.begin(), hw.joints.end());
+        jointEdges.insert(jointEdges.end(), hw.jointEdges.begin(), hw.jointEdges.end());
+        freeSlots.insert(freeSlots.end(), hw.freeSlots.begin(), hw.freeSlots.end());
+        wstate[w] = hw.state;
+        b->worlds[w] = hw.info;
+        b->proxyIds[w] = std::move(hw.proxyIds);
+        b->resident[w] = 1;
+        b->needsInitialPairs[w] = 1;
+        lds = std::max(lds, std::min(hw.info.islandScratchWords, kIslandLdsBudgetWords));
     }
-    const u32 nb = u32(bodies.size()), nf = u32(fixtures.size());
-    std::vector<u32> moved(nf, 1u);  // CreateProxy buffers every new proxy (broad_phase.cpp:78-84)
+    for (u32 s = 0; s < kSpaces; ++s) b->spaces[s].Reset(0);
     BatchView& V = b->view;
-    V.worldCount = world_count;
-#define ALLOC(field, count, init)                                   \
-    do {                                                            \
-        int _r = DevAlloc(b, &field, (count), (init));              \
-        if (_r) { bx_batch_destroy(b); return _r; }                 \
-    } while (0)
-    WorldInfo* dWorlds;
-    ALLOC(dWorlds, world_count, b->worlds.data());
-    V.worlds = dWorlds;
-    ALLOC(V.wstate, world_count, wstate.data());
-    HullInfo* dh;
-    ALLOC(dh, L.info.size(), L.info.data());
-    V.hulls = dh;
-    Vec3* dv;
-    ALLOC(dv, L.vertices.size(), L.vertices.data());
-    V.hullVertices = dv;
-    HalfEdge* de;
-    ALLOC(de, L.edges.size(), L.edges.data());
-    V.hullEdges = de;
-    u32* dfc;
-    ALLOC(dfc, L.faces.size(), L.faces.data());
-    V.hullFaces = dfc;
-    Plane* dp;
-    ALLOC(dp, L.planes.size(), L.planes.data());
-    V.hullPlanes = dp;
-    ALLOC(V.bodies, nb, bodies.data());
-    ALLOC(V.fixtures, nf, fixtures.data());
-    u32* dfl;
-    ALLOC(dfl, fixtureList.size(), fixtureList.data());
-    V.fixtureList = dfl;
-    u32* dpo;
-    ALLOC(dpo, proxyOrder.size(), proxyOrder.data());
-    V.proxyOrder = dpo;
-    ALLOC(V.fatAABB, nf, fat.data());
-    ALLOC(V.moved, nf, moved.data());
-    ALLOC(V.joints, joints.size(), joints.data());
-    JointEdge* dje;
-    ALLOC(dje, jointEdges.size(), jointEdges.data());
-    V.jointEdges = dje;
-    ALLOC(V.contacts, contactTotal, (Contact*)nullptr);
-    ALLOC(V.contactOrder, contactTotal, (u32*)nullptr);
-    ALLOC(V.freeSlots, contactTotal, freeSlots.data());
-    ALLOC(V.contactAlive, contactTotal, (u32*)nullptr);
-    ALLOC(V.edgeOffset, nf + world_count, (u32*)nullptr);
-    ALLOC(V.edgeSlots, 2 * size_t(contactTotal), (u32*)nullptr);
-    ALLOC(V.pairs, pairTotal, (uint2*)nullptr);
-    ALLOC(V.pairCounts, nf + world_count, (u32*)nullptr);
-    ALLOC(V.islands, nb, (IslandInfo*)nullptr);
-    ALLOC(V.islandBodies, islandBodyTotal, (u32*)nullptr);
-    ALLOC(V.islandContacts, contactTotal, (uint4*)nullptr);
-    ALLOC(V.islandJoints, joints.size(), (uint4*)nullptr);
-    ALLOC(V.jointCons, joints.size(), (JointCon*)nullptr);
-    ALLOC(V.dfsStack, nb, (u32*)nullptr);
-    ALLOC(V.velBodies, islandBodyTotal, (VelBody*)nullptr);
-    ALLOC(V.posBodies, islandBodyTotal, (PosBody*)nullptr);
-    ALLOC(V.velCons, contactTotal, (VelCon*)nullptr);
-    ALLOC(V.posCons, contactTotal, (PosCon*)nullptr);
-    ALLOC(V.islandList, nb, (uint2*)nullptr);
-    ALLOC(V.islandListCount, 1, (u32*)nullptr);
-    {
-        // island builder: LDS when a world's scratch fits, otherwise a global area
-        const u32 kLdsWords = 64 * 1024 / 4;
-        u32 need = 0, scratch = 0;
-        for (u32 w = 0; w < world_count; ++w) {
-            WorldInfo& wi = b->worlds[w];
-            u32 words = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
-            wi.islandScratchWords = words;
-            if (words <= kLdsWords) {
-                wi.islandScratchBase = kNull;
-                need = std::max(need, words);
-            } else {
-                wi.islandScratchBase = scratch;
-                scratch += words;
-            }
-        }
-        V.islandLdsWords = need;
-        ALLOC(V.islandScratch, scratch, (u32*)nullptr);
-        HIP_CHECK(hipMemcpy((void*)V.worlds, b->worlds.data(), world_count * sizeof(WorldInfo), hipMemcpyHostToDevice));
+    rc = AllocArrays(b, next, world_count);
+    if (!rc) rc = UploadLibrary(b, *b->library);
+    if (rc) { bx_batch_destroy(b); return rc; }
+    V.islandLdsWords = lds;
+#define PUT(dev, host)                                                                                      \
+    if (!host.empty()) {                                                                                    \
+        hipError_t _e = hipMemcpy((void*)(dev), host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice); \
+        if (_e != hipSuccess) { bx_batch_destroy(b); return Fail(BX_E_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(_e)); } \
     }
-#undef ALLOC
+    PUT(V.bodies, bodies);
+    PUT(V.fixtures, fixtures);
+    PUT(V.fixtureList, fixtureList);
+    PUT(V.proxyOrder, proxyOrder);
+    PUT(V.fatAABB, fat);
+    PUT(V.moved, moved);
+    PUT(V.joints, joints);
+    PUT(V.jointEdges, jointEdges);
+    PUT(V.freeSlots, freeSlots);
+    PUT(V.worlds, b->worlds);
+    PUT(V.wstate, wstate);
+#undef PUT
+    b->newContacts = world_count > 0;
     *out = b;
     return BX_OK;
 }
 
+int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_pool_opts* opts, bx_batch** out) {
+    if (!ctx || !out || !opts) return Fail(BX_E_INVALID, "null argument");
+    if (opts->hbm_budget_bytes == 0) return Fail(BX_E_INVALID, "zero memory budget");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    const u32 slots = opts->max_worlds ? opts->max_worlds : 4096u;
+    // Split the budget over the spaces in the proportions of worlds of one fixture per body, about
+    // 200 bodies, the default contact capacity (12 x fixtures + 64) and the given joint density.
+    const double fixturesPerBody = 1.0 + 1.0 / 200.0;
+    const double jointsPerBody = (opts->joints_per_1000_bodies ? opts->joints_per_1000_bodies : 50u) / 1000.0;
+    const double contactsPerBody = opts->max_contacts_per_world ? opts->max_contacts_per_world / 200.0
+                                                                : 12.0 * fixturesPerBody + 64.0 / 200.0;
+    const double pairsPerBody = opts->max_pairs_per_world ? opts->max_pairs_per_world / 200.0 : 2.0 * contactsPerBody;
+    double per[kSpaces];
+    per[kSpaceBody] = 1.0;
+    per[kSpaceFixture] = fixturesPerBody;
+    per[kSpaceJoint] = jointsPerBody;
+    per[kSpaceContact] = contactsPerBody;
+    per[kSpacePair] = pairsPerBody;
+    per[kSpaceIslandBody] = 1.0 + contactsPerBody;
+    per[kSpaceScratch] = 5.0 + 4.0 * fixturesPerBody + 5.0 * contactsPerBody;
+    u64 ub[kSpaces];
+    UnitBytes(ub);
+    double bytesPerBody = 0;
+    for (u32 s = 0; s < kSpaces; ++s) bytesPerBody += per[s] * double(ub[s]);
+    double fixed = double(slots) * (sizeof(WorldInfo) + sizeof(WorldState) + 8.0 * kListRing);
+    double bodiesCap = (double(opts->hbm_budget_bytes) - fixed) / bytesPerBody;
+    // every index of every array stays below 2^31
+    for (u32 s = 0; s < kSpaces; ++s) bodiesCap = std::min(bodiesCap, double(0x7FFFFFFF / 2) / per[s]);
+    if (bodiesCap < 64.0) return Fail(BX_E_INVALID, "memory budget too small");
+    u64 cap[kSpaces];
+    for (u32 s = 0; s < kSpaces; ++s) cap[s] = u64(bodiesCap * per[s]) + 1;
+    bx_batch* b = new bx_batch();
+    b->ctx = ctx;
+    b->pool = true;
+    b->library = new HostLibrary();
+    int rc = BuildLibrary(library, *b->library);
+    if (rc) { bx_batch_destroy(b); return rc; }
+    b->worldOpts.max_contacts_per_world = opts->max_contacts_per_world;
+    b->worldOpts.max_pairs_per_world = opts->max_pairs_per_world;
+    InitSlots(b, slots);
+    rc = AllocArrays(b, cap, slots);
+    if (!rc) rc = UploadLibrary(b, *b->library);
+    if (rc) { bx_batch_destroy(b); return rc; }
+    for (u32 s = 0; s < kSpaces; ++s) b->spaces[s].Reset(cap[s]);
+    b->freeWorldSlots.resize(slots);
+    for (u32 i = 0; i < slots; ++i) b->freeWorldSlots[i] = slots - 1 - i;  // lowest slot first
+    b->view.islandLdsWords = kIslandLdsBudgetWords;
+    b->newContacts = false;
+    *out = b;
+    return BX_OK;
+}
+
+// Units a world descriptor needs in each space (the sizes BuildWorld gives it).
+static void DescUnits(const bx_world_desc& d, const bx_batch_opts* o, u32 units[kSpaces]) {
+    WorldInfo wi;
+    std::memset(&wi, 0, sizeof(wi));
+    wi.bodyCount = d.body_count;
+    wi.fixtureCount = d.fixture_count;
+    wi.jointCount = d.joint_count;
+    u64 cap = o && o->max_contacts_per_world ? o->max_contacts_per_world : 12ull * d.fixture_count + 64;
+    wi.contactCap = u32(cap);
+    wi.pairCap = u32(o && o->max_pairs_per_world ? o->max_pairs_per_world : 2 * cap);
+    wi.islandBodyCap = d.body_count + wi.contactCap;
+    wi.islandScratchWords = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+    WorldUnits(wi, units);
+}
+
+// Takes a free slot and ranges for a world of the given units; BX_E_CAPACITY when they are not free.
+static int Reserve(bx_batch* b, const u32 units[kSpaces], u32 base[kSpaces], u32* slot) {
+    if (b->freeWorldSlots.empty()) return Fail(BX_E_CAPACITY, "no free world slot in the pool");
+    u64 got[kSpaces];
+    for (u32 s = 0; s < kSpaces; ++s) {
+        if (!b->spaces[s].Alloc(units[s], got[s])) {
+            for (u32 t = 0; t < s; ++t) b->spaces[t].Free(got[t], units[t]);
+            return Fail(BX_E_CAPACITY, "pool memory budget exhausted");
+        }
+        base[s] = u32(got[s]);
+    }
+    *slot = b->freeWorldSlots.back();
+    b->freeWorldSlots.pop_back();
+    return BX_OK;
+}
+
+int bx_pool_admit(bx_batch* b, const bx_world_desc* world, uint32_t* slot) {
+    if (!b || !world || !slot) return Fail(BX_E_INVALID, "null argument");
+    if (!b->pool) return Fail(BX_E_INVALID, "not a pool");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    // reserve first (from the counts alone), so a world that does not fit costs no host build
+    u32 units[kSpaces], base[kSpaces], s;
+    DescUnits(*world, &b->worldOpts, units);
+    int rc = Reserve(b, units, base, &s);
+    if (rc) return rc;
+    HostWorld hw;
+    rc = BuildWorld(*world, *b->library, &b->worldOpts, hw);
+    if (!rc) {
+        u32 built[kSpaces];
+        WorldUnits(hw.info, built);
+        if (std::memcmp(built, units, sizeof(units)) != 0) rc = Fail(BX_E_INVALID, "world sizes changed while building");
+    }
+    if (!rc) {
+        SetBases(hw.info, base);
+        rc = PlaceWorld(b, s, hw);
+    }
+    if (rc) {
+        for (u32 k = 0; k < kSpaces; ++k) b->spaces[k].Free(base[k], units[k]);
+        b->freeWorldSlots.push_back(s);
+        b->resident[s] = 0;
+        return rc;
+    }
+    *slot = s;
+    return BX_OK;
+}
+
+int bx_pool_release(bx_batch* b, uint32_t slot) {
+    if (!b || !b->pool) return Fail(BX_E_INVALID, "not a pool");
+    if (!IsResident(b, slot)) return Fail(BX_E_INVALID, "slot holds no world");
+    u32 units[kSpaces], base[kSpaces];
+    WorldUnits(b->worlds[slot], units);
+    GetBases(b->worlds[slot], base);
+    for (u32 s = 0; s < kSpaces; ++s) b->spaces[s].Free(base[s], units[s]);
+    b->resident[slot] = 0;
+    b->needsInitialPairs[slot] = 0;
+    b->proxyIds[slot].clear();
+    b->freeWorldSlots.push_back(slot);
+    return BX_OK;
+}
+
+uint64_t bx_pool_world_bytes(bx_batch* b, const bx_world_desc* d) {
+    if (!d) return 0;
+    u32 units[kSpaces];
+    DescUnits(*d, b ? &b->worldOpts : nullptr, units);
+    u64 ub[kSpaces];
+    UnitBytes(ub);
+    u64 bytes = 0;
+    for (u32 s = 0; s < kSpaces; ++s) bytes += units[s] * ub[s];
+    return bytes;
+}
+
+int bx_pool_get_stats(bx_batch* b, bx_pool_stats* out) {
+    if (!b || !out || !b->pool) return Fail(BX_E_INVALID, "not a pool");
+    std::memset(out, 0, sizeof(*out));
+    u64 ub[kSpaces];
+    UnitBytes(ub);
+    for (u32 s = 0; s < kSpaces; ++s) {
+        out->capacity_bytes += b->spaces[s].capacity * ub[s];
+        out->used_bytes += (b->spaces[s].capacity - b->spaces[s].FreeUnits()) * ub[s];
+    }
+    for (u32 w = 0; w < b->worldCount; ++w) out->resident_worlds += b->resident[w];
+    out->free_slots = u32(b->freeWorldSlots.size());
+    out->world_slots = b->worldCount;
+    return BX_OK;
+}
+
+// Snapshot layout: header, proxy ids, then every per-world array's range in kArrays order.
+struct SnapshotHeader {
+    u32 magic, version, needsInitialPairs, proxyCount;
+    WorldInfo info;
+    WorldState state;
+};
+static const u32 kSnapshotMagic = 0x53584231u;  // "1BXS"
+
+static u64 SnapshotBytes(const WorldInfo& wi) {
+    u32 units[kSpaces];
+    WorldUnits(wi, units);
+    u64 bytes = sizeof(SnapshotHeader) + u64(wi.fixtureCount) * sizeof(u32);
+    for (const ArraySpec& a : kArrays) bytes += u64(units[a.space]) * a.perUnit * a.elemSize;
+    return bytes;
+}
+
+int bx_batch_snapshot(bx_batch* b, uint32_t world, void* out, uint64_t capacity, uint64_t* size) {
+    if (!b || !size) return Fail(BX_E_INVALID, "null argument");
+    if (!IsResident(b, world)) return Fail(BX_E_INVALID, "world not resident");
+    const WorldInfo& wi = b->worlds[world];
+    *size = SnapshotBytes(wi);
+    if (!out) return BX_OK;
+    if (capacity < *size) return Fail(BX_E_INVALID, "snapshot buffer too small");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    SnapshotHeader h;
+    std::memset(&h, 0, sizeof(h));
+    h.magic = kSnapshotMagic;
+    h.version = 1;
+    h.needsInitialPairs = b->needsInitialPairs[world];
+    h.proxyCount = wi.fixtureCount;
+    h.info = wi;
+    HIP_CHECK(hipMemcpyAsync(&h.state, b->view.wstate + world, sizeof(WorldState), hipMemcpyDeviceToHost, st));
+    char* p = static_cast<char*>(out) + sizeof(h) + u64(wi.fixtureCount) * sizeof(u32);
+    u32 units[kSpaces], base[kSpaces];
+    WorldUnits(wi, units);
+    GetBases(wi, base);
+    for (const ArraySpec& a : kArrays) {
+        u64 bytes = u64(units[a.space]) * a.perUnit * a.elemSize;
+        if (bytes)
+            HIP_CHECK(hipMemcpyAsync(p, ArrayPtr(b->view, a) + u64(base[a.space]) * a.perUnit * a.elemSize, bytes,
+                                     hipMemcpyDeviceToHost, st));
+        p += bytes;
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::memcpy(out, &h, sizeof(h));
+    if (wi.fixtureCount)
+        std::memcpy(static_cast<char*>(out) + sizeof(h), b->proxyIds[world].data(), wi.fixtureCount * sizeof(u32));
+    return BX_OK;
+}
+
+int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* slot) {
+    if (!b || !snapshot || !slot) return Fail(BX_E_INVALID, "null argument");
+    if (!b->pool) return Fail(BX_E_INVALID, "not a pool");
+    SnapshotHeader h;
+    if (size < sizeof(h)) return Fail(BX_E_INVALID, "snapshot too short");
+    std::memcpy(&h, snapshot, sizeof(h));
+    if (h.magic != kSnapshotMagic || h.version != 1 || h.proxyCount != h.info.fixtureCount ||
+        size != SnapshotBytes(h.info))
+        return Fail(BX_E_INVALID, "not a world snapshot of this library version");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    WorldInfo wi = h.info;
+    u32 units[kSpaces], base[kSpaces], s;
+    WorldUnits(wi, units);
+    int rc = Reserve(b, units, base, &s);
+    if (rc) return rc;
+    SetBases(wi, base);
+    const char* p = static_cast<const char*>(snapshot) + sizeof(h) + u64(wi.fixtureCount) * sizeof(u32);
+    for (const ArraySpec& a : kArrays) {
+        u64 bytes = u64(units[a.space]) * a.perUnit * a.elemSize;
+        hipError_t e = bytes ? hipMemcpyAsync(ArrayPtr(b->view, a) + u64(base[a.space]) * a.perUnit * a.elemSize, p,
+                                              bytes, hipMemcpyHostToDevice, st)
+                             : hipSuccess;
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            for (u32 k = 0; k < kSpaces; ++k) b->spaces[k].Free(base[k], units[k]);
+            b->freeWorldSlots.push_back(s);
+            return Fail(BX_E_DEVICE, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+        }
+        p += bytes;
+    }
+    HIP_CHECK(hipMemcpyAsync((void*)(b->view.worlds + s), &wi, sizeof(wi), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(b->view.wstate + s, &h.state, sizeof(h.state), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    b->worlds[s] = wi;
+    const u32* ids = reinterpret_cast<const u32*>(static_cast<const char*>(snapshot) + sizeof(h));
+    b->proxyIds[s].assign(ids, ids + wi.fixtureCount);
+    b->resident[s] = 1;
+    b->needsInitialPairs[s] = h.needsInitialPairs ? 1 : 0;
+    if (h.needsInitialPairs) b->newContacts = true;
+    *slot = s;
+    return BX_OK;
+}
 
 int bx_batch_step(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count) {
     if (!b || !cfg) return Fail(BX_E_INVALID, "null argument");
     HIP_CHECK(hipSetDevice(b->ctx->device));
-    if (b->worldCount == 0 || step_count == 0) return BX_OK;
-    return StepRange(b, cfg, 0, b->worldCount, step_count, true);
+    if (step_count == 0) return BX_OK;
+    if (!b->pool) return b->worldCount ? StepAll(b, cfg, step_count) : BX_OK;
+    std::vector<u32> list;
+    for (u32 w = 0; w < b->worldCount; ++w)
+        if (b->resident[w]) list.push_back(w);
+    for (u32 s = 0; s < step_count; ++s) {
+        int rc = StepList(b, cfg, list.data(), u32(list.size()));
+        if (rc) return rc;
+    }
+    return BX_OK;
+}
+
+int bx_batch_step_list(bx_batch* b, const bx_step_cfg* cfg, const uint32_t* worlds, uint32_t count) {
+    if (!b || !cfg || (!worlds && count)) return Fail(BX_E_INVALID, "null argument");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    return StepList(b, cfg, worlds, count);
 }
 
 int bx_batch_step_subset(bx_batch* b, const bx_step_cfg* cfg, const uint32_t* worlds, const uint32_t* step_counts,
                          uint32_t count) {
     if (!b || !cfg || (!worlds && count)) return Fail(BX_E_INVALID, "null argument");
     HIP_CHECK(hipSetDevice(b->ctx->device));
-    if (b->newContacts) {
-        // the first step of a batch runs the creation-time FindNewContacts on every world
-        int rc = StepRange(b, cfg, 0, b->worldCount, 0, false);
+    u32 most = 0;
+    for (u32 i = 0; i < count; ++i) most = std::max(most, step_counts ? step_counts[i] : 1u);
+    // launch s steps every world with more than s steps to go
+    std::vector<u32> list;
+    for (u32 s = 0; s < most; ++s) {
+        list.clear();
+        for (u32 i = 0; i < count; ++i)
+            if ((step_counts ? step_counts[i] : 1u) > s) list.push_back(worlds[i]);
+        int rc = StepList(b, cfg, list.data(), u32(list.size()));
         if (rc) return rc;
     }
-    for (u32 i = 0; i < count; ++i) {
-        if (worlds[i] >= b->worldCount) return Fail(BX_E_INVALID, "world index out of range");
-        u32 n = step_counts ? step_counts[i] : 1u;
-        bool saved = b->newContacts;
-        int rc = StepRange(b, cfg, worlds[i], 1, n, false);
-        b->newContacts = saved;
-        if (rc) return rc;
-    }
-    b->newContacts = false;
     return BX_OK;
 }
 
@@ -723,7 +1309,7 @@ int bx_batch_get_counts(bx_batch* b, uint32_t world, bx_world_counts* out) {
 }
 
 int bx_batch_get_bodies(bx_batch* b, uint32_t world, bx_body_state* out, uint32_t capacity) {
-    if (!b || world >= b->worldCount || !out) return Fail(BX_E_INVALID, "bad argument");
+    if (!b || !IsResident(b, world) || !out) return Fail(BX_E_INVALID, "bad argument or world not resident");
     const WorldInfo& wi = b->worlds[world];
     std::vector<Body> bodies;
     int rc = Download(b, bodies, b->view.bodies + wi.bodyBase, wi.bodyCount);
@@ -835,7 +1421,7 @@ int bx_batch_get_island_contacts(bx_batch* b, uint32_t world, uint32_t* out, uin
 }
 
 int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, uint32_t count) {
-    if (!b || world >= b->worldCount || !in) return Fail(BX_E_INVALID, "bad argument");
+    if (!b || !IsResident(b, world) || !in) return Fail(BX_E_INVALID, "bad argument or world not resident");
     const WorldInfo& wi = b->worlds[world];
     if (count != wi.bodyCount) return Fail(BX_E_INVALID, "body count mismatch");
     std::vector<Body> bodies;
@@ -906,7 +1492,7 @@ uint32_t bx_struct_size(const char* name) {
 #define S(T) if (n == #T) return uint32_t(sizeof(T));
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
-    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts)
+    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats)
 #undef S
     return 0;
 }

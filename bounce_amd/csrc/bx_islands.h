@@ -57,13 +57,12 @@ __device__ void BuildAdjacency(const BatchView& B, const WorldInfo& wi, const Wo
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
-    extern __shared__ u32 lds[];
-    const u32 w = blockIdx.x;
-    const WorldInfo wi = B.worlds[w];
-    WorldState& ws = B.wstate[w];
+// The island builder of one world over its working arrays at `base`: LDS (kLds, DS instructions)
+// when the world's arrays fit the block's LDS at the current contact count, else its global area.
+template <bool kLds>
+__device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const WorldInfo& wi, WorldState& ws,
+                                             u32* base) {
     const u32 nb = wi.bodyCount, nf = wi.fixtureCount, n = ws.contactCount;
-    u32* base = (wi.islandScratchBase == kNull) ? lds : B.islandScratch + wi.islandScratchBase;
     u32* bflags = base;                 // flags | type << 8
     u32* bfix = bflags + nb;            // fixtureBegin | fixtureCount << 16
     u32* bjoint = bfix + nb;            // jointBegin | jointCount << 16
@@ -73,10 +72,10 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
     u32* fbody = flist + nf;            // body | sensor << 31
     u32* off = fbody + nf;              // nf + 1
     u32* cursor = off + nf + 1;         // nf + 1
-    u32* cfa = cursor + nf + 1;         // by order position
-    u32* cfb = cfa + wi.contactCap;
-    u32* cfl = cfb + wi.contactCap;
-    u32* edges = cfl + wi.contactCap;   // 2 * contactCap
+    u32* cfa = cursor + nf + 1;         // by order position, n entries each
+    u32* cfb = cfa + n;
+    u32* cfl = cfb + n;
+    u32* edges = cfl + n;               // 2 n
     const u32* order = B.contactOrder + wi.contactBase;
 
     // 1. stage (island flags cleared: world.cpp:265-277)
@@ -231,5 +230,16 @@ __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
         b.flags = bflags[i] & 0xFFu;
         b.islandIndex = bidx[i];
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
+    extern __shared__ u32 lds[];
+    const u32 w = WorldOfBlock(B);
+    const WorldInfo wi = B.worlds[w];
+    WorldState& ws = B.wstate[w];
+    if (IslandScratchWords(wi.bodyCount, wi.fixtureCount, ws.contactCount) <= B.islandLdsWords)
+        BuildIslands<true>(B, w, wi, ws, lds);
+    else
+        BuildIslands<false>(B, w, wi, ws, B.islandScratch + wi.islandScratchBase);
 }
 

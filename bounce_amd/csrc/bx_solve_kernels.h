@@ -18,7 +18,7 @@ constexpr u32 kSolveLanes = 8;     // islands per 64-wide workgroup in the sweep
 constexpr u32 kSlabBodies = 256;   // velocities of islands up to this size live in LDS
 
 __global__ void __launch_bounds__(kBlock) k_integrate_vel(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     const WorldState& ws = B.wstate[w];
     const u32 n = Min(ws.islandBodyCount, wi.islandBodyCap);
@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate_vel(BatchView B, StepParam
 }
 
 __global__ void __launch_bounds__(kBlock) k_init_contacts(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     const WorldState& ws = B.wstate[w];
     const scalar dtRatio = ws.inv_dt0 * sp.dt;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams s
 }
 
 __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     const WorldState& ws = B.wstate[w];
     if ((wi.flags & 0x2u) == 0) {  // b3ContactSolver::StoreImpulses (contact_solver.cpp:488-515)
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams s
 }
 
 __global__ void __launch_bounds__(kBlock) k_finish(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     const WorldState& ws = B.wstate[w];
     const u32 n = Min(ws.islandBodyCount, wi.islandBodyCap);

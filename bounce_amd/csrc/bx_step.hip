@@ -198,13 +198,13 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
 }
 
 __global__ void __launch_bounds__(kBlock) k_update_contacts(BatchView B) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     WorldState& ws = B.wstate[w];
     const u32 n = ws.contactCount;
     u32* order = B.contactOrder + wi.contactBase;
     u32* alive = B.contactAlive + wi.contactBase;
-    if (w == 0 && threadIdx.x == 0) *B.islandListCount = 0;  // the island builder refills it
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.islandListCount = 0;  // the island builder refills it
     // The reference visits contacts newest-first and a contact can wake a sleeping body that a
     // later contact tests for activity; with every non-static body awake the visits commute.
     int asleep = 0;
@@ -275,7 +275,7 @@ __device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transfor
 }
 
 __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     const bool solved = sp.dt > scalar(0);
     if (threadIdx.x == 0 && solved) B.wstate[w].inv_dt0 = sp.inv_dt;  // world.cpp:233-236
@@ -333,7 +333,7 @@ __device__ __forceinline__ int RegisterLookup(u32 t1, u32 t2) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildAdjacency) {
-    const u32 w = blockIdx.x;
+    const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     WorldState& ws = B.wstate[w];
     if (rebuildAdjacency) {
@@ -450,11 +450,16 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
 }
 
 // ================================================================== host launch
-void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const StepParams& sp, bool findFirst,
-                u32 maxIslands, hipStream_t stream, hipEvent_t* ev) {
+// Launches one step of worldCount worlds: worlds firstWorld.. when worldList is null, else the
+// worlds listed in the device array worldList (any subset of the batch, in any order).
+void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
+                bool findFirst, u32 maxIslands, hipStream_t stream, hipEvent_t* ev) {
     BatchView B = view;
-    B.worlds += firstWorld;
-    B.wstate += firstWorld;
+    B.worldList = worldList;
+    if (!worldList) {
+        B.worlds += firstWorld;
+        B.wstate += firstWorld;
+    }
     B.worldCount = worldCount;
     dim3 grid(worldCount);
     // ev (when given) holds kStepEvents events: one before the first kernel and one after each
@@ -495,6 +500,15 @@ void LaunchStep(const BatchView& view, u32 firstWorld, u32 worldCount, const Ste
         mark(10);
         mark(11);
     }
+}
+
+// The creation-time pair search (b3World::Step's FindNewContacts after shapes were added) of the
+// worlds listed in the device array worldList.
+void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream) {
+    BatchView B = view;
+    B.worldList = worldList;
+    B.worldCount = worldCount;
+    hipLaunchKernelGGL(k_find_pairs, dim3(worldCount), dim3(kBlock), 0, stream, B, 1u);
 }
 
 }  // namespace bx

@@ -176,9 +176,9 @@ struct WorldInfo {
     u32 flags;                         // BX_WORLD_* flags
     u32 serialIslands;                 // islands must be solved in order (see bx_step.hip)
     Vec3 gravity;
-    u32 islandScratchBase;             // words in BatchView::islandScratch, kNull when LDS suffices
-    u32 islandScratchWords;            // words the island builder needs for this world
-    u32 edgeBase;                      // fixtureBase + world index: CSR offsets (fixtureCount + 1 entries)
+    u32 islandScratchBase;             // words in BatchView::islandScratch (used when LDS is too small)
+    u32 islandScratchWords;            // words the island builder needs for this world at full capacity
+    u32 edgeBase;                      // CSR offsets of the contact edges (fixtureCount + 1 entries)
 };
 
 // Per-world mutable counters.
@@ -243,6 +243,7 @@ struct PosCon {
 // Everything a step kernel needs; all pointers are device pointers.
 struct BatchView {
     u32 worldCount;
+    const u32* worldList;     // launched worlds (block i steps world worldList[i]); null: block i steps world i
     const WorldInfo* worlds;
     WorldState* wstate;
     // shape library
@@ -284,8 +285,13 @@ struct BatchView {
     PosCon* posCons;          // per world: contactCap
     uint2* islandList;        // islands of the launched worlds: {world, island} (island = kNull: all, in order)
     u32* islandListCount;
-    u32* islandScratch;       // island-builder scratch for worlds too large for LDS
+    u32* islandScratch;       // island-builder working arrays of worlds whose arrays do not fit LDS
     u32 islandLdsWords;       // LDS words available to the island builder per world
 };
+
+// World stepped by this workgroup in the one-workgroup-per-world kernels.
+__device__ __forceinline__ u32 WorldOfBlock(const BatchView& B) {
+    return B.worldList ? B.worldList[blockIdx.x] : blockIdx.x;
+}
 
 }  // namespace bx

@@ -49,6 +49,14 @@ def load(path=capi.LIB_PATH):
         "bx_kernel_name": (C.c_char_p, [u]),
         "bx_struct_size": (u, [C.c_char_p]),
         "bx_batch_device_bytes": (C.c_uint64, [P]),
+        "bx_batch_step_list": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), u]),
+        "bx_pool_create": (C.c_int, [P, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_pool_opts), C.POINTER(P)]),
+        "bx_pool_admit": (C.c_int, [P, C.POINTER(capi.bx_world_desc), C.POINTER(u)]),
+        "bx_pool_release": (C.c_int, [P, u]),
+        "bx_pool_world_bytes": (C.c_uint64, [P, C.POINTER(capi.bx_world_desc)]),
+        "bx_pool_get_stats": (C.c_int, [P, C.POINTER(capi.bx_pool_stats)]),
+        "bx_batch_snapshot": (C.c_int, [P, u, P, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "bx_pool_restore": (C.c_int, [P, P, C.c_uint64, C.POINTER(u)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -129,6 +137,20 @@ class Batch:
         cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
         _check(_lib.bx_batch_step(self.h, C.byref(cfg), n))
 
+    def step_list(self, worlds, dt=1.0 / 60.0, vit=8, pit=2):
+        """One step of each listed world, all in one launch sequence."""
+        cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
+        w = np.ascontiguousarray(worlds, dtype=np.uint32)
+        _check(_lib.bx_batch_step_list(self.h, C.byref(cfg), w.ctypes.data_as(C.POINTER(C.c_uint32)), len(w)))
+
+    def snapshot(self, world):
+        """The world's whole device state as bytes (bx_pool_restore resumes it in a pool)."""
+        size = C.c_uint64(0)
+        _check(_lib.bx_batch_snapshot(self.h, world, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _check(_lib.bx_batch_snapshot(self.h, world, buf, size.value, C.byref(size)))
+        return buf.raw
+
     def step_subset(self, worlds, counts, dt=1.0 / 60.0, vit=8, pit=2):
         cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
         w = np.ascontiguousarray(worlds, dtype=np.uint32)
@@ -144,8 +166,11 @@ class Batch:
         _check(_lib.bx_batch_get_counts(self.h, world, C.byref(c)))
         return c
 
+    def _body_count(self, world):
+        return self.scene.worlds[world].body_count
+
     def bodies(self, world=0):
-        n = self.scene.worlds[world].body_count
+        n = self._body_count(world)
         out = np.zeros(n, dtype=BODY_DT)
         _check(_lib.bx_batch_get_bodies(self.h, world, out.ctypes.data_as(C.POINTER(capi.bx_body_state)), n))
         return out
@@ -183,3 +208,65 @@ class Batch:
 
     def device_bytes(self):
         return int(_lib.bx_batch_device_bytes(self.h))
+
+
+class PoolFull(BxError):
+    """The pool has no free slot or not enough free memory for the world right now."""
+
+
+class Pool(Batch):
+    """Resident worlds in a fixed device-memory budget (bx_pool_*): worlds are admitted into slots
+    while they fit, stepped in any mix with step_list, read like a batch's worlds and released."""
+
+    def __init__(self, ctx, scene, budget_bytes, max_worlds=0, max_contacts=0, max_pairs=0):
+        self.ctx = ctx
+        self.scene = scene  # shape library; its worlds are not admitted by construction
+        self._lib = scene.library()
+        opts = capi.bx_pool_opts()
+        opts.hbm_budget_bytes = int(budget_bytes)
+        opts.max_worlds = max_worlds
+        opts.max_contacts_per_world = max_contacts
+        opts.max_pairs_per_world = max_pairs
+        h = C.c_void_p()
+        _check(_lib.bx_pool_create(ctx.h, C.byref(self._lib), C.byref(opts), C.byref(h)))
+        self.h = h
+        self.slots = {}  # slot -> World
+        ctx._batches.add(self)
+
+    def _body_count(self, slot):
+        return self.slots[slot].body_count
+
+    def world_bytes(self, world):
+        d = world.desc()
+        return int(_lib.bx_pool_world_bytes(self.h, C.byref(d)))
+
+    def admit(self, world):
+        """Slot of the admitted world; PoolFull when it does not fit now."""
+        d = world.desc()
+        slot = C.c_uint32(0)
+        rc = _lib.bx_pool_admit(self.h, C.byref(d), C.byref(slot))
+        if rc == capi.BX_E_CAPACITY:
+            raise PoolFull(_lib.bx_last_error().decode())
+        _check(rc)
+        self.slots[slot.value] = world
+        return slot.value
+
+    def release(self, slot):
+        _check(_lib.bx_pool_release(self.h, slot))
+        del self.slots[slot]
+
+    def restore(self, world, image):
+        """Admit a snapshot taken of `world`; PoolFull when it does not fit now."""
+        slot = C.c_uint32(0)
+        buf = C.create_string_buffer(image, len(image))
+        rc = _lib.bx_pool_restore(self.h, buf, len(image), C.byref(slot))
+        if rc == capi.BX_E_CAPACITY:
+            raise PoolFull(_lib.bx_last_error().decode())
+        _check(rc)
+        self.slots[slot.value] = world
+        return slot.value
+
+    def stats(self):
+        st = capi.bx_pool_stats()
+        _check(_lib.bx_pool_get_stats(self.h, C.byref(st)))
+        return st

@@ -256,6 +256,24 @@ typedef struct bx_batch_opts {
     uint32_t reserved[5];
 } bx_batch_opts;
 
+typedef struct bx_pool_opts {
+    uint64_t hbm_budget_bytes;         /* device memory for the resident worlds' state, allocated once */
+    uint32_t max_worlds;               /* resident world slots (0: 4096) */
+    uint32_t max_contacts_per_world;   /* 0: 12 x fixtures + 64 */
+    uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
+    uint32_t joints_per_1000_bodies;   /* share of the budget kept for joints (0: 50) */
+    uint32_t reserved[4];
+} bx_pool_opts;
+
+typedef struct bx_pool_stats {
+    uint64_t capacity_bytes;           /* per-world state the pool can hold */
+    uint64_t used_bytes;               /* held by resident worlds */
+    uint32_t resident_worlds;
+    uint32_t free_slots;
+    uint32_t world_slots;
+    uint32_t reserved[5];
+} bx_pool_stats;
+
 /* Library version and build information ("bxstep gfx950 ..."). */
 const char* bx_version(void);
 
@@ -277,9 +295,38 @@ void bx_batch_destroy(bx_batch* batch);
    Asynchronous on the context stream; call bx_batch_sync before reading state. */
 int bx_batch_step(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
 
-/* Advance only the worlds whose indices are listed (serving: a mix of resident worlds). */
+/* Advance world worlds[i] by step_counts[i] steps (1 each when step_counts is NULL). Step s launches
+   once for every listed world with more than s steps to go, so the worlds share each launch. */
 int bx_batch_step_subset(bx_batch* batch, const bx_step_cfg* cfg, const uint32_t* worlds,
                          const uint32_t* step_counts, uint32_t count);
+
+/* One step of each listed world (distinct, resident) in one launch sequence: the unit of
+   continuous batching. A world's result does not depend on which other worlds share the launch. */
+int bx_batch_step_list(bx_batch* batch, const bx_step_cfg* cfg, const uint32_t* worlds, uint32_t count);
+
+/* ---------------------------------------------------------------------------------------------
+ * Pools (serving): a batch with world slots and a fixed device-memory budget, allocated once.
+ * Worlds are admitted into free slots while their state fits the budget, stepped in any mix with
+ * bx_batch_step_list, read with the bx_batch_get_* calls (world = slot) and released. The pool's
+ * memory is split into index spaces (bodies, fixtures, joints, contact slots, pairs, island
+ * entries, island-builder words); a world holds one range in each.
+ * ------------------------------------------------------------------------------------------- */
+int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_pool_opts* opts,
+                   bx_batch** out);
+/* Admit a world (built as bx_batch_create would build it) into a free slot. BX_E_CAPACITY when no
+   slot or not enough of the budget is free now; the pool is unchanged then. */
+int bx_pool_admit(bx_batch* pool, const bx_world_desc* world, uint32_t* slot);
+/* Free a slot and its memory. Launches already enqueued that list the slot still run correctly. */
+int bx_pool_release(bx_batch* pool, uint32_t slot);
+/* Device bytes a world would hold while resident in this pool (NULL pool: default capacities). */
+uint64_t bx_pool_world_bytes(bx_batch* pool, const bx_world_desc* world);
+int bx_pool_get_stats(bx_batch* pool, bx_pool_stats* out);
+
+/* Whole device state of a resident world as a relocatable byte image (eviction and resume): with
+   out == NULL only *size is written. bx_pool_restore admits an image into a free slot of a pool
+   built with the same shape library; stepping continues exactly as if the world had stayed. */
+int bx_batch_snapshot(bx_batch* batch, uint32_t world, void* out, uint64_t capacity, uint64_t* size);
+int bx_pool_restore(bx_batch* pool, const void* snapshot, uint64_t size, uint32_t* slot);
 
 int bx_batch_sync(bx_batch* batch);
 
