@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--serve-worlds", type=int, default=512)
     p.add_argument("--serve-rate", type=float, default=1000.0)
-    p.add_argument("--serve-seconds", type=float, default=5.0)
+    p.add_argument("--serve-seconds", type=float, default=0.0)
     p.add_argument("--serve-budget-gb", type=float, default=1.0)
     return p.parse_args()
 

@@ -419,6 +419,13 @@ static int GetState(bx_batch* b, u32 world, WorldState& ws) {
     return BX_OK;
 }
 
+// Default contact-slot capacity of a world: the reference grows its contact list without bound;
+// here a world holds a fixed number of slots, enough for the densest piles of the mixed-shapes
+// workload (up to about 12.5 fat-AABB overlaps per fixture while bodies settle). A step that needs
+// more sets status bit 0x2 and the world no longer follows the reference.
+static const u32 kContactsPerFixture = 16, kContactSlack = 64;
+static u64 DefaultContactCap(u32 fixtures) { return u64(kContactsPerFixture) * fixtures + kContactSlack; }
+
 // One world built on the host from its descriptor. Every index in these arrays is world-local;
 // `info` holds sizes, capacities and flags (its bases are set when the world is placed).
 struct HostWorld {
@@ -623,7 +630,7 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
     }
     // capacities
-    u64 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : 12ull * d.fixture_count + 64;
+    u64 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : DefaultContactCap(d.fixture_count);
     u64 pcap = opts && opts->max_pairs_per_world ? opts->max_pairs_per_world : 2 * cap;
     if (cap > (1u << 24) || pcap > (1u << 25)) return Fail(BX_E_INVALID, "contact capacity too large");
     wi.contactCap = u32(cap);
@@ -991,11 +998,11 @@ int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_po
     HIP_CHECK(hipSetDevice(ctx->device));
     const u32 slots = opts->max_worlds ? opts->max_worlds : 4096u;
     // Split the budget over the spaces in the proportions of worlds of one fixture per body, about
-    // 200 bodies, the default contact capacity (12 x fixtures + 64) and the given joint density.
+    // 200 bodies, the default contact capacity and the given joint density.
     const double fixturesPerBody = 1.0 + 1.0 / 200.0;
     const double jointsPerBody = (opts->joints_per_1000_bodies ? opts->joints_per_1000_bodies : 50u) / 1000.0;
     const double contactsPerBody = opts->max_contacts_per_world ? opts->max_contacts_per_world / 200.0
-                                                                : 12.0 * fixturesPerBody + 64.0 / 200.0;
+                                                                : double(kContactsPerFixture) * fixturesPerBody + double(kContactSlack) / 200.0;
     const double pairsPerBody = opts->max_pairs_per_world ? opts->max_pairs_per_world / 200.0 : 2.0 * contactsPerBody;
     double per[kSpaces];
     per[kSpaceBody] = 1.0;
@@ -1044,7 +1051,7 @@ static void DescUnits(const bx_world_desc& d, const bx_batch_opts* o, u32 units[
     wi.bodyCount = d.body_count;
     wi.fixtureCount = d.fixture_count;
     wi.jointCount = d.joint_count;
-    u64 cap = o && o->max_contacts_per_world ? o->max_contacts_per_world : 12ull * d.fixture_count + 64;
+    u64 cap = o && o->max_contacts_per_world ? o->max_contacts_per_world : DefaultContactCap(d.fixture_count);
     wi.contactCap = u32(cap);
     wi.pairCap = u32(o && o->max_pairs_per_world ? o->max_pairs_per_world : 2 * cap);
     wi.islandBodyCap = d.body_count + wi.contactCap;

@@ -14,7 +14,32 @@
 //   k_solve_position  per island (lane)  position iterations (island.cpp:231-248)
 //   k_finish          per body/island    copy back + sleep (island.cpp:249-306)
 
-constexpr u32 kSolveLanes = 8;     // islands per 64-wide workgroup in the sweep kernels
+#ifndef BX_TWO_BUFFERS
+#define BX_TWO_BUFFERS 1
+#endif
+#ifndef BX_SOLVE_LANES
+#define BX_SOLVE_LANES 16
+#endif
+#ifndef BX_SOLVE_WAVES
+#define BX_SOLVE_WAVES 1
+#endif
+constexpr u32 kSolveLanes = BX_SOLVE_LANES;  // islands per workgroup in the velocity sweeps
+constexpr u32 kSolveWaves = BX_SOLVE_WAVES;  // wavefronts per workgroup in the velocity sweeps
+static_assert(kSolveLanes % kSolveWaves == 0 && kSolveLanes / kSolveWaves <= 64, "islands per wavefront");
+#ifndef BX_POS_LANES
+#define BX_POS_LANES 64
+#endif
+#ifndef BX_POS_TWO_BUFFERS
+#define BX_POS_TWO_BUFFERS 0
+#endif
+constexpr u32 kPosLanes = BX_POS_LANES;      // islands per 64-wide workgroup in the position sweeps
+#ifndef BX_POS_SPREAD_LDS
+#define BX_POS_SPREAD_LDS 0
+#endif
+// LDS requested (and not used) by the position-sweep workgroups so that at most one runs per CU:
+// the sweep of a wavefront is a serial chain whose time grows with every other wavefront sharing
+// its CU, so the islands are spread over all CUs instead.
+constexpr u32 kPosSpreadLds = BX_POS_SPREAD_LDS;
 constexpr u32 kSlabBodies = 256;   // velocities of islands up to this size live in LDS
 
 __global__ void __launch_bounds__(kBlock) k_integrate_vel(BatchView B, StepParams sp) {
@@ -70,6 +95,69 @@ __global__ void __launch_bounds__(kBlock) k_init_contacts(BatchView B, StepParam
 // Warm start and velocity iterations of one island on one lane. kInLds selects at compile time
 // whether the velocities live in the lane's LDS slab (LDS instructions, which do not share the
 // in-order vector-memory counter with the constraint prefetch) or stay in global memory.
+// One contact of a velocity sweep on body velocities `vel`: read the two bodies, request the
+// constraint `ahead` (the next one, clamped to the last), apply `op`, write the bodies back.
+// Two register buffers alternate between the current and the requested constraint, so the
+// prefetched constraint is never copied between registers.
+template <class V, class Op>
+__device__ __forceinline__ void VelocityStep(VelCon& cur, VelCon& ahead, const VelCon* src, V* vel, Op op) {
+    Vec3 vA = vel[cur.indexA].v, wA = vel[cur.indexA].w, vB = vel[cur.indexB].v, wB = vel[cur.indexB].w;
+    ahead = *src;
+    op(cur, vA, wA, vB, wB);
+    vel[cur.indexA].v = vA;
+    vel[cur.indexA].w = wA;
+    vel[cur.indexB].v = vB;
+    vel[cur.indexB].w = wB;
+}
+
+// A sweep over the island's contacts in order. kTwoBuffers unrolls by two so that the two
+// register buffers swap roles instead of the prefetched constraint being copied each contact
+// (twice the code, used for the iteration sweeps that dominate).
+template <bool kTwoBuffers, class V, class Op>
+__device__ __forceinline__ void ContactSweep(VelCon* vcs, u32 nc, V* vel, Op op) {
+    VelCon a = vcs[0], b;
+    if (!kTwoBuffers) {
+        for (u32 k = 0; k < nc; ++k) {
+            VelocityStep(a, b, vcs + Min(k + 1, nc - 1), vel, op);
+            op.after(a, vcs[k]);
+            a = b;
+        }
+        return;
+    }
+    for (u32 k = 0;;) {
+        VelocityStep(a, b, vcs + Min(k + 1, nc - 1), vel, op);
+        op.after(a, vcs[k]);
+        if (++k == nc) break;
+        VelocityStep(b, a, vcs + Min(k + 1, nc - 1), vel, op);
+        op.after(b, vcs[k]);
+        if (++k == nc) break;
+    }
+}
+
+struct WarmStartOp {
+    __device__ __forceinline__ void operator()(VelCon& vc, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) const {
+        WarmStartContact(vc, vA, wA, vB, wB);
+    }
+    __device__ __forceinline__ void after(const VelCon&, VelCon&) const {}
+};
+
+struct SolveOp {
+    __device__ __forceinline__ void operator()(VelCon& vc, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) const {
+        SolveContactVelocity(vc, vA, wA, vB, wB);
+    }
+    // the impulses accumulated by this sweep (the next sweep and StoreImpulses read them)
+    __device__ __forceinline__ void after(const VelCon& vc, VelCon& dst) const {
+        dst.tangentImpulse = vc.tangentImpulse;
+        dst.motorImpulse = vc.motorImpulse;
+#pragma unroll
+        for (u32 p = 0; p < kMaxManifoldPoints; ++p)
+            if (p < vc.pointCount) dst.points[p].normalImpulse = vc.points[p].normalImpulse;
+    }
+};
+
+// Warm start and velocity iterations of one island on one lane. kInLds selects at compile time
+// whether the velocities live in the lane's LDS slab (LDS instructions, which do not share the
+// in-order vector-memory counter with the constraint prefetch) or stay in global memory.
 template <bool kInLds>
 __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jcs, u32 nj, VelBody* vel,
                                                u32 iterations, bool warm, scalar dt) {
@@ -77,45 +165,25 @@ __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jc
     // requested (clamped index, no branch), so waiting for the bodies never waits for the prefetch.
     // Order of island.cpp:154-181: contact warm start, joint warm start, then per iteration all
     // joints followed by all contacts.
-    if (warm && nc > 0) {
-        VelCon next = vcs[0];
-        for (u32 k = 0; k < nc; ++k) {
-            const VelCon vc = next;
-            Vec3 vA = vel[vc.indexA].v, wA = vel[vc.indexA].w, vB = vel[vc.indexB].v, wB = vel[vc.indexB].w;
-            next = vcs[Min(k + 1, nc - 1)];
-            WarmStartContact(vc, vA, wA, vB, wB);
-            vel[vc.indexA].v = vA; vel[vc.indexA].w = wA;
-            vel[vc.indexB].v = vB; vel[vc.indexB].w = wB;
-        }
-    }
+    if (warm && nc > 0) ContactSweep<false>(vcs, nc, vel, WarmStartOp());
     if (warm)
         for (u32 k = 0; k < nj; ++k) WarmStartJoint(jcs[k], vel);
     for (u32 it = 0; it < iterations && (nc > 0 || nj > 0); ++it) {
         for (u32 k = 0; k < nj; ++k) SolveJointVelocity(jcs[k], vel, dt);
-        if (nc == 0) continue;
-        VelCon next = vcs[0];
-        for (u32 k = 0; k < nc; ++k) {
-            VelCon vc = next;
-            Vec3 vA = vel[vc.indexA].v, wA = vel[vc.indexA].w, vB = vel[vc.indexB].v, wB = vel[vc.indexB].w;
-            next = vcs[Min(k + 1, nc - 1)];
-            SolveContactVelocity(vc, vA, wA, vB, wB);
-            vel[vc.indexA].v = vA; vel[vc.indexA].w = wA;
-            vel[vc.indexB].v = vB; vel[vc.indexB].w = wB;
-            VelCon& dst = vcs[k];
-            dst.tangentImpulse = vc.tangentImpulse;
-            dst.motorImpulse = vc.motorImpulse;
-#pragma unroll
-            for (u32 p = 0; p < kMaxManifoldPoints; ++p)
-                if (p < vc.pointCount) dst.points[p].normalImpulse = vc.points[p].normalImpulse;
-        }
+        if (nc > 0) ContactSweep<kInLds && BX_TWO_BUFFERS>(vcs, nc, vel, SolveOp());
     }
 }
 
-__global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams sp) {
+__global__ void __launch_bounds__(64 * kSolveWaves) k_solve_velocity(BatchView B, StepParams sp) {
     __shared__ VelBody slabs[kSolveLanes][kSlabBodies];
-    if (threadIdx.x >= kSolveLanes) return;
+    // kSolveLanes islands per workgroup, spread over its kSolveWaves wavefronts (the first
+    // kSolveLanes / kSolveWaves lanes of each)
+    constexpr u32 kPerWave = kSolveLanes / kSolveWaves;
+    const u32 lane = threadIdx.x & 63u;
+    if (lane >= kPerWave) return;
+    const u32 slot = (threadIdx.x >> 6) * kPerWave + lane;
     const u32 count = *B.islandListCount;
-    for (u32 i = blockIdx.x * kSolveLanes + threadIdx.x; i < count; i += gridDim.x * kSolveLanes) {
+    for (u32 i = blockIdx.x * kSolveLanes + slot; i < count; i += gridDim.x * kSolveLanes) {
         const uint2 e = B.islandList[i];
         const WorldInfo wi = B.worlds[e.x];
         const WorldState& ws = B.wstate[e.x];
@@ -128,7 +196,7 @@ __global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams s
             VelCon* vcs = B.velCons + wi.contactBase + isl.contactStart;
             JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
             if (isl.bodyCount <= kSlabBodies) {
-                VelBody* slab = slabs[threadIdx.x];
+                VelBody* slab = slabs[slot];
                 for (u32 b = 0; b < isl.bodyCount; ++b) slab[b] = gvel[b];
                 VelocitySweeps<true>(vcs, isl.contactCount, jcs, isl.jointCount, slab, sp.velocityIterations, warm,
                                      sp.dt);
@@ -173,20 +241,36 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
                           B.bodies[wi.bodyBase + ib[i]], sp.dt);
 }
 
+// One contact of a position sweep: read the two bodies, request the constraint `ahead`, solve.
+__device__ __forceinline__ scalar PositionStep(const PosCon& cur, PosCon& ahead, const PosCon* src, PosBody* pos,
+                                               scalar minSeparation) {
+    PosBody A = pos[cur.indexA];
+    PosBody Bb = pos[cur.indexB];
+    ahead = *src;
+    minSeparation = SolveContactPosition(cur, A.x, A.q, A.I, Bb.x, Bb.q, Bb.I, minSeparation);
+    pos[cur.indexA] = A;
+    pos[cur.indexB] = Bb;
+    return minSeparation;
+}
+
 __device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, const JointCon* jcs, u32 nj, PosBody* pos,
                                                u32 iterations) {
     for (u32 it = 0; it < iterations; ++it) {
         scalar minSeparation = 0;
-        PosCon next;
-        if (nc > 0) next = pcs[0];
-        for (u32 k = 0; k < nc; ++k) {
-            const PosCon pc = next;
-            PosBody A = pos[pc.indexA];
-            PosBody Bb = pos[pc.indexB];
-            next = pcs[Min(k + 1, nc - 1)];
-            minSeparation = SolveContactPosition(pc, A.x, A.q, A.I, Bb.x, Bb.q, Bb.I, minSeparation);
-            pos[pc.indexA] = A;
-            pos[pc.indexB] = Bb;
+        if (nc > 0 && BX_POS_TWO_BUFFERS) {  // two register buffers swap roles (no copy of the prefetch)
+            PosCon a = pcs[0], b;
+            for (u32 k = 0;;) {
+                minSeparation = PositionStep(a, b, pcs + Min(k + 1, nc - 1), pos, minSeparation);
+                if (++k == nc) break;
+                minSeparation = PositionStep(b, a, pcs + Min(k + 1, nc - 1), pos, minSeparation);
+                if (++k == nc) break;
+            }
+        } else if (nc > 0) {
+            PosCon a = pcs[0], b;
+            for (u32 k = 0; k < nc; ++k) {
+                minSeparation = PositionStep(a, b, pcs + Min(k + 1, nc - 1), pos, minSeparation);
+                a = b;
+            }
         }
         bool contactsSolved = minSeparation >= scalar(-3) * kLinearSlop;
         bool jointsSolved = true;  // joint_solver.cpp:55-64
@@ -200,8 +284,9 @@ __device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, const 
 }
 
 __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams sp) {
+    if (threadIdx.x >= kPosLanes) return;
     const u32 count = *B.islandListCount;
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    for (u32 i = blockIdx.x * kPosLanes + threadIdx.x; i < count; i += gridDim.x * kPosLanes) {
         const uint2 e = B.islandList[i];
         const WorldInfo wi = B.worlds[e.x];
         const WorldState& ws = B.wstate[e.x];

@@ -197,7 +197,10 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_update_contacts(BatchView B) {
+#ifndef BX_UC_BLOCKS_PER_CU
+#define BX_UC_BLOCKS_PER_CU 4
+#endif
+__global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts(BatchView B) {
     const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     WorldState& ws = B.wstate[w];
@@ -477,16 +480,16 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         mark(3);
         // one lane per island; islands <= bodies of the launched worlds
         u32 vblocks = std::max(1u, std::min((maxIslands + kSolveLanes - 1) / kSolveLanes, 16384u));
-        u32 pblocks = std::max(1u, std::min((maxIslands + 63u) / 64u, 16384u));
+        u32 pblocks = std::max(1u, std::min((maxIslands + kPosLanes - 1) / kPosLanes, 16384u));
         hipLaunchKernelGGL(k_integrate_vel, grid, dim3(kBlock), 0, stream, B, sp);
         mark(4);
         hipLaunchKernelGGL(k_init_contacts, grid, dim3(kBlock), 0, stream, B, sp);
         mark(5);
-        hipLaunchKernelGGL(k_solve_velocity, dim3(vblocks), dim3(64), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_solve_velocity, dim3(vblocks), dim3(64 * kSolveWaves), 0, stream, B, sp);
         mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
         mark(7);
-        hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), 0, stream, B, sp);
+        hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), kPosSpreadLds, stream, B, sp);
         mark(8);
         hipLaunchKernelGGL(k_finish, grid, dim3(kBlock), 0, stream, B, sp);
         mark(9);

@@ -250,7 +250,7 @@ typedef struct bx_step_cfg {
 } bx_step_cfg;
 
 typedef struct bx_batch_opts {
-    uint32_t max_contacts_per_world;   /* 0: 12 x fixtures + 64 */
+    uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t solver_mode;              /* 0: auto, 1: one lane per island, 2: scheduled wavefront */
     uint32_t reserved[5];
@@ -259,7 +259,7 @@ typedef struct bx_batch_opts {
 typedef struct bx_pool_opts {
     uint64_t hbm_budget_bytes;         /* device memory for the resident worlds' state, allocated once */
     uint32_t max_worlds;               /* resident world slots (0: 4096) */
-    uint32_t max_contacts_per_world;   /* 0: 12 x fixtures + 64 */
+    uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t joints_per_1000_bodies;   /* share of the budget kept for joints (0: 50) */
     uint32_t reserved[4];
