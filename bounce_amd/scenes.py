@@ -330,6 +330,52 @@ def mixed_drop(n_bodies=60, n_worlds=1, seed=4, kinds=("sphere", "capsule", "box
     return s
 
 
+def triangle_ground_world(rng, n_bodies, box_hull, grid=6, size=12.0, bumps=0.3, kinds=("sphere", "capsule", "box"),
+                          extent=(8.0, 12.0, 8.0), hull_choices=None):
+    """Bodies of the given kinds (as in mixed_drop_world) dropped on a static ground made of isolated
+    triangle fixtures (b3TriangleShape, no edge wings): a grid x grid lattice over a size x size
+    square at y = 0 whose vertex heights are U[-bumps, bumps], two triangles per cell. The ground
+    body sits at the origin with the identity orientation (triangle AABBs are computed in the
+    shape's frame)."""
+    nt = 2 * grid * grid
+    ground = mixed_drop_world(rng, n_bodies, 0, box_hull, kinds, extent, 0.0, hull_choices)
+    bodies = ground.bodies
+    bodies[0]["position"] = (0.0, 0.0, 0.0)
+    xs = np.linspace(-size / 2, size / 2, grid + 1)
+    h = rng.uniform(-bumps, bumps, (grid + 1, grid + 1))
+    v = lambda i, j: (F32(xs[i]), F32(h[i, j]), F32(xs[j]))
+    tris = fixture_array(nt)
+    k = 0
+    for i in range(grid):
+        for j in range(grid):
+            a, b, c, d = v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)
+            for t in ((a, c, b), (a, d, c)):  # counter-clockwise seen from above: normals point up
+                tris[k]["body"] = 0
+                tris[k]["shape_type"] = capi.BX_SHAPE_TRIANGLE
+                tris[k]["vertex1"], tris[k]["vertex2"], tris[k]["vertex3"] = t
+                k += 1
+    fixtures = np.concatenate([tris, ground.fixtures[1:]])
+    return World(bodies, fixtures)
+
+
+def triangle_drop(n_bodies=40, n_worlds=1, seed=14, grid=6, kinds=("sphere", "capsule", "box"), bumps=0.3,
+                  extent=(8.0, 12.0, 8.0), hull_count=0):
+    """Spheres, capsules and boxes (and random hulls when hull_count > 0) dropped on a bumpy ground
+    of isolated triangles (the triangle narrowphase mix)."""
+    s = SceneSet("triangle_drop")
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    choices = None
+    if hull_count:
+        base = len(s.hulls)
+        s.hulls.extend(hull_lib.random_hull_library(hull_count, seed=seed + 1))
+        choices = np.arange(base, base + hull_count)
+        kinds = tuple(kinds) + ("hull",)
+    rng = np.random.default_rng(seed)
+    for _ in range(n_worlds):
+        s.worlds.append(triangle_ground_world(rng, n_bodies, b, grid, 12.0, bumps, kinds, extent, choices))
+    return s
+
+
 def chain_world(rng, links=10, ground_hull=0, link_hull=1, motor=True, limit=False, extra=0, origin=(0.0, 12.0, 0.0),
                 motor_speed=0.5 * np.pi, max_torque=50.0, lower=-0.25 * np.pi, upper=0.25 * np.pi):
     """A chain of `links` boxes hanging from the static ground body by revolute joints about z,

@@ -730,8 +730,11 @@ static void BuildCapsuleFaceContact(Manifold& m, const Transform& xf1, const Hul
     m.pointCount = pointCount;
 }
 
-void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2,
-                           const Shape& s2) {  // collide_hull_capsule.cpp:107-181
+// b3CollideHullAndCapsule (collide_hull_capsule.cpp:107-181) for a convex shape s1 whose polyhedron is
+// h1. b3CollideTriangleAndCapsule (collide_triangle_capsule.cpp:107-278) is the same sequence on the
+// triangle's two-face hull (b3TriangleHull) when the triangle has no edge wings.
+void CollideConvexAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Hull& h1, const Transform& xf2,
+                             const Shape& s2) {
     scalar r1 = s1.radius, r2 = s2.radius;
     scalar totalRadius = r1 + r2;
     GJKProxy p1, p2;
@@ -739,7 +742,6 @@ void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, c
     p2.Set(s2);
     GJKOutput q = GJK(xf1, p1, xf2, p2, false);
     if (q.distance > totalRadius) return;
-    const Hull& h1 = *s1.hull;
     if (q.distance > scalar(0)) {
         Vec3 c1 = q.point1, c2 = q.point2;
         Vec3 N1 = (c2 - c1) / q.distance;
@@ -770,6 +772,11 @@ void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, c
         BuildCapsuleEdgeContact(m, xf1, h1, eq.index1, xf2, s2.v1, s2.v2);
     else
         BuildCapsuleFaceContact(m, xf1, h1, fq.index, r1, xf2, s2.v1, s2.v2, r2);
+}
+
+void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2,
+                           const Shape& s2) {  // collide_hull_capsule.cpp:107-181
+    CollideConvexAndCapsule(m, xf1, s1, *s1.hull, xf2, s2);
 }
 
 }  // namespace orc

@@ -174,6 +174,9 @@ void CollideCapsules(Manifold& m, const Transform& xf1, const Shape& s1, const T
 void CollideHullAndSphere(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2);
 void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2);
 bool ConvexTestOverlap(const Transform& xfA, const Shape& sA, const Transform& xfB, const Shape& sB);
+void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Shape& t1, const Transform& xf2, const Shape& s2);
+void CollideTriangleAndCapsule(Manifold& m, const Transform& xf1, const Shape& t1, const Transform& xf2, const Shape& s2);
+void CollideHullAndTriangle(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& t2);
 
 void ClosestPointsOnSegments(Vec3& C1, Vec3& C2, const Vec3& P1, const Vec3& Q1, const Vec3& P2, const Vec3& Q2);
 Vec3 ClosestPointOnSegment(const Vec3& Q, const Vec3& A, const Vec3& B);

@@ -575,6 +575,39 @@ void CollideHulls(Manifold& m, const Transform& xf1, const Shape& s1, const Tran
     }
 }
 
+// b3CollideHulls without the feature cache (collide_hulls.cpp:189-264), as b3CollideHullAndTriangle
+// calls it on the triangle's two-face hull.
+void CollideHullsUncached(Manifold& m, const Transform& xf1, const Hull& h1, scalar r1, const Transform& xf2,
+                          const Hull& h2, scalar r2) {
+    scalar totalRadius = r1 + r2;
+    FaceQuery fq1 = QueryFaceSeparation(xf1, h1, xf2, h2);
+    if (fq1.separation > totalRadius) return;
+    FaceQuery fq2 = QueryFaceSeparation(xf2, h2, xf1, h1);
+    if (fq2.separation > totalRadius) return;
+    EdgeQuery eq = QueryEdgeSeparation(xf1, h1, xf2, h2);
+    if (eq.separation > totalRadius) return;
+    const scalar kRelEdgeTol = scalar(0.90);
+    const scalar kRelFaceTol = scalar(0.98);
+    const scalar kAbsTol = scalar(0.5) * kLinearSlop;
+    if (eq.separation > kRelEdgeTol * Max(fq1.separation, fq2.separation) + kAbsTol) {
+        BuildEdgeContact(m, xf1, eq.index1, h1, xf2, eq.index2, h2);
+    } else {
+        if (fq2.separation > kRelFaceTol * fq1.separation + kAbsTol)
+            BuildFaceContact(m, xf2, fq2.index, h2, r2, xf1, h1, r1, true);
+        else
+            BuildFaceContact(m, xf1, fq1.index, h1, r1, xf2, h2, r2, false);
+    }
+    if (m.pointCount > 0) return;  // the heuristic succeeded
+    if (eq.separation > Max(fq1.separation, fq2.separation)) {
+        BuildEdgeContact(m, xf1, eq.index1, h1, xf2, eq.index2, h2);
+    } else {
+        if (fq2.separation > fq1.separation)
+            BuildFaceContact(m, xf2, fq2.index, h2, r2, xf1, h1, r1, true);
+        else
+            BuildFaceContact(m, xf1, fq1.index, h1, r1, xf2, h2, r2, false);
+    }
+}
+
 // ------------------------------------------------------------------ shape dispatch
 void ComputeAABB(const Shape& s, const Transform& xf, AABB* out) {
     switch (s.type) {
@@ -604,6 +637,14 @@ void ComputeAABB(const Shape& s, const Transform& xf, AABB* out) {
         out->upperBound = Max(c1, c2) + r;
         break;
     }
+    case e_triangle: {  // triangle_shape.cpp:75-83: the local bounds, the transform is not applied
+        Vec3 lower = Min(s.v1, Min(s.v2, s.v3));
+        Vec3 upper = Max(s.v1, Max(s.v2, s.v3));
+        Vec3 r(s.radius, s.radius, s.radius);
+        out->lowerBound = lower - r;
+        out->upperBound = upper + r;
+        break;
+    }
     default:
         out->lowerBound.SetZero();
         out->upperBound.SetZero();
@@ -616,6 +657,11 @@ void ComputeMass(const Shape& s, scalar density, MassData* out) {
     case e_hull: HullMass(*s.hull, density, out); break;
     case e_sphere: SphereMass(s.v1, s.radius, density, out); break;
     case e_capsule: CapsuleMass(s.v1, s.v2, s.radius, density, out); break;
+    case e_triangle:  // triangle_shape.cpp:67-73
+        out->center = (s.v1 + s.v2 + s.v3) / scalar(3);
+        out->mass = 0;
+        out->I.SetZero();
+        break;
     default:
         out->mass = 0; out->center.SetZero(); out->I.SetZero();
         break;

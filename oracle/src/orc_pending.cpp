@@ -21,5 +21,4 @@ void MeshContactCollide(World&, Contact*) { Missing("mesh contacts"); }
 #include "../../include/bxstep.h"
 namespace orc {
 bool BuildMesh(Mesh&, const bx_mesh_desc&) { Missing("meshes"); }
-void InitTriangleShape(Shape&, const Vec3&, const Vec3&, const Vec3&) { Missing("triangle shapes"); }
 }  // namespace orc

@@ -243,6 +243,12 @@ void World::Collide(Contact* c) {
         CollideHullAndSphere(m, xfA, fA.shape, xfB, fB.shape);
     } else if (tA == e_hull && tB == e_capsule) {
         CollideHullAndCapsule(m, xfA, fA.shape, xfB, fB.shape);
+    } else if (tA == e_triangle && tB == e_sphere) {  // triangle_sphere_contact.cpp
+        CollideTriangleAndSphere(m, xfA, fA.shape, xfB, fB.shape);
+    } else if (tA == e_triangle && tB == e_capsule) {  // triangle_capsule_contact.cpp
+        CollideTriangleAndCapsule(m, xfA, fA.shape, xfB, fB.shape);
+    } else if (tA == e_hull && tB == e_triangle) {  // hull_triangle_contact.cpp
+        CollideHullAndTriangle(m, xfA, fA.shape, xfB, fB.shape);
     }
     c->manifoldCount = 1;  // convex_contact.cpp:44-52
 }

@@ -43,6 +43,10 @@ SCENES = {
     "chain10_limit": (lambda: scenes.chains(1, 10, motor=False, limit=True), 240, (240,)),
     "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300, (300,)),
     "mixed_world200": (lambda: scenes.mixed_worlds(1, sizes=[200], hull_count=64), 180, (90, 180)),
+    "triangles40_hulls": (lambda: scenes.triangle_drop(40, hull_count=8), 240, (120, 240)),
+    "triangles60_round": (lambda: scenes.triangle_drop(60, seed=15, kinds=("sphere", "capsule")), 240, (120, 240)),
+    "triangles50_boxes_sleep": (lambda: scenes.with_flags(scenes.triangle_drop(50, seed=16, kinds=("box",)), 1),
+                                300, (150, 300)),
 }
 
 
