@@ -342,6 +342,11 @@ static void ResetMass(Body& b, const std::vector<Fixture>& fixtures, const std::
         if (f.shapeType == BX_SHAPE_HULL) HullMass(L.View(f.shapeIndex), f.density, &md);
         else if (f.shapeType == BX_SHAPE_SPHERE) SphereMass(f.v1, f.radius, f.density, &md);
         else if (f.shapeType == BX_SHAPE_CAPSULE) CapsuleMass(f.v1, f.v2, f.radius, f.density, &md);
+        else if (f.shapeType == BX_SHAPE_TRIANGLE) {  // triangle_shape.cpp:67-73
+            md.center = (f.v1 + f.v2 + f.v3) / scalar(3);
+            md.mass = 0;
+            md.I.SetZero();
+        }
         else { md.mass = 0; md.center.SetZero(); md.I.SetZero(); }
         localCenter += md.mass * md.center;
         b.mass += md.mass;
@@ -396,6 +401,10 @@ static void HostAABB(const Fixture& f, const Transform& xf, const HostLibrary& L
         Vec3 r(f.radius, f.radius, f.radius);
         out.lowerBound = Min(c1, c2) - r;
         out.upperBound = Max(c1, c2) + r;
+    } else if (f.shapeType == BX_SHAPE_TRIANGLE) {  // triangle_shape.cpp:75-83 (local bounds)
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = Min(f.v1, Min(f.v2, f.v3)) - r;
+        out.upperBound = Max(f.v1, Max(f.v2, f.v3)) + r;
     } else {
         out.lowerBound.SetZero();
         out.upperBound.SetZero();
@@ -511,6 +520,9 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         case BX_SHAPE_SPHERE:
         case BX_SHAPE_CAPSULE:
             f.radius = fd.radius;
+            break;
+        case BX_SHAPE_TRIANGLE:  // b3TriangleShape::Set: hull skin radius, no edge wings
+            f.radius = kHullRadius;
             break;
         default:
             return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");

@@ -12,13 +12,13 @@
 
 namespace bx {
 
-// GJK proxy (gjk_proxy.h): hull vertices in global memory, or up to two inline vertices.
+// GJK proxy (gjk_proxy.h): hull vertices in memory, or up to three inline vertices.
 struct ProxyD {
     const Vec3* g;
-    Vec3 a, b;
+    Vec3 a, b, c;
     u32 count;
     scalar radius;
-    BX_HD Vec3 Get(u32 i) const { return g ? g[i] : (i == 0u ? a : b); }
+    BX_HD Vec3 Get(u32 i) const { return g ? g[i] : (i == 0u ? a : (i == 1u ? b : c)); }
     BX_HD u32 Support(const Vec3& d) const {  // gjk_proxy.h:72-86: first index of the maximum
         u32 maxIndex = 0;
         scalar maxProjection = Dot(d, Get(0));
@@ -35,12 +35,13 @@ BX_HD ProxyD MakeProxy(const Fixture& f, const HullView* h) {  // gjk.cpp:37-95
     p.g = nullptr;
     p.a = f.v1;
     p.b = f.v2;
+    p.c = f.v3;
     p.radius = f.radius;
     if (f.shapeType == 3u) {
         p.g = h->vertices;
         p.count = h->vertexCount;
-    } else {
-        p.count = f.shapeType == 1u ? 2u : 1u;
+    } else {  // sphere 1, capsule 2, triangle 3 vertices (gjk.cpp:37-95)
+        p.count = f.shapeType == 1u ? 2u : (f.shapeType == 2u ? 3u : 1u);
     }
     return p;
 }

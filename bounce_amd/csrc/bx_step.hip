@@ -19,6 +19,7 @@
 #include <algorithm>
 #include "bx_collide.h"
 #include "bx_convex.h"
+#include "bx_triangle.h"
 #include "bx_joint.h"
 #include "bx_solver.h"
 
@@ -162,6 +163,16 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
         } else if (fA.shapeType == 3u && fB.shapeType == 1u) {  // hull_capsule_contact.cpp:42-45
             HullView hA = GetHull(B, fA.shapeIndex);
             CollideHullAndCapsule(m, xfA, hA, fA, xfB, fB);
+        } else if (fA.shapeType == 2u && fB.shapeType == 0u) {  // triangle_sphere_contact.cpp
+            CollideTriangleAndSphere(m, xfA, fA, xfB, fB);
+        } else if (fA.shapeType == 2u && fB.shapeType == 1u) {  // triangle_capsule_contact.cpp
+            CollideTriangleAndCapsule(m, xfA, fA, xfB, fB);
+        } else if (fA.shapeType == 3u && fB.shapeType == 2u) {  // hull_triangle_contact.cpp
+            HullView hA = GetHull(B, fA.shapeIndex);
+            FaceScratch scratch;
+            u32 overflow = 0;
+            CollideHullAndTriangle(m, xfA, hA, fA.radius, xfB, fB, scratch, overflow);
+            if (overflow) atomicOr(&ws.status, 0x2u);
         } else {
             atomicOr(&ws.status, 0x200u);  // shape pair without a device narrowphase (meshes, triangles)
         }
@@ -271,6 +282,10 @@ __device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transfor
         Vec3 r(f.radius, f.radius, f.radius);
         out.lowerBound = Min(c1, c2) - r;
         out.upperBound = Max(c1, c2) + r;
+    } else if (f.shapeType == 2u) {  // triangle_shape.cpp:75-83: local bounds, the transform is unused
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = Min(f.v1, Min(f.v2, f.v3)) - r;
+        out.upperBound = Max(f.v1, Max(f.v2, f.v3)) + r;
     } else {
         out.lowerBound.SetZero();
         out.upperBound.SetZero();

@@ -41,6 +41,9 @@ SCENES = {
     "chain10_motor": (lambda: scenes.chains(1, 10), 240),
     "chain10_free": (lambda: scenes.chains(1, 10, motor=False), 150),
     "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300),
+    "triangles40_hulls": (lambda: scenes.triangle_drop(40, hull_count=8), 240),
+    "triangles60_round": (lambda: scenes.triangle_drop(60, seed=15, kinds=("sphere", "capsule")), 240),
+    "triangles50_boxes_sleep": (lambda: scenes.with_flags(scenes.triangle_drop(50, seed=16, kinds=("box",)), 1), 300),
 }
 
 
@@ -105,6 +108,25 @@ def test_batched_mixed_shapes_match_single(gpu_ctx):
     batch = device.Batch(gpu_ctx, scene)
     refs = [CpuWorld("oracle", scene, w) for w in range(16)]
     for _ in range(120):
+        batch.step()
+        for r in refs:
+            r.step()
+    for w, r in enumerate(refs):
+        diffs = compare_worlds(DeviceWorld(batch, w), r)
+        assert not diffs, f"world {w}: {diffs}"
+        assert batch.counts(w).status == 0
+
+
+def test_batched_triangle_grounds_match_single(gpu_ctx):
+    """Six triangle-ground worlds (spheres, capsules, boxes and random hulls on isolated triangles)
+    in one launch equal the oracle world by world."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    scene = scenes.triangle_drop(25, n_worlds=6, seed=31, hull_count=8)
+    print("[parity] batched triangle grounds x6: 150 steps", flush=True)
+    batch = device.Batch(gpu_ctx, scene)
+    refs = [CpuWorld("oracle", scene, w) for w in range(6)]
+    for _ in range(150):
         batch.step()
         for r in refs:
             r.step()
