@@ -376,6 +376,60 @@ def triangle_drop(n_bodies=40, n_worlds=1, seed=14, grid=6, kinds=("sphere", "ca
     return s
 
 
+def grid_mesh(rng, cells=8, size=12.0, bumps=0.3):
+    """A heightfield triangle mesh: (cells + 1)^2 vertices over a size x size square at heights
+    U[-bumps, bumps], two counter-clockwise (upward-facing) triangles per cell."""
+    xs = np.linspace(-size / 2, size / 2, cells + 1)
+    h = rng.uniform(-bumps, bumps, (cells + 1, cells + 1))
+    v = np.zeros(((cells + 1) * (cells + 1), 3), dtype=F32)
+    for i in range(cells + 1):
+        for j in range(cells + 1):
+            v[i * (cells + 1) + j] = (xs[i], h[i, j], xs[j])
+    t = []
+    for i in range(cells):
+        for j in range(cells):
+            a, b = i * (cells + 1) + j, (i + 1) * (cells + 1) + j
+            c, d = (i + 1) * (cells + 1) + j + 1, i * (cells + 1) + j + 1
+            t.append((a, c, b))
+            t.append((a, d, c))
+    return v, np.array(t, dtype=np.uint32)
+
+
+def mesh_ground_world(rng, n_bodies, box_hull, mesh_index, kinds=("sphere", "capsule", "box"), extent=(8.0, 12.0, 8.0),
+                      hull_choices=None, scale=(1.0, 1.0, 1.0), sensors=0.0):
+    """Bodies of the given kinds (as in mixed_drop_world) dropped on a static mesh fixture
+    (b3MeshShape with the given scale) at the origin."""
+    w = mixed_drop_world(rng, n_bodies, 0, box_hull, kinds, extent, sensors, hull_choices)
+    w.bodies[0]["position"] = (0.0, 0.0, 0.0)
+    f = w.fixtures[0]
+    f["shape_type"] = capi.BX_SHAPE_MESH
+    f["shape_index"] = mesh_index
+    f["mesh_scale"] = [F32(x) for x in scale]
+    f["is_sensor"] = 0
+    w.fixtures[0] = f
+    return w
+
+
+def mesh_drop(n_bodies=40, n_worlds=1, seed=24, cells=8, kinds=("sphere", "capsule", "box"), bumps=0.3,
+              extent=(8.0, 12.0, 8.0), hull_count=0, scale=(1.0, 1.0, 1.0), sensors=0.0):
+    """Spheres, capsules and boxes (and random hulls when hull_count > 0) dropped on a bumpy
+    heightfield mesh (the mesh-contact mix: static tree query, per-triangle narrowphase with edge
+    wings, clustering into at most three manifolds)."""
+    s = SceneSet("mesh_drop")
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    choices = None
+    if hull_count:
+        base = len(s.hulls)
+        s.hulls.extend(hull_lib.random_hull_library(hull_count, seed=seed + 1))
+        choices = np.arange(base, base + hull_count)
+        kinds = tuple(kinds) + ("hull",)
+    rng = np.random.default_rng(seed)
+    s.meshes.append(grid_mesh(rng, cells, 12.0, bumps))
+    for _ in range(n_worlds):
+        s.worlds.append(mesh_ground_world(rng, n_bodies, b, 0, kinds, extent, choices, scale, sensors))
+    return s
+
+
 def chain_world(rng, links=10, ground_hull=0, link_hull=1, motor=True, limit=False, extra=0, origin=(0.0, 12.0, 0.0),
                 motor_speed=0.5 * np.pi, max_torque=50.0, lower=-0.25 * np.pi, upper=0.25 * np.pi):
     """A chain of `links` boxes hanging from the static ground body by revolute joints about z,

@@ -377,6 +377,11 @@ GJKOutput GJK(const Transform& xf1, const GJKProxy& proxy1, const Transform& xf2
     simplex.ClosestPoints(&out.point1, &out.point2);
     out.distance = Distance(out.point1, out.point2);
     out.iterations = iter;
+    out.simplexCount = simplex.count;
+    for (u32 i = 0; i < simplex.count; ++i) {
+        out.simplexIndex1[i] = simplex.v[i].index1;
+        out.simplexIndex2[i] = simplex.v[i].index2;
+    }
     if (applyRadius) {
         scalar r1 = proxy1.radius, r2 = proxy2.radius;
         if (out.distance > r1 + r2 && out.distance > kEpsilon) {
@@ -733,6 +738,44 @@ static void BuildCapsuleFaceContact(Manifold& m, const Transform& xf1, const Hul
 // b3CollideHullAndCapsule (collide_hull_capsule.cpp:107-181) for a convex shape s1 whose polyhedron is
 // h1. b3CollideTriangleAndCapsule (collide_triangle_capsule.cpp:107-278) is the same sequence on the
 // triangle's two-face hull (b3TriangleHull) when the triangle has no edge wings.
+// b3UniqueCount (math.h:84-104)
+static u32 UniqueCount(const u32* v, u32 n) {
+    u32 count = 0;
+    for (u32 i = 0; i < n; ++i) {
+        u32 j;
+        for (j = 0; j < n; ++j)
+            if (v[i] == v[j]) break;
+        if (i == j) ++count;
+    }
+    return count;
+}
+
+// The proxy-1 side of b3GetFeaturePair (gjk_feature_pair.cpp:21-208) for the final simplex of q:
+// returns count1 and, when it is 2, the two vertex indices of the edge on proxy 1.
+static u32 FeatureCount1(const GJKOutput& q, u32& e0, u32& e1) {
+    u32 n = q.simplexCount;
+    const u32* i1 = q.simplexIndex1;
+    const u32* i2 = q.simplexIndex2;
+    u32 u1 = UniqueCount(i1, n), u2 = UniqueCount(i2, n);
+    if (n == 1) return 1;
+    if (n == 2) {
+        if ((u1 == 2 && u2 == 2) || (u1 != 1 && u2 == 1)) { e0 = i1[0]; e1 = i1[1]; return 2; }
+        if (u1 == 1) return 1;
+        return 0;
+    }
+    if (n == 3) {
+        if (u1 == 1 && u2 == 3) return 1;
+        if (u1 == 3 && u2 == 1) return 3;
+        if (u1 == 2 && (u2 == 2 || u2 == 3)) {
+            e0 = i1[0];
+            e1 = i1[0] == i1[1] ? i1[2] : i1[1];
+            return 2;
+        }
+        if (u1 == 3) return 3;
+    }
+    return 0;
+}
+
 void CollideConvexAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Hull& h1, const Transform& xf2,
                              const Shape& s2) {
     scalar r1 = s1.radius, r2 = s2.radius;
@@ -754,6 +797,40 @@ void CollideConvexAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1,
         if (L < kTol * kTol) {
             BuildCapsuleFaceContact(m, xf1, h1, index1, r1, xf2, s2.v1, s2.v2, r2);
             if (m.pointCount == 2) return;
+        }
+        if (s1.type == e_triangle && (s1.hasE1 || s1.hasE2 || s1.hasE3)) {
+            // collide_triangle_capsule.cpp:162-241: the closest feature is a triangle edge with a wing
+            u32 v1 = 0, v2 = 0;
+            if (FeatureCount1(q, v1, v2) == 2) {
+                const Vec3 vertices[3] = {s1.v1, s1.v2, s1.v3};
+                const bool hasWing[3] = {s1.hasE1, s1.hasE2, s1.hasE3};
+                const Vec3 wings[3] = {s1.e1, s1.e2, s1.e3};
+                u32 edgeIndex = 0;
+                for (u32 i = 0; i < 3; ++i) {
+                    u32 j = i + 1 < 3 ? i + 1 : 0;
+                    if ((v1 == i && v2 == j) || (v2 == i && v1 == j)) { edgeIndex = i; break; }
+                }
+                if (hasWing[edgeIndex]) {
+                    u32 ev1 = edgeIndex;
+                    u32 ev2 = edgeIndex + 1 < 3 ? edgeIndex + 1 : 0;
+                    Vec3 Q = MulT(xf1, c2);
+                    Vec3 A = wings[edgeIndex], B = vertices[ev2], C = vertices[ev1];
+                    scalar wABC[4];
+                    Barycentric3(wABC, A, B, C, Q);
+                    if (wABC[0] > scalar(0) && wABC[1] > scalar(0) && wABC[2] > scalar(0)) return;
+                    Vec3 center = (A + B + C) / scalar(3);
+                    Plane plane = h1.planes[0];
+                    scalar distance = Distance(center, plane);
+                    const scalar kCoplanarTol = scalar(0.005);
+                    if (distance > -kCoplanarTol && distance < kCoplanarTol) {
+                        Vec3 n = plane.normal;
+                        if (Dot(n, localN1) < scalar(0)) n = -n;
+                        Vec3 nc2 = c1 + q.distance * Mul(xf1.rotation, n);
+                        localN1 = n;
+                        c2 = nc2;
+                    }
+                }
+            }
         }
         m.pointCount = 1;
         m.points[0].localNormal1 = localN1;

@@ -162,7 +162,8 @@ struct GJKProxy {
     void Set(const Shape& s);
     u32 GetSupportIndex(const Vec3& d) const;
 };
-struct GJKOutput { Vec3 point1, point2; scalar distance; u32 iterations; };
+// simplexCount / simplexIndex1: the final simplex (the cache b3GJK writes, gjk.cpp:781, 868-876)
+struct GJKOutput { Vec3 point1, point2; scalar distance; u32 iterations; u32 simplexCount; u32 simplexIndex1[4]; u32 simplexIndex2[4]; };
 GJKOutput GJK(const Transform& xf1, const GJKProxy& p1, const Transform& xf2, const GJKProxy& p2, bool applyRadius);
 
 // Narrowphase (collide/*.cpp)
@@ -174,6 +175,10 @@ void CollideCapsules(Manifold& m, const Transform& xf1, const Shape& s1, const T
 void CollideHullAndSphere(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2);
 void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2);
 bool ConvexTestOverlap(const Transform& xfA, const Shape& sA, const Transform& xfB, const Shape& sB);
+// b3ClusterVertex and b3ReducePolygon (cluster.h, cluster.cpp:23-200)
+struct ClusterVertex { Vec3 position; u32 clipIndex; };
+typedef std::vector<ClusterVertex> ClusterPolygon;
+void ReducePolygon(ClusterPolygon& out, const ClusterPolygon& in, const Vec3& N, u32 initialPoint);
 void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Shape& t1, const Transform& xf2, const Shape& s2);
 void CollideTriangleAndCapsule(Manifold& m, const Transform& xf1, const Shape& t1, const Transform& xf2, const Shape& s2);
 void CollideHullAndTriangle(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& t2);

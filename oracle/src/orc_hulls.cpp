@@ -313,8 +313,6 @@ static void ClipPolygonToFaceSidePlanes(ClipPolygon& out, const ClipPolygon& in,
 }
 
 // ------------------------------------------------------------------ reduction (cluster.cpp:23-200)
-struct ClusterVertex { Vec3 position; u32 clipIndex; };
-typedef std::vector<ClusterVertex> ClusterPolygon;
 
 static inline bool IsCCW(const Vec3& A, const Vec3& B, const Vec3& C, const Vec3& N) {
     Vec3 n = Cross(B - A, C - A);
@@ -635,6 +633,11 @@ void ComputeAABB(const Shape& s, const Transform& xf, AABB* out) {
         Vec3 r(s.radius, s.radius, s.radius);
         out->lowerBound = Min(c1, c2) - r;
         out->upperBound = Max(c1, c2) + r;
+        break;
+    }
+    case e_mesh: {  // mesh_shape.cpp:44-50
+        void MeshAABB(const Shape& s, const Transform& xf, AABB* out);
+        MeshAABB(s, xf, out);
         break;
     }
     case e_triangle: {  // triangle_shape.cpp:75-83: the local bounds, the transform is not applied

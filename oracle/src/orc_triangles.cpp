@@ -5,7 +5,8 @@
 //   triangle_hull.h:24-72           b3TriangleHull: the triangle as a two-face hull
 //   geometry.h:27-94                b3BarycentricCoordinates (segment and triangle forms)
 //   collide_triangle_sphere.cpp:24-296
-//   collide_triangle_capsule.cpp:107-278 (triangles without edge wings)
+//   collide_triangle_capsule.cpp:107-278 (the hull-capsule sequence on the triangle hull, with the
+//                                   edge-wing step in CollideConvexAndCapsule)
 //   collide_hull_triangle.cpp:24-124
 #include <cstdio>
 #include <cstdlib>
@@ -137,10 +138,6 @@ void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Shape& t1
 
 void CollideTriangleAndCapsule(Manifold& m, const Transform& xf1, const Shape& t1, const Transform& xf2,
                                const Shape& s2) {  // collide_triangle_capsule.cpp:107-278
-    if (t1.hasE1 || t1.hasE2 || t1.hasE3) {
-        std::fprintf(stderr, "oracle: triangle-capsule with edge wings is not restated yet\n");
-        std::abort();
-    }
     Hull h;
     MakeTriangleHull(h, t1.v1, t1.v2, t1.v3);
     CollideConvexAndCapsule(m, xf1, t1, h, xf2, s2);

@@ -23,6 +23,9 @@ bool SolveJointsPosition(World& w, std::vector<u32>& joints, Position* pos);
 void MeshContactFindPairs(World& w, Contact* c);
 void MeshContactSynchronize(World& w, Contact* c);
 void MeshContactCollide(World& w, Contact* c);
+void MeshContactCreate(World& w, Contact* c);
+bool MeshContactTestOverlap(World& w, Contact* c);
+void MeshAABB(const Shape& s, const Transform& xf, AABB* out);
 
 World::~World() {
     for (Contact* c : contactList) delete c;
@@ -180,7 +183,7 @@ void World::AddPair(u32 fA, u32 fB) {  // contact_manager.cpp:38-141
     contactList.insert(contactList.begin(), c);
     fixtures[fA].contacts.insert(fixtures[fA].contacts.begin(), c);
     fixtures[fB].contacts.insert(fixtures[fB].contacts.begin(), c);
-    if (c->isMesh) MeshContactSynchronize(*this, c);
+    if (c->isMesh) MeshContactCreate(*this, c);
 }
 
 void World::FindNewContacts() {  // contact_manager.cpp:153-163, broad_phase.h:159-207
@@ -262,7 +265,7 @@ void World::UpdateContact(Contact* c) {  // contact.cpp:183-285
     Body& bA = bodies[fA.body];
     Body& bB = bodies[fB.body];
     if (sensor) {
-        touching = ConvexTestOverlap(bA.xf, fA.shape, bB.xf, fB.shape);
+        touching = c->isMesh ? MeshContactTestOverlap(*this, c) : ConvexTestOverlap(bA.xf, fA.shape, bB.xf, fB.shape);
         c->manifoldCount = 0;
     } else {
         u32 oldCount = c->manifoldCount;

@@ -47,6 +47,12 @@ SCENES = {
     "triangles60_round": (lambda: scenes.triangle_drop(60, seed=15, kinds=("sphere", "capsule")), 240, (120, 240)),
     "triangles50_boxes_sleep": (lambda: scenes.with_flags(scenes.triangle_drop(50, seed=16, kinds=("box",)), 1),
                                 300, (150, 300)),
+    "mesh40_round": (lambda: scenes.mesh_drop(40, kinds=("sphere", "capsule")), 240, (120, 240)),
+    "mesh40_boxes": (lambda: scenes.mesh_drop(40, seed=25, kinds=("box",)), 240, (120, 240)),
+    "mesh40_hulls": (lambda: scenes.mesh_drop(40, seed=26, hull_count=8), 240, (120, 240)),
+    "mesh30_scaled": (lambda: scenes.mesh_drop(30, seed=27, scale=(1.5, 0.5, 1.2)), 240, (240,)),
+    "mesh40_sensors": (lambda: scenes.mesh_drop(40, seed=28, sensors=0.3), 240, (240,)),
+    "mesh50_sleep": (lambda: scenes.with_flags(scenes.mesh_drop(50, seed=29, hull_count=4), 1), 300, (150, 300)),
 }
 
 
