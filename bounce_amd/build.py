@@ -14,7 +14,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIB_DIR, "libbxstep.so")
 SOURCES = ["bx_step.hip", "bx_api.hip"]
-HEADERS = ["bx_math.h", "bx_types.h", "bx_collide.h", "bx_convex.h", "bx_triangle.h", "bx_joint.h", "bx_solver.h", "bx_shapes.h", "bx_islands.h",
+HEADERS = ["bx_math.h", "bx_types.h", "bx_collide.h", "bx_convex.h", "bx_triangle.h", "bx_mesh.h", "bx_joint.h", "bx_solver.h", "bx_shapes.h", "bx_islands.h",
            "bx_solve_kernels.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -114,7 +114,7 @@ class bx_step_cfg(C.Structure):
 
 class bx_batch_opts(C.Structure):
     _fields_ = [("max_contacts_per_world", u32), ("max_pairs_per_world", u32), ("solver_mode", u32),
-                ("reserved", _arr(u32, 5))]
+                ("mesh_triangles_per_contact", u32), ("reserved", _arr(u32, 4))]
 
 
 class bx_pool_opts(C.Structure):

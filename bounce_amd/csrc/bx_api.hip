@@ -75,6 +75,10 @@ enum Space : u32 {
     kSpacePair,        // pairBase: pairCap units
     kSpaceIslandBody,  // islandBodyBase: islandBodyCap units
     kSpaceScratch,     // islandScratchBase: island-builder words at full capacity
+    kSpaceCon,         // conBase: island constraint entries (3 x contactCap with meshes, else contactCap)
+    kSpaceMesh,        // meshBase: mesh contact records, contactCap units in worlds with a mesh fixture
+    kSpaceMeshTri,     // meshTriBase: contactCap x meshTriCap triangle indices
+    kSpaceMeshScratch, // meshScratchBase: clustering stripes of the world's kBlock lanes
     kSpaces
 };
 
@@ -86,6 +90,10 @@ static void WorldUnits(const WorldInfo& wi, u32 u[kSpaces]) {
     u[kSpacePair] = wi.pairCap;
     u[kSpaceIslandBody] = wi.islandBodyCap;
     u[kSpaceScratch] = wi.islandScratchWords;
+    u[kSpaceCon] = wi.conCap;
+    u[kSpaceMesh] = wi.meshTriCap ? wi.contactCap : 0u;
+    u[kSpaceMeshTri] = wi.meshTriCap ? wi.contactCap * wi.meshTriCap : 0u;
+    u[kSpaceMeshScratch] = wi.meshTriCap ? kMeshLanes * MeshLaneStride(wi.meshTriCap) : 0u;
 }
 
 static void SetBases(WorldInfo& wi, const u32 base[kSpaces]) {
@@ -97,6 +105,10 @@ static void SetBases(WorldInfo& wi, const u32 base[kSpaces]) {
     wi.pairBase = base[kSpacePair];
     wi.islandBodyBase = base[kSpaceIslandBody];
     wi.islandScratchBase = base[kSpaceScratch];
+    wi.conBase = base[kSpaceCon];
+    wi.meshBase = base[kSpaceMesh];
+    wi.meshTriBase = base[kSpaceMeshTri];
+    wi.meshScratchBase = base[kSpaceMeshScratch];
 }
 
 static void GetBases(const WorldInfo& wi, u32 base[kSpaces]) {
@@ -107,6 +119,10 @@ static void GetBases(const WorldInfo& wi, u32 base[kSpaces]) {
     base[kSpacePair] = wi.pairBase;
     base[kSpaceIslandBody] = wi.islandBodyBase;
     base[kSpaceScratch] = wi.islandScratchBase;
+    base[kSpaceCon] = wi.conBase;
+    base[kSpaceMesh] = wi.meshBase;
+    base[kSpaceMeshTri] = wi.meshTriBase;
+    base[kSpaceMeshScratch] = wi.meshScratchBase;
 }
 
 // Every per-world device array: its BatchView member, element size, space and elements per unit.
@@ -124,8 +140,10 @@ static const ArraySpec kArrays[] = {
     BX_ARRAY(jointEdges, kSpaceJoint, 2),
     BX_ARRAY(contacts, kSpaceContact, 1), BX_ARRAY(contactOrder, kSpaceContact, 1),
     BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1),
-    BX_ARRAY(islandContacts, kSpaceContact, 1), BX_ARRAY(velCons, kSpaceContact, 1),
-    BX_ARRAY(posCons, kSpaceContact, 1), BX_ARRAY(edgeSlots, kSpaceContact, 2),
+    BX_ARRAY(edgeSlots, kSpaceContact, 2),
+    BX_ARRAY(islandContacts, kSpaceCon, 1), BX_ARRAY(velCons, kSpaceCon, 1), BX_ARRAY(posCons, kSpaceCon, 1),
+    BX_ARRAY(meshContacts, kSpaceMesh, 1), BX_ARRAY(meshTriLists, kSpaceMeshTri, 1),
+    BX_ARRAY(meshScratch, kSpaceMeshScratch, 1),
     BX_ARRAY(pairs, kSpacePair, 1),
     BX_ARRAY(islandBodies, kSpaceIslandBody, 1), BX_ARRAY(velBodies, kSpaceIslandBody, 1),
     BX_ARRAY(posBodies, kSpaceIslandBody, 1),
@@ -254,6 +272,10 @@ struct HostLibrary {
     std::vector<HalfEdge> edges;
     std::vector<u32> faces;
     std::vector<Plane> planes;
+    std::vector<MeshInfo> meshes;
+    std::vector<Vec3> meshVertices;
+    std::vector<MeshTri> meshTris;
+    std::vector<TreeNode> meshNodes;
     HullView View(u32 i) const {
         const HullInfo& hi = info[i];
         HullView h;
@@ -268,6 +290,96 @@ struct HostLibrary {
         return h;
     }
 };
+
+// b3StaticTree::Build (static_tree.cpp:40-170) over the triangle AABBs (mesh.h:80-96) of one mesh,
+// appended to L.meshNodes: nodes in pre-order (a node, its first child's subtree, then its second
+// child's), each interior node splitting its triangles, sorted by AABB centre along the longest
+// axis of its AABB with std::sort, at the first centre beyond the AABB centre (at the middle when
+// that leaves one side empty). Returns the depth of the deepest node.
+static u32 BuildMeshTree(const std::vector<AABB>& boxes, std::vector<TreeNode>& nodes) {
+    const u32 n = u32(boxes.size());
+    std::vector<u32> idx(n);
+    for (u32 i = 0; i < n; ++i) idx[i] = i;
+    struct Job { u32 first, count, parent, side, depth; };
+    std::vector<Job> jobs{{0u, n, kNull, 0u, 0u}};
+    const u32 base = u32(nodes.size());
+    u32 maxDepth = 0;
+    while (!jobs.empty()) {
+        Job j = jobs.back();
+        jobs.pop_back();
+        const u32 id = u32(nodes.size()) - base;
+        nodes.push_back(TreeNode{});
+        if (j.parent != kNull) {
+            TreeNode& p = nodes[base + j.parent];
+            (j.side == 1u ? p.child1 : p.child2) = id;
+        }
+        maxDepth = std::max(maxDepth, j.depth);
+        u32* r = idx.data() + j.first;
+        TreeNode& node = nodes.back();
+        if (j.count <= 1) {
+            node.aabb = boxes[r[0]];
+            node.child1 = node.child2 = kNull;
+            node.index = r[0];
+            continue;
+        }
+        AABB a = boxes[r[0]];
+        for (u32 i = 1; i < j.count; ++i) a = Combine(a, boxes[r[i]]);
+        node.aabb = a;
+        node.index = kNull;
+        Vec3 d = a.upperBound - a.lowerBound;  // b3AABB longest axis (aabb.h:164-184)
+        u32 axis = 0;
+        scalar longest = d.x;
+        if (d.y > longest) { longest = d.y; axis = 1; }
+        if (d.z > longest) axis = 2;
+        auto coord = [&](u32 t) { Vec3 c = boxes[t].GetCenter(); return axis == 0 ? c.x : (axis == 1 ? c.y : c.z); };
+        std::sort(r, r + j.count, [&](u32 t1, u32 t2) { return coord(t1) < coord(t2); });
+        Vec3 ac = a.GetCenter();
+        scalar split = axis == 0 ? ac.x : (axis == 1 ? ac.y : ac.z);
+        u32 k = 0;
+        while (k < j.count && !(coord(r[k]) > split)) ++k;
+        if (k == 0 || k == j.count) k = j.count / 2;
+        jobs.push_back(Job{j.first + k, j.count - k, id, 2u, j.depth + 1});  // built after the first child
+        jobs.push_back(Job{j.first, k, id, 1u, j.depth + 1});
+    }
+    return maxDepth;
+}
+
+// b3Mesh::BuildAdjacency (mesh.cpp:35-112): for each triangle in order and each of its edges whose
+// wing is unset, the first later triangle (and its first edge) sharing the edge in either
+// direction sets both wings: this edge's to the other triangle's vertex off the edge, and the
+// other edge's (overwriting it) to this triangle's. Found through a sorted edge list instead of a
+// scan over the later triangles.
+static void BuildMeshWings(MeshTri* t, u32 n) {
+    struct EdgeRef { u64 key; u32 tri, edge; };
+    auto vert = [&](const MeshTri& x, u32 i) { return i == 0 ? x.v1 : (i == 1 ? x.v2 : x.v3); };
+    auto wing = [&](MeshTri& x, u32 i) -> u32& { return i == 0 ? x.u1 : (i == 1 ? x.u2 : x.u3); };
+    std::vector<EdgeRef> refs;
+    refs.reserve(size_t(3) * n);
+    for (u32 i = 0; i < n; ++i) {
+        t[i].u1 = t[i].u2 = t[i].u3 = kNull;
+        for (u32 e = 0; e < 3; ++e) {
+            u32 a = vert(t[i], e), b = vert(t[i], e + 1 < 3 ? e + 1 : 0);
+            refs.push_back(EdgeRef{(u64(std::min(a, b)) << 32) | std::max(a, b), i, e});
+        }
+    }
+    auto less = [](const EdgeRef& x, const EdgeRef& y) {
+        return x.key != y.key ? x.key < y.key : (x.tri != y.tri ? x.tri < y.tri : x.edge < y.edge);
+    };
+    std::sort(refs.begin(), refs.end(), less);
+    for (u32 i = 0; i < n; ++i) {
+        for (u32 e = 0; e < 3; ++e) {
+            if (wing(t[i], e) != kNull) continue;
+            u32 a = vert(t[i], e), b = vert(t[i], e + 1 < 3 ? e + 1 : 0);
+            EdgeRef probe{(u64(std::min(a, b)) << 32) | std::max(a, b), i + 1, 0};
+            auto it = std::lower_bound(refs.begin(), refs.end(), probe, less);
+            if (it == refs.end() || it->key != probe.key) continue;
+            u32 e3 = e + 2 < 3 ? e + 2 : e - 1;            // vertex of triangle i off edge e
+            u32 f3 = it->edge + 2 < 3 ? it->edge + 2 : it->edge - 1;
+            wing(t[it->tri], it->edge) = vert(t[i], e3);
+            wing(t[i], e) = vert(t[it->tri], f3);
+        }
+    }
+}
 
 static int BuildLibrary(const bx_shape_library* lib, HostLibrary& L) {
     if (!lib) return BX_OK;
@@ -314,7 +426,40 @@ static int BuildLibrary(const bx_shape_library* lib, HostLibrary& L) {
             if (d.edges[5 * k + 1] != (k ^ 1u)) return Fail(BX_E_INVALID, "hull twins must be adjacent half-edges");
         L.info.push_back(hi);
     }
-    if (lib->mesh_count > 0) return Fail(BX_E_UNSUPPORTED, "meshes are not supported by the device path yet");
+    if (lib->mesh_count && !lib->meshes) return Fail(BX_E_INVALID, "null mesh array");
+    for (u32 i = 0; i < lib->mesh_count; ++i) {
+        const bx_mesh_desc& d = lib->meshes[i];
+        if (d.vertex_count == 0 || d.triangle_count == 0 || !d.vertices || !d.triangles)
+            return Fail(BX_E_INVALID, "mesh " + std::to_string(i) + ": empty");
+        if (d.triangle_count > (1u << 26)) return Fail(BX_E_INVALID, "mesh " + std::to_string(i) + ": too many triangles");
+        MeshInfo mi;
+        std::memset(&mi, 0, sizeof(mi));
+        mi.vBase = u32(L.meshVertices.size());
+        mi.vCount = d.vertex_count;
+        mi.tBase = u32(L.meshTris.size());
+        mi.tCount = d.triangle_count;
+        mi.nBase = u32(L.meshNodes.size());
+        mi.root = 0;  // pre-order
+        for (u32 k = 0; k < d.vertex_count; ++k) L.meshVertices.push_back(V3(d.vertices + 3 * k));
+        std::vector<AABB> boxes(d.triangle_count);
+        for (u32 k = 0; k < d.triangle_count; ++k) {
+            MeshTri t;
+            t.v1 = d.triangles[3 * k];
+            t.v2 = d.triangles[3 * k + 1];
+            t.v3 = d.triangles[3 * k + 2];
+            if (t.v1 >= d.vertex_count || t.v2 >= d.vertex_count || t.v3 >= d.vertex_count)
+                return Fail(BX_E_INVALID, "mesh " + std::to_string(i) + ": vertex index out of range");
+            Vec3 a = V3(d.vertices + 3 * t.v1), b = V3(d.vertices + 3 * t.v2), c = V3(d.vertices + 3 * t.v3);
+            boxes[k].lowerBound = Min(a, Min(b, c));  // b3Mesh::GetTriangleAABB (mesh.h:80-96)
+            boxes[k].upperBound = Max(a, Max(b, c));
+            boxes[k].Extend(kLinearSlop);
+            L.meshTris.push_back(t);
+        }
+        BuildMeshWings(L.meshTris.data() + mi.tBase, d.triangle_count);
+        u32 depth = BuildMeshTree(boxes, L.meshNodes);
+        if (depth + 2 > kTreeStack) return Fail(BX_E_INVALID, "mesh " + std::to_string(i) + ": tree too deep");
+        L.meshes.push_back(mi);
+    }
     return BX_OK;
 }
 
@@ -405,6 +550,16 @@ static void HostAABB(const Fixture& f, const Transform& xf, const HostLibrary& L
         Vec3 r(f.radius, f.radius, f.radius);
         out.lowerBound = Min(f.v1, Min(f.v2, f.v3)) - r;
         out.upperBound = Max(f.v1, Max(f.v2, f.v3)) + r;
+    } else if (f.shapeType == BX_SHAPE_MESH) {  // mesh_shape.cpp:44-50
+        const MeshInfo& mi = L.meshes[f.shapeIndex];
+        const Vec3* v = L.meshVertices.data() + mi.vBase;
+        out.lowerBound = out.upperBound = Mul(xf, Mul(f.v1, v[0]));
+        for (u32 i = 1; i < mi.vCount; ++i) {
+            Vec3 p = Mul(xf, Mul(f.v1, v[i]));
+            out.lowerBound = Min(out.lowerBound, p);
+            out.upperBound = Max(out.upperBound, p);
+        }
+        out.Extend(f.radius);
     } else {
         out.lowerBound.SetZero();
         out.upperBound.SetZero();
@@ -434,6 +589,35 @@ static int GetState(bx_batch* b, u32 world, WorldState& ws) {
 // more sets status bit 0x2 and the world no longer follows the reference.
 static const u32 kContactsPerFixture = 16, kContactSlack = 64;
 static u64 DefaultContactCap(u32 fixtures) { return u64(kContactsPerFixture) * fixtures + kContactSlack; }
+
+// Triangles a mesh contact holds by default: the fat AABB of a shape of about a metre over a
+// heightfield of 1 m cells, moving a few metres per step, overlaps well under this many.
+static const u32 kMeshTrianglesPerContact = 64;
+
+// The sizes of a world that follow from its counts and the options: contact slots, pairs, island
+// entries, island-builder words, and for a world with a mesh fixture the mesh contact records and
+// three constraint entries per contact slot (a mesh contact has up to three manifolds).
+static int SetCapacities(WorldInfo& wi, bool hasMesh, const bx_batch_opts* o) {
+    u64 cap = o && o->max_contacts_per_world ? o->max_contacts_per_world : DefaultContactCap(wi.fixtureCount);
+    u64 pcap = o && o->max_pairs_per_world ? o->max_pairs_per_world : 2 * cap;
+    u64 tcap = hasMesh ? (o && o->mesh_triangles_per_contact ? o->mesh_triangles_per_contact : kMeshTrianglesPerContact) : 0;
+    if (cap > (1u << 24) || pcap > (1u << 25)) return Fail(BX_E_INVALID, "contact capacity too large");
+    if (tcap > (1u << 20) || cap * tcap > 0x7FFFFFFFull || u64(kMeshLanes) * MeshLaneStride(u32(tcap)) > 0x7FFFFFFFull)
+        return Fail(BX_E_INVALID, "mesh triangle capacity too large");
+    wi.contactCap = u32(cap);
+    wi.pairCap = u32(pcap);
+    wi.islandBodyCap = wi.bodyCount + wi.contactCap;
+    wi.islandScratchWords = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+    wi.conCap = hasMesh ? kMaxManifolds * wi.contactCap : wi.contactCap;
+    wi.meshTriCap = u32(tcap);
+    return BX_OK;
+}
+
+static bool HasMesh(const bx_world_desc& d) {
+    for (u32 i = 0; i < d.fixture_count; ++i)
+        if (d.fixtures[i].shape_type == BX_SHAPE_MESH) return true;
+    return false;
+}
 
 // One world built on the host from its descriptor. Every index in these arrays is world-local;
 // `info` holds sizes, capacities and flags (its bases are set when the world is placed).
@@ -523,6 +707,15 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
             break;
         case BX_SHAPE_TRIANGLE:  // b3TriangleShape::Set: hull skin radius, no edge wings
             f.radius = kHullRadius;
+            break;
+        case BX_SHAPE_MESH:  // b3MeshShape: the mesh by index, its scale in v1, hull skin radius
+            if (fd.shape_index >= L.meshes.size()) return Fail(BX_E_INVALID, "mesh index out of range");
+            if (fd.mesh_scale[0] == 0.0f || fd.mesh_scale[1] == 0.0f || fd.mesh_scale[2] == 0.0f)
+                return Fail(BX_E_INVALID, "mesh scale must be nonzero");
+            f.radius = kHullRadius;
+            f.v1 = V3(fd.mesh_scale);
+            f.v2.SetZero();
+            f.v3.SetZero();
             break;
         default:
             return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");
@@ -642,13 +835,9 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
     }
     // capacities
-    u64 cap = opts && opts->max_contacts_per_world ? opts->max_contacts_per_world : DefaultContactCap(d.fixture_count);
-    u64 pcap = opts && opts->max_pairs_per_world ? opts->max_pairs_per_world : 2 * cap;
-    if (cap > (1u << 24) || pcap > (1u << 25)) return Fail(BX_E_INVALID, "contact capacity too large");
-    wi.contactCap = u32(cap);
-    wi.pairCap = u32(pcap);
-    wi.islandBodyCap = d.body_count + wi.contactCap;
-    wi.islandScratchWords = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+    int rc = SetCapacities(wi, HasMesh(d), opts);
+    if (rc) return rc;
+    const u32 cap = wi.contactCap;
     hw.freeSlots.resize(cap);
     for (u32 k = 0; k < cap; ++k) hw.freeSlots[k] = u32(cap) - 1 - k;
     std::memset(&hw.state, 0, sizeof(hw.state));
@@ -708,6 +897,18 @@ static int UploadLibrary(bx_batch* b, const HostLibrary& L) {
     Plane* dp;
     if ((rc = DevAlloc(b, &dp, L.planes.size(), L.planes.data()))) return rc;
     V.hullPlanes = dp;
+    MeshInfo* dm;
+    if ((rc = DevAlloc(b, &dm, L.meshes.size(), L.meshes.data()))) return rc;
+    V.meshes = dm;
+    Vec3* dmv;
+    if ((rc = DevAlloc(b, &dmv, L.meshVertices.size(), L.meshVertices.data()))) return rc;
+    V.meshVertices = dmv;
+    MeshTri* dmt;
+    if ((rc = DevAlloc(b, &dmt, L.meshTris.size(), L.meshTris.data()))) return rc;
+    V.meshTris = dmt;
+    TreeNode* dn;
+    if ((rc = DevAlloc(b, &dn, L.meshNodes.size(), L.meshNodes.data()))) return rc;
+    V.meshNodes = dn;
     return BX_OK;
 }
 
@@ -1024,6 +1225,8 @@ int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_po
     per[kSpacePair] = pairsPerBody;
     per[kSpaceIslandBody] = 1.0 + contactsPerBody;
     per[kSpaceScratch] = 5.0 + 4.0 * fixturesPerBody + 5.0 * contactsPerBody;
+    per[kSpaceCon] = contactsPerBody;
+    per[kSpaceMesh] = per[kSpaceMeshTri] = per[kSpaceMeshScratch] = 0.0;  // pools hold no mesh worlds
     u64 ub[kSpaces];
     UnitBytes(ub);
     double bytesPerBody = 0;
@@ -1063,11 +1266,7 @@ static void DescUnits(const bx_world_desc& d, const bx_batch_opts* o, u32 units[
     wi.bodyCount = d.body_count;
     wi.fixtureCount = d.fixture_count;
     wi.jointCount = d.joint_count;
-    u64 cap = o && o->max_contacts_per_world ? o->max_contacts_per_world : DefaultContactCap(d.fixture_count);
-    wi.contactCap = u32(cap);
-    wi.pairCap = u32(o && o->max_pairs_per_world ? o->max_pairs_per_world : 2 * cap);
-    wi.islandBodyCap = d.body_count + wi.contactCap;
-    wi.islandScratchWords = IslandScratchWords(wi.bodyCount, wi.fixtureCount, wi.contactCap);
+    SetCapacities(wi, d.fixtures && HasMesh(d), o);
     WorldUnits(wi, units);
 }
 
@@ -1090,6 +1289,8 @@ static int Reserve(bx_batch* b, const u32 units[kSpaces], u32 base[kSpaces], u32
 int bx_pool_admit(bx_batch* b, const bx_world_desc* world, uint32_t* slot) {
     if (!b || !world || !slot) return Fail(BX_E_INVALID, "null argument");
     if (!b->pool) return Fail(BX_E_INVALID, "not a pool");
+    if (world->fixture_count && !world->fixtures) return Fail(BX_E_INVALID, "null fixture array");
+    if (HasMesh(*world)) return Fail(BX_E_UNSUPPORTED, "pools do not hold worlds with mesh fixtures");
     HIP_CHECK(hipSetDevice(b->ctx->device));
     // reserve first (from the counts alone), so a world that does not fit costs no host build
     u32 units[kSpaces], base[kSpaces], s;
@@ -1185,7 +1386,7 @@ int bx_batch_snapshot(bx_batch* b, uint32_t world, void* out, uint64_t capacity,
     SnapshotHeader h;
     std::memset(&h, 0, sizeof(h));
     h.magic = kSnapshotMagic;
-    h.version = 1;
+    h.version = 2;
     h.needsInitialPairs = b->needsInitialPairs[world];
     h.proxyCount = wi.fixtureCount;
     h.info = wi;
@@ -1214,7 +1415,7 @@ int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* 
     SnapshotHeader h;
     if (size < sizeof(h)) return Fail(BX_E_INVALID, "snapshot too short");
     std::memcpy(&h, snapshot, sizeof(h));
-    if (h.magic != kSnapshotMagic || h.version != 1 || h.proxyCount != h.info.fixtureCount ||
+    if (h.magic != kSnapshotMagic || h.version != 2 || h.proxyCount != h.info.fixtureCount ||
         size != SnapshotBytes(h.info))
         return Fail(BX_E_INVALID, "not a world snapshot of this library version");
     HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -1364,9 +1565,15 @@ int bx_batch_get_contacts(bx_batch* b, uint32_t world, bx_contact_state* out, ui
     std::vector<Fixture> fixtures;
     rc = Download(b, fixtures, b->view.fixtures + wi.fixtureBase, wi.fixtureCount);
     if (rc) return rc;
+    std::vector<MeshContact> mesh;
+    if (wi.meshTriCap) {
+        rc = Download(b, mesh, b->view.meshContacts + wi.meshBase, wi.contactCap);
+        if (rc) return rc;
+    }
     u32 n = std::min<u32>(capacity, ws.contactCount);
     for (u32 i = 0; i < n; ++i) {
-        const Contact& c = contacts[order[ws.contactCount - 1 - i]];  // world list: newest first
+        const u32 slot = order[ws.contactCount - 1 - i];  // world list: newest first
+        const Contact& c = contacts[slot];
         bx_contact_state& s = out[i];
         std::memset(&s, 0, sizeof(s));
         s.fixture_a = c.fixtureA;
@@ -1379,9 +1586,9 @@ int bx_batch_get_contacts(bx_batch* b, uint32_t world, bx_contact_state* out, ui
             s.cache[2] = c.cache.index1;
             s.cache[3] = c.cache.index2;
         }
-        if (c.manifoldCount > 0) {
-            const Manifold& m = c.manifold;
-            bx_manifold_state& ms = s.manifolds[0];
+        for (u32 j = 0; j < std::min(c.manifoldCount, kMaxManifolds); ++j) {
+            const Manifold& m = j == 0 ? c.manifold : mesh[slot].extra[j - 1];
+            bx_manifold_state& ms = s.manifolds[j];
             ms.point_count = m.pointCount;
             ms.tangent_impulse[0] = m.tangentImpulse.x;
             ms.tangent_impulse[1] = m.tangentImpulse.y;
@@ -1428,14 +1635,18 @@ int bx_batch_get_island_contacts(bx_batch* b, uint32_t world, uint32_t* out, uin
     std::vector<u32> order;
     rc = Download(b, order, b->view.contactOrder + wi.contactBase, ws.contactCount);
     if (rc) return rc;
-    std::vector<uint4> ic;
-    rc = Download(b, ic, b->view.islandContacts + wi.contactBase, ws.islandContactCount);
+    std::vector<uint4> entries;
+    rc = Download(b, entries, b->view.islandContacts + wi.conBase, std::min(ws.islandContactCount, wi.conCap));
     if (rc) return rc;
     std::vector<u32> listIndex(wi.contactCap, BX_NULL_INDEX);
     for (u32 k = 0; k < ws.contactCount; ++k) listIndex[order[k]] = ws.contactCount - 1 - k;
-    u32 n = std::min<u32>(capacity, u32(ic.size()));
-    for (u32 i = 0; i < n; ++i) out[i] = listIndex[ic[i].x];
-    if (count) *count = u32(ic.size());
+    u32 total = 0;
+    for (const uint4& e : entries) {  // one entry per manifold: the contact is listed at its first
+        if (e.x >> 30) continue;
+        if (total < capacity) out[total] = listIndex[e.x];
+        ++total;
+    }
+    if (count) *count = total;
     return BX_OK;
 }
 

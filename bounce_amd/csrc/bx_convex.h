@@ -55,6 +55,18 @@ struct SVert {
 struct GJKOut {
     Vec3 point1, point2;
     scalar distance;
+    // the final simplex: vertex count and the proxy vertex indices of its first three vertices
+    u32 simplexCount;
+    u32 a1, b1, c1, a2, b2, c2;
+};
+
+// Edge wings of a mesh triangle (b3TriangleShape::m_hasE1..3 / m_e1..3, triangle_shape.h:60-70):
+// bit i of mask set when edge i (v1v2, v2v3, v3v1) has a wing vertex e(i+1).
+struct TriWings {
+    Vec3 e1, e2, e3;
+    u32 mask;
+    BX_HD bool Has(u32 i) const { return (mask >> i) & 1u; }
+    BX_HD Vec3 Get(u32 i) const { return i == 0u ? e1 : (i == 1u ? e2 : e3); }
 };
 
 namespace gjk {
@@ -322,6 +334,13 @@ BX_HD GJKOut GJK(const Transform& xf1, const ProxyD& proxy1, const Transform& xf
         ++s.count;
     }
     GJKOut out;
+    out.simplexCount = s.count;
+    out.a1 = s.v0.i1;
+    out.b1 = s.v1.i1;
+    out.c1 = s.v2.i1;
+    out.a2 = s.v0.i2;
+    out.b2 = s.v1.i2;
+    out.c2 = s.v2.i2;
     s.ClosestPoints(&out.point1, &out.point2);
     out.distance = Distance(out.point1, out.point2);
     if (applyRadius) {
@@ -605,8 +624,74 @@ BX_HD void CapsuleEdgeContact(Manifold& m, const Transform& xf1, const HullView&
     SetPoint(m.points[0], MulC(xf1.rotation, N), MulT(xf1, point1), MulT(xf2, point2), key);
 }
 
+// geometry.h:27-43 (segment AB; the divisor is |AB|^2)
+BX_HD void BaryGeom2(scalar out[3], const Vec3& A, const Vec3& B, const Vec3& Q) {
+    Vec3 AB = B - A;
+    Vec3 QA = A - Q;
+    Vec3 QB = B - Q;
+    scalar divisor = Dot(AB, AB);
+    out[0] = Dot(QB, AB);
+    out[1] = -Dot(QA, AB);
+    out[2] = divisor;
+}
+
+// geometry.h:45-70 (triangle ABC; the divisor is |AB x AC|^2)
+BX_HD void BaryGeom3(scalar out[4], const Vec3& A, const Vec3& B, const Vec3& C, const Vec3& Q) {
+    Vec3 AB = B - A;
+    Vec3 AC = C - A;
+    Vec3 QA = A - Q;
+    Vec3 QB = B - Q;
+    Vec3 QC = C - Q;
+    Vec3 QB_x_QC = Cross(QB, QC);
+    Vec3 QC_x_QA = Cross(QC, QA);
+    Vec3 QA_x_QB = Cross(QA, QB);
+    Vec3 AB_x_AC = Cross(AB, AC);
+    scalar divisor = Dot(AB_x_AC, AB_x_AC);
+    out[0] = Dot(QB_x_QC, AB_x_AC);
+    out[1] = Dot(QC_x_QA, AB_x_AC);
+    out[2] = Dot(QA_x_QB, AB_x_AC);
+    out[3] = divisor;
+}
+
+// b3UniqueCount (math.h:84-104) of the first n <= 3 of (a, b, c).
+BX_HD u32 UniqueCount3(u32 n, u32 a, u32 b, u32 c) {
+    if (n == 1u) return 1u;
+    if (n == 2u) return a == b ? 1u : 2u;
+    u32 count = 1u;
+    if (b != a) ++count;
+    if (c != a && c != b) ++count;
+    return count;
+}
+
+// The proxy-1 side of b3GetFeaturePair (gjk_feature_pair.cpp:21-208) for the final simplex of q:
+// the number of proxy-1 vertices of the closest feature and, when it is an edge, its two vertices.
+BX_HD u32 FeatureCount1(const GJKOut& q, u32& e0, u32& e1) {
+    const u32 n = q.simplexCount;
+    if (n == 1u) return 1u;
+    if (n == 2u) {
+        u32 u1 = UniqueCount3(2u, q.a1, q.b1, 0u), u2 = UniqueCount3(2u, q.a2, q.b2, 0u);
+        if ((u1 == 2u && u2 == 2u) || (u1 != 1u && u2 == 1u)) { e0 = q.a1; e1 = q.b1; return 2u; }
+        if (u1 == 1u) return 1u;
+        return 0u;
+    }
+    if (n == 3u) {
+        u32 u1 = UniqueCount3(3u, q.a1, q.b1, q.c1), u2 = UniqueCount3(3u, q.a2, q.b2, q.c2);
+        if (u1 == 1u && u2 == 3u) return 1u;
+        if (u1 == 3u && u2 == 1u) return 3u;
+        if (u1 == 2u && (u2 == 2u || u2 == 3u)) {
+            e0 = q.a1;
+            e1 = q.a1 == q.b1 ? q.c1 : q.b1;
+            return 2u;
+        }
+        if (u1 == 3u) return 3u;
+    }
+    return 0u;
+}
+
+// collide_hull_capsule.cpp:107-181; with `wings`, the triangle-capsule sequence of a mesh triangle
+// (collide_triangle_capsule.cpp:107-278), whose closest-edge step consults the edge wings.
 BX_HD void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const HullView& h1, const Fixture& s1,
-                                 const Transform& xf2, const Fixture& s2) {  // collide_hull_capsule.cpp:107-181
+                                 const Transform& xf2, const Fixture& s2, const TriWings* wings = nullptr) {
     scalar r1 = s1.radius, r2 = s2.radius;
     scalar totalRadius = r1 + r2;
     ProxyD p1 = MakeProxy(s1, &h1), p2 = MakeProxy(s2, nullptr);
@@ -624,6 +709,37 @@ BX_HD void CollideHullAndCapsule(Manifold& m, const Transform& xf1, const HullVi
         if (L < kTol * kTol) {
             CapsuleFaceContact(m, xf1, h1, index1, r1, xf2, s2, r2);
             if (m.pointCount == 2) return;
+        }
+        u32 v1 = 0, v2 = 0;
+        if (wings && wings->mask && FeatureCount1(q, v1, v2) == 2u) {
+            // collide_triangle_capsule.cpp:162-241: the closest feature is a triangle edge
+            u32 edgeIndex = 0;
+            for (u32 i = 0; i < 3u; ++i) {
+                u32 j = i + 1u < 3u ? i + 1u : 0u;
+                if ((v1 == i && v2 == j) || (v2 == i && v1 == j)) { edgeIndex = i; break; }
+            }
+            if (wings->Has(edgeIndex)) {
+                u32 ev1 = edgeIndex;
+                u32 ev2 = edgeIndex + 1u < 3u ? edgeIndex + 1u : 0u;
+                Vec3 Q = MulT(xf1, c2);
+                Vec3 A = wings->Get(edgeIndex);
+                Vec3 B = ev2 == 0u ? s1.v1 : (ev2 == 1u ? s1.v2 : s1.v3);
+                Vec3 C = ev1 == 0u ? s1.v1 : (ev1 == 1u ? s1.v2 : s1.v3);
+                scalar wABC[4];
+                BaryGeom3(wABC, A, B, C, Q);
+                if (wABC[0] > scalar(0) && wABC[1] > scalar(0) && wABC[2] > scalar(0)) return;
+                Vec3 center = (A + B + C) / scalar(3);
+                Plane plane = h1.planes[0];
+                scalar distance = Distance(center, plane);
+                const scalar kCoplanarTol = scalar(0.005);
+                if (distance > -kCoplanarTol && distance < kCoplanarTol) {
+                    Vec3 n = plane.normal;
+                    if (Dot(n, localN1) < scalar(0)) n = -n;
+                    Vec3 nc2 = c1 + q.distance * Mul(xf1.rotation, n);
+                    localN1 = n;
+                    c2 = nc2;
+                }
+            }
         }
         OnePoint(m, localN1, MulT(xf1, c1), MulT(xf2, c2));
         return;

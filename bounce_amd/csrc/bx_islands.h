@@ -74,7 +74,7 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
     u32* cursor = off + nf + 1;         // nf + 1
     u32* cfa = cursor + nf + 1;         // by order position, n entries each
     u32* cfb = cfa + n;
-    u32* cfl = cfb + n;
+    u32* cfl = cfb + n;                 // flags (bit 1 touching) | manifold count << 2
     u32* edges = cfl + n;               // 2 n
     const u32* order = B.contactOrder + wi.contactBase;
 
@@ -98,7 +98,7 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
         const Contact& c = B.contacts[wi.contactBase + order[i]];
         cfa[i] = c.fixtureA;
         cfb[i] = c.fixtureB;
-        cfl[i] = c.flags & ~1u;
+        cfl[i] = (c.flags & 2u) | (Min(c.manifoldCount, kMaxManifolds) << 2);
         atomicAdd(&off[c.fixtureA], 1u);
         atomicAdd(&off[c.fixtureB], 1u);
     }
@@ -129,7 +129,7 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
     if (threadIdx.x == 0) {
         IslandInfo* islands = B.islands + wi.bodyBase;
         u32* ibodies = B.islandBodies + wi.islandBodyBase;
-        uint4* icontacts = B.islandContacts + wi.contactBase;
+        uint4* icontacts = B.islandContacts + wi.conBase;
         uint4* ijoints = B.islandJoints + wi.jointBase;
         u32 islandCount = 0, nbo = 0, nc = 0, nj = 0, overflow = 0;
         for (u32 s = nb; s-- > 0;) {
@@ -164,7 +164,10 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
                         if ((cf & 2u) == 0) continue;
                         u32 fa = cfa[k], fbb = cfb[k];
                         if ((fbody[fa] | fbody[fbb]) & 0x80000000u) continue;  // sensors
-                        icontacts[nc++] = make_uint4(order[k], fbody[fa], fbody[fbb], 0u);
+                        // one constraint entry per manifold (a mesh contact has up to three,
+                        // solved one after the other as b3ContactSolver does)
+                        for (u32 j = 0; j < (cf >> 2); ++j)
+                            icontacts[nc++] = make_uint4(order[k] | (j << 30), fbody[fa], fbody[fbb], 0u);
                         cfl[k] = cf | 1u;
                         u32 other = (fa == f ? fbody[fbb] : fbody[fa]) & 0x7FFFFFFFu;
                         if (bflags[other] & kIsland) continue;

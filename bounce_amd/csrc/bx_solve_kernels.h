@@ -65,20 +65,22 @@ __global__ void __launch_bounds__(kBlock) k_init_contacts(BatchView B, StepParam
     const WorldState& ws = B.wstate[w];
     const scalar dtRatio = ws.inv_dt0 * sp.dt;
     const bool warm = (wi.flags & 0x2u) == 0;
-    const uint4* ic = B.islandContacts + wi.contactBase;
+    const uint4* ic = B.islandContacts + wi.conBase;
     const Body* bodies = B.bodies + wi.bodyBase;
     for (u32 k = threadIdx.x; k < ws.islandContactCount; k += blockDim.x) {
         const uint4 e = ic[k];
-        const Contact& c = B.contacts[wi.contactBase + e.x];
+        const u32 slot = e.x & 0x3FFFFFFFu;
+        Contact& c = B.contacts[wi.contactBase + slot];
         const Fixture& fA = B.fixtures[wi.fixtureBase + c.fixtureA];
         const Fixture& fB = B.fixtures[wi.fixtureBase + c.fixtureB];
         const VelBody* vel = B.velBodies + wi.islandBodyBase + e.w;
         const PosBody* pos = B.posBodies + wi.islandBodyBase + e.w;
         VelCon vc;
         PosCon pc;
-        InitContact(vc, pc, c, fA, fB, bodies[fA.body], bodies[fB.body], e.y, e.z, vel, pos, dtRatio, warm);
-        B.velCons[wi.contactBase + k] = vc;
-        B.posCons[wi.contactBase + k] = pc;
+        InitContact(vc, pc, ContactManifold(B, wi, c, slot, e.x >> 30), c.manifoldCount, fA, fB, bodies[fA.body],
+                    bodies[fB.body], e.y, e.z, vel, pos, dtRatio, warm);
+        B.velCons[wi.conBase + k] = vc;
+        B.posCons[wi.conBase + k] = pc;
     }
     // b3JointSolver::InitializeVelocityConstraints: independent per joint (revolute_joint.cpp:78-231)
     const uint4* ij = B.islandJoints + wi.jointBase;
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(64 * kSolveWaves) k_solve_velocity(BatchView B
         for (u32 k = first; k < last; ++k) {
             const IslandInfo isl = B.islands[wi.bodyBase + k];
             VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;
-            VelCon* vcs = B.velCons + wi.contactBase + isl.contactStart;
+            VelCon* vcs = B.velCons + wi.conBase + isl.contactStart;
             JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
             if (isl.bodyCount <= kSlabBodies) {
                 VelBody* slab = slabs[slot];
@@ -214,14 +216,16 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
     const WorldInfo wi = B.worlds[w];
     const WorldState& ws = B.wstate[w];
     if ((wi.flags & 0x2u) == 0) {  // b3ContactSolver::StoreImpulses (contact_solver.cpp:488-515)
-        const uint4* ic = B.islandContacts + wi.contactBase;
+        const uint4* ic = B.islandContacts + wi.conBase;
         for (u32 k = threadIdx.x; k < ws.islandContactCount; k += blockDim.x) {
-            Contact& c = B.contacts[wi.contactBase + ic[k].x];
+            const u32 slot = ic[k].x & 0x3FFFFFFFu;
+            Contact& c = B.contacts[wi.contactBase + slot];
             if (c.manifoldCount == 0) continue;
-            const VelCon& vc = B.velCons[wi.contactBase + k];
-            c.manifold.tangentImpulse = vc.tangentImpulse;
-            c.manifold.motorImpulse = vc.motorImpulse;
-            for (u32 p = 0; p < c.manifold.pointCount; ++p) c.manifold.points[p].normalImpulse = vc.points[p].normalImpulse;
+            Manifold& m = ContactManifold(B, wi, c, slot, ic[k].x >> 30);
+            const VelCon& vc = B.velCons[wi.conBase + k];
+            m.tangentImpulse = vc.tangentImpulse;
+            m.motorImpulse = vc.motorImpulse;
+            for (u32 p = 0; p < m.pointCount; ++p) m.points[p].normalImpulse = vc.points[p].normalImpulse;
         }
     }
     // the joints keep their impulses for the next step's warm start (revolute_joint.h members)
@@ -294,7 +298,7 @@ __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams s
         const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
         for (u32 k = first; k < last; ++k) {
             IslandInfo& isl = B.islands[wi.bodyBase + k];
-            isl.positionsSolved = PositionSweeps(B.posCons + wi.contactBase + isl.contactStart, isl.contactCount,
+            isl.positionsSolved = PositionSweeps(B.posCons + wi.conBase + isl.contactStart, isl.contactCount,
                                                  B.jointCons + wi.jointBase + isl.jointStart, isl.jointCount,
                                                  B.posBodies + wi.islandBodyBase + isl.bodyStart,
                                                  sp.positionIterations) ? 1u : 0u;

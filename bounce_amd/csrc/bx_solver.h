@@ -60,12 +60,13 @@ BX_HD void IntegratePosition(VelBody& vb, PosBody& pb, const Body& b, scalar h) 
 }
 
 // contact_solver.cpp:28-150 (constructor) and 152-270 (InitializeVelocityConstraints) for one
-// convex contact, reading the island state right after velocity integration.
-BX_HD void InitContact(VelCon& vc, PosCon& pc, const Contact& c, const Fixture& fA, const Fixture& fB,
-                       const Body& bA, const Body& bB, u32 indexA, u32 indexB, const VelBody* vel,
+// manifold of a contact, reading the island state right after velocity integration. A convex
+// contact has one manifold; the manifolds of a mesh contact become consecutive constraints on the
+// same two bodies, which the sweeps then visit in b3ContactSolver's manifold order.
+BX_HD void InitContact(VelCon& vc, PosCon& pc, const Manifold& m, u32 manifoldCount, const Fixture& fA,
+                       const Fixture& fB, const Body& bA, const Body& bB, u32 indexA, u32 indexB, const VelBody* vel,
                        const PosBody* pos, scalar dtRatio, bool warmStarting) {
-    const Manifold& m = c.manifold;
-    const u32 pointCount = c.manifoldCount > 0 ? m.pointCount : 0;
+    const u32 pointCount = manifoldCount > 0 ? m.pointCount : 0;
     pc.indexA = indexA;
     pc.indexB = indexB;
     pc.pointCount = pointCount;

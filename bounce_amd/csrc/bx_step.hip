@@ -88,7 +88,105 @@ __device__ __forceinline__ void WakeBody(Body& b) {  // b3Body::SetAwake(true)
     }
 }
 
+// ================================================================== shape AABBs
+__device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transform& xf, AABB& out) {
+    if (f.shapeType == 3u) {  // hull_shape.cpp:160-164, aabb.h:53-62
+        HullView h = GetHull(B, f.shapeIndex);
+        out.lowerBound = out.upperBound = Mul(xf, h.vertices[0]);
+        for (u32 i = 1; i < h.vertexCount; ++i) {
+            Vec3 v = Mul(xf, h.vertices[i]);
+            out.lowerBound = Min(out.lowerBound, v);
+            out.upperBound = Max(out.upperBound, v);
+        }
+        out.Extend(f.radius);
+    } else if (f.shapeType == 0u) {  // sphere_shape.cpp:51-57
+        Vec3 c = Mul(xf, f.v1);
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = c - r;
+        out.upperBound = c + r;
+    } else if (f.shapeType == 1u) {  // capsule_shape.cpp:124-131
+        Vec3 c1 = Mul(xf, f.v1), c2 = Mul(xf, f.v2);
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = Min(c1, c2) - r;
+        out.upperBound = Max(c1, c2) + r;
+    } else if (f.shapeType == 2u) {  // triangle_shape.cpp:75-83: local bounds, the transform is unused
+        Vec3 r(f.radius, f.radius, f.radius);
+        out.lowerBound = Min(f.v1, Min(f.v2, f.v3)) - r;
+        out.upperBound = Max(f.v1, Max(f.v2, f.v3)) + r;
+    } else if (f.shapeType == 4u) {  // mesh_shape.cpp:44-50, aabb.h:65-74
+        const MeshInfo mi = B.meshes[f.shapeIndex];
+        const Vec3* v = B.meshVertices + mi.vBase;
+        out.lowerBound = out.upperBound = Mul(xf, Mul(f.v1, v[0]));
+        for (u32 i = 1; i < mi.vCount; ++i) {
+            Vec3 p = Mul(xf, Mul(f.v1, v[i]));
+            out.lowerBound = Min(out.lowerBound, p);
+            out.upperBound = Max(out.upperBound, p);
+        }
+        out.Extend(f.radius);
+    } else {
+        out.lowerBound.SetZero();
+        out.upperBound.SetZero();
+    }
+}
+
+#include "bx_mesh.h"
+
 // ================================================================== update contacts
+// b3Manifold::Initialize(const b3Manifold&) (collision.cpp:40-61): impulses of `old` carried over to
+// the points of `m` with the same contact id.
+__device__ __forceinline__ void MatchManifold(Manifold& m, const Manifold& old) {
+    m.tangentImpulse = old.tangentImpulse;
+    m.motorImpulse = old.motorImpulse;
+    for (u32 i = 0; i < m.pointCount; ++i) {
+        ManifoldPoint& p1 = m.points[i];
+        for (u32 j = 0; j < old.pointCount; ++j) {
+            const ManifoldPoint& p2 = old.points[j];
+            if (p2.triangleKey == p1.triangleKey && p2.key[0] == p1.key[0] && p2.key[1] == p1.key[1] &&
+                p2.key[2] == p1.key[2] && p2.key[3] == p1.key[3]) {
+                p1.normalImpulse = p2.normalImpulse;
+                p1.persisting = 1;
+                break;
+            }
+        }
+    }
+}
+
+// b3Contact::Update (contact.cpp:218-262) of a mesh contact: b3Manifold::Initialize of its
+// manifolds, b3MeshContact::Collide, then every new manifold matched against every old one in
+// order (a point takes the impulse of its match in the last old manifold that has one; tangent and
+// motor impulses come from the last old manifold). Returns touching.
+__device__ __noinline__ bool UpdateMeshManifolds(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot,
+                                                 Contact& c, const Fixture& fA, const Fixture& fB,
+                                                 const Transform& xfA, const Transform& xfB) {
+    MeshObs* lane = B.meshScratch + wi.meshScratchBase + threadIdx.x * MeshLaneStride(wi.meshTriCap);
+    Manifold* nm = reinterpret_cast<Manifold*>(lane);
+    for (u32 i = 0; i < kMaxManifolds; ++i) {
+        Manifold& m = nm[i];
+        m.pointCount = 0;
+        m.tangentImpulse.SetZero();
+        m.motorImpulse = m.motorSpeed = m.tangentSpeed1 = m.tangentSpeed2 = 0;
+        for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+            m.points[k].normalImpulse = 0;
+            m.points[k].persisting = 0;
+            m.points[k].edgeContact = 0;
+        }
+    }
+    const u32 oldCount = c.manifoldCount;
+    u32 overflow = 0;
+    const u32 count = MeshCollide(B, wi, slot, fA, fB, xfA, xfB, lane, nm, overflow);
+    if (overflow) atomicOr(&ws.status, 0x2u);
+    if ((wi.flags & 0x2u) == 0)
+        for (u32 i = 0; i < count; ++i)
+            for (u32 j = 0; j < oldCount; ++j) MatchManifold(nm[i], ContactManifold(B, wi, c, slot, j));
+    bool touching = false;
+    for (u32 i = 0; i < count; ++i) {
+        if (nm[i].pointCount > 0) touching = true;
+        ContactManifold(B, wi, c, slot, i) = nm[i];
+    }
+    c.manifoldCount = count;
+    return touching;
+}
+
 // b3ContactManager::UpdateContacts (contact_manager.cpp:165-227) + b3Contact::Update
 // (contact.cpp:183-285) for one contact. Returns false if the contact is destroyed.
 __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot, bool applyWakes) {
@@ -98,10 +196,16 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     Body& bA = B.bodies[wi.bodyBase + fA.body];
     Body& bB = B.bodies[wi.bodyBase + fB.body];
     bool sensor = fA.isSensor || fB.isSensor;
+    const bool isMesh = fA.shapeType == kMeshShape || fB.shapeType == kMeshShape;
     auto destroy = [&]() {
-        if (applyWakes && !sensor && c.manifoldCount > 0 && c.manifold.pointCount > 0) {
-            WakeBody(bA);
-            WakeBody(bB);
+        if (applyWakes && !sensor) {
+            for (u32 j = 0; j < c.manifoldCount; ++j) {
+                if (ContactManifold(B, wi, c, slot, j).pointCount > 0) {
+                    WakeBody(bA);
+                    WakeBody(bB);
+                    break;
+                }
+            }
         }
         return false;
     };
@@ -113,7 +217,16 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
 
     bool wasTouching = (c.flags & 2u) != 0;
     bool touching = false;
-    if (sensor) {
+    if (sensor && isMesh) {
+        touching = MeshContactTestOverlap(B, wi, slot, c, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q));
+        c.manifoldCount = 0;
+    } else if (isMesh) {
+        touching = UpdateMeshManifolds(B, wi, ws, slot, c, fA, fB, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q));
+        if (touching != wasTouching && applyWakes) {
+            WakeBody(bA);
+            WakeBody(bB);
+        }
+    } else if (sensor) {
         // b3ConvexContact::TestOverlap (convex_contact.cpp:34-43)
         Transform xfA(bA.xfT, bA.q), xfB(bB.xfT, bB.q);
         HullView hA, hB;
@@ -174,28 +287,10 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
             CollideHullAndTriangle(m, xfA, hA, fA.radius, xfB, fB, scratch, overflow);
             if (overflow) atomicOr(&ws.status, 0x2u);
         } else {
-            atomicOr(&ws.status, 0x200u);  // shape pair without a device narrowphase (meshes, triangles)
+            atomicOr(&ws.status, 0x200u);  // shape pair without a device narrowphase
         }
         c.manifoldCount = 1;
-        if (wi.flags & 0x2u) {
-            // warm starting disabled: nothing to match
-        } else if (oldCount > 0) {
-            // b3Manifold::Initialize(const b3Manifold&) (collision.cpp:40-61)
-            m.tangentImpulse = old.tangentImpulse;
-            m.motorImpulse = old.motorImpulse;
-            for (u32 i = 0; i < m.pointCount; ++i) {
-                ManifoldPoint& p1 = m.points[i];
-                for (u32 j = 0; j < old.pointCount; ++j) {
-                    const ManifoldPoint& p2 = old.points[j];
-                    if (p2.triangleKey == p1.triangleKey && p2.key[0] == p1.key[0] && p2.key[1] == p1.key[1] &&
-                        p2.key[2] == p1.key[2] && p2.key[3] == p1.key[3]) {
-                        p1.normalImpulse = p2.normalImpulse;
-                        p1.persisting = 1;
-                        break;
-                    }
-                }
-            }
-        }
+        if ((wi.flags & 0x2u) == 0 && oldCount > 0) MatchManifold(m, old);  // no matching without warm starting
         c.manifold = m;
         touching = m.pointCount > 0;
         if (touching != wasTouching && applyWakes) {
@@ -262,36 +357,6 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
 #include "bx_solve_kernels.h"
 
 // ================================================================== fixture sync + clear forces
-__device__ void ComputeAABB(const BatchView& B, const Fixture& f, const Transform& xf, AABB& out) {
-    if (f.shapeType == 3u) {  // hull_shape.cpp:160-164, aabb.h:53-62
-        HullView h = GetHull(B, f.shapeIndex);
-        out.lowerBound = out.upperBound = Mul(xf, h.vertices[0]);
-        for (u32 i = 1; i < h.vertexCount; ++i) {
-            Vec3 v = Mul(xf, h.vertices[i]);
-            out.lowerBound = Min(out.lowerBound, v);
-            out.upperBound = Max(out.upperBound, v);
-        }
-        out.Extend(f.radius);
-    } else if (f.shapeType == 0u) {  // sphere_shape.cpp:51-57
-        Vec3 c = Mul(xf, f.v1);
-        Vec3 r(f.radius, f.radius, f.radius);
-        out.lowerBound = c - r;
-        out.upperBound = c + r;
-    } else if (f.shapeType == 1u) {  // capsule_shape.cpp:124-131
-        Vec3 c1 = Mul(xf, f.v1), c2 = Mul(xf, f.v2);
-        Vec3 r(f.radius, f.radius, f.radius);
-        out.lowerBound = Min(c1, c2) - r;
-        out.upperBound = Max(c1, c2) + r;
-    } else if (f.shapeType == 2u) {  // triangle_shape.cpp:75-83: local bounds, the transform is unused
-        Vec3 r(f.radius, f.radius, f.radius);
-        out.lowerBound = Min(f.v1, Min(f.v2, f.v3)) - r;
-        out.upperBound = Max(f.v1, Max(f.v2, f.v3)) + r;
-    } else {
-        out.lowerBound.SetZero();
-        out.upperBound.SetZero();
-    }
-}
-
 __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
     const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
@@ -334,6 +399,18 @@ __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
         if ((wi.flags & 0x4u) == 0) {  // b3World::ClearForces
             b.force.SetZero();
             b.torque.SetZero();
+        }
+    }
+    if (solved && wi.meshTriCap) {
+        // every mesh contact follows its shapes (world.cpp: after the fixtures are synchronized)
+        const u32 n = B.wstate[w].contactCount;
+        const u32* order = B.contactOrder + wi.contactBase;
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+            const u32 slot = order[i];
+            const Contact& c = B.contacts[wi.contactBase + slot];
+            if (B.fixtures[wi.fixtureBase + c.fixtureA].shapeType == kMeshShape ||
+                B.fixtures[wi.fixtureBase + c.fixtureB].shapeType == kMeshShape)
+                MeshContactSynchronize(B, wi, slot, c);
         }
     }
 }
@@ -451,19 +528,33 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
                 c.cache.index1 = 0;
                 c.cache.index2 = 0;
                 B.contactOrder[wi.contactBase + n0 + r] = slot;
+                if (B.fixtures[wi.fixtureBase + fa].shapeType == kMeshShape ||
+                    B.fixtures[wi.fixtureBase + fb].shapeType == kMeshShape)
+                    MeshContactCreate(B, wi, slot, c);
             }
         }
         created += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (created > free0) {
-            ws.status |= 0x2u;
-            created = free0;
-        }
-        ws.contactCount = n0 + created;
-        ws.freeCount = free0 - created;
+        if (created > free0) atomicOr(&ws.status, 0x2u);
+        ws.contactCount = n0 + Min(created, free0);
+        ws.freeCount = free0 - Min(created, free0);
         ws.pairCount = npairs;
+    }
+    if (wi.meshTriCap) {
+        // b3ContactManager::FindNewContacts: every mesh contact refreshes its triangles
+        // (contact_manager.cpp:153-163, mesh_contact.cpp:228-258)
+        const u32 n = n0 + Min(created, free0);
+        bool ok = true;
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+            const u32 slot = order[i];
+            const Contact& c = B.contacts[wi.contactBase + slot];
+            if (B.fixtures[wi.fixtureBase + c.fixtureA].shapeType == kMeshShape ||
+                B.fixtures[wi.fixtureBase + c.fixtureB].shapeType == kMeshShape)
+                ok = MeshContactFindPairs(B, wi, slot, c) && ok;
+        }
+        if (!ok) atomicOr(&ws.status, 0x2u);
     }
 }
 

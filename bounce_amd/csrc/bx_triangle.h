@@ -8,8 +8,9 @@
 //                                      the triangle hull)
 //   collide_hull_triangle.cpp:24-124 and collide_hulls.cpp:189-264 (b3CollideHulls without the
 //                                      feature cache on the triangle hull)
-// Triangle fixtures of the device path have no edge wings (b3TriangleShape::Set), so the wing
-// corrections of the reference never apply here.
+// Triangle fixtures have no edge wings (b3TriangleShape::Set); the child triangles of a mesh
+// (b3MeshShape::GetChildTriangle, mesh_shape.cpp:142-175) carry them, and the functions below take
+// them as an optional TriWings.
 #pragma once
 
 namespace bx {
@@ -55,35 +56,6 @@ struct TriangleHull {
     }
 };
 
-// geometry.h:27-43 (segment AB; the divisor is |AB|^2)
-BX_HD void BaryGeom2(scalar out[3], const Vec3& A, const Vec3& B, const Vec3& Q) {
-    Vec3 AB = B - A;
-    Vec3 QA = A - Q;
-    Vec3 QB = B - Q;
-    scalar divisor = Dot(AB, AB);
-    out[0] = Dot(QB, AB);
-    out[1] = -Dot(QA, AB);
-    out[2] = divisor;
-}
-
-// geometry.h:45-70 (triangle ABC; the divisor is |AB x AC|^2)
-BX_HD void BaryGeom3(scalar out[4], const Vec3& A, const Vec3& B, const Vec3& C, const Vec3& Q) {
-    Vec3 AB = B - A;
-    Vec3 AC = C - A;
-    Vec3 QA = A - Q;
-    Vec3 QB = B - Q;
-    Vec3 QC = C - Q;
-    Vec3 QB_x_QC = Cross(QB, QC);
-    Vec3 QC_x_QA = Cross(QC, QA);
-    Vec3 QA_x_QB = Cross(QA, QB);
-    Vec3 AB_x_AC = Cross(AB, AC);
-    scalar divisor = Dot(AB_x_AC, AB_x_AC);
-    out[0] = Dot(QB_x_QC, AB_x_AC);
-    out[1] = Dot(QC_x_QA, AB_x_AC);
-    out[2] = Dot(QA_x_QB, AB_x_AC);
-    out[3] = divisor;
-}
-
 // One point at P on the triangle towards the sphere centre Q.
 BX_HD void TriangleSpherePoint(Manifold& m, const Vec3& P, const Vec3& Q, const Vec3& center2, scalar radius) {
     Vec3 d = Q - P;
@@ -95,8 +67,17 @@ BX_HD void TriangleSpherePoint(Manifold& m, const Vec3& P, const Vec3& Q, const 
     OnePoint(m, n, P, center2);
 }
 
+// Is Q inside the triangle (A1, B1, C1) adjacent across an edge?
+BX_HD bool InAdjacentFace(const Vec3& A1, const Vec3& B1, const Vec3& C1, const Vec3& Q) {
+    scalar w[4];
+    BaryGeom3(w, A1, B1, C1, Q);
+    return w[0] > scalar(0) && w[1] > scalar(0) && w[2] > scalar(0);
+}
+
+// collide_triangle_sphere.cpp:24-296. On an edge with a wing, a sphere centre over the adjacent
+// triangle makes no point (that triangle's face region has it).
 BX_HD void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Fixture& t1, const Transform& xf2,
-                                    const Fixture& s2) {
+                                    const Fixture& s2, const TriWings* wings = nullptr) {
     Vec3 Q = MulT(xf1, Mul(xf2, s2.v1));
     Vec3 A = t1.v1, B = t1.v2, C = t1.v3;
     scalar radius = t1.radius + s2.radius;
@@ -111,16 +92,19 @@ BX_HD void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Fix
     BaryGeom3(wABC, A, B, C, Q);
     if (wAB[0] > scalar(0) && wAB[1] > scalar(0) && wABC[3] * wABC[2] <= scalar(0)) {  // edge AB
         scalar den = scalar(1) / wAB[2];
+        if (wings && wings->Has(0u) && InAdjacentFace(wings->e1, B, A, Q)) return;
         TriangleSpherePoint(m, den * (wAB[0] * A + wAB[1] * B), Q, s2.v1, radius);
         return;
     }
     if (wBC[0] > scalar(0) && wBC[1] > scalar(0) && wABC[3] * wABC[0] <= scalar(0)) {  // edge BC
         scalar den = scalar(1) / wBC[2];
+        if (wings && wings->Has(1u) && InAdjacentFace(wings->e2, C, B, Q)) return;
         TriangleSpherePoint(m, den * (wBC[0] * B + wBC[1] * C), Q, s2.v1, radius);
         return;
     }
     if (wCA[0] > scalar(0) && wCA[1] > scalar(0) && wABC[3] * wABC[1] <= scalar(0)) {  // edge CA
         scalar den = scalar(1) / wCA[2];
+        if (wings && wings->Has(2u) && InAdjacentFace(wings->e3, A, C, Q)) return;
         TriangleSpherePoint(m, den * (wCA[0] * C + wCA[1] * A), Q, s2.v1, radius);
         return;
     }
@@ -130,10 +114,10 @@ BX_HD void CollideTriangleAndSphere(Manifold& m, const Transform& xf1, const Fix
 }
 
 BX_HD void CollideTriangleAndCapsule(Manifold& m, const Transform& xf1, const Fixture& t1, const Transform& xf2,
-                                     const Fixture& s2) {
+                                     const Fixture& s2, const TriWings* wings = nullptr) {
     TriangleHull th;
     th.Set(t1.v1, t1.v2, t1.v3);
-    CollideHullAndCapsule(m, xf1, th.View(), t1, xf2, s2);
+    CollideHullAndCapsule(m, xf1, th.View(), t1, xf2, s2, wings);
 }
 
 // b3CollideHulls without the feature cache (collide_hulls.cpp:189-264).
@@ -166,11 +150,49 @@ BX_HD void CollideHullsUncached(Manifold& m, const Transform& xf1, const HullVie
     }
 }
 
+// collide_hull_triangle.cpp:24-124: with wings, the normal of a point on an edge shared with a
+// coplanar wing triangle is snapped to the triangle plane.
 BX_HD void CollideHullAndTriangle(Manifold& m, const Transform& xf1, const HullView& h1, scalar r1,
-                                  const Transform& xf2, const Fixture& t2, FaceScratch& s, u32& overflow) {
+                                  const Transform& xf2, const Fixture& t2, FaceScratch& s, u32& overflow,
+                                  const TriWings* wings = nullptr) {
     TriangleHull th;
     th.Set(t2.v1, t2.v2, t2.v3);
     CollideHullsUncached(m, xf1, h1, r1, xf2, th.View(), t2.radius, s, overflow);
+    if (!wings || !wings->mask) return;
+    Plane plane2 = Mul(xf2, th.p[0]);
+    Vec3 centroid1 = Mul(xf1, h1.centroid);
+    Vec3 centroid2 = Mul(xf2, th.centroid);
+    for (u32 i = 0; i < m.pointCount; ++i) {
+        ManifoldPoint& mp = m.points[i];
+        u32 e2;
+        if (mp.edgeContact) {
+            e2 = mp.key[2];
+        } else {
+            e2 = mp.key[2] == kNull ? mp.key[3] : mp.key[2];
+        }
+        if (e2 >= 6u) continue;  // no triangle edge (B3_NULL_EDGE)
+        u32 edgeIndex = e2 >> 1;  // half-edges 2k, 2k + 1 lie on edge k
+        if (!wings->Has(edgeIndex)) continue;
+        u32 ev1 = edgeIndex;
+        u32 ev2 = edgeIndex + 1u < 3u ? edgeIndex + 1u : 0u;
+        Vec3 A = wings->Get(edgeIndex);
+        Vec3 B = ev2 == 0u ? t2.v1 : (ev2 == 1u ? t2.v2 : t2.v3);
+        Vec3 C = ev1 == 0u ? t2.v1 : (ev1 == 1u ? t2.v2 : t2.v3);
+        Vec3 center = (A + B + C) / scalar(3);
+        scalar distance = Distance(center, th.p[0]);
+        const scalar kCoplanarTol = scalar(0.005);
+        if (distance > -kCoplanarTol && distance < kCoplanarTol) {
+            Vec3 n1 = Mul(xf1.rotation, mp.localNormal1);
+            Vec3 c1 = Mul(xf1, mp.localPoint1);
+            Vec3 c2 = Mul(xf2, mp.localPoint2);
+            scalar sep = Dot(c2 - c1, n1);
+            Vec3 n = plane2.normal;
+            if (Dot(n, centroid2 - centroid1) < scalar(0)) n = -n;
+            Vec3 nc1 = c2 - sep * n;
+            mp.localNormal1 = MulC(xf1.rotation, n);
+            mp.localPoint1 = MulT(xf1, nc1);
+        }
+    }
 }
 
 }  // namespace bx

@@ -62,6 +62,25 @@ struct HullView {
     }
 };
 
+// Triangle meshes of the shape library (b3Mesh, mesh.h:30-96; b3StaticTree, static_tree.h:30-110).
+struct MeshInfo {
+    u32 vBase, vCount;      // vertices
+    u32 tBase, tCount;      // triangles
+    u32 nBase, root;        // tree nodes, root node (mesh-local)
+    u32 pad0, pad1;
+};
+
+// b3MeshTriangle with its edge wings (b3MeshTriangleWings, mesh.cpp:35-112): the vertex across
+// edge (v1, v2), (v2, v3), (v3, v1) in the adjacent triangle, kNull on a boundary edge.
+struct MeshTri { u32 v1, v2, v3, u1, u2, u3; };
+
+// b3StaticTree node: a leaf has child1 == kNull and index = its triangle.
+struct TreeNode {
+    AABB aabb;
+    u32 child1, child2, index, pad;
+};
+constexpr u32 kTreeStack = 64;  // traversal stack of a tree query; the host rejects deeper trees
+
 // Body flags: b3Body::e_* (body.h:379-384).
 constexpr u32 kAwake = 0x1, kIsland = 0x2, kAutoSleep = 0x4;
 constexpr u32 kFixedX = 0x8, kFixedY = 0x10, kFixedZ = 0x20;
@@ -121,6 +140,31 @@ struct Contact {
     Manifold manifold;  // convex contacts carry one manifold (convex_contact.cpp:23-30)
 };
 
+// State of a mesh contact besides its first manifold (b3MeshContact, mesh_contact.h:50-80): the
+// fat AABB of the other shape in the mesh's unscaled frame, whether it moved since the triangle
+// query, the queried triangle count (the triangles themselves are in BatchView::meshTriLists) and
+// manifolds 2 and 3 of the up to kMaxManifolds cluster manifolds.
+struct MeshContact {
+    AABB meshAABB;
+    u32 moved, triangleCount;
+    Manifold extra[kMaxManifolds - 1];
+};
+
+// One contact point of a mesh contact's per-triangle manifolds during clustering (cluster.cpp:
+// 471-584): world normal, world point (moved onto the cluster plane during reduction), separation,
+// and (triangle list position << 2 | point).
+struct MeshObs {
+    Vec3 n, p;
+    scalar sep;
+    u32 tag;
+};
+
+// Lanes of the per-world contact update and the stripe of MeshObs each lane owns there: its three
+// cluster manifolds, up to four observations per triangle and the member list of one cluster.
+constexpr u32 kMeshLanes = 256;
+constexpr u32 kMeshManifoldObs = u32((kMaxManifolds * sizeof(Manifold) + sizeof(MeshObs) - 1) / sizeof(MeshObs));
+BX_HD u32 MeshLaneStride(u32 triCap) { return kMeshManifoldObs + 4u * triCap + (4u * triCap + 7u) / 8u; }
+
 struct JointEdge { u32 joint, other; };
 
 // Revolute joint (revolute_joint.h / revolute_joint.cpp): definition and the warm-start state that
@@ -179,6 +223,11 @@ struct WorldInfo {
     u32 islandScratchBase;             // words in BatchView::islandScratch (used when LDS is too small)
     u32 islandScratchWords;            // words the island builder needs for this world at full capacity
     u32 edgeBase;                      // CSR offsets of the contact edges (fixtureCount + 1 entries)
+    u32 conBase, conCap;               // island constraint entries (one per manifold of an island contact)
+    u32 meshBase;                      // mesh contact records (contactCap when the world has a mesh fixture)
+    u32 meshTriBase;                   // mesh contact triangle lists (contactCap x meshTriCap)
+    u32 meshTriCap;                    // triangles a mesh contact can hold (0: no mesh fixture)
+    u32 meshScratchBase;               // clustering stripes (kMeshLanes x MeshLaneStride(meshTriCap))
 };
 
 // Per-world mutable counters.
@@ -252,6 +301,10 @@ struct BatchView {
     const HalfEdge* hullEdges;
     const u32* hullFaces;
     const Plane* hullPlanes;
+    const MeshInfo* meshes;
+    const Vec3* meshVertices;
+    const MeshTri* meshTris;
+    const TreeNode* meshNodes;
     // bodies / fixtures / joints
     Body* bodies;
     Fixture* fixtures;
@@ -275,14 +328,18 @@ struct BatchView {
     // islands
     IslandInfo* islands;      // per world: up to bodyCount islands (base = bodyBase)
     u32* islandBodies;        // per world: islandBodyCap
-    uint4* islandContacts;    // per world: contactCap entries {slot, localA, localB, 0}
+    uint4* islandContacts;    // per world: conCap entries {slot | manifold << 30, localA, localB, island bodyStart}
     uint4* islandJoints;      // per world: jointCount entries {joint, localA, localB, island bodyStart}
     JointCon* jointCons;      // per world: jointCount entries, island order
     u32* dfsStack;            // per world: bodyCount
     VelBody* velBodies;       // per world: islandBodyCap
     PosBody* posBodies;       // per world: islandBodyCap
-    VelCon* velCons;          // per world: contactCap
-    PosCon* posCons;          // per world: contactCap
+    VelCon* velCons;          // per world: conCap
+    PosCon* posCons;          // per world: conCap
+    // mesh contacts (worlds with a mesh fixture)
+    MeshContact* meshContacts; // per contact slot
+    u32* meshTriLists;        // per contact slot: meshTriCap triangle indices, tree query order
+    MeshObs* meshScratch;     // per world: kBlock x 4 x meshTriCap
     uint2* islandList;        // islands of the launched worlds: {world, island} (island = kNull: all, in order)
     u32* islandListCount;
     u32* islandScratch;       // island-builder working arrays of worlds whose arrays do not fit LDS

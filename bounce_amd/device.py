@@ -103,7 +103,7 @@ PAIR_DT = np.dtype(capi.bx_pair)
 class Batch:
     """A batch of worlds resident on the device."""
 
-    def __init__(self, ctx, scene, max_contacts=0, max_pairs=0, solver_mode=0):
+    def __init__(self, ctx, scene, max_contacts=0, max_pairs=0, solver_mode=0, mesh_triangles=0):
         self.ctx = ctx
         self.scene = scene
         self._lib = scene.library()
@@ -112,6 +112,7 @@ class Batch:
         opts.max_contacts_per_world = max_contacts
         opts.max_pairs_per_world = max_pairs
         opts.solver_mode = solver_mode
+        opts.mesh_triangles_per_contact = mesh_triangles
         h = C.c_void_p()
         _check(_lib.bx_batch_create(ctx.h, C.byref(self._lib), self._descs, len(scene.worlds), C.byref(opts),
                                     C.byref(h)))

@@ -253,7 +253,8 @@ typedef struct bx_batch_opts {
     uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t solver_mode;              /* 0: auto, 1: one lane per island, 2: scheduled wavefront */
-    uint32_t reserved[5];
+    uint32_t mesh_triangles_per_contact; /* triangles a mesh contact can overlap (0: 64) */
+    uint32_t reserved[4];
 } bx_batch_opts;
 
 typedef struct bx_pool_opts {

@@ -44,6 +44,11 @@ SCENES = {
     "triangles40_hulls": (lambda: scenes.triangle_drop(40, hull_count=8), 240),
     "triangles60_round": (lambda: scenes.triangle_drop(60, seed=15, kinds=("sphere", "capsule")), 240),
     "triangles50_boxes_sleep": (lambda: scenes.with_flags(scenes.triangle_drop(50, seed=16, kinds=("box",)), 1), 300),
+    "mesh40": (lambda: scenes.mesh_drop(40), 200),
+    "mesh30_hulls_bumpy": (lambda: scenes.mesh_drop(30, hull_count=8, bumps=0.8, cells=12), 200),
+    "mesh30_scaled": (lambda: scenes.mesh_drop(30, scale=(1.5, 0.8, 1.2), seed=5), 200),
+    "mesh30_sensors": (lambda: scenes.mesh_drop(30, sensors=0.3, seed=7), 200),
+    "mesh30_sleep": (lambda: scenes.with_flags(scenes.mesh_drop(30, seed=3), 1), 300),
 }
 
 
@@ -134,6 +139,43 @@ def test_batched_triangle_grounds_match_single(gpu_ctx):
         diffs = compare_worlds(DeviceWorld(batch, w), r)
         assert not diffs, f"world {w}: {diffs}"
         assert batch.counts(w).status == 0
+
+
+def test_batched_mesh_grounds_match_single(gpu_ctx):
+    """Six worlds on one shared heightfield mesh (spheres, capsules, boxes and random hulls; mesh
+    contacts with up to three cluster manifolds) in one launch equal the oracle world by world."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    scene = scenes.mesh_drop(25, n_worlds=6, seed=33, hull_count=8)
+    print("[parity] batched mesh grounds x6: 150 steps", flush=True)
+    batch = device.Batch(gpu_ctx, scene)
+    refs = [CpuWorld("oracle", scene, w) for w in range(6)]
+    for _ in range(150):
+        batch.step()
+        for r in refs:
+            r.step()
+    for w, r in enumerate(refs):
+        diffs = compare_worlds(DeviceWorld(batch, w), r)
+        assert not diffs, f"world {w}: {diffs}"
+        assert batch.counts(w).status == 0
+
+
+def test_fine_mesh_with_large_triangle_lists(gpu_ctx):
+    """A 32 x 32-cell mesh of 0.375 m cells under 60 mixed bodies: mesh contacts overlap more than
+    the default 64 triangles, so the batch is built with 256 per contact."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    scene = scenes.mesh_drop(60, cells=32, bumps=0.8, hull_count=8, seed=9)
+    print("[parity] fine mesh: 200 steps", flush=True)
+    batch = device.Batch(gpu_ctx, scene, mesh_triangles=256)
+    ref = CpuWorld("oracle", scene)
+    for s in range(200):
+        batch.step()
+        ref.step()
+        if s % 20 == 19:
+            diffs = compare_worlds(DeviceWorld(batch, 0), ref)
+            assert not diffs, f"step {s}: {diffs}"
+    assert batch.counts(0).status == 0
 
 
 def test_batched_mixed_worlds_match_single(gpu_ctx):
