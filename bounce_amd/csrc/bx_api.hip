@@ -1183,6 +1183,8 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     if (!rc) rc = UploadLibrary(b, *b->library);
     if (rc) { bx_batch_destroy(b); return rc; }
     V.islandLdsWords = lds;
+    V.meshWorlds = 0;
+    for (u32 w = 0; w < world_count; ++w) V.meshWorlds += b->worlds[w].meshTriCap ? 1u : 0u;
 #define PUT(dev, host)                                                                                      \
     if (!host.empty()) {                                                                                    \
         hipError_t _e = hipMemcpy((void*)(dev), host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice); \

@@ -59,7 +59,7 @@ __device__ void BuildAdjacency(const BatchView& B, const WorldInfo& wi, const Wo
 
 // The island builder of one world over its working arrays at `base`: LDS (kLds, DS instructions)
 // when the world's arrays fit the block's LDS at the current contact count, else its global area.
-template <bool kLds>
+template <bool kLds, bool kMesh>
 __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const WorldInfo& wi, WorldState& ws,
                                              u32* base) {
     const u32 nb = wi.bodyCount, nf = wi.fixtureCount, n = ws.contactCount;
@@ -165,9 +165,14 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
                         u32 fa = cfa[k], fbb = cfb[k];
                         if ((fbody[fa] | fbody[fbb]) & 0x80000000u) continue;  // sensors
                         // one constraint entry per manifold (a mesh contact has up to three,
-                        // solved one after the other as b3ContactSolver does)
-                        for (u32 j = 0; j < (cf >> 2); ++j)
-                            icontacts[nc++] = make_uint4(order[k] | (j << 30), fbody[fa], fbody[fbb], 0u);
+                        // solved one after the other as b3ContactSolver does; a touching convex
+                        // contact has one)
+                        if (kMesh) {
+                            for (u32 j = 0; j < (cf >> 2); ++j)
+                                icontacts[nc++] = make_uint4(order[k] | (j << 30), fbody[fa], fbody[fbb], 0u);
+                        } else {
+                            icontacts[nc++] = make_uint4(order[k], fbody[fa], fbody[fbb], 0u);
+                        }
                         cfl[k] = cf | 1u;
                         u32 other = (fa == f ? fbody[fbb] : fbody[fa]) & 0x7FFFFFFFu;
                         if (bflags[other] & kIsland) continue;
@@ -235,14 +240,15 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
     }
 }
 
+template <bool kMesh>
 __global__ void __launch_bounds__(kBlock) k_islands(BatchView B) {
     extern __shared__ u32 lds[];
     const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
     WorldState& ws = B.wstate[w];
     if (IslandScratchWords(wi.bodyCount, wi.fixtureCount, ws.contactCount) <= B.islandLdsWords)
-        BuildIslands<true>(B, w, wi, ws, lds);
+        BuildIslands<true, kMesh>(B, w, wi, ws, lds);
     else
-        BuildIslands<false>(B, w, wi, ws, B.islandScratch + wi.islandScratchBase);
+        BuildIslands<false, kMesh>(B, w, wi, ws, B.islandScratch + wi.islandScratchBase);
 }
 

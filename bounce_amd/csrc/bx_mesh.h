@@ -177,8 +177,9 @@ __device__ bool MeshContactTestOverlap(const BatchView& B, const WorldInfo& wi, 
 
 // The manifold of one child triangle (hull_mesh_contact.cpp, mesh_sphere_contact.cpp,
 // mesh_capsule_contact.cpp), from a fresh manifold, its points tagged with the triangle.
-__device__ void MeshTriangleManifold(const BatchView& B, const Fixture& fA, const Fixture& fB, const Transform& xfA,
-                                     const Transform& xfB, u32 triangle, Manifold& m, u32& overflow) {
+__device__ __noinline__ void MeshTriangleManifold(const BatchView& B, const Fixture& fA, const Fixture& fB,
+                                                  const Transform& xfA, const Transform& xfB, u32 triangle, Manifold& m,
+                                                  FaceScratch& scratch, u32& overflow) {
     for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
         ManifoldPoint& p = m.points[k];
         p.localNormal1 = p.localPoint1 = p.localPoint2 = Vec3(scalar(0), scalar(0), scalar(0));
@@ -200,7 +201,6 @@ __device__ void MeshTriangleManifold(const BatchView& B, const Fixture& fA, cons
     } else {
         MeshChild(B, fB, triangle, t, w);
         HullView hA = GetHull(B, fA.shapeIndex);
-        FaceScratch scratch;
         CollideHullAndTriangle(m, xfA, hA, fA.radius, xfB, t, scratch, overflow, &w);
     }
     for (u32 k = 0; k < m.pointCount; ++k) m.points[k].triangleKey = triangle;
@@ -400,10 +400,11 @@ __device__ u32 ReducePolygonIndexed(ClusterVertex out[kMaxManifoldPoints], Pos p
 
 // b3MeshContact::Collide (mesh_contact.cpp:328-371) + b3ClusterSolver::Run (cluster.cpp:471-584):
 // the cluster manifolds of a mesh contact into nm[0, returned count), which arrive initialized
-// (b3Manifold::Initialize). `lane` is the lane's stripe of the world's clustering scratch.
+// (b3Manifold::Initialize). `lane` is the lane's stripe of the world's clustering scratch and
+// `scratch` the caller's hull clipping buffers.
 __device__ __noinline__ u32 MeshCollide(const BatchView& B, const WorldInfo& wi, u32 slot, const Fixture& fA,
                                         const Fixture& fB, const Transform& xfA, const Transform& xfB, MeshObs* lane,
-                                        Manifold* nm, u32& overflow) {
+                                        Manifold* nm, FaceScratch& scratch, u32& overflow) {
     const u32 nt = B.meshContacts[wi.meshBase + slot].triangleCount;
     const u32* tris = MeshTriangles(B, wi, slot);
     MeshObs* obs = lane + kMeshManifoldObs;
@@ -413,7 +414,7 @@ __device__ __noinline__ u32 MeshCollide(const BatchView& B, const WorldInfo& wi,
     u32 no = 0;
     for (u32 i = 0; i < nt; ++i) {
         Manifold m;
-        MeshTriangleManifold(B, fA, fB, xfA, xfB, tris[i], m, overflow);
+        MeshTriangleManifold(B, fA, fB, xfA, xfB, tris[i], m, scratch, overflow);
         for (u32 j = 0; j < m.pointCount; ++j) {
             const ManifoldPoint& p = m.points[j];
             // b3WorldManifold point (collision.cpp:63-75)
@@ -459,7 +460,7 @@ __device__ __noinline__ u32 MeshCollide(const BatchView& B, const WorldInfo& wi,
         for (u32 j = 0; j < nq; ++j) {
             const u32 tag = obs[members[quad[j].clipIndex]].tag & 0x0FFFFFFFu;
             Manifold m;
-            MeshTriangleManifold(B, fA, fB, xfA, xfB, tris[tag >> 2], m, overflow);
+            MeshTriangleManifold(B, fA, fB, xfA, xfB, tris[tag >> 2], m, scratch, overflow);
             const u32 pt = tag & 3u;
             out.points[j] = pt == 0u ? m.points[0] : (pt == 1u ? m.points[1] : (pt == 2u ? m.points[2] : m.points[3]));
         }

@@ -157,7 +157,7 @@ __device__ __forceinline__ void MatchManifold(Manifold& m, const Manifold& old) 
 // motor impulses come from the last old manifold). Returns touching.
 __device__ __noinline__ bool UpdateMeshManifolds(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot,
                                                  Contact& c, const Fixture& fA, const Fixture& fB,
-                                                 const Transform& xfA, const Transform& xfB) {
+                                                 const Transform& xfA, const Transform& xfB, FaceScratch& scratch) {
     MeshObs* lane = B.meshScratch + wi.meshScratchBase + threadIdx.x * MeshLaneStride(wi.meshTriCap);
     Manifold* nm = reinterpret_cast<Manifold*>(lane);
     for (u32 i = 0; i < kMaxManifolds; ++i) {
@@ -173,7 +173,7 @@ __device__ __noinline__ bool UpdateMeshManifolds(const BatchView& B, const World
     }
     const u32 oldCount = c.manifoldCount;
     u32 overflow = 0;
-    const u32 count = MeshCollide(B, wi, slot, fA, fB, xfA, xfB, lane, nm, overflow);
+    const u32 count = MeshCollide(B, wi, slot, fA, fB, xfA, xfB, lane, nm, scratch, overflow);
     if (overflow) atomicOr(&ws.status, 0x2u);
     if ((wi.flags & 0x2u) == 0)
         for (u32 i = 0; i < count; ++i)
@@ -188,7 +188,9 @@ __device__ __noinline__ bool UpdateMeshManifolds(const BatchView& B, const World
 }
 
 // b3ContactManager::UpdateContacts (contact_manager.cpp:165-227) + b3Contact::Update
-// (contact.cpp:183-285) for one contact. Returns false if the contact is destroyed.
+// (contact.cpp:183-285) for one contact. Returns false if the contact is destroyed. kMesh compiles
+// the mesh contact path in (launches over batches that hold a mesh fixture).
+template <bool kMesh>
 __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot, bool applyWakes) {
     Contact& c = B.contacts[wi.contactBase + slot];
     const Fixture fA = B.fixtures[wi.fixtureBase + c.fixtureA];
@@ -196,7 +198,8 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     Body& bA = B.bodies[wi.bodyBase + fA.body];
     Body& bB = B.bodies[wi.bodyBase + fB.body];
     bool sensor = fA.isSensor || fB.isSensor;
-    const bool isMesh = fA.shapeType == kMeshShape || fB.shapeType == kMeshShape;
+    const bool isMesh = kMesh && (fA.shapeType == kMeshShape || fB.shapeType == kMeshShape);
+    FaceScratch scratch;  // hull clipping buffers of every narrowphase below
     auto destroy = [&]() {
         if (applyWakes && !sensor) {
             for (u32 j = 0; j < c.manifoldCount; ++j) {
@@ -221,7 +224,8 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
         touching = MeshContactTestOverlap(B, wi, slot, c, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q));
         c.manifoldCount = 0;
     } else if (isMesh) {
-        touching = UpdateMeshManifolds(B, wi, ws, slot, c, fA, fB, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q));
+        touching = UpdateMeshManifolds(B, wi, ws, slot, c, fA, fB, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q),
+                                       scratch);
         if (touching != wasTouching && applyWakes) {
             WakeBody(bA);
             WakeBody(bB);
@@ -258,7 +262,6 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
             Transform xf0A = SweepTransform(bA.x0, bA.q0, bA.x, bA.q, bA.localCenter, scalar(0));
             Transform xf0B = SweepTransform(bB.x0, bB.q0, bB.x, bB.q, bB.localCenter, scalar(0));
             HullView hA = GetHull(B, fA.shapeIndex), hB = GetHull(B, fB.shapeIndex);
-            FaceScratch scratch;
             u32 overflow = 0;
             FeatureCache cache = c.cache;
             CollideHulls(m, xfA, hA, fA.radius, xfB, hB, fB.radius, cache, xf0A, xf0B, scratch, overflow);
@@ -282,7 +285,6 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
             CollideTriangleAndCapsule(m, xfA, fA, xfB, fB);
         } else if (fA.shapeType == 3u && fB.shapeType == 2u) {  // hull_triangle_contact.cpp
             HullView hA = GetHull(B, fA.shapeIndex);
-            FaceScratch scratch;
             u32 overflow = 0;
             CollideHullAndTriangle(m, xfA, hA, fA.radius, xfB, fB, scratch, overflow);
             if (overflow) atomicOr(&ws.status, 0x2u);
@@ -306,6 +308,7 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
 #ifndef BX_UC_BLOCKS_PER_CU
 #define BX_UC_BLOCKS_PER_CU 4
 #endif
+template <bool kMesh>
 __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts(BatchView B) {
     const u32 w = WorldOfBlock(B);
     const WorldInfo wi = B.worlds[w];
@@ -323,9 +326,10 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
     }
     asleep = __syncthreads_or(asleep);
     if (!asleep) {
-        for (u32 i = threadIdx.x; i < n; i += blockDim.x) alive[i] = UpdateOneContact(B, wi, ws, order[i], true) ? 1u : 0u;
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            alive[i] = UpdateOneContact<kMesh>(B, wi, ws, order[i], true) ? 1u : 0u;
     } else if (threadIdx.x == 0) {
-        for (u32 k = n; k-- > 0;) alive[k] = UpdateOneContact(B, wi, ws, order[k], true) ? 1u : 0u;
+        for (u32 k = n; k-- > 0;) alive[k] = UpdateOneContact<kMesh>(B, wi, ws, order[k], true) ? 1u : 0u;
     }
     __syncthreads();
     // stable compaction of the order list; destroyed slots go back on the free stack
@@ -579,10 +583,12 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
     mark(0);
     if (findFirst) hipLaunchKernelGGL(k_find_pairs, grid, dim3(kBlock), 0, stream, B, 1u);
     mark(1);
-    hipLaunchKernelGGL(k_update_contacts, grid, dim3(kBlock), 0, stream, B);
+    if (B.meshWorlds) hipLaunchKernelGGL(k_update_contacts<true>, grid, dim3(kBlock), 0, stream, B);
+    else hipLaunchKernelGGL(k_update_contacts<false>, grid, dim3(kBlock), 0, stream, B);
     mark(2);
     if (sp.dt > scalar(0)) {
-        hipLaunchKernelGGL(k_islands, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
+        if (B.meshWorlds) hipLaunchKernelGGL(k_islands<true>, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
+        else hipLaunchKernelGGL(k_islands<false>, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
         mark(3);
         // one lane per island; islands <= bodies of the launched worlds
         u32 vblocks = std::max(1u, std::min((maxIslands + kSolveLanes - 1) / kSolveLanes, 16384u));

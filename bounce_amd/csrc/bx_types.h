@@ -344,6 +344,7 @@ struct BatchView {
     u32* islandListCount;
     u32* islandScratch;       // island-builder working arrays of worlds whose arrays do not fit LDS
     u32 islandLdsWords;       // LDS words available to the island builder per world
+    u32 meshWorlds;           // nonzero when some world holds a mesh fixture (mesh kernel variants)
 };
 
 // World stepped by this workgroup in the one-workgroup-per-world kernels.
