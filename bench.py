@@ -2,11 +2,15 @@
 """Benchmark of the batched b3World::Step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds 4096] [--levels 20]
+    python bench.py --workload mixed --worlds 2048
 
-Workload: the batched-stacking configuration, 4,096 independent worlds, each a 20-level box
-pyramid (210 unit boxes, friction 0.6, seeded +-1 mm jitter) on a static ground box, stepped
+Workload (default): the batched-stacking configuration, 4,096 independent worlds, each a 20-level
+box pyramid (210 unit boxes, friction 0.6, seeded +-1 mm jitter) on a static ground box, stepped
 together at dt = 1/60 s with 8 velocity / 2 position iterations in FP32 (sleeping off). A "step"
 is one b3World::Step of every world of the batch; inputs are resident in HBM before timing.
+`--workload mixed` runs the mixed-shapes batch instead: per-world body counts lognormal (median
+200, clipped to 20..2000), spheres, capsules and random hulls of 8-16 points dropped on a ground box,
+and a 10-link revolute chain with a motor per world (scenes.mixed_worlds, seed 8).
 
 Printed (one JSON line): body-steps/s of the batched path (value), the plain per-world GPU path
 (one world per launch sequence, measured on a sample of worlds), their ratio, p50/p99 step
@@ -48,6 +52,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=60)
     p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--workload", choices=("stacking", "mixed"), default="stacking")
     p.add_argument("--worlds", type=int, default=4096)
     p.add_argument("--levels", type=int, default=20)
     p.add_argument("--plain-worlds", type=int, default=16)
@@ -62,7 +67,7 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(args, scene_factory, body_count_per_world):
+def cpu_baseline(args, scene_factory):
     """Reference Step (oracle/_ref build) on host cores, one world per thread, bounded sample."""
     try:
         from oracle import oracle as O
@@ -73,7 +78,7 @@ def cpu_baseline(args, scene_factory, body_count_per_world):
     threads = min(os.cpu_count() or 1, 16)
     sample = scene_factory(args.cpu_worlds)
     secs = O.bench(lib, sample, warmup=0, steps=args.cpu_steps, threads=threads)
-    value = args.cpu_worlds * body_count_per_world * args.cpu_steps / secs if secs > 0 else None
+    value = sample.body_count * args.cpu_steps / secs if secs > 0 else None
     return {"value": value, "unit": "body-steps/s", "cores": threads, "kind": kind,
             "sample": f"{args.cpu_worlds} worlds x {args.cpu_steps} steps from step 0, one world per thread"}
 
@@ -109,6 +114,8 @@ def main():
     from bounce_amd import device, scenes
 
     def factory(n):
+        if args.workload == "mixed":
+            return scenes.mixed_worlds(n)
         return scenes.pyramids(n, levels=args.levels, seed=2)
 
     scene = factory(args.worlds)
@@ -155,7 +162,7 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tj = json.load(f)
-        if tj.get("worlds") == args.worlds and tj.get("levels") == args.levels:
+        if args.workload == "stacking" and tj.get("worlds") == args.worlds and tj.get("levels") == args.levels:
             traffic = tj["bytes_per_launch"].get(dominant)
     batch.close()
     # plain per-world path: the same kernels, one world per launch sequence
@@ -178,7 +185,7 @@ def main():
                  "sample": f"{args.plain_worlds} worlds x {plain_steps} steps, one world per launch"}
         pb.close()
     ctx.close()
-    cpu = None if args.no_cpu else cpu_baseline(args, factory, bodies_per_world)
+    cpu = None if args.no_cpu else cpu_baseline(args, factory)
     serving = None if args.serve_seconds <= 0 else serve(args)
     out = {
         "metric": "body-steps/sec on 1 MI355X, batched vs plain per-world; p99 step latency",
@@ -192,9 +199,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded pyramids)",
-        "config": {"workload": f"batched stacking: {args.worlds} worlds x {args.levels}-level box pyramid "
-                               f"({bodies_per_world} bodies incl. ground), dt=1/60, 8/2 iterations, sleep off",
+        "data": "synthetic (seeded pyramids)" if args.workload == "stacking" else "synthetic (seeded mixed worlds)",
+        "config": {"workload": (f"batched stacking: {args.worlds} worlds x {args.levels}-level box pyramid "
+                                f"({bodies_per_world} bodies incl. ground), dt=1/60, 8/2 iterations, sleep off")
+                   if args.workload == "stacking" else
+                   (f"batched mixed shapes: {args.worlds} worlds, {total_bodies} bodies (lognormal median 200, "
+                    f"20..2000 per world; spheres, capsules, 8-16 point hulls, a 10-link motor chain), "
+                    f"dt=1/60, 8/2 iterations, sleep off"),
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
         "plain_per_world": plain,
         "batched_over_plain": value / plain["value"] if plain else None,

@@ -643,7 +643,9 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
     wi.serialIslands = 0;
     if ((d.body_count && !d.bodies) || (d.fixture_count && !d.fixtures) || (d.joint_count && !d.joints))
         return Fail(BX_E_INVALID, "null descriptor array");
-    if (d.fixture_count > 0xFFFFu || d.body_count > 0xFFFFFFu || d.joint_count > 0xFFFFu)
+    // the island builder packs a body's first fixture-list entry and first joint edge (two per
+    // joint) with their counts into 16-bit halves
+    if (d.fixture_count > 0xFFFFu || d.body_count > 0xFFFFFFu || d.joint_count > 0x7FFFu)
         return Fail(BX_E_INVALID, "world too large");
     // bodies (b3Body::b3Body, body.cpp:25-103)
     std::vector<Body>& bodies = hw.bodies;
