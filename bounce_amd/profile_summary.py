@@ -28,7 +28,7 @@ STEP_KERNELS = ["k_update_contacts", "k_islands", "k_integrate_vel", "k_init_con
 
 
 def short(name):
-    m = re.search(r"bx::(k_\w+)\(", name)
+    m = re.search(r"bx::(k_\w+)(?:<[^>]*>)?\(", name)
     if m:
         return m.group(1)
     m = re.search(r"(k_copy|k_fill)\(", name)
