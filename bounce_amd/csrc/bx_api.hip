@@ -777,8 +777,6 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= d.body_count || jd.body_b >= d.body_count ||
             jd.body_a == jd.body_b)
             return Fail(BX_E_INVALID, "bad joint descriptor");
-        if (jd.enable_limit)
-            return Fail(BX_E_UNSUPPORTED, "revolute joint limits are not supported by the device path");
         Joint j;
         std::memset(&j, 0, sizeof(j));
         j.bodyA = jd.body_a;
@@ -805,7 +803,7 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
             j.referenceRotation = Q4(jd.reference_rotation);
         }
         j.enableMotor = jd.enable_motor ? 1u : 0u;
-        j.enableLimit = 0;
+        j.enableLimit = jd.enable_limit ? 1u : 0u;
         j.lowerAngle = jd.lower_angle;
         j.upperAngle = jd.upper_angle;
         j.motorSpeed = jd.motor_speed;

@@ -1,10 +1,13 @@
-// bx_calibrate.hip -- plain device copy of a known size, for calibrating HBM traffic counters.
-//
-// libbxcal.so is measurement support, not part of the step: a profiler's FETCH_SIZE and
-// WRITE_SIZE readings for k_copy (which reads and writes exactly `bytes` once each) give the
-// factor that turns the same counters read for the step's kernels into bytes.
+// bx_calibrate.hip -- measurement and check support around the step library:
+//   * a plain device copy of a known size, for calibrating HBM traffic counters: a profiler's
+//     FETCH_SIZE and WRITE_SIZE readings for k_copy (which reads and writes exactly `bytes` once
+//     each) give the factor that turns the same counters read for the step's kernels into bytes;
+//   * the revolute joint's limit angle (bx::JointAngle, the step's own function) evaluated on the
+//     device over given inputs, so tests can compare it with the host's atan2.
+// libbxcal.so is not part of the step.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "bx_joint.h"
 
 namespace {
 
@@ -16,6 +19,11 @@ __global__ void k_fill(uint4* p, size_t n, uint32_t v) {
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
         dst[i] = src[i];
+}
+
+__global__ void k_joint_angles(const float* s, const float* c, float* out, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = bx::JointAngle(s[i], c[i]);
 }
 
 }  // namespace
@@ -48,6 +56,31 @@ double bxcal_copy(uint64_t bytes, int reps) {
     (void)hipFree(b);
     if (err != hipSuccess) return -3.0;
     return double(ms) * 1e3 / reps;
+}
+
+// out[i] = bx::JointAngle(s[i], c[i]) computed on the device (host arrays of n floats). Returns 0,
+// or a negative value on a HIP error.
+int bxcal_joint_angles(const float* s, const float* c, float* out, uint64_t n) {
+    if (n == 0) return 0;
+    if (!s || !c || !out) return -1;
+    float *ds = nullptr, *dc = nullptr, *dout = nullptr;
+    const size_t bytes = size_t(n) * sizeof(float);
+    int rc = 0;
+    if (hipMalloc(&ds, bytes) != hipSuccess || hipMalloc(&dc, bytes) != hipSuccess ||
+        hipMalloc(&dout, bytes) != hipSuccess) {
+        rc = -2;
+    } else if (hipMemcpy(ds, s, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = -3;
+    } else {
+        k_joint_angles<<<1024, 256>>>(ds, dc, dout, size_t(n));
+        if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = -4;
+    }
+    if (ds) (void)hipFree(ds);
+    if (dc) (void)hipFree(dc);
+    if (dout) (void)hipFree(dout);
+    return rc;
 }
 
 }  // extern "C"

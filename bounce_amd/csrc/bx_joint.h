@@ -1,10 +1,11 @@
 // bx_joint.h -- the revolute joint's solver rows for the device island sweeps.
 //
 // Same binary32 expression trees as revolute_joint.cpp:78-531 (InitializeVelocityConstraints,
-// WarmStart, SolveVelocityConstraints, SolvePositionConstraints). Joint limits need the joint
-// angle through atan2 (revolute_joint.cpp:126, 437), whose last bit the device library does not
-// share with the host's; worlds with limited joints are rejected at upload, so the limit rows are
-// kept here only for completeness of the state machine (enableLimit is always 0 on the device).
+// WarmStart, SolveVelocityConstraints, SolvePositionConstraints). The joint angle of the limit rows
+// (revolute_joint.cpp:135, 434) is the reference's atan2 of the two binary32 cosines promoted to
+// double, rounded to binary32: JointAngle evaluates it in double as well, so the binary32 angle is
+// the reference's unless the two double results fall on opposite sides of a binary32 rounding
+// boundary (their last-bit difference makes that a chance of about 2^-26 per evaluation).
 #pragma once
 #include "bx_types.h"
 
@@ -55,6 +56,12 @@ BX_HD Quat Mat33Quat(const Mat33& m) {
     return r;
 }
 
+// b3RevoluteJoint's limit states (revolute_joint.h)
+constexpr u32 kInactiveLimit = 0, kAtLowerLimit = 1, kAtUpperLimit = 2, kEqualLimits = 3;
+
+// The relative angle of the joint frames: atan2(s, c) on the binary32 arguments promoted to double.
+BX_HD scalar JointAngle(scalar s, scalar c) { return scalar(::atan2(double(s), double(c))); }
+
 // revolute_joint.cpp:78-231. `pos` is the island's position array, `indexA/B` the joint bodies'
 // island-local indices in this island.
 BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
@@ -76,7 +83,7 @@ BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u3
     jc.lowerAngle = j.lowerAngle;
     jc.upperAngle = j.upperAngle;
     jc.enableMotor = j.enableMotor;
-    jc.enableLimit = 0;
+    jc.enableLimit = j.enableLimit;
     jc.motorSpeed = j.motorSpeed;
     jc.maxMotorTorque = j.maxMotorTorque;
     Quat qA = pos[indexA].q;
@@ -89,10 +96,10 @@ BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u3
     Quat fB = qB * j.localRotationB;
     Mat33 RfA = fA.GetRotationMatrix();
     Mat33 RfB = fB.GetRotationMatrix();
-    Vec3 w1 = RfA.z;
+    Vec3 u1 = RfA.x, v1 = RfA.y, w1 = RfA.z;
     Vec3 u2 = RfB.x, v2 = RfB.y;
     scalar motorImpulse = j.motorImpulse;
-    if (j.enableMotor) {
+    if (j.enableMotor || j.enableLimit) {
         jc.motorAxis = w1;
         jc.motorAxis.Normalize();
         scalar K = Dot((iA + iB) * jc.motorAxis, jc.motorAxis);
@@ -100,12 +107,32 @@ BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u3
     } else {
         jc.motorAxis.SetZero();
         jc.motorMass = scalar(0);
-        motorImpulse = scalar(0);
     }
-    // without limits the limit state is inactive and its impulse zero (revolute_joint.cpp:163-167)
-    j.limitState = 0;
-    jc.limitState = 0;
-    scalar limitImpulse = scalar(0);
+    if (!j.enableMotor) motorImpulse = scalar(0);
+    // the limit state carries over from the last step; entering a state clears the limit impulse
+    // (revolute_joint.cpp:122-168)
+    u32 limitState = j.limitState;
+    scalar limitImpulse = j.limitImpulse;
+    if (j.enableLimit) {
+        scalar c = Dot(u2, u1);
+        scalar s = Dot(u2, v1);
+        scalar angle = JointAngle(s, c);
+        if (Abs(j.upperAngle - j.lowerAngle) < scalar(2) * kAngularSlop) {
+            if (limitState != kEqualLimits) { limitState = kEqualLimits; limitImpulse = scalar(0); }
+        } else if (angle <= j.lowerAngle) {
+            if (limitState != kAtLowerLimit) { limitState = kAtLowerLimit; limitImpulse = scalar(0); }
+        } else if (angle >= j.upperAngle) {
+            if (limitState != kAtUpperLimit) { limitState = kAtUpperLimit; limitImpulse = scalar(0); }
+        } else {
+            limitState = kInactiveLimit;
+            limitImpulse = scalar(0);
+        }
+    } else {
+        limitState = kInactiveLimit;
+        limitImpulse = scalar(0);
+    }
+    j.limitState = limitState;
+    jc.limitState = limitState;
     {
         jc.rA = Mul(qA, j.localAnchorA - jc.localCenterA);
         jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
@@ -153,8 +180,13 @@ template <class V>
 BX_HD void WarmStartJoint(const JointCon& jc, V* vel) {
     Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
     Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
-    if (jc.enableMotor) {  // limitState is never e_equalLimits without limits
+    if (jc.enableMotor && jc.limitState != kEqualLimits) {
         Vec3 P = jc.motorAxis * jc.motorImpulse;
+        wA -= jc.iA * P;
+        wB += jc.iB * P;
+    }
+    if (jc.enableLimit && jc.limitState != kInactiveLimit) {
+        Vec3 P = jc.motorAxis * jc.limitImpulse;
         wA -= jc.iA * P;
         wB += jc.iB * P;
     }
@@ -181,7 +213,7 @@ BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
     Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
     Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
     Mat33 iA = jc.iA, iB = jc.iB;
-    if (jc.enableMotor) {
+    if (jc.enableMotor && jc.limitState != kEqualLimits) {
         scalar dw = Dot(wB - wA, jc.motorAxis);
         scalar Cdot = dw - jc.motorSpeed;
         scalar impulse = -jc.motorMass * Cdot;
@@ -190,6 +222,24 @@ BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
         jc.motorImpulse = Clamp(jc.motorImpulse + impulse, -maxImpulse, maxImpulse);
         impulse = jc.motorImpulse - oldImpulse;
         Vec3 P = jc.motorAxis * impulse;
+        wA -= iA * P;
+        wB += iB * P;
+    }
+    if (jc.enableLimit && jc.limitState != kInactiveLimit) {
+        scalar Cdot = Dot(wB - wA, jc.motorAxis);
+        scalar impulse = -jc.motorMass * Cdot;
+        if (jc.limitState == kEqualLimits) {
+            jc.limitImpulse += impulse;
+        } else if (jc.limitState == kAtLowerLimit) {
+            scalar oldImpulse = jc.limitImpulse;
+            jc.limitImpulse = Max(jc.limitImpulse + impulse, scalar(0));
+            impulse = jc.limitImpulse - oldImpulse;
+        } else if (jc.limitState == kAtUpperLimit) {
+            scalar oldImpulse = jc.limitImpulse;
+            jc.limitImpulse = Min(jc.limitImpulse + impulse, scalar(0));
+            impulse = jc.limitImpulse - oldImpulse;
+        }
+        Vec3 P = impulse * jc.motorAxis;
         wA -= iA * P;
         wB += iB * P;
     }
@@ -219,7 +269,7 @@ BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
     vel[jc.indexB].w = wB;
 }
 
-// revolute_joint.cpp:369-531 without the limit row. `P` is an array of PosBody.
+// revolute_joint.cpp:369-531. `P` is an array of PosBody.
 template <class P>
 BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
     Vec3 xA = pos[jc.indexA].x;
@@ -248,6 +298,37 @@ BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
         iA = RotateToFrame(jc.localInvIA, qA);
         xB += mB * impulse;
         qB += Derivative(qB, iB * Cross(rB, impulse));
+        qB.Normalize();
+        iB = RotateToFrame(jc.localInvIB, qB);
+    }
+    scalar limitError = scalar(0);
+    if (jc.enableLimit) {
+        Quat fA = qA * jc.localRotationA;
+        Quat fB = qB * jc.localRotationB;
+        Mat33 RfA = fA.GetRotationMatrix();
+        Mat33 RfB = fB.GetRotationMatrix();
+        Vec3 u1 = RfA.x, v1 = RfA.y, w1 = RfA.z;
+        Vec3 u2 = RfB.x;
+        Vec3 limitAxis = w1;
+        scalar K = Dot((iA + iB) * limitAxis, limitAxis);
+        scalar limitMass = K > scalar(0) ? scalar(1) / K : scalar(0);
+        scalar c = Dot(u2, u1);
+        scalar s = Dot(u2, v1);
+        scalar angle = JointAngle(s, c);
+        scalar C = scalar(0);
+        if (Abs(jc.upperAngle - jc.lowerAngle) < scalar(2) * kAngularSlop)
+            C = Clamp(angle - jc.lowerAngle, -kMaxAngularCorrection, kMaxAngularCorrection);
+        else if (angle <= jc.lowerAngle)
+            C = Clamp(angle - jc.lowerAngle + kAngularSlop, -kMaxAngularCorrection, scalar(0));
+        else if (angle >= jc.upperAngle)
+            C = Clamp(angle - jc.upperAngle - kAngularSlop, scalar(0), kMaxAngularCorrection);
+        limitError = Abs(C);
+        scalar limitImpulse = -C * limitMass;
+        Vec3 Pl = limitImpulse * limitAxis;
+        qA -= Derivative(qA, iA * Pl);
+        qA.Normalize();
+        iA = RotateToFrame(jc.localInvIA, qA);
+        qB += Derivative(qB, iB * Pl);
         qB.Normalize();
         iB = RotateToFrame(jc.localInvIB, qB);
     }
@@ -289,7 +370,7 @@ BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
     pos[jc.indexB].q = qB;
     pos[jc.indexA].I = iA;
     pos[jc.indexB].I = iB;
-    return linearError <= kLinearSlop && angularError <= kCosSlop;  // limitError is 0 without limits
+    return linearError <= kLinearSlop && limitError <= kAngularSlop && angularError <= kCosSlop;
 }
 
 }  // namespace bx

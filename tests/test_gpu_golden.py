@@ -5,8 +5,7 @@ Each fixture holds, for every step, a SHA-256 of the reference FP32 build's full
 device library steps the same scene and its state hash must equal the reference's at every step:
 the product is pinned to the reference directly, not only through the oracle.
 
-Two fixtures are not replayed: chain10_limit (revolute limits need atan2, which the device path
-rejects) and mixed150_dense_sleep (kept off the GPU; see tests/test_gpu_parity.py).
+One fixture is not replayed: mixed150_dense_sleep (kept off the GPU; see tests/test_gpu_parity.py).
 """
 import os
 
@@ -18,7 +17,7 @@ from tests.golden.make_golden import SCENES, state_hash
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NOT_REPLAYED = {"chain10_limit", "mixed150_dense_sleep"}
+NOT_REPLAYED = {"mixed150_dense_sleep"}
 
 
 class DeviceWorld:

@@ -41,6 +41,9 @@ SCENES = {
     "chain10_motor": (lambda: scenes.chains(1, 10), 240),
     "chain10_free": (lambda: scenes.chains(1, 10, motor=False), 150),
     "chain10_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, extra=20), 1), 300),
+    "chain10_limit": (lambda: scenes.chains(1, 10, motor=False, limit=True), 240),
+    "chain10_motor_limit": (lambda: scenes.chains(1, 10, motor=True, limit=True), 240),
+    "chain10_limit_extra_sleep": (lambda: scenes.with_flags(scenes.chains(1, 10, limit=True, extra=20), 1), 300),
     "triangles40_hulls": (lambda: scenes.triangle_drop(40, hull_count=8), 240),
     "triangles60_round": (lambda: scenes.triangle_drop(60, seed=15, kinds=("sphere", "capsule")), 240),
     "triangles50_boxes_sleep": (lambda: scenes.with_flags(scenes.triangle_drop(50, seed=16, kinds=("box",)), 1), 300),
