@@ -39,6 +39,7 @@ def command(out=LIB, extra=()):
 CAL_LIB = os.path.join(LIB_DIR, "libbxcal.so")
 FACADE_LIB = os.path.join(LIB_DIR, "libbounce_amd.so")
 HELLO_BIN = os.path.join(LIB_DIR, "bx_hello_world")
+MESH_BIN = os.path.join(LIB_DIR, "bx_mesh_ground")
 LIB_DIR_RPATH = "$ORIGIN"
 
 
@@ -55,6 +56,8 @@ def build(verbose=True):
                                      "-L", LIB_DIR, "-lbxstep", "-Wl,-rpath,$ORIGIN"]))
     jobs.append((HELLO_BIN, host + ["-o", HELLO_BIN + ".tmp", os.path.join(REPO, "tests", "drop_in", "hello_world.cpp"),
                                     "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
+    jobs.append((MESH_BIN, host + ["-o", MESH_BIN + ".tmp", os.path.join(REPO, "tests", "drop_in", "mesh_ground.cpp"),
+                                   "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
     for out, cmd in jobs:
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -100,7 +100,11 @@ b3Fixture* b3Body::CreateFixture(const b3FixtureDef& def) {
     case b3Shape::e_sphere: f->m_shape = new b3SphereShape(*static_cast<const b3SphereShape*>(def.shape)); break;
     case b3Shape::e_capsule: f->m_shape = new b3CapsuleShape(*static_cast<const b3CapsuleShape*>(def.shape)); break;
     case b3Shape::e_hull: f->m_shape = new b3HullShape(*static_cast<const b3HullShape*>(def.shape)); break;
-    default: delete f; return nullptr;  // triangles and meshes are not supported by the device step
+    case b3Shape::e_triangle:
+        f->m_shape = new b3TriangleShape(*static_cast<const b3TriangleShape*>(def.shape));
+        break;
+    case b3Shape::e_mesh: f->m_shape = new b3MeshShape(*static_cast<const b3MeshShape*>(def.shape)); break;
+    default: delete f; return nullptr;
     }
     f->m_type = def.shape->GetType();
     f->m_body = this;
@@ -224,10 +228,36 @@ int b3World::Upload() {
         hd[i].planes = &h->planes[0].normal.x;
         Copy3(hd[i].centroid, h->centroid);
     }
+    // every distinct b3Mesh once; the descriptor takes the triangles' vertex indices only
+    std::vector<const b3Mesh*> meshes;
+    std::unordered_map<const b3Mesh*, uint32> meshIndex;
+    for (b3Fixture* f : m_fixtures) {
+        if (f->m_type != b3Shape::e_mesh) continue;
+        const b3Mesh* m = static_cast<const b3MeshShape*>(f->m_shape)->m_mesh;
+        if (meshIndex.emplace(m, uint32(meshes.size())).second) meshes.push_back(m);
+    }
+    std::vector<std::vector<uint32>> meshTris(meshes.size());
+    std::vector<bx_mesh_desc> md(meshes.size());
+    for (size_t i = 0; i < meshes.size(); ++i) {
+        const b3Mesh* m = meshes[i];
+        std::vector<uint32>& t = meshTris[i];
+        t.resize(size_t(3) * m->triangleCount);
+        for (uint32 k = 0; k < m->triangleCount; ++k) {
+            t[3 * k] = m->triangles[k].v1;
+            t[3 * k + 1] = m->triangles[k].v2;
+            t[3 * k + 2] = m->triangles[k].v3;
+        }
+        md[i].vertex_count = m->vertexCount;
+        md[i].triangle_count = m->triangleCount;
+        md[i].vertices = m->vertexCount ? &m->vertices[0].x : nullptr;
+        md[i].triangles = t.data();
+    }
     bx_shape_library lib;
     std::memset(&lib, 0, sizeof(lib));
     lib.hull_count = uint32(hd.size());
     lib.hulls = hd.data();
+    lib.mesh_count = uint32(md.size());
+    lib.meshes = md.data();
     std::vector<bx_body_desc> bd(m_bodies.size());
     for (size_t i = 0; i < m_bodies.size(); ++i) {
         const b3Body* b = m_bodies[i];
@@ -266,6 +296,15 @@ int b3World::Upload() {
         } else if (f->m_type == b3Shape::e_capsule) {
             Copy3(d.vertex1, static_cast<const b3CapsuleShape*>(f->m_shape)->m_vertex1);
             Copy3(d.vertex2, static_cast<const b3CapsuleShape*>(f->m_shape)->m_vertex2);
+        } else if (f->m_type == b3Shape::e_triangle) {
+            const b3TriangleShape* t = static_cast<const b3TriangleShape*>(f->m_shape);
+            Copy3(d.vertex1, t->m_vertex1);
+            Copy3(d.vertex2, t->m_vertex2);
+            Copy3(d.vertex3, t->m_vertex3);
+        } else if (f->m_type == b3Shape::e_mesh) {
+            const b3MeshShape* m = static_cast<const b3MeshShape*>(f->m_shape);
+            d.shape_index = meshIndex[m->m_mesh];
+            Copy3(d.mesh_scale, m->m_scale);
         } else {
             d.shape_index = hullIndex[static_cast<const b3HullShape*>(f->m_shape)->m_hull];
         }

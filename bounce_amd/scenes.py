@@ -164,6 +164,45 @@ def hello_world():
     return s
 
 
+def mesh_ground_program():
+    """tests/drop_in/mesh_ground.cpp: a sphere (r 0.5), a capsule (1 m, r 0.3) and a 1 m box dropped
+    on a static 6 x 6-cell heightfield mesh of 1 m cells, heights ((7i + 3j) mod 5) / 10 - 0.2."""
+    s = SceneSet("mesh_ground_program")
+    box = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    n = 6
+    v = np.zeros(((n + 1) * (n + 1), 3), dtype=F32)
+    for i in range(n + 1):
+        for j in range(n + 1):
+            v[i * (n + 1) + j] = (F32(-3.0) + F32(i), F32((7 * i + 3 * j) % 5) * F32(0.1) - F32(0.2), F32(-3.0) + F32(j))
+    t = []
+    for i in range(n):
+        for j in range(n):
+            a, b = i * (n + 1) + j, (i + 1) * (n + 1) + j
+            c, d = (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            t.append((a, c, b))
+            t.append((a, d, c))
+    s.meshes.append((v, np.array(t, dtype=np.uint32)))
+    bodies = body_array(4)
+    fixtures = fixture_array(4)
+    fixtures[0]["shape_type"] = capi.BX_SHAPE_MESH
+    fixtures[0]["shape_index"] = 0
+    for k, pos in enumerate([(0.3, 2.5, -0.2), (-1.2, 3.0, 0.8), (1.1, 3.5, 1.3)], start=1):
+        bodies[k]["type"] = capi.BX_BODY_DYNAMIC
+        bodies[k]["position"] = pos
+        fixtures[k]["body"] = k
+        fixtures[k]["density"] = 1.0
+    fixtures[1]["shape_type"] = capi.BX_SHAPE_SPHERE
+    fixtures[1]["radius"] = F32(0.5)
+    fixtures[2]["shape_type"] = capi.BX_SHAPE_CAPSULE
+    fixtures[2]["vertex1"] = (-0.5, 0.0, 0.0)
+    fixtures[2]["vertex2"] = (0.5, 0.0, 0.0)
+    fixtures[2]["radius"] = F32(0.3)
+    fixtures[3]["shape_type"] = capi.BX_SHAPE_HULL
+    fixtures[3]["shape_index"] = box
+    s.worlds.append(World(bodies, fixtures, gravity=(0.0, F32(-9.8), 0.0)))
+    return s
+
+
 def drop_boxes_world(rng, n_boxes, ground_hull, box_hull, extent=(20.0, 50.0, 20.0)):
     bodies = body_array(n_boxes + 1)
     fixtures = fixture_array(n_boxes + 1)

@@ -2,8 +2,9 @@
 
 CPU: programs written against the reference's API compile and link against the drop-in headers
 (the reference's own examples/hello_world/main.cpp when the reference tree is present), and the
-drop-in b3BoxHull builds the reference's box table bit for bit. GPU: a hello-world program built
-against the drop-in API steps on the device and prints the oracle's poses on every step.
+drop-in b3BoxHull builds the reference's box table bit for bit. GPU: a hello-world program and a
+program dropping a sphere, a capsule and a box on a b3Mesh heightfield, built against the drop-in
+API, step on the device and print the oracle's poses on every step.
 """
 import os
 import subprocess
@@ -18,6 +19,7 @@ INC = os.path.join(REPO, "bounce_amd", "include")
 LIB = os.path.join(REPO, "bounce_amd", "lib")
 REF_HELLO = "/root/reference/examples/hello_world/main.cpp"
 HELLO_BIN = os.path.join(LIB, "bx_hello_world")
+MESH_BIN = os.path.join(LIB, "bx_mesh_ground")
 
 
 def _have_libs():
@@ -88,3 +90,35 @@ def test_hello_world_program_on_device():
         assert lines[i] == expect
     printed = [l for l in out.stdout.splitlines() if l.startswith("position = ")]
     assert printed[-1] == "position = 0.00 5.02 0.00"  # what the reference's hello_world prints at step 60
+
+
+def mesh_program_lines(exe):
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    return [l for l in out.stdout.splitlines() if l.startswith("step ")]
+
+
+def expected_mesh_lines():
+    from oracle.oracle import CpuWorld
+    w = CpuWorld("oracle", scenes.mesh_ground_program())
+    lines = []
+    for i in range(120):
+        w.step()
+        b = w.bodies()
+        for k in (1, 2, 3):
+            vals = tuple(float(x) for x in b[k]["position"]) + tuple(float(x) for x in b[k]["orientation"])
+            lines.append("step %d body %d position = %.9g %.9g %.9g orientation = %.9g %.9g %.9g %.9g"
+                         % ((i + 1, k) + vals))
+    return lines
+
+
+@pytest.mark.gpu
+def test_mesh_program_on_device():
+    """The drop-in mesh program prints, on every step, exactly the oracle's poses of its bodies."""
+    assert os.path.exists(MESH_BIN), "bx_mesh_ground is built by bounce_amd/build.py"
+    print("[drop-in] bx_mesh_ground", flush=True)
+    got = mesh_program_lines(MESH_BIN)
+    want = expected_mesh_lines()
+    assert len(got) == len(want) == 360
+    for g, w in zip(got, want):
+        assert g == w
