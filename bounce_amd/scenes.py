@@ -231,6 +231,21 @@ def drop_boxes(n_boxes=1000, n_worlds=1, seed=1, extent=(20.0, 50.0, 20.0)):
     return s
 
 
+def box_worlds(n_worlds, seed=8, median=200, sigma=0.8, clip=(50, 5000)):
+    """Worlds of unit boxes dropped on a ground box, per-world counts round(LogNormal(ln median,
+    sigma)) clipped to `clip`, each in a column whose footprint grows with the count (about one box
+    per 2 m^2 of ground, 20 m tall)."""
+    s = SceneSet("box_worlds")
+    g = s.add_hull(hull_lib.box_hull(50.0, 0.5, 50.0))
+    b = s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.round(rng.lognormal(np.log(median), sigma, n_worlds)), clip[0], clip[1]).astype(int)
+    for n in sizes:
+        side = float(min(90.0, max(6.0, np.sqrt(2.0 * n / 4.0))))
+        s.worlds.append(drop_boxes_world(rng, int(n), g, b, extent=(side, 20.0, side)))
+    return s
+
+
 def pyramid_world(rng, levels, ground_hull, box_hull, friction=0.6, jitter=1e-3):
     n = levels * (levels + 1) // 2
     bodies = body_array(n + 1)

@@ -145,12 +145,17 @@ class Server:
                 del self.active[w]
         return len(slots)
 
-    def run(self, requests, clock=None, sleep=time.sleep, idle_sleep=2e-4):
-        """Serve `requests` (arrival in seconds from the start) in real time (or on `clock`)."""
+    def run(self, requests, clock=None, sleep=time.sleep, idle_sleep=2e-4, progress=None, progress_every=2.0):
+        """Serve `requests` (arrival in seconds from the start) in real time (or on `clock`).
+        `progress(server)` is called about every `progress_every` seconds of wall time."""
         t0 = time.perf_counter()
         clock = clock or (lambda: time.perf_counter() - t0)
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        last = time.perf_counter()
         while pending or self.waiting or self.active:
+            if progress is not None and time.perf_counter() - last >= progress_every:
+                last = time.perf_counter()
+                progress(self)
             now = clock()
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
