@@ -1731,4 +1731,60 @@ uint32_t bx_struct_size(const char* name) {
 
 uint64_t bx_batch_device_bytes(bx_batch* b) { return b ? uint64_t(b->deviceBytes) : 0; }
 
+// Host-side mesh queries over the structures the device path builds at upload (no device needed).
+static int BuildOneMesh(const bx_mesh_desc* mesh, HostLibrary& L) {
+    if (!mesh) return Fail(BX_E_INVALID, "null mesh");
+    bx_shape_library lib;
+    std::memset(&lib, 0, sizeof(lib));
+    lib.mesh_count = 1;
+    lib.meshes = mesh;
+    return BuildLibrary(&lib, L);
+}
+
+int bx_mesh_query_aabbs(const bx_mesh_desc* mesh, const float* aabbs, uint32_t n, uint32_t* counts, uint32_t* out,
+                        uint32_t cap_per) {
+    if (n && (!aabbs || !counts || (cap_per && !out))) return Fail(BX_E_INVALID, "null argument");
+    HostLibrary L;
+    int rc = BuildOneMesh(mesh, L);
+    if (rc) return rc;
+    const TreeNode* nodes = L.meshNodes.data();
+    std::vector<u32> stack;
+    for (u32 k = 0; k < n; ++k) {
+        AABB box;
+        box.lowerBound = V3(aabbs + 6 * k);
+        box.upperBound = V3(aabbs + 6 * k + 3);
+        u32 found = 0;
+        stack.assign(1, L.meshes[0].root);
+        while (!stack.empty()) {  // the device's traversal (MeshContactFindPairs): stack order
+            u32 id = stack.back();
+            stack.pop_back();
+            if (id == kNull) continue;
+            const TreeNode& node = nodes[id];
+            if (!TestOverlap(node.aabb, box)) continue;
+            if (node.child1 == kNull) {
+                if (found < cap_per) out[size_t(k) * cap_per + found] = node.index;
+                ++found;
+            } else {
+                stack.push_back(node.child1);
+                stack.push_back(node.child2);
+            }
+        }
+        counts[k] = found;
+    }
+    return BX_OK;
+}
+
+int bx_mesh_wings(const bx_mesh_desc* mesh, uint32_t* out) {
+    if (!out) return Fail(BX_E_INVALID, "null argument");
+    HostLibrary L;
+    int rc = BuildOneMesh(mesh, L);
+    if (rc) return rc;
+    for (size_t i = 0; i < L.meshTris.size(); ++i) {
+        out[3 * i] = L.meshTris[i].u1;
+        out[3 * i + 1] = L.meshTris[i].u2;
+        out[3 * i + 2] = L.meshTris[i].u3;
+    }
+    return BX_OK;
+}
+
 }  // extern "C"

@@ -361,6 +361,19 @@ uint32_t bx_struct_size(const char* name);
 /* Device memory held by a batch, in bytes. */
 uint64_t bx_batch_device_bytes(bx_batch* batch);
 
+/* ---------------------------------------------------------------------------------------------
+ * Mesh structures as the device path builds them at upload (host only, no device needed)
+ * ------------------------------------------------------------------------------------------- */
+/* b3StaticTree::QueryAABB (static_tree.h:215-255) on the mesh's tree as b3Mesh::BuildTree builds it
+   (mesh.cpp:21-33, static_tree.cpp:40-170), for n boxes aabbs[6k..6k+5] (lower xyz, upper xyz):
+   counts[k] overlapping triangles, the first cap_per of them in out[k * cap_per ..] in the order a
+   mesh contact visits them. */
+int bx_mesh_query_aabbs(const bx_mesh_desc* mesh, const float* aabbs, uint32_t n, uint32_t* counts, uint32_t* out,
+                        uint32_t cap_per);
+/* b3Mesh::BuildAdjacency (mesh.cpp:35-112): the wing vertices u1, u2, u3 of every triangle
+   (BX_NULL_INDEX on a boundary edge) into out[3 * triangle_count]. */
+int bx_mesh_wings(const bx_mesh_desc* mesh, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,6 +144,8 @@ bool BuildMesh(Mesh& m, const bx_mesh_desc& d) {
 }
 
 // b3StaticTree::QueryAABB (static_tree.h:215-255): leaves reported in stack order
+static void QueryTree(const Mesh& m, const AABB& aabb, std::vector<u32>& out);
+void QueryMeshTree(const Mesh& m, const AABB& aabb, std::vector<u32>& out) { QueryTree(m, aabb, out); }
 static void QueryTree(const Mesh& m, const AABB& aabb, std::vector<u32>& out) {
     std::vector<u32> stack;
     stack.push_back(m.root);
@@ -497,3 +499,39 @@ void MeshContactCollide(World& w, Contact* c) {  // mesh_contact.cpp:328-371
 }
 
 }  // namespace orc
+
+extern "C" {
+
+// b3StaticTree::QueryAABB of mesh `d` (built as b3Mesh::BuildTree builds it) for n boxes
+// (aabbs[6 k .. 6 k + 5]: lower xyz, upper xyz): counts[k] triangles, the first cap_per of them in
+// out[k * cap_per ..]. Returns 0, or -1 for an invalid mesh.
+int orc_mesh_query(const bx_mesh_desc* d, const float* aabbs, uint32_t n, uint32_t* counts, uint32_t* out,
+                   uint32_t cap_per) {
+    orc::Mesh m;
+    if (!orc::BuildMesh(m, *d)) return -1;
+    std::vector<orc::u32> tris;
+    for (uint32_t k = 0; k < n; ++k) {
+        orc::AABB box;
+        box.lowerBound = orc::Vec3(aabbs[6 * k], aabbs[6 * k + 1], aabbs[6 * k + 2]);
+        box.upperBound = orc::Vec3(aabbs[6 * k + 3], aabbs[6 * k + 4], aabbs[6 * k + 5]);
+        tris.clear();
+        orc::QueryMeshTree(m, box, tris);
+        counts[k] = uint32_t(tris.size());
+        for (uint32_t i = 0; i < tris.size() && i < cap_per; ++i) out[size_t(k) * cap_per + i] = tris[i];
+    }
+    return 0;
+}
+
+// The wing vertices (u1, u2, u3) of every triangle after b3Mesh::BuildAdjacency.
+int orc_mesh_wings(const bx_mesh_desc* d, uint32_t* out) {
+    orc::Mesh m;
+    if (!orc::BuildMesh(m, *d)) return -1;
+    for (size_t i = 0; i < m.triangles.size(); ++i) {
+        out[3 * i] = m.triangles[i].u1;
+        out[3 * i + 1] = m.triangles[i].u2;
+        out[3 * i + 2] = m.triangles[i].u3;
+    }
+    return 0;
+}
+
+}  // extern "C"
