@@ -1420,6 +1420,7 @@ int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* 
     if (h.magic != kSnapshotMagic || h.version != 2 || h.proxyCount != h.info.fixtureCount ||
         size != SnapshotBytes(h.info))
         return Fail(BX_E_INVALID, "not a world snapshot of this library version");
+    if (h.info.meshTriCap) return Fail(BX_E_UNSUPPORTED, "pools do not hold worlds with mesh fixtures");
     HIP_CHECK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     WorldInfo wi = h.info;
