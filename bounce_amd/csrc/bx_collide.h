@@ -249,9 +249,12 @@ BX_HD void ClipPolygonToPlane(ClipPolygon& out, const ClipPolygon& in, const Cli
     }
 }
 
-// Clip `poly` in place against the side planes of face `index` (two buffers ping-pong).
-BX_HD void ClipPolygonToFaceSidePlanes(ClipPolygon& poly, ClipPolygon& tmp, const Transform& xf, scalar r, u32 index,
-                                       const HullView& h, u32& overflow) {
+// Clip `poly` against the side planes of face `index`, alternating between `poly` and `tmp`;
+// returns the buffer holding the result (its count is 0 once a plane leaves nothing).
+BX_HD const ClipPolygon* ClipPolygonToFaceSidePlanes(ClipPolygon& poly, ClipPolygon& tmp, const Transform& xf,
+                                                     scalar r, u32 index, const HullView& h, u32& overflow) {
+    ClipPolygon* in = &poly;
+    ClipPolygon* out = &tmp;
     u32 begin = h.faces[index];
     u32 e = begin;
     do {
@@ -263,12 +266,14 @@ BX_HD void ClipPolygonToFaceSidePlanes(ClipPolygon& poly, ClipPolygon& tmp, cons
         ClipPlane cp;
         cp.edge = edgeId;
         cp.plane = Mul(xf, plane);
-        ClipPolygonToPlane(tmp, poly, cp, overflow);
-        poly.count = tmp.count;
-        for (u32 i = 0; i < tmp.count; ++i) poly.v[i] = tmp.v[i];
-        if (poly.count == 0) return;
+        ClipPolygonToPlane(*out, *in, cp, overflow);
+        ClipPolygon* t = in;
+        in = out;
+        out = t;
+        if (in->count == 0) return in;
         e = edge.next;
     } while (e != begin);
+    return in;
 }
 
 // ------------------------------------------------------------------ reduction (cluster.cpp:23-200)
@@ -402,13 +407,13 @@ BX_HD void BuildFaceContact(Manifold& m, const Transform& xf1, u32 index1, const
     Vec3 normal1 = MulC(xf2.rotation, plane1.normal);
     u32 index2 = h2.GetSupportFace(-normal1);
     BuildPolygon(s.poly, xf2, index2, h2, overflow);
-    ClipPolygonToFaceSidePlanes(s.poly, s.tmp, xf1, totalRadius, index1, h1, overflow);
-    if (s.poly.count == 0) return;
+    const ClipPolygon& poly = *ClipPolygonToFaceSidePlanes(s.poly, s.tmp, xf1, totalRadius, index1, h1, overflow);
+    if (poly.count == 0) return;
     u32 n1 = 0;
     u32 minIndex = 0;
     scalar minSeparation = kMaxScalar;
-    for (u32 i = 0; i < s.poly.count; ++i) {
-        Vec3 p2 = s.poly.v[i].position;
+    for (u32 i = 0; i < poly.count; ++i) {
+        Vec3 p2 = poly.v[i].position;
         scalar separation = Distance(p2, plane1);
         if (separation <= totalRadius) {
             if (separation < minSeparation) { minIndex = n1; minSeparation = separation; }
@@ -422,7 +427,7 @@ BX_HD void BuildFaceContact(Manifold& m, const Transform& xf1, u32 index1, const
     Vec3 orientedNormal = flipNormal ? -normal : normal;
     u32 pointCount = ReducePolygon(s.reduced, s.polygon1, n1, orientedNormal, minIndex);
     for (u32 i = 0; i < pointCount; ++i) {
-        const ClipVertex& v2 = s.poly.v[s.reduced[i].clipIndex];
+        const ClipVertex& v2 = poly.v[s.reduced[i].clipIndex];
         Vec3 v1 = ClosestPointOnPlane(v2.position, plane1);
         ManifoldPoint& mp = m.points[i];
         if (flipNormal) {

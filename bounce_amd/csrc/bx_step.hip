@@ -240,11 +240,13 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
         c.manifoldCount = 0;
     } else {
         const u32 oldCount = c.manifoldCount;
-        Manifold old = c.manifold;
+        // the last step's manifold, read in place: the contact record is written only at the end
+        const Manifold& old = c.manifold;
+        // b3Manifold::Initialize (collision.cpp:23-38). Every narrowphase writes the geometry and id
+        // of each point it reports, so the new manifold starts from the initialized fields only;
+        // the record's unused point slots keep their old contents, as the reference's reused
+        // manifold storage does.
         Manifold m;
-        // b3Manifold::Initialize (collision.cpp:23-38); point geometry and ids keep their old
-        // contents exactly as the reference's reused manifold storage does.
-        m = old;
         m.pointCount = 0;
         m.tangentImpulse.SetZero();
         m.motorImpulse = 0;
@@ -293,7 +295,14 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
         }
         c.manifoldCount = 1;
         if ((wi.flags & 0x2u) == 0 && oldCount > 0) MatchManifold(m, old);  // no matching without warm starting
-        c.manifold = m;
+        Manifold& dst = c.manifold;
+        dst.pointCount = m.pointCount;
+        dst.tangentImpulse = m.tangentImpulse;
+        dst.motorImpulse = m.motorImpulse;
+        dst.motorSpeed = m.motorSpeed;
+        dst.tangentSpeed1 = m.tangentSpeed1;
+        dst.tangentSpeed2 = m.tangentSpeed2;
+        for (u32 i = 0; i < m.pointCount; ++i) dst.points[i] = m.points[i];
         touching = m.pointCount > 0;
         if (touching != wasTouching && applyWakes) {
             WakeBody(bA);
