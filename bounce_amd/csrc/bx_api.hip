@@ -387,6 +387,8 @@ static int BuildLibrary(const bx_shape_library* lib, HostLibrary& L) {
         const bx_hull_desc& d = lib->hulls[i];
         if (d.vertex_count < 4 || d.edge_count < 6 || d.face_count < 4 || (d.edge_count & 1u))
             return Fail(BX_E_INVALID, "hull " + std::to_string(i) + ": invalid counts");
+        if (d.edge_count >= 0xFFFFu)  // feature ids are clipped in 16 bits
+            return Fail(BX_E_INVALID, "hull " + std::to_string(i) + ": more than 65,534 half-edges");
         HullInfo hi;
         hi.vBase = u32(L.vertices.size());
         hi.vCount = d.vertex_count;

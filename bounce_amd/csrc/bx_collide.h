@@ -15,7 +15,19 @@ namespace bx {
 
 constexpr u32 kMaxClip = 32;
 
-struct ClipVertex { Vec3 position; u32 pair[4]; };
+// b3ClipVertex with its b3FeaturePair held in 16 bits per id while clipping (the ids are hull
+// half-edge indices, below 0xFFFF by the upload checks, or the null edge, held as 0xFFFF); they
+// are widened back to the reference's 32-bit ids when a manifold point is written.
+struct ClipVertex { Vec3 position; uint16_t pair[4]; };
+constexpr uint16_t kNullId16 = 0xFFFFu;
+BX_HD uint16_t Id16(u32 x) { return x == kNull ? kNullId16 : uint16_t(x); }
+BX_HD u32 Id32(uint16_t x) { return x == kNullId16 ? kNull : u32(x); }
+BX_HD void MakeClipKey(uint16_t out[4], u32 a, u32 b, u32 c, u32 d) {
+    out[0] = Id16(a); out[1] = Id16(b); out[2] = Id16(c); out[3] = Id16(d);
+}
+BX_HD void WidenKey(u32 out[4], const uint16_t in[4]) {
+    out[0] = Id32(in[0]); out[1] = Id32(in[1]); out[2] = Id32(in[2]); out[3] = Id32(in[3]);
+}
 struct ClipPlane { Plane plane; u32 edge; };
 struct ClusterVertex { Vec3 position; u32 clipIndex; };
 
@@ -208,7 +220,7 @@ BX_HD void BuildPolygon(ClipPolygon& out, const Transform& xf, u32 index, const 
         if (out.count < kMaxClip) {
             ClipVertex& v = out.v[out.count++];
             v.position = Mul(xf, h.vertices[edge.origin]);
-            MakeKey(v.pair, kNull, kNull, edge.prev, twin.twin);
+            MakeClipKey(v.pair, kNull, kNull, edge.prev, twin.twin);
         } else {
             overflow = 1;
         }
@@ -234,13 +246,13 @@ BX_HD void ClipPolygonToPlane(ClipPolygon& out, const ClipPolygon& in, const Cli
             scalar fraction = d1 / (d1 - d2);
             ClipVertex v;
             v.position = v1.position + fraction * (v2.position - v1.position);
-            MakeKey(v.pair, plane.edge, kNull, v1.pair[3], kNull);
+            MakeClipKey(v.pair, plane.edge, kNull, Id32(v1.pair[3]), kNull);
             PushClip(out, v, overflow);
         } else if (d1 > scalar(0) && d2 <= scalar(0)) {
             scalar fraction = d1 / (d1 - d2);
             ClipVertex v;
             v.position = v1.position + fraction * (v2.position - v1.position);
-            MakeKey(v.pair, kNull, plane.edge, kNull, v2.pair[2]);
+            MakeClipKey(v.pair, kNull, plane.edge, kNull, Id32(v2.pair[2]));
             PushClip(out, v, overflow);
             PushClip(out, v2, overflow);
         }
@@ -432,7 +444,7 @@ BX_HD void BuildFaceContact(Manifold& m, const Transform& xf1, u32 index1, const
         ManifoldPoint& mp = m.points[i];
         if (flipNormal) {
             u32 key[4];
-            MakeKey(key, v2.pair[2], v2.pair[3], v2.pair[0], v2.pair[1]);
+            MakeKey(key, Id32(v2.pair[2]), Id32(v2.pair[3]), Id32(v2.pair[0]), Id32(v2.pair[1]));
             mp.localNormal1 = MulC(xf2.rotation, orientedNormal);
             mp.localPoint1 = MulT(xf2, v2.position);
             mp.localPoint2 = MulT(xf1, v1);
@@ -441,7 +453,9 @@ BX_HD void BuildFaceContact(Manifold& m, const Transform& xf1, u32 index1, const
             mp.localNormal1 = MulC(xf1.rotation, orientedNormal);
             mp.localPoint1 = MulT(xf1, v1);
             mp.localPoint2 = MulT(xf2, v2.position);
-            SetPointId(mp, v2.pair);
+            u32 key[4];
+            WidenKey(key, v2.pair);
+            SetPointId(mp, key);
         }
         mp.edgeContact = 0;
     }

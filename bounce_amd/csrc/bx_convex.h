@@ -384,9 +384,9 @@ struct SegClip {  // two b3ClipVertex of a clipped segment
 BX_HD SegClip BuildSegment(const Vec3& v1, const Vec3& v2) {  // clip.cpp:22-30
     SegClip s;
     s.c0.position = v1;
-    MakeKey(s.c0.pair, kNull, kNull, 0u, 0u);
+    MakeClipKey(s.c0.pair, kNull, kNull, 0u, 0u);
     s.c1.position = v2;
-    MakeKey(s.c1.pair, kNull, kNull, 1u, 0u);
+    MakeClipKey(s.c1.pair, kNull, kNull, 1u, 0u);
     return s;
 }
 
@@ -401,7 +401,7 @@ BX_HD u32 ClipSegmentToPlane(SegClip& io, const ClipPlane& plane) {
         scalar fraction = d1 / (d1 - d2);
         ClipVertex x;
         x.position = io.c0.position + fraction * (io.c1.position - io.c0.position);
-        MakeKey(x.pair, plane.edge, kNull, io.c0.pair[3], kNull);
+        MakeClipKey(x.pair, plane.edge, kNull, Id32(io.c0.pair[3]), kNull);
         io.c1 = x;
         return 2;
     }
@@ -409,7 +409,7 @@ BX_HD u32 ClipSegmentToPlane(SegClip& io, const ClipPlane& plane) {
         scalar fraction = d1 / (d1 - d2);
         ClipVertex x;
         x.position = io.c0.position + fraction * (io.c1.position - io.c0.position);
-        MakeKey(x.pair, kNull, plane.edge, kNull, io.c1.pair[2]);
+        MakeClipKey(x.pair, kNull, plane.edge, kNull, Id32(io.c1.pair[2]));
         io.c0 = io.c1;
         io.c1 = x;
         return 2;
@@ -519,6 +519,12 @@ BX_HD void SetPoint(ManifoldPoint& p, const Vec3& n1, const Vec3& p1, const Vec3
     p.localPoint2 = p2;
     p.triangleKey = kNull;
     MakeKey(p.key, key[0], key[1], key[2], key[3]);
+}
+
+BX_HD void SetPoint(ManifoldPoint& p, const Vec3& n1, const Vec3& p1, const Vec3& p2, const uint16_t key[4]) {
+    u32 wide[4];
+    WidenKey(wide, key);
+    SetPoint(p, n1, p1, p2, wide);
 }
 
 BX_HD void CollideCapsules(Manifold& m, const Transform& xf1, const Fixture& s1, const Transform& xf2,
