@@ -114,7 +114,7 @@ __device__ __forceinline__ void VelocityStep(VelCon& cur, VelCon& ahead, const V
 
 // A sweep over the island's contacts in order. kTwoBuffers unrolls by two so that the two
 // register buffers swap roles instead of the prefetched constraint being copied each contact
-// (twice the code, used for the iteration sweeps that dominate).
+// (twice the code; used for the warm start and the iteration sweeps on LDS velocities).
 template <bool kTwoBuffers, class V, class Op>
 __device__ __forceinline__ void ContactSweep(VelCon* vcs, u32 nc, V* vel, Op op) {
     VelCon a = vcs[0], b;
@@ -167,7 +167,7 @@ __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jc
     // requested (clamped index, no branch), so waiting for the bodies never waits for the prefetch.
     // Order of island.cpp:154-181: contact warm start, joint warm start, then per iteration all
     // joints followed by all contacts.
-    if (warm && nc > 0) ContactSweep<false>(vcs, nc, vel, WarmStartOp());
+    if (warm && nc > 0) ContactSweep<kInLds && BX_TWO_BUFFERS>(vcs, nc, vel, WarmStartOp());
     if (warm)
         for (u32 k = 0; k < nj; ++k) WarmStartJoint(jcs[k], vel);
     for (u32 it = 0; it < iterations && (nc > 0 || nj > 0); ++it) {
