@@ -12,7 +12,9 @@ is one b3World::Step of every world of the batch; inputs are resident in HBM bef
 200, clipped to 20..2000), spheres, capsules and random hulls of 8-16 points dropped on a ground box,
 and a 10-link revolute chain with a motor per world (scenes.mixed_worlds, seed 8).
 
-Printed (one JSON line): body-steps/s of the batched path (value), the plain per-world GPU path
+Printed (one JSON line): body-steps/s of the batched path (value; the step's kernels launched
+directly, with HIP events between them), the same steps of a fresh copy of the batch as replays
+of one captured HIP graph (graph_replay), the plain per-world GPU path
 (one world per launch sequence, measured on a sample of worlds), their ratio, p50/p99 step
 latency, the roofline of the dominant kernel (the island solver) from HIP events on the
 library's stream, the reference CPU Step timed on this host's cores (one world per thread)
@@ -56,6 +58,8 @@ def parse():
     p.add_argument("--worlds", type=int, default=4096)
     p.add_argument("--levels", type=int, default=20)
     p.add_argument("--plain-worlds", type=int, default=16)
+    p.add_argument("--no-graph", dest="graph", action="store_false",
+                   help="skip the graph-replay leg (the K steps again as replays of one captured step)")
     p.add_argument("--cpu-worlds", type=int, default=256)
     p.add_argument("--cpu-steps", type=int, default=60)
     p.add_argument("--latency-steps", type=int, default=20)
@@ -165,6 +169,21 @@ def main():
         if args.workload == "stacking" and tj.get("worlds") == args.worlds and tj.get("levels") == args.levels:
             traffic = tj["bytes_per_launch"].get(dominant)
     batch.close()
+    # the same W + K steps of a fresh copy of the batch, each a replay of one captured step
+    # (bx_batch_step_graph; the capture happens in the first warm-up step)
+    graph = None
+    if args.graph:
+        gb = device.Batch(ctx, scene)
+        for _ in range(args.warmup):
+            gb.step_graph()
+        gb.sync()
+        g0 = time.perf_counter()
+        gb.step_graph(n=args.steps)
+        gb.sync()
+        g1 = time.perf_counter()
+        graph = {"value": total_bodies * args.steps / (g1 - g0), "unit": "body-steps/s",
+                 "ms_per_step": (g1 - g0) / args.steps * 1e3, "steps": args.steps, "warmup": args.warmup}
+        gb.close()
     # plain per-world path: the same kernels, one world per launch sequence
     plain = None
     if args.plain_worlds > 0:
@@ -207,6 +226,7 @@ def main():
                     f"20..2000 per world; spheres, capsules, 8-16 point hulls, a 10-link motor chain), "
                     f"dt=1/60, 8/2 iterations, sleep off"),
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
+        "graph_replay": graph,
         "plain_per_world": plain,
         "batched_over_plain": value / plain["value"] if plain else None,
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),

@@ -239,6 +239,10 @@ struct bx_batch {
     hipEvent_t listEv[kListRing] = {};
     bool listUsed[kListRing] = {};
     u32 listNext = 0;
+    // one captured step of every world, replayed by bx_batch_step_graph (captured with graphParams)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    StepParams graphParams{};
 };
 
 static bool IsResident(const bx_batch* b, u32 world) { return world < b->worldCount && b->resident[world]; }
@@ -1080,6 +1084,57 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     return BX_OK;
 }
 
+static void DropGraph(bx_batch* b) {
+    if (b->graphExec) hipGraphExecDestroy(b->graphExec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+}
+
+// `steps` steps of every world of a (non-pool) batch as replays of one captured step. The graph
+// holds the step's kernel launches with their arguments by value; the batch's device arrays are
+// fixed at creation, so it stays valid for the batch's lifetime and is captured again only when
+// the step configuration changes. The creation-time pair search runs before it, as a launch of its
+// own. The step's kernels write no memory through copies or fills, so the graph has kernel nodes
+// only.
+static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
+    const StepParams sp = MakeParams(cfg);
+    u32 maxIslands = 0;
+    for (u32 i = 0; i < b->worldCount; ++i) maxIslands += b->worlds[i].bodyCount;
+    if (b->newContacts) {
+        std::vector<u32> all(b->worldCount);
+        for (u32 i = 0; i < b->worldCount; ++i) all[i] = i;
+        int rc = RunInitialPairs(b, all.data(), b->worldCount);
+        if (rc) return rc;
+    }
+    const StepParams& g = b->graphParams;
+    const bool same = b->graphExec && g.dt == sp.dt && g.inv_dt == sp.inv_dt &&
+                      g.velocityIterations == sp.velocityIterations && g.positionIterations == sp.positionIterations;
+    if (!same) {
+        DropGraph(b);
+        hipStream_t st = b->ctx->stream;
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        LaunchStep(b->view, nullptr, 0, b->worldCount, sp, false, maxIslands, st, nullptr);
+        hipError_t launch = hipGetLastError();
+        hipError_t e = hipStreamEndCapture(st, &b->graph);
+        if (launch != hipSuccess || e != hipSuccess) {
+            DropGraph(b);
+            return Fail(BX_E_DEVICE, std::string("step capture: ") +
+                                         hipGetErrorString(launch != hipSuccess ? launch : e));
+        }
+        e = hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            DropGraph(b);
+            return Fail(BX_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        b->graphParams = sp;
+    }
+    for (u32 s = 0; s < steps; ++s) HIP_CHECK(hipGraphLaunch(b->graphExec, b->ctx->stream));
+    b->ctx->timedSteps = 0;  // replayed steps carry no per-kernel events
+    return BX_OK;
+}
+
 extern "C" {
 
 const char* bx_version(void) { return "bxstep 0.1 gfx950"; }
@@ -1119,6 +1174,7 @@ void bx_batch_destroy(bx_batch* b) {
     for (void* p : b->allocations) hipFree(p);
     for (hipEvent_t e : b->listEv)
         if (e) hipEventDestroy(e);
+    DropGraph(b);
     if (b->listHost) hipHostFree(b->listHost);
     delete b->library;
     delete b;
@@ -1471,6 +1527,14 @@ int bx_batch_step(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count) {
         if (rc) return rc;
     }
     return BX_OK;
+}
+
+int bx_batch_step_graph(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count) {
+    if (!b || !cfg) return Fail(BX_E_INVALID, "null argument");
+    if (b->pool) return Fail(BX_E_UNSUPPORTED, "graph replay steps a batch, not a pool");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    if (step_count == 0 || b->worldCount == 0) return BX_OK;
+    return StepGraph(b, cfg, step_count);
 }
 
 int bx_batch_step_list(bx_batch* b, const bx_step_cfg* cfg, const uint32_t* worlds, uint32_t count) {

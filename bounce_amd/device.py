@@ -36,6 +36,7 @@ def load(path=capi.LIB_PATH):
                                       C.POINTER(capi.bx_batch_opts), C.POINTER(P)]),
         "bx_batch_destroy": (None, [P]),
         "bx_batch_step": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
+        "bx_batch_step_graph": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
         "bx_batch_step_subset": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), C.POINTER(u), u]),
         "bx_batch_sync": (C.c_int, [P]),
         "bx_batch_get_counts": (C.c_int, [P, u, C.POINTER(capi.bx_world_counts)]),
@@ -137,6 +138,11 @@ class Batch:
     def step(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
         cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
         _check(_lib.bx_batch_step(self.h, C.byref(cfg), n))
+
+    def step_graph(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
+        """n steps, each a replay of one captured step (same results as step; untimed)."""
+        cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
+        _check(_lib.bx_batch_step_graph(self.h, C.byref(cfg), n))
 
     def step_list(self, worlds, dt=1.0 / 60.0, vit=8, pit=2):
         """One step of each listed world, all in one launch sequence."""

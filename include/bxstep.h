@@ -296,6 +296,13 @@ void bx_batch_destroy(bx_batch* batch);
    Asynchronous on the context stream; call bx_batch_sync before reading state. */
 int bx_batch_step(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
 
+/* The same steps as bx_batch_step (b3World::Step, world.cpp:187-242, step_count times) with the same
+   results, each step a replay of one step captured as a HIP graph on the first call (captured
+   again when cfg changes), so a step costs one graph launch on the host. The steps are not timed
+   per kernel (bx_batch_get_timings and bx_batch_get_kernel_timings then report no timed step).
+   Batches only: a pool returns BX_E_UNSUPPORTED. */
+int bx_batch_step_graph(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
+
 /* Advance world worlds[i] by step_counts[i] steps (1 each when step_counts is NULL). Step s launches
    once for every listed world with more than s steps to go, so the worlds share each launch. */
 int bx_batch_step_subset(bx_batch* batch, const bx_step_cfg* cfg, const uint32_t* worlds,
