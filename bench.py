@@ -152,6 +152,7 @@ def main():
     contacts = np.mean([c.contact_count for c in counts]) * args.worlds
     status = int(np.bitwise_or.reduce([c.status for c in counts]))
     device_bytes = batch.device_bytes()
+    lanes, tuned_ms = batch.solver_layout()
     dom_bytes = KERNEL_BYTES[dominant](island_contacts, island_bodies, contacts)
     achieved = dom_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
     # step latency: single steps, each bracketed by events on the stream
@@ -226,6 +227,10 @@ def main():
                     f"20..2000 per world; spheres, capsules, 8-16 point hulls, a 10-link motor chain), "
                     f"dt=1/60, 8/2 iterations, sleep off"),
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
+        "solver_layout": {"islands_per_wavefront": lanes, "layouts": [16, 32, 64],
+                          "tuned_k_solve_velocity_ms": [round(t, 3) for t in tuned_ms],
+                          "note": "solver_mode 0: each layout timed twice on warm-up steps 4-9 (order "
+                                  "16 32 64 64 32 16), the one with the smallest sample kept"},
         "graph_replay": graph,
         "plain_per_world": plain,
         "batched_over_plain": value / plain["value"] if plain else None,

@@ -216,6 +216,16 @@ struct RangeAllocator {
 // them have finished (one event per region).
 static const u32 kListRing = 4;
 
+// Tuning of the velocity-sweep layout (solver_mode 0) on a batch's first steps: kTuneSkip steps
+// with the first layout (contacts and their points are still appearing), then every layout twice
+// in the palindrome kTuneOrder; a layout's time is the smaller of its two samples. The first steps
+// after an upload run slower (the device is coming out of idle), so a layout timed only early would
+// lose to one timed later; with the first layout sampled first and last, such a start cannot move
+// the choice away from it, while a layout that is faster by more than the early slowdown still wins.
+static const u32 kTuneSkip = 3;
+static const u32 kTuneOrder[2 * kSolveLayouts] = {0, 1, 2, 2, 1, 0};
+static const u32 kTuneSteps = kTuneSkip + 2 * kSolveLayouts;
+
 struct bx_batch {
     bx_context* ctx = nullptr;
     u32 worldCount = 0;  // world slots
@@ -243,6 +253,11 @@ struct bx_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     StepParams graphParams{};
+    u32 graphLayout = 0;
+    // velocity-sweep layout tuning (solver_mode 0): steps of the tuning sequence done, and the
+    // time of each layout (the smaller of its two samples)
+    u32 tuneStep = kTuneSteps;
+    float tuneMs[kSolveLayouts] = {};
 };
 
 static bool IsResident(const bx_batch* b, u32 world) { return world < b->worldCount && b->resident[world]; }
@@ -1077,7 +1092,29 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     b->ctx->timedSteps = timedSteps;
     for (u32 s = 0; s < steps; ++s) {
         hipEvent_t* ev = s < timedSteps ? b->ctx->ev.data() + size_t(12) * s : nullptr;
-        LaunchStep(b->view, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+        if (b->tuneStep >= kTuneSkip && b->tuneStep < kTuneSteps && ev && sp.dt > scalar(0)) {
+            // solver_mode 0: this step runs and times one velocity-sweep layout of the tuning
+            // sequence (events 5 and 6 bracket k_solve_velocity); after the last, the fastest is kept
+            const u32 layout = kTuneOrder[b->tuneStep - kTuneSkip];
+            BatchView v = b->view;
+            v.solveLayout = layout;
+            LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipEventSynchronize(ev[6]));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[5], ev[6]));
+            const bool first = b->tuneStep - kTuneSkip < kSolveLayouts;
+            b->tuneMs[layout] = first ? ms : std::min(b->tuneMs[layout], ms);
+            if (++b->tuneStep == kTuneSteps) {
+                u32 best = 0;
+                for (u32 l = 1; l < kSolveLayouts; ++l)
+                    if (b->tuneMs[l] < b->tuneMs[best]) best = l;
+                b->view.solveLayout = best;
+            }
+        } else {
+            if (b->tuneStep < kTuneSkip && sp.dt > scalar(0)) ++b->tuneStep;
+            LaunchStep(b->view, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+        }
         findFirst = false;
     }
     HIP_CHECK(hipGetLastError());
@@ -1091,7 +1128,8 @@ static void DropGraph(bx_batch* b) {
     b->graph = nullptr;
 }
 
-// `steps` steps of every world of a (non-pool) batch as replays of one captured step. The graph
+// `steps` steps of every world of a (non-pool) batch as replays of one captured step (after the
+// directly launched steps that still time velocity-sweep layouts under solver_mode 0). The graph
 // holds the step's kernel launches with their arguments by value; the batch's device arrays are
 // fixed at creation, so it stays valid for the batch's lifetime and is captured again only when
 // the step configuration changes. The creation-time pair search runs before it, as a launch of its
@@ -1099,6 +1137,12 @@ static void DropGraph(bx_batch* b) {
 // only.
 static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     const StepParams sp = MakeParams(cfg);
+    if (b->tuneStep < kTuneSteps && sp.dt > scalar(0)) {  // solver_mode 0: time the layouts first
+        const u32 k = std::min(steps, kTuneSteps - b->tuneStep);
+        int rc = StepAll(b, cfg, k);
+        if (rc || k == steps) return rc;
+        steps -= k;
+    }
     u32 maxIslands = 0;
     for (u32 i = 0; i < b->worldCount; ++i) maxIslands += b->worlds[i].bodyCount;
     if (b->newContacts) {
@@ -1109,7 +1153,8 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     }
     const StepParams& g = b->graphParams;
     const bool same = b->graphExec && g.dt == sp.dt && g.inv_dt == sp.inv_dt &&
-                      g.velocityIterations == sp.velocityIterations && g.positionIterations == sp.positionIterations;
+                      g.velocityIterations == sp.velocityIterations && g.positionIterations == sp.positionIterations &&
+                      b->graphLayout == b->view.solveLayout;
     if (!same) {
         DropGraph(b);
         hipStream_t st = b->ctx->stream;
@@ -1129,6 +1174,7 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
             return Fail(BX_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
         b->graphParams = sp;
+        b->graphLayout = b->view.solveLayout;
     }
     for (u32 s = 0; s < steps; ++s) HIP_CHECK(hipGraphLaunch(b->graphExec, b->ctx->stream));
     b->ctx->timedSteps = 0;  // replayed steps carry no per-kernel events
@@ -1243,6 +1289,14 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     V.islandLdsWords = lds;
     V.meshWorlds = 0;
     for (u32 w = 0; w < world_count; ++w) V.meshWorlds += b->worlds[w].meshTriCap ? 1u : 0u;
+    // velocity-sweep layout: fixed by solver_mode 1..3, else timed on the first steps (StepAll)
+    const u32 mode = opts ? opts->solver_mode : 0u;
+    if (mode > kSolveLayouts) {
+        bx_batch_destroy(b);
+        return Fail(BX_E_INVALID, "solver_mode must be 0 (tuned) or 1..3");
+    }
+    V.solveLayout = mode ? mode - 1 : 0u;
+    b->tuneStep = mode ? kTuneSteps : 0u;
 #define PUT(dev, host)                                                                                      \
     if (!host.empty()) {                                                                                    \
         hipError_t _e = hipMemcpy((void*)(dev), host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice); \
@@ -1535,6 +1589,13 @@ int bx_batch_step_graph(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count
     HIP_CHECK(hipSetDevice(b->ctx->device));
     if (step_count == 0 || b->worldCount == 0) return BX_OK;
     return StepGraph(b, cfg, step_count);
+}
+
+int bx_batch_get_solver_layout(bx_batch* b, uint32_t* islands_per_wavefront, float* tuned_ms, uint32_t count) {
+    if (!b || !islands_per_wavefront || (!tuned_ms && count)) return Fail(BX_E_INVALID, "null argument");
+    *islands_per_wavefront = kSolveLayoutLanes[std::min(b->view.solveLayout, kSolveLayouts - 1)];
+    for (u32 i = 0; i < count; ++i) tuned_ms[i] = i < kSolveLayouts ? b->tuneMs[i] : 0.0f;
+    return int(kSolveLayouts);
 }
 
 int bx_batch_step_list(bx_batch* b, const bx_step_cfg* cfg, const uint32_t* worlds, uint32_t count) {

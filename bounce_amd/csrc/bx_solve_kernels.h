@@ -17,15 +17,6 @@
 #ifndef BX_TWO_BUFFERS
 #define BX_TWO_BUFFERS 1
 #endif
-#ifndef BX_SOLVE_LANES
-#define BX_SOLVE_LANES 16
-#endif
-#ifndef BX_SOLVE_WAVES
-#define BX_SOLVE_WAVES 1
-#endif
-constexpr u32 kSolveLanes = BX_SOLVE_LANES;  // islands per workgroup in the velocity sweeps
-constexpr u32 kSolveWaves = BX_SOLVE_WAVES;  // wavefronts per workgroup in the velocity sweeps
-static_assert(kSolveLanes % kSolveWaves == 0 && kSolveLanes / kSolveWaves <= 64, "islands per wavefront");
 #ifndef BX_POS_LANES
 #define BX_POS_LANES 64
 #endif
@@ -40,7 +31,6 @@ constexpr u32 kPosLanes = BX_POS_LANES;      // islands per 64-wide workgroup in
 // the sweep of a wavefront is a serial chain whose time grows with every other wavefront sharing
 // its CU, so the islands are spread over all CUs instead.
 constexpr u32 kPosSpreadLds = BX_POS_SPREAD_LDS;
-constexpr u32 kSlabBodies = 256;   // velocities of islands up to this size live in LDS
 
 __global__ void __launch_bounds__(kBlock) k_integrate_vel(BatchView B, StepParams sp) {
     const u32 w = WorldOfBlock(B);
@@ -176,16 +166,16 @@ __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jc
     }
 }
 
-__global__ void __launch_bounds__(64 * kSolveWaves) k_solve_velocity(BatchView B, StepParams sp) {
-    __shared__ VelBody slabs[kSolveLanes][kSlabBodies];
-    // kSolveLanes islands per workgroup, spread over its kSolveWaves wavefronts (the first
-    // kSolveLanes / kSolveWaves lanes of each)
-    constexpr u32 kPerWave = kSolveLanes / kSolveWaves;
-    const u32 lane = threadIdx.x & 63u;
-    if (lane >= kPerWave) return;
-    const u32 slot = (threadIdx.x >> 6) * kPerWave + lane;
+// kLanes islands per workgroup of one wavefront (its first kLanes lanes), each island's velocities
+// in an LDS slab of kSlab bodies when they fit.
+template <u32 kLanes, u32 kSlab>
+__global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams sp) {
+    static_assert(kLanes <= 64 && kLanes * kSlab * sizeof(VelBody) <= 96 * 1024, "velocity sweep layout");
+    __shared__ VelBody slabs[kLanes][kSlab];
+    const u32 slot = threadIdx.x;
+    if (slot >= kLanes) return;
     const u32 count = *B.islandListCount;
-    for (u32 i = blockIdx.x * kSolveLanes + slot; i < count; i += gridDim.x * kSolveLanes) {
+    for (u32 i = blockIdx.x * kLanes + slot; i < count; i += gridDim.x * kLanes) {
         const uint2 e = B.islandList[i];
         const WorldInfo wi = B.worlds[e.x];
         const WorldState& ws = B.wstate[e.x];
@@ -197,7 +187,7 @@ __global__ void __launch_bounds__(64 * kSolveWaves) k_solve_velocity(BatchView B
             VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;
             VelCon* vcs = B.velCons + wi.conBase + isl.contactStart;
             JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
-            if (isl.bodyCount <= kSlabBodies) {
+            if (isl.bodyCount <= kSlab) {
                 VelBody* slab = slabs[slot];
                 for (u32 b = 0; b < isl.bodyCount; ++b) slab[b] = gvel[b];
                 VelocitySweeps<true>(vcs, isl.contactCount, jcs, isl.jointCount, slab, sp.velocityIterations, warm,

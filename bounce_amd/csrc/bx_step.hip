@@ -600,13 +600,20 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         else hipLaunchKernelGGL(k_islands<false>, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
         mark(3);
         // one lane per island; islands <= bodies of the launched worlds
-        u32 vblocks = std::max(1u, std::min((maxIslands + kSolveLanes - 1) / kSolveLanes, 16384u));
+        static_assert(kSolveLayoutLanes[0] == 16 && kSolveLayoutSlab[0] == 256 && kSolveLayoutLanes[1] == 32 &&
+                          kSolveLayoutSlab[1] == 128 && kSolveLayoutLanes[2] == 64 && kSolveLayoutSlab[2] == 64,
+                      "launched layouts");
+        const u32 layout = std::min(B.solveLayout, kSolveLayouts - 1);
+        const u32 lanes = kSolveLayoutLanes[layout];
+        u32 vblocks = std::max(1u, std::min((maxIslands + lanes - 1) / lanes, 16384u));
         u32 pblocks = std::max(1u, std::min((maxIslands + kPosLanes - 1) / kPosLanes, 16384u));
         hipLaunchKernelGGL(k_integrate_vel, grid, dim3(kBlock), 0, stream, B, sp);
         mark(4);
         hipLaunchKernelGGL(k_init_contacts, grid, dim3(kBlock), 0, stream, B, sp);
         mark(5);
-        hipLaunchKernelGGL(k_solve_velocity, dim3(vblocks), dim3(64 * kSolveWaves), 0, stream, B, sp);
+        if (layout == 0) hipLaunchKernelGGL((k_solve_velocity<16, 256>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        else if (layout == 1) hipLaunchKernelGGL((k_solve_velocity<32, 128>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        else hipLaunchKernelGGL((k_solve_velocity<64, 64>), dim3(vblocks), dim3(64), 0, stream, B, sp);
         mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
         mark(7);

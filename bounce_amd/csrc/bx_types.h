@@ -289,6 +289,17 @@ struct PosCon {
     Vec3 localPointB[kMaxManifoldPoints];
 };
 
+// Layouts of the velocity sweeps (k_solve_velocity): islands per 64-wide workgroup (one lane each)
+// and the largest island whose velocities live in the lane's LDS slab; every layout requests 96 KB
+// of LDS, so one workgroup runs per CU. The layout changes only which lane solves which island,
+// never the order of operations inside an island, so all layouts give identical results. A
+// wavefront's sweep is a serial chain whose instruction count does not depend on how many of its
+// lanes are active: few islands per wavefront spread a small island count over more CUs, many
+// islands per wavefront need fewer wavefronts when islands are many.
+constexpr u32 kSolveLayouts = 3;
+constexpr u32 kSolveLayoutLanes[kSolveLayouts] = {16, 32, 64};
+constexpr u32 kSolveLayoutSlab[kSolveLayouts] = {256, 128, 64};
+
 // Everything a step kernel needs; all pointers are device pointers.
 struct BatchView {
     u32 worldCount;
@@ -345,6 +356,7 @@ struct BatchView {
     u32* islandScratch;       // island-builder working arrays of worlds whose arrays do not fit LDS
     u32 islandLdsWords;       // LDS words available to the island builder per world
     u32 meshWorlds;           // nonzero when some world holds a mesh fixture (mesh kernel variants)
+    u32 solveLayout;          // layout of the velocity sweeps (index into kSolveLayoutLanes)
 };
 
 // World stepped by this workgroup in the one-workgroup-per-world kernels.

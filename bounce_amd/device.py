@@ -37,6 +37,7 @@ def load(path=capi.LIB_PATH):
         "bx_batch_destroy": (None, [P]),
         "bx_batch_step": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
         "bx_batch_step_graph": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
+        "bx_batch_get_solver_layout": (C.c_int, [P, C.POINTER(u), C.POINTER(C.c_float), u]),
         "bx_batch_step_subset": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), C.POINTER(u), u]),
         "bx_batch_sync": (C.c_int, [P]),
         "bx_batch_get_counts": (C.c_int, [P, u, C.POINTER(capi.bx_world_counts)]),
@@ -212,6 +213,13 @@ class Batch:
         t = (C.c_float * 11)()
         n = _check(_lib.bx_batch_get_kernel_timings(self.h, t, 11))
         return {_lib.bx_kernel_name(i).decode(): float(t[i]) for i in range(n)}
+
+    def solver_layout(self):
+        """(islands per wavefront of the velocity sweeps, [k_solve_velocity ms timed per layout])."""
+        lanes = C.c_uint32(0)
+        t = (C.c_float * 8)()
+        n = _check(_lib.bx_batch_get_solver_layout(self.h, C.byref(lanes), t, 8))
+        return lanes.value, [float(t[i]) for i in range(n)]
 
     def device_bytes(self):
         return int(_lib.bx_batch_device_bytes(self.h))

@@ -252,7 +252,10 @@ typedef struct bx_step_cfg {
 typedef struct bx_batch_opts {
     uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
-    uint32_t solver_mode;              /* 0: auto, 1: one lane per island, 2: scheduled wavefront */
+    uint32_t solver_mode;              /* velocity-sweep layout, one lane per island: 1, 2, 3 = 16, 32, 64
+                                          islands per wavefront; 0: tuned on the batch's steps 4-9
+                                          (each layout timed twice, order 16 32 64 64 32 16), the
+                                          one with the smallest sample is kept */
     uint32_t mesh_triangles_per_contact; /* triangles a mesh contact can overlap (0: 64) */
     uint32_t reserved[4];
 } bx_batch_opts;
@@ -302,6 +305,12 @@ int bx_batch_step(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
    per kernel (bx_batch_get_timings and bx_batch_get_kernel_timings then report no timed step).
    Batches only: a pool returns BX_E_UNSUPPORTED. */
 int bx_batch_step_graph(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
+
+/* The velocity-sweep layout in use: islands per wavefront (16 until solver_mode 0 has timed every
+   layout), and the time of k_solve_velocity in ms measured for each layout by solver_mode 0 (the
+   smaller of its two samples; 0 before the tuning steps) in tuned_ms[0..count). Returns the number of layouts (3). No counterpart
+   in the reference, whose solver runs one island at a time on the CPU (island.cpp:154-181). */
+int bx_batch_get_solver_layout(bx_batch* batch, uint32_t* islands_per_wavefront, float* tuned_ms, uint32_t count);
 
 /* Advance world worlds[i] by step_counts[i] steps (1 each when step_counts is NULL). Step s launches
    once for every listed world with more than s steps to go, so the worlds share each launch. */

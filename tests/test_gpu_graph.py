@@ -77,7 +77,8 @@ def test_graph_replay_limited_chains_with_sleep(gpu_ctx):
 
 def test_graph_replay_reports_no_kernel_times_and_refuses_pools(gpu_ctx):
     from bounce_amd import device
-    batch = device.Batch(gpu_ctx, scenes.pyramids(2, levels=5, seed=1))
+    # a fixed layout: under solver_mode 0 the first nine steps run directly (timed) to tune it
+    batch = device.Batch(gpu_ctx, scenes.pyramids(2, levels=5, seed=1), solver_mode=1)
     batch.step(n=2)
     assert batch.kernel_timings()
     batch.step_graph(n=2)

@@ -1,0 +1,95 @@
+"""Velocity-sweep layouts (bx_batch_opts.solver_mode) give identical results, bit-exact.
+
+solver_mode 1, 2, 3 put 16, 32 or 64 islands on one wavefront (one lane each) with LDS slabs of
+256, 128 or 64 bodies; larger islands keep their velocities in global memory. solver_mode 0 runs
+each layout on one of the first timed steps and keeps the fastest. A layout only decides which
+lane solves which island, so every mode must give the same state hash for every world at every
+checkpoint, on islands that fit every slab, islands that fit only the larger slabs, joints,
+meshes and many small islands.
+"""
+import pytest
+
+from bounce_amd import scenes
+from tests.golden.make_golden import state_hash
+
+pytestmark = pytest.mark.gpu
+
+
+class DeviceWorld:
+    def __init__(self, batch, w):
+        self.batch, self.w = batch, w
+
+    def bodies(self):
+        return self.batch.bodies(self.w)
+
+    def contacts(self):
+        return self.batch.contacts(self.w)
+
+    def pairs(self):
+        return self.batch.pairs(self.w)
+
+    def island_contacts(self):
+        return self.batch.island_contacts(self.w)
+
+
+CASES = {
+    # 10-, 12- and 16-level pyramids: islands of 56, 79 and 137 bodies
+    "pyramids": lambda: scenes.concat([scenes.pyramids(2, levels=10, seed=3), scenes.pyramids(2, levels=12, seed=4),
+                                       scenes.pyramids(2, levels=16, seed=5)]),
+    "drop200": lambda: scenes.drop_boxes(200, extent=(10.0, 25.0, 10.0), seed=9),
+    "limited_chains_sleep": lambda: scenes.with_flags(scenes.chains(2, 10, limit=True, extra=20), 1),
+    "mesh_grounds": lambda: scenes.mesh_drop(20, n_worlds=2, seed=37, hull_count=4),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_layouts_match(gpu_ctx, name):
+    from bounce_amd import device
+    scene = CASES[name]()
+    worlds = len(scene.worlds)
+    print(f"[layouts] {name}: {worlds} worlds, modes 0-3, 1+2+57 steps", flush=True)
+    batches = {m: device.Batch(gpu_ctx, scene, solver_mode=m) for m in range(4)}
+    for k in (1, 2, 57):
+        hashes = {}
+        for m, b in batches.items():
+            b.step(n=k)
+            b.sync()
+            hashes[m] = [state_hash(DeviceWorld(b, w)) for w in range(worlds)]
+        for m in (0, 2, 3):
+            for w in range(worlds):
+                assert hashes[m][w] == hashes[1][w], f"{name}: mode {m} differs from mode 1 in world {w}"
+    for m in (1, 2, 3):
+        lanes, times = batches[m].solver_layout()
+        assert lanes == (16, 32, 64)[m - 1] and times == [0.0, 0.0, 0.0]
+    lanes, times = batches[0].solver_layout()
+    assert lanes in (16, 32, 64) and all(t > 0.0 for t in times)
+    assert lanes == (16, 32, 64)[min(range(3), key=lambda i: times[i])]
+    for b in batches.values():
+        for w in range(worlds):
+            assert b.counts(w).status == 0
+        b.close()
+
+
+def test_graph_replay_uses_the_tuned_layout(gpu_ctx):
+    from bounce_amd import device
+    scene = scenes.pyramids(4, levels=12, seed=6)
+    print("[layouts] graph replay after tuning: 2 + 20 steps", flush=True)
+    direct = device.Batch(gpu_ctx, scene, solver_mode=0)
+    replay = device.Batch(gpu_ctx, scene, solver_mode=0)
+    direct.step(n=2)
+    replay.step_graph(n=2)  # two of the three tuning steps, launched directly
+    direct.step(n=20)
+    replay.step_graph(n=20)  # the third tuning step, then replays with the kept layout
+    direct.sync()
+    replay.sync()
+    assert all(t > 0.0 for t in replay.solver_layout()[1])
+    for w in range(4):
+        assert state_hash(DeviceWorld(direct, w)) == state_hash(DeviceWorld(replay, w))
+    direct.close()
+    replay.close()
+
+
+def test_invalid_solver_mode_is_rejected(gpu_ctx):
+    from bounce_amd import device
+    with pytest.raises(device.BxError):
+        device.Batch(gpu_ctx, scenes.pyramids(1, levels=4), solver_mode=4)
