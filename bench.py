@@ -230,7 +230,8 @@ def main():
         "solver_layout": {"islands_per_wavefront": lanes, "layouts": [16, 32, 64],
                           "tuned_k_solve_velocity_ms": [round(t, 3) for t in tuned_ms],
                           "note": "solver_mode 0: each layout timed twice on warm-up steps 4-9 (order "
-                                  "16 32 64 64 32 16), the one with the smallest sample kept"},
+                                  "16 32 64 64 32 16), the one with the smallest sample kept (16 unless "
+                                  "another is faster by more than 5%)"},
         "graph_replay": graph,
         "plain_per_world": plain,
         "batched_over_plain": value / plain["value"] if plain else None,

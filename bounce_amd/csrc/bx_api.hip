@@ -225,6 +225,9 @@ static const u32 kListRing = 4;
 static const u32 kTuneSkip = 3;
 static const u32 kTuneOrder[2 * kSolveLayouts] = {0, 1, 2, 2, 1, 0};
 static const u32 kTuneSteps = kTuneSkip + 2 * kSolveLayouts;
+// Layouts within a few percent of each other are a tie in one or two samples; the first layout is
+// kept unless another is faster by more than this fraction.
+static const float kTuneMargin = 0.05f;
 
 struct bx_batch {
     bx_context* ctx = nullptr;
@@ -1106,9 +1109,10 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
             const bool first = b->tuneStep - kTuneSkip < kSolveLayouts;
             b->tuneMs[layout] = first ? ms : std::min(b->tuneMs[layout], ms);
             if (++b->tuneStep == kTuneSteps) {
+                // another layout replaces the first only when faster by more than kTuneMargin
                 u32 best = 0;
                 for (u32 l = 1; l < kSolveLayouts; ++l)
-                    if (b->tuneMs[l] < b->tuneMs[best]) best = l;
+                    if (b->tuneMs[l] < b->tuneMs[best] && b->tuneMs[l] < b->tuneMs[0] * (1.0f - kTuneMargin)) best = l;
                 b->view.solveLayout = best;
             }
         } else {

@@ -35,6 +35,12 @@ def short(name):
     return m.group(1) if m else name
 
 
+def instance(name):
+    """The kernel's short name with its template arguments (k_solve_velocity<16u, 256u>)."""
+    m = re.search(r"bx::(k_\w+(?:<[^>]*>)?)\(", name)
+    return m.group(1) if m else short(name)
+
+
 def per_kernel_calls(rows, key):
     out = {}
     for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
@@ -87,7 +93,7 @@ def main():
              "## rocprofv3 --stats (all calls of the bench process)", "",
              "| kernel | calls | average ms | share % |", "|---|---|---|---|"]
     for r in stats:
-        lines.append(f"| {short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) * 1e-6:.4f} | "
+        lines.append(f"| {instance(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) * 1e-6:.4f} | "
                      f"{float(r['Percentage']):.2f} |")
     lines += ["", f"## Timed region (steps {a.warmup + 1}..{a.warmup + a.steps}): kernel trace vs the bench's "
               "HIP events", "", "| kernel | trace average ms | HIP-event average ms | ratio |", "|---|---|---|---|"]

@@ -255,7 +255,8 @@ typedef struct bx_batch_opts {
     uint32_t solver_mode;              /* velocity-sweep layout, one lane per island: 1, 2, 3 = 16, 32, 64
                                           islands per wavefront; 0: tuned on the batch's steps 4-9
                                           (each layout timed twice, order 16 32 64 64 32 16), the
-                                          one with the smallest sample is kept */
+                                          one with the smallest sample is kept; 16 unless another
+                                          is faster by more than 5% */
     uint32_t mesh_triangles_per_contact; /* triangles a mesh contact can overlap (0: 64) */
     uint32_t reserved[4];
 } bx_batch_opts;

@@ -63,7 +63,9 @@ def test_layouts_match(gpu_ctx, name):
         assert lanes == (16, 32, 64)[m - 1] and times == [0.0, 0.0, 0.0]
     lanes, times = batches[0].solver_layout()
     assert lanes in (16, 32, 64) and all(t > 0.0 for t in times)
-    assert lanes == (16, 32, 64)[min(range(3), key=lambda i: times[i])]
+    best = min(range(3), key=lambda i: times[i])
+    expect = best if times[best] < times[0] * (1.0 - 0.05) else 0  # 16 unless faster by more than 5%
+    assert lanes == (16, 32, 64)[expect]
     for b in batches.values():
         for w in range(worlds):
             assert b.counts(w).status == 0
