@@ -10,7 +10,9 @@ together at dt = 1/60 s with 8 velocity / 2 position iterations in FP32 (sleepin
 is one b3World::Step of every world of the batch; inputs are resident in HBM before timing.
 `--workload mixed` runs the mixed-shapes batch instead: per-world body counts lognormal (median
 200, clipped to 20..2000), spheres, capsules and random hulls of 8-16 points dropped on a ground box,
-and a 10-link revolute chain with a motor per world (scenes.mixed_worlds, seed 8).
+and a 10-link revolute chain with a motor per world (scenes.mixed_worlds, seed 8). `--workload drops`
+runs worlds of 30 unit boxes dropped on a ground box (many small islands), and `--solver-mode`
+fixes the velocity-sweep layout (default 0: tuned on the batch's first steps).
 
 Printed (one JSON line): body-steps/s of the batched path (value; the step's kernels launched
 directly, with HIP events between them), the same steps of a fresh copy of the batch as replays
@@ -54,7 +56,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=60)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--workload", choices=("stacking", "mixed"), default="stacking")
+    p.add_argument("--workload", choices=("stacking", "mixed", "drops"), default="stacking")
+    p.add_argument("--solver-mode", type=int, default=0,
+                   help="velocity-sweep layout: 1, 2, 3 = 16, 32, 64 islands per wavefront; 0 = tuned")
     p.add_argument("--worlds", type=int, default=4096)
     p.add_argument("--levels", type=int, default=20)
     p.add_argument("--plain-worlds", type=int, default=16)
@@ -118,6 +122,8 @@ def main():
     from bounce_amd import device, scenes
 
     def factory(n):
+        if args.workload == "drops":
+            return scenes.drop_boxes(30, n_worlds=n, seed=21, extent=(12.0, 6.0, 12.0))
         if args.workload == "mixed":
             return scenes.mixed_worlds(n)
         return scenes.pyramids(n, levels=args.levels, seed=2)
@@ -126,7 +132,7 @@ def main():
     bodies_per_world = scene.worlds[0].body_count
     total_bodies = scene.body_count
     ctx = device.Context(0)
-    batch = device.Batch(ctx, scene)
+    batch = device.Batch(ctx, scene, solver_mode=args.solver_mode)
     # warmup
     for _ in range(args.warmup):
         batch.step()
@@ -174,7 +180,7 @@ def main():
     # (bx_batch_step_graph; the capture happens in the first warm-up step)
     graph = None
     if args.graph:
-        gb = device.Batch(ctx, scene)
+        gb = device.Batch(ctx, scene, solver_mode=args.solver_mode)
         for _ in range(args.warmup):
             gb.step_graph()
         gb.sync()
@@ -219,10 +225,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded pyramids)" if args.workload == "stacking" else "synthetic (seeded mixed worlds)",
+        "data": {"stacking": "synthetic (seeded pyramids)", "mixed": "synthetic (seeded mixed worlds)",
+                 "drops": "synthetic (seeded dropped boxes)"}[args.workload],
         "config": {"workload": (f"batched stacking: {args.worlds} worlds x {args.levels}-level box pyramid "
                                 f"({bodies_per_world} bodies incl. ground), dt=1/60, 8/2 iterations, sleep off")
                    if args.workload == "stacking" else
+                   (f"batched drops: {args.worlds} worlds x 30 unit boxes dropped from up to 6 m on a ground box "
+                    f"(many small islands), dt=1/60, 8/2 iterations, sleep off")
+                   if args.workload == "drops" else
                    (f"batched mixed shapes: {args.worlds} worlds, {total_bodies} bodies (lognormal median 200, "
                     f"20..2000 per world; spheres, capsules, 8-16 point hulls, a 10-link motor chain), "
                     f"dt=1/60, 8/2 iterations, sleep off"),
