@@ -119,7 +119,8 @@ class bx_batch_opts(C.Structure):
 
 class bx_pool_opts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_worlds", u32), ("max_contacts_per_world", u32),
-                ("max_pairs_per_world", u32), ("joints_per_1000_bodies", u32), ("reserved", _arr(u32, 4))]
+                ("max_pairs_per_world", u32), ("joints_per_1000_bodies", u32), ("solver_mode", u32),
+                ("reserved", _arr(u32, 3))]
 
 
 class bx_pool_stats(C.Structure):

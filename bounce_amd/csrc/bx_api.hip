@@ -1373,6 +1373,11 @@ int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_po
     for (u32 i = 0; i < slots; ++i) b->freeWorldSlots[i] = slots - 1 - i;  // lowest slot first
     b->view.islandLdsWords = kIslandLdsBudgetWords;
     b->newContacts = false;
+    if (opts->solver_mode > kSolveLayouts) {
+        bx_batch_destroy(b);
+        return Fail(BX_E_INVALID, "solver_mode must be 0 (16 islands per wavefront) or 1..3");
+    }
+    b->view.solveLayout = opts->solver_mode ? opts->solver_mode - 1 : 0u;
     *out = b;
     return BX_OK;
 }

@@ -233,7 +233,7 @@ class Pool(Batch):
     """Resident worlds in a fixed device-memory budget (bx_pool_*): worlds are admitted into slots
     while they fit, stepped in any mix with step_list, read like a batch's worlds and released."""
 
-    def __init__(self, ctx, scene, budget_bytes, max_worlds=0, max_contacts=0, max_pairs=0):
+    def __init__(self, ctx, scene, budget_bytes, max_worlds=0, max_contacts=0, max_pairs=0, solver_mode=0):
         self.ctx = ctx
         self.scene = scene  # shape library; its worlds are not admitted by construction
         self._lib = scene.library()
@@ -242,6 +242,7 @@ class Pool(Batch):
         opts.max_worlds = max_worlds
         opts.max_contacts_per_world = max_contacts
         opts.max_pairs_per_world = max_pairs
+        opts.solver_mode = solver_mode
         h = C.c_void_p()
         _check(_lib.bx_pool_create(ctx.h, C.byref(self._lib), C.byref(opts), C.byref(h)))
         self.h = h

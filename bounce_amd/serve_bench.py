@@ -27,7 +27,7 @@ def catalog(n_worlds, seed=8, clip=(50, 5000), kind="mixed"):
 
 
 def run(n_worlds=1024, budget_gb=2.0, rates=(1000.0,), seconds=8.0, seed=1, max_slots=4096, log=sys.stderr,
-        kind="mixed"):
+        kind="mixed", solver_mode=0):
     """Yields one report per rate as soon as that rate has been served."""
     cat = catalog(n_worlds, kind=kind)
     ctx = device.Context(0)
@@ -37,7 +37,8 @@ def run(n_worlds=1024, budget_gb=2.0, rates=(1000.0,), seconds=8.0, seed=1, max_
               file=log, flush=True)
 
     for rate in rates:
-        pool = device.Pool(ctx, cat, budget_bytes=int(budget_gb * (1 << 30)), max_worlds=max_slots)
+        pool = device.Pool(ctx, cat, budget_bytes=int(budget_gb * (1 << 30)), max_worlds=max_slots,
+                           solver_mode=solver_mode)
         srv = serving.Server(pool, cat.worlds)
         reqs = serving.poisson_requests(rate, seconds, n_worlds, max_steps=16, seed=seed)
         print(f"[serve] rate {rate}/s: {len(reqs)} requests over {n_worlds} worlds", file=log, flush=True)
@@ -45,6 +46,7 @@ def run(n_worlds=1024, budget_gb=2.0, rates=(1000.0,), seconds=8.0, seed=1, max_
         wall = srv.run(reqs, progress=progress)
         rep = srv.report(wall)
         rep.update({"rate_per_s": rate, "worlds": n_worlds, "budget_gb": budget_gb, "catalog": kind,
+                    "solver_mode": solver_mode,
                     "catalog_bodies": int(cat.body_count), "serve_s": time.perf_counter() - t})
         st = pool.stats()
         rep["pool_capacity_gb"] = st.capacity_bytes / 2 ** 30
@@ -62,9 +64,10 @@ def main(argv=None):
     ap.add_argument("--seconds", type=float, default=8.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--catalog", choices=("mixed", "boxes"), default="mixed")
+    ap.add_argument("--solver-mode", type=int, default=0, help="1, 2, 3 = 16, 32, 64 islands per wavefront")
     a = ap.parse_args(argv)
     rates = [float(x) for x in a.rates.split(",")]
-    for rep in run(a.worlds, a.budget_gb, rates, a.seconds, a.seed, kind=a.catalog):
+    for rep in run(a.worlds, a.budget_gb, rates, a.seconds, a.seed, kind=a.catalog, solver_mode=a.solver_mode):
         print(json.dumps(rep), flush=True)
 
 

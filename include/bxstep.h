@@ -267,7 +267,10 @@ typedef struct bx_pool_opts {
     uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t joints_per_1000_bodies;   /* share of the budget kept for joints (0: 50) */
-    uint32_t reserved[4];
+    uint32_t solver_mode;              /* velocity-sweep layout: 1, 2, 3 = 16, 32, 64 islands per
+                                          wavefront; 0: 16 (pools are not tuned: their world mix
+                                          changes from tick to tick) */
+    uint32_t reserved[3];
 } bx_pool_opts;
 
 typedef struct bx_pool_stats {

@@ -95,3 +95,25 @@ def test_invalid_solver_mode_is_rejected(gpu_ctx):
     from bounce_amd import device
     with pytest.raises(device.BxError):
         device.Batch(gpu_ctx, scenes.pyramids(1, levels=4), solver_mode=4)
+
+
+def test_pool_layouts_match(gpu_ctx):
+    """Pools take a fixed layout (bx_pool_opts.solver_mode); 16 and 64 islands per wavefront give the
+    same state for worlds stepped in a changing mix."""
+    from bounce_amd import device
+    scene = scenes.drop_boxes(30, n_worlds=6, seed=21, extent=(12.0, 6.0, 12.0))
+    print("[layouts] pools at 16 and 64 islands per wavefront: 40 ticks", flush=True)
+    pools = [device.Pool(gpu_ctx, scene, budget_bytes=64 << 20, max_worlds=6, solver_mode=m) for m in (1, 3)]
+    slots = [[p.admit(w) for w in scene.worlds] for p in pools]
+    for t in range(40):
+        pick = [i for i in range(6) if (t + i) % 3 != 0]
+        for p, s in zip(pools, slots):
+            p.step_list([s[i] for i in pick])
+    for i in range(6):
+        a = state_hash(DeviceWorld(pools[0], slots[0][i]))
+        b = state_hash(DeviceWorld(pools[1], slots[1][i]))
+        assert a == b, f"world {i}"
+    for p in pools:
+        p.close()
+    with pytest.raises(device.BxError):
+        device.Pool(gpu_ctx, scene, budget_bytes=64 << 20, max_worlds=2, solver_mode=4)
