@@ -2,7 +2,6 @@
 """Benchmark of the batched b3World::Step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds 4096] [--levels 20]
-    python bench.py --workload mixed --worlds 2048
 
 Workload (default): the batched-stacking configuration, 4,096 independent worlds, each a 20-level
 box pyramid (210 unit boxes, friction 0.6, seeded +-1 mm jitter) on a static ground box, stepped
@@ -72,6 +71,7 @@ def parse():
     p.add_argument("--serve-rate", type=float, default=1000.0)
     p.add_argument("--serve-seconds", type=float, default=0.0)
     p.add_argument("--serve-budget-gb", type=float, default=1.0)
+    p.add_argument("--serve-catalog", choices=("boxes", "mixed"), default="boxes")
     return p.parse_args()
 
 
@@ -97,7 +97,7 @@ def serve(args):
     import subprocess
     cmd = [sys.executable, "-m", "bounce_amd.serve_bench", "--worlds", str(args.serve_worlds),
            "--budget-gb", str(args.serve_budget_gb), "--rates", str(args.serve_rate),
-           "--seconds", str(args.serve_seconds)]
+           "--seconds", str(args.serve_seconds), "--catalog", args.serve_catalog]
     try:
         r = subprocess.run(cmd, cwd=REPO, capture_output=True, text=True, timeout=600)
     except subprocess.TimeoutExpired:
@@ -108,7 +108,7 @@ def serve(args):
         return {"error": f"exit {r.returncode}"}
     rep = json.loads(lines[-1])
     rep["workload"] = (f"Poisson requests at {args.serve_rate}/s for {args.serve_seconds} s, each 1-16 steps of "
-                       f"one of {args.serve_worlds} mixed-shape worlds (50-5000 bodies), "
+                       f"one of {args.serve_worlds} {args.serve_catalog} worlds (50-5000 bodies), "
                        f"pool budget {args.serve_budget_gb} GiB")
     return rep
 
@@ -119,7 +119,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    from bounce_amd import device, scenes
+    from bounce_amd import device, gates, scenes
+    if args.workload == "mixed":
+        gates.require_mixed_gpu("bench.py --workload mixed")
+    if args.serve_seconds > 0 and args.serve_catalog == "mixed":
+        gates.require_mixed_gpu("bench.py --serve-catalog mixed")
 
     def factory(n):
         if args.workload == "drops":

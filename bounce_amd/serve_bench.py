@@ -17,7 +17,7 @@ import time
 
 import numpy as np
 
-from bounce_amd import device, scenes, serving
+from bounce_amd import device, gates, scenes, serving
 
 
 def catalog(n_worlds, seed=8, clip=(50, 5000), kind="mixed"):
@@ -27,7 +27,7 @@ def catalog(n_worlds, seed=8, clip=(50, 5000), kind="mixed"):
 
 
 def run(n_worlds=1024, budget_gb=2.0, rates=(1000.0,), seconds=8.0, seed=1, max_slots=4096, log=sys.stderr,
-        kind="mixed", solver_mode=0):
+        kind="boxes", solver_mode=0):
     """Yields one report per rate as soon as that rate has been served."""
     cat = catalog(n_worlds, kind=kind)
     ctx = device.Context(0)
@@ -63,10 +63,12 @@ def main(argv=None):
     ap.add_argument("--rates", default="500,1000,2000")
     ap.add_argument("--seconds", type=float, default=8.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--catalog", choices=("mixed", "boxes"), default="mixed")
+    ap.add_argument("--catalog", choices=("mixed", "boxes"), default="boxes")
     ap.add_argument("--solver-mode", type=int, default=0, help="1, 2, 3 = 16, 32, 64 islands per wavefront")
     a = ap.parse_args(argv)
     rates = [float(x) for x in a.rates.split(",")]
+    if a.catalog == "mixed":
+        gates.require_mixed_gpu("serve_bench --catalog mixed")
     for rep in run(a.worlds, a.budget_gb, rates, a.seconds, a.seed, kind=a.catalog, solver_mode=a.solver_mode):
         print(json.dumps(rep), flush=True)
 
