@@ -177,6 +177,7 @@ __global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams s
     const u32 count = *B.islandListCount;
     for (u32 i = blockIdx.x * kLanes + slot; i < count; i += gridDim.x * kLanes) {
         const uint2 e = B.islandList[i];
+        BX_WORLD_HOOK(B, e.x);
         const WorldInfo wi = B.worlds[e.x];
         const WorldState& ws = B.wstate[e.x];
         const bool warm = (wi.flags & 0x2u) == 0;
@@ -282,6 +283,7 @@ __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams s
     const u32 count = *B.islandListCount;
     for (u32 i = blockIdx.x * kPosLanes + threadIdx.x; i < count; i += gridDim.x * kPosLanes) {
         const uint2 e = B.islandList[i];
+        BX_WORLD_HOOK(B, e.x);
         const WorldInfo wi = B.worlds[e.x];
         const WorldState& ws = B.wstate[e.x];
         const u32 first = e.y == kNull ? 0u : e.y;

@@ -359,9 +359,18 @@ struct BatchView {
     u32 solveLayout;          // layout of the velocity sweeps (index into kSolveLayoutLanes)
 };
 
+// BX_WORLD_HOOK(view, w) marks where a workgroup or lane starts work on world w; it compiles to
+// nothing in the device library (the CPU test build uses it to check that a world's kernels touch
+// only that world's ranges).
+#ifndef BX_WORLD_HOOK
+#define BX_WORLD_HOOK(view, w)
+#endif
+
 // World stepped by this workgroup in the one-workgroup-per-world kernels.
 __device__ __forceinline__ u32 WorldOfBlock(const BatchView& B) {
-    return B.worldList ? B.worldList[blockIdx.x] : blockIdx.x;
+    const u32 w = B.worldList ? B.worldList[blockIdx.x] : blockIdx.x;
+    BX_WORLD_HOOK(B, w);
+    return w;
 }
 
 }  // namespace bx
