@@ -31,9 +31,17 @@ typedef uint8_t u8;
 typedef uint64_t u64;
 
 static thread_local std::string g_error;
+static thread_local const char* g_errorStatic = nullptr;  // set instead of g_error where nothing may allocate
 
 static int Fail(int code, const std::string& msg) {
     g_error = msg;
+    g_errorStatic = nullptr;
+    return code;
+}
+
+// Failure with a message of static storage (no allocation on the error path).
+static int FailStatic(int code, const char* msg) {
+    g_errorStatic = msg;
     return code;
 }
 
@@ -41,6 +49,13 @@ static int Fail(int code, const std::string& msg) {
     do {                                                                                                 \
         hipError_t _e = (expr);                                                                          \
         if (_e != hipSuccess) return Fail(BX_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+// HIP_CHECK for paths that allocate nothing, even when they fail: the message is HIP's own.
+#define HIP_CHECK_S(expr)                                          \
+    do {                                                           \
+        hipError_t _e = (expr);                                    \
+        if (_e != hipSuccess) return FailStatic(BX_E_DEVICE, hipGetErrorString(_e)); \
     } while (0)
 
 // Steps of one bx_batch_step call whose kernels are timed with HIP events (12 events per step,
@@ -52,7 +67,16 @@ struct bx_context {
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;  // 12 per timed step of the last bx_batch_step call
     u32 timedSteps = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of `scratch` (bx_batch_step_noalloc)
 };
+
+// bx_batch_step_noalloc keeps the launch's island work list (one {world, island} entry per island, at
+// most one per body) and its counter in this file-scope device array instead of in memory the batch
+// allocated. Every device the library runs on holds one copy until the process ends; a context looks
+// up its device's copy once, at creation. Calls that overlap on one device share it (and race).
+static const u32 kNoAllocIslands = 1u << 21;
+static const size_t kNoAllocListBytes = size_t(kNoAllocIslands) * sizeof(uint2);
+static __device__ __align__(256) unsigned char scratch[kNoAllocListBytes + 256];
 
 static int EnsureEvents(bx_context* ctx, u32 steps) {
     size_t need = size_t(12) * steps;
@@ -1003,31 +1027,35 @@ static StepParams MakeParams(const bx_step_cfg* cfg) {
 static int NextListRegion(bx_batch* b, u32** host, u32** dev, u32* r) {
     *r = b->listNext;
     b->listNext = (b->listNext + 1) % kListRing;
-    if (b->listUsed[*r]) HIP_CHECK(hipEventSynchronize(b->listEv[*r]));
+    if (b->listUsed[*r]) HIP_CHECK_S(hipEventSynchronize(b->listEv[*r]));
     size_t off = size_t(*r) * 2 * std::max(1u, b->worldCount);
     *host = b->listHost + off;
     *dev = b->listDev + off;
     return BX_OK;
 }
 
-// The creation-time pair search of every listed world that has not had it yet.
+// The creation-time pair search of every listed world (every world when `worlds` is null) that has
+// not had it yet.
 static int RunInitialPairs(bx_batch* b, const u32* worlds, u32 count) {
     if (!b->newContacts) return BX_OK;
     u32 *h, *d, r;
     int rc = NextListRegion(b, &h, &d, &r);
     if (rc) return rc;
     u32 n = 0;
-    for (u32 i = 0; i < count; ++i)
-        if (b->needsInitialPairs[worlds[i]]) {
-            h[n++] = worlds[i];
-            b->needsInitialPairs[worlds[i]] = 0;
+    if (!worlds) count = b->worldCount;
+    for (u32 i = 0; i < count; ++i) {
+        const u32 w = worlds ? worlds[i] : i;
+        if (b->needsInitialPairs[w]) {
+            h[n++] = w;
+            b->needsInitialPairs[w] = 0;
         }
-    if (n) {
-        HIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
-        LaunchInitialPairs(b->view, d, n, b->ctx->stream);
-        HIP_CHECK(hipGetLastError());
     }
-    HIP_CHECK(hipEventRecord(b->listEv[r], b->ctx->stream));
+    if (n) {
+        HIP_CHECK_S(hipMemcpyAsync(d, h, n * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
+        LaunchInitialPairs(b->view, d, n, b->ctx->stream);
+        HIP_CHECK_S(hipGetLastError());
+    }
+    HIP_CHECK_S(hipEventRecord(b->listEv[r], b->ctx->stream));
     b->listUsed[r] = true;
     bool any = false;
     for (u32 w = 0; w < b->worldCount && !any; ++w) any = b->needsInitialPairs[w] != 0;
@@ -1083,9 +1111,7 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
             std::fill(b->needsInitialPairs.begin(), b->needsInitialPairs.end(), 0);
             b->newContacts = false;
         } else {
-            std::vector<u32> all(b->worldCount);
-            for (u32 i = 0; i < b->worldCount; ++i) all[i] = i;
-            int rc = RunInitialPairs(b, all.data(), b->worldCount);
+            int rc = RunInitialPairs(b, nullptr, 0);
             if (rc) return rc;
         }
     }
@@ -1150,9 +1176,7 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     u32 maxIslands = 0;
     for (u32 i = 0; i < b->worldCount; ++i) maxIslands += b->worlds[i].bodyCount;
     if (b->newContacts) {
-        std::vector<u32> all(b->worldCount);
-        for (u32 i = 0; i < b->worldCount; ++i) all[i] = i;
-        int rc = RunInitialPairs(b, all.data(), b->worldCount);
+        int rc = RunInitialPairs(b, nullptr, 0);
         if (rc) return rc;
     }
     const StepParams& g = b->graphParams;
@@ -1188,7 +1212,7 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
 extern "C" {
 
 const char* bx_version(void) { return "bxstep 0.1 gfx950"; }
-const char* bx_last_error(void) { return g_error.c_str(); }
+const char* bx_last_error(void) { return g_errorStatic ? g_errorStatic : g_error.c_str(); }
 
 int bx_context_create(int device, bx_context** out) {
     if (!out) return Fail(BX_E_INVALID, "null out");
@@ -1199,6 +1223,15 @@ int bx_context_create(int device, bx_context** out) {
     bx_context* ctx = new bx_context();
     ctx->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // this device's copy of the no-allocation scratch array (the array's name is a host-side stand-in)
+    void* sym = nullptr;
+    hipError_t se = hipGetSymbolAddress(&sym, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sym) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return Fail(BX_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+    }
+    ctx->scratch = static_cast<unsigned char*>(sym);
     int rc = EnsureEvents(ctx, 1);
     if (rc) {
         (void)hipStreamDestroy(ctx->stream);
@@ -1589,6 +1622,45 @@ int bx_batch_step(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count) {
         int rc = StepList(b, cfg, list.data(), u32(list.size()));
         if (rc) return rc;
     }
+    return BX_OK;
+}
+
+int bx_batch_step_noalloc(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_count) {
+    if (!b || !cfg) return FailStatic(BX_E_INVALID, "null argument");
+    if (b->pool) return FailStatic(BX_E_UNSUPPORTED, "bx_batch_step_noalloc steps a batch, not a pool");
+    if (!b->ctx->scratch) return FailStatic(BX_E_DEVICE, "bx_batch_step_noalloc: no scratch array on this device");
+    HIP_CHECK_S(hipSetDevice(b->ctx->device));
+    if (step_count == 0 || b->worldCount == 0) return BX_OK;
+    u64 bodies = 0;
+    bool allNew = true;
+    for (u32 w = 0; w < b->worldCount; ++w) {
+        bodies += b->worlds[w].bodyCount;
+        allNew = allNew && b->needsInitialPairs[w];
+    }
+    if (bodies > kNoAllocIslands)
+        return FailStatic(BX_E_CAPACITY, "bx_batch_step_noalloc: the batch holds more than 2,097,152 bodies, the "
+                                         "island work list of the no-allocation scratch array");
+    bool findFirst = false;
+    if (b->newContacts) {
+        if (allNew) {  // the creation-time pair search runs as the first step's first kernel
+            findFirst = true;
+            for (u32 w = 0; w < b->worldCount; ++w) b->needsInitialPairs[w] = 0;
+            b->newContacts = false;
+        } else {
+            int rc = RunInitialPairs(b, nullptr, 0);
+            if (rc) return rc;
+        }
+    }
+    BatchView v = b->view;
+    v.islandList = reinterpret_cast<uint2*>(b->ctx->scratch);
+    v.islandListCount = reinterpret_cast<u32*>(b->ctx->scratch + kNoAllocListBytes);  // reset by the contact update
+    const StepParams sp = MakeParams(cfg);
+    for (u32 s = 0; s < step_count; ++s) {
+        LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, u32(bodies), b->ctx->stream, nullptr);
+        findFirst = false;
+    }
+    HIP_CHECK_S(hipGetLastError());
+    b->ctx->timedSteps = 0;
     return BX_OK;
 }
 

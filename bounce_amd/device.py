@@ -37,6 +37,7 @@ def load(path=capi.LIB_PATH):
         "bx_batch_destroy": (None, [P]),
         "bx_batch_step": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
         "bx_batch_step_graph": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
+        "bx_batch_step_noalloc": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), u]),
         "bx_batch_get_solver_layout": (C.c_int, [P, C.POINTER(u), C.POINTER(C.c_float), u]),
         "bx_batch_step_subset": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), C.POINTER(u), u]),
         "bx_batch_sync": (C.c_int, [P]),
@@ -144,6 +145,11 @@ class Batch:
         """n steps, each a replay of one captured step (same results as step; untimed)."""
         cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
         _check(_lib.bx_batch_step_graph(self.h, C.byref(cfg), n))
+
+    def step_noalloc(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
+        """n steps through bx_batch_step_noalloc (same results as step; allocates nothing, untimed)."""
+        cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
+        _check(_lib.bx_batch_step_noalloc(self.h, C.byref(cfg), n))
 
     def step_list(self, worlds, dt=1.0 / 60.0, vit=8, pit=2):
         """One step of each listed world, all in one launch sequence."""

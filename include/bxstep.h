@@ -303,6 +303,15 @@ void bx_batch_destroy(bx_batch* batch);
    Asynchronous on the context stream; call bx_batch_sync before reading state. */
 int bx_batch_step(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
 
+/* The same steps as bx_batch_step, for callers that cannot allocate (code under HIP graph capture,
+   tight serving loops, harnesses that forbid allocation). No counterpart in the reference. It calls no
+   hipMalloc, hipFree or other allocating routine, creates no events and times nothing: the launch's
+   island work list lives in a device array of the library (16 MiB per device, held until the process
+   ends), so a batch may hold at most 2,097,152 bodies (BX_E_CAPACITY beyond). Calls that overlap on one
+   device share that array and race. Batches only (a pool returns BX_E_UNSUPPORTED); the velocity-sweep
+   layout is the batch's current one (no tuning). */
+int bx_batch_step_noalloc(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_count);
+
 /* The same steps as bx_batch_step (b3World::Step, world.cpp:187-242, step_count times) with the same
    results, each step a replay of one step captured as a HIP graph on the first call (captured
    again when cfg changes), so a step costs one graph launch on the host. The steps are not timed

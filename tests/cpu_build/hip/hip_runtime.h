@@ -22,6 +22,8 @@
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (reinterpret_cast<const void*>(&(x)))
 // Block-scope thread_local is static per host thread; one workgroup runs at a time per host
 // thread, so these are the workgroup's shared memory. `extern __shared__ T x[]` names the dynamic
 // LDS buffer of the namespace (bx::lds), defined by the CPU build.
@@ -116,3 +118,8 @@ hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, si
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
 hipError_t hipGraphExecDestroy(hipGraphExec_t e);
 hipError_t hipGraphDestroy(hipGraph_t g);
+// A __device__ variable is an ordinary host object here: its address is the symbol's.
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
