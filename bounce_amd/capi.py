@@ -18,6 +18,9 @@ BX_BODY_ISLAND = 0x2
 BX_BODY_AUTO_SLEEP = 0x4
 BX_BODY_FIXED_ROT_X, BX_BODY_FIXED_ROT_Y, BX_BODY_FIXED_ROT_Z = 0x8, 0x10, 0x20
 BX_WORLD_SLEEPING = 0x1
+(BX_EDIT_LINEAR_VELOCITY, BX_EDIT_ANGULAR_VELOCITY, BX_EDIT_FORCE, BX_EDIT_FORCE_TO_CENTER, BX_EDIT_TORQUE,
+ BX_EDIT_LINEAR_IMPULSE, BX_EDIT_ANGULAR_IMPULSE, BX_EDIT_TRANSFORM, BX_EDIT_AWAKE, BX_EDIT_SLEEPING_ALLOWED,
+ BX_EDIT_GRAVITY_SCALE, BX_EDIT_LINEAR_DAMPING, BX_EDIT_ANGULAR_DAMPING) = range(1, 14)
 BX_WORLD_NO_WARM_START = 0x2
 BX_WORLD_NO_CLEAR_FORCES = 0x4
 BX_JOINT_REVOLUTE = 4
@@ -80,6 +83,22 @@ class bx_body_state(C.Structure):
     _fields_ = [("position", _arr(f32, 3)), ("orientation", _arr(f32, 4)), ("world_center", _arr(f32, 3)),
                 ("linear_velocity", _arr(f32, 3)), ("angular_velocity", _arr(f32, 3)),
                 ("sleep_time", f32), ("flags", u32), ("island_index", u32)]
+
+
+class bx_body_edit(C.Structure):
+    _fields_ = [("body", u32), ("op", u32), ("wake", u32), ("reserved", u32), ("v", _arr(f32, 3)), ("p", _arr(f32, 4))]
+
+
+def body_edits(rows):
+    """bx_body_edit records from (body, op, v, p, wake) tuples (p and wake optional)."""
+    out = (bx_body_edit * max(1, len(rows)))()
+    for i, r in enumerate(rows):
+        body, op, v = r[0], r[1], r[2]
+        p = r[3] if len(r) > 3 and r[3] is not None else (0.0, 0.0, 0.0, 0.0)
+        out[i].body, out[i].op, out[i].wake = body, op, int(r[4]) if len(r) > 4 else 1
+        out[i].v[:] = [float(x) for x in v]
+        out[i].p[:] = [float(x) for x in (list(p) + [0.0] * 4)[:4]]
+    return out
 
 
 class bx_manifold_point_state(C.Structure):

@@ -1861,8 +1861,41 @@ int bx_batch_get_island_contacts(bx_batch* b, uint32_t world, uint32_t* out, uin
     return BX_OK;
 }
 
+// b3Body::SetAwake (body.h:588-607) on a body record.
+static void HostSetAwake(Body& x, bool flag) {
+    if (flag) {
+        if ((x.flags & kAwake) == 0) {
+            x.flags |= kAwake;
+            x.sleepTime = 0;
+        }
+    } else {
+        x.flags &= ~kAwake;
+        x.sleepTime = 0;
+        x.force.SetZero();
+        x.torque.SetZero();
+        x.v.SetZero();
+        x.w.SetZero();
+    }
+}
+
+// b3Body::SetLinearVelocity / SetAngularVelocity (body.h:652-685): ignored for a static body; a
+// nonzero velocity wakes the body.
+static void HostSetVelocity(Body& x, Vec3& dst, const Vec3& v) {
+    if (x.type == kStatic) return;
+    if (Dot(v, v) > scalar(0)) HostSetAwake(x, true);
+    dst = v;
+}
+
+static int UploadBodies(bx_batch* b, const WorldInfo& wi, const std::vector<Body>& bodies) {
+    HIP_CHECK(hipMemcpyAsync(b->view.bodies + wi.bodyBase, bodies.data(), bodies.size() * sizeof(Body),
+                             hipMemcpyHostToDevice, b->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    return BX_OK;
+}
+
 int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, uint32_t count) {
     if (!b || !IsResident(b, world) || !in) return Fail(BX_E_INVALID, "bad argument or world not resident");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
     const WorldInfo& wi = b->worlds[world];
     if (count != wi.bodyCount) return Fail(BX_E_INVALID, "body count mismatch");
     std::vector<Body> bodies;
@@ -1870,13 +1903,113 @@ int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, ui
     if (rc) return rc;
     for (u32 i = 0; i < count; ++i) {
         Body& x = bodies[i];
-        x.v = V3(in[i].linear_velocity);
-        x.w = V3(in[i].angular_velocity);
+        HostSetVelocity(x, x.v, V3(in[i].linear_velocity));
+        HostSetVelocity(x, x.w, V3(in[i].angular_velocity));
     }
-    HIP_CHECK(hipMemcpyAsync(b->view.bodies + wi.bodyBase, bodies.data(), count * sizeof(Body), hipMemcpyHostToDevice,
-                             b->ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
-    return BX_OK;
+    return UploadBodies(b, wi, bodies);
+}
+
+int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits, uint32_t count) {
+    if (!b || !IsResident(b, world) || (!edits && count)) return Fail(BX_E_INVALID, "bad argument or world not resident");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    const WorldInfo& wi = b->worlds[world];
+    bool transform = false;
+    for (u32 k = 0; k < count; ++k) {
+        if (edits[k].body >= wi.bodyCount) return Fail(BX_E_INVALID, "edit names a body out of range");
+        if (edits[k].op < BX_EDIT_LINEAR_VELOCITY || edits[k].op > BX_EDIT_ANGULAR_DAMPING)
+            return Fail(BX_E_INVALID, "unknown body edit");
+        transform = transform || edits[k].op == BX_EDIT_TRANSFORM;
+    }
+    if (count == 0) return BX_OK;
+    std::vector<Body> bodies;
+    int rc = Download(b, bodies, b->view.bodies + wi.bodyBase, wi.bodyCount);
+    if (rc) return rc;
+    // SetTransform moves the body's fixture proxies: their fixtures, fat AABBs and moved flags
+    std::vector<u32> flist, moved;
+    std::vector<Fixture> fixtures;
+    std::vector<AABB> fat;
+    if (transform) {
+        if ((rc = Download(b, flist, b->view.fixtureList + wi.fixtureListBase, wi.fixtureCount))) return rc;
+        if ((rc = Download(b, fixtures, b->view.fixtures + wi.fixtureBase, wi.fixtureCount))) return rc;
+        if ((rc = Download(b, fat, b->view.fatAABB + wi.fixtureBase, wi.fixtureCount))) return rc;
+        if ((rc = Download(b, moved, b->view.moved + wi.fixtureBase, wi.fixtureCount))) return rc;
+    }
+    for (u32 k = 0; k < count; ++k) {
+        const bx_body_edit& e = edits[k];
+        Body& x = bodies[e.body];
+        const Vec3 v = V3(e.v), p(e.p[0], e.p[1], e.p[2]);
+        const bool dynamic = x.type == kDynamic;
+        // the Apply* methods: dynamic bodies only; wake first if asked; act only on an awake body
+        auto wakeFirst = [&]() {
+            if (e.wake && (x.flags & kAwake) == 0) HostSetAwake(x, true);
+            return (x.flags & kAwake) != 0;
+        };
+        switch (e.op) {
+        case BX_EDIT_LINEAR_VELOCITY: HostSetVelocity(x, x.v, v); break;
+        case BX_EDIT_ANGULAR_VELOCITY: HostSetVelocity(x, x.w, v); break;
+        case BX_EDIT_FORCE:  // body.h:727-744
+            if (dynamic && wakeFirst()) {
+                x.force += v;
+                x.torque += Cross(p - x.x, v);
+            }
+            break;
+        case BX_EDIT_FORCE_TO_CENTER:
+            if (dynamic && wakeFirst()) x.force += v;
+            break;
+        case BX_EDIT_TORQUE:
+            if (dynamic && wakeFirst()) x.torque += v;
+            break;
+        case BX_EDIT_LINEAR_IMPULSE:  // body.h:782-799
+            if (dynamic && wakeFirst()) {
+                x.v += x.invMass * v;
+                x.w += x.worldInvI * Cross(p - x.x, v);
+            }
+            break;
+        case BX_EDIT_ANGULAR_IMPULSE:
+            if (dynamic && wakeFirst()) x.w += x.worldInvI * v;
+            break;
+        case BX_EDIT_TRANSFORM: {  // body.cpp:194-214
+            const Transform xf(v, Quat(Vec3(e.p[0], e.p[1], e.p[2]), e.p[3]));
+            x.xfT = xf.translation;
+            x.q = xf.rotation;
+            x.x = Mul(xf, x.localCenter);
+            x.x0 = x.x;
+            x.q0 = x.q;
+            x.worldInvI = RotateToFrame(x.invI, x.q);
+            for (u32 i = 0; i < x.fixtureCount; ++i) {  // b3Fixture::Synchronize(broadPhase, xf, xf)
+                const u32 f = flist[x.fixtureBegin + i];
+                AABB a1, a2;
+                HostAABB(fixtures[f], xf, *b->library, a1);
+                HostAABB(fixtures[f], xf, *b->library, a2);
+                if (MoveFatAABB(fat[f], Combine(a1, a2), a2.GetCenter() - a1.GetCenter())) moved[f] = 1u;
+            }
+            b->needsInitialPairs[world] = 1;  // m_newContacts: the next step begins with FindNewContacts
+            b->newContacts = true;
+            break;
+        }
+        case BX_EDIT_AWAKE: HostSetAwake(x, e.wake != 0); break;
+        case BX_EDIT_SLEEPING_ALLOWED:  // body.h:829-840
+            if (e.wake) {
+                x.flags |= kAutoSleep;
+            } else {
+                x.flags &= ~kAutoSleep;
+                HostSetAwake(x, true);
+            }
+            break;
+        case BX_EDIT_GRAVITY_SCALE:
+            if (x.type != kStatic) x.gravityScale = v;
+            break;
+        case BX_EDIT_LINEAR_DAMPING: x.linearDamping = v; break;
+        case BX_EDIT_ANGULAR_DAMPING: x.angularDamping = v; break;
+        }
+    }
+    if (transform) {
+        HIP_CHECK(hipMemcpyAsync(b->view.fatAABB + wi.fixtureBase, fat.data(), fat.size() * sizeof(AABB),
+                                 hipMemcpyHostToDevice, b->ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(b->view.moved + wi.fixtureBase, moved.data(), moved.size() * sizeof(u32),
+                                 hipMemcpyHostToDevice, b->ctx->stream));
+    }
+    return UploadBodies(b, wi, bodies);
 }
 
 int bx_batch_get_timings(bx_batch* b, float* ms, uint32_t count) {

@@ -389,24 +389,8 @@ __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
                 ComputeAABB(B, fx, xf1, a1);
                 ComputeAABB(B, fx, xf2, a2);
                 AABB aabb = Combine(a1, a2);
-                Vec3 d = kAabbMultiplier * (a2.GetCenter() - a1.GetCenter());
-                // b3DynamicTree::MoveProxy (dynamic_tree.cpp:139-207)
-                AABB fat = aabb;
-                fat.Extend(kAabbExtension);
-                if (d.x < scalar(0)) fat.lowerBound.x += d.x; else fat.upperBound.x += d.x;
-                if (d.y < scalar(0)) fat.lowerBound.y += d.y; else fat.upperBound.y += d.y;
-                if (d.z < scalar(0)) fat.lowerBound.z += d.z; else fat.upperBound.z += d.z;
-                AABB& tree = B.fatAABB[wi.fixtureBase + f];
-                bool keep = false;
-                if (tree.Contains(aabb)) {
-                    AABB huge = fat;
-                    huge.Extend(scalar(4) * kAabbExtension);
-                    keep = huge.Contains(tree);
-                }
-                if (!keep) {
-                    tree = fat;
+                if (MoveFatAABB(B.fatAABB[wi.fixtureBase + f], aabb, a2.GetCenter() - a1.GetCenter()))
                     B.moved[wi.fixtureBase + f] = 1u;
-                }
             }
         }
         if ((wi.flags & 0x4u) == 0) {  // b3World::ClearForces

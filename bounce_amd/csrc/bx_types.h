@@ -81,6 +81,25 @@ struct TreeNode {
 };
 constexpr u32 kTreeStack = 64;  // traversal stack of a tree query; the host rejects deeper trees
 
+// b3DynamicTree::MoveProxy (dynamic_tree.cpp:139-207) of a proxy whose fat AABB is `tree`: the shape
+// now covers `aabb` and is predicted to move by `displacement`. Returns whether the proxy moved (its fat
+// AABB is replaced by the new, extended and displaced one, and it is buffered for the pair search).
+BX_HD bool MoveFatAABB(AABB& tree, const AABB& aabb, const Vec3& displacement) {
+    const Vec3 d = kAabbMultiplier * displacement;
+    AABB fat = aabb;
+    fat.Extend(kAabbExtension);
+    if (d.x < scalar(0)) fat.lowerBound.x += d.x; else fat.upperBound.x += d.x;
+    if (d.y < scalar(0)) fat.lowerBound.y += d.y; else fat.upperBound.y += d.y;
+    if (d.z < scalar(0)) fat.lowerBound.z += d.z; else fat.upperBound.z += d.z;
+    if (tree.Contains(aabb)) {
+        AABB huge = fat;
+        huge.Extend(scalar(4) * kAabbExtension);
+        if (huge.Contains(tree)) return false;
+    }
+    tree = fat;
+    return true;
+}
+
 // Body flags: b3Body::e_* (body.h:379-384).
 constexpr u32 kAwake = 0x1, kIsland = 0x2, kAutoSleep = 0x4;
 constexpr u32 kFixedX = 0x8, kFixedY = 0x10, kFixedZ = 0x20;

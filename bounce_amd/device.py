@@ -47,6 +47,7 @@ def load(path=capi.LIB_PATH):
         "bx_batch_get_pairs": (C.c_int, [P, u, C.POINTER(capi.bx_pair), u]),
         "bx_batch_get_island_contacts": (C.c_int, [P, u, C.POINTER(u), u, C.POINTER(u)]),
         "bx_batch_set_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
+        "bx_batch_edit_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_edit), u]),
         "bx_batch_get_timings": (C.c_int, [P, C.POINTER(C.c_float), u]),
         "bx_batch_get_kernel_timings": (C.c_int, [P, C.POINTER(C.c_float), u]),
         "bx_kernel_name": (C.c_char_p, [u]),
@@ -145,6 +146,11 @@ class Batch:
         """n steps, each a replay of one captured step (same results as step; untimed)."""
         cfg = capi.bx_step_cfg(C.c_float(dt), vit, pit)
         _check(_lib.bx_batch_step_graph(self.h, C.byref(cfg), n))
+
+    def edit_bodies(self, world, rows):
+        """Body edits between steps: (body, op, v, p, wake) tuples, op a capi.BX_EDIT_* value."""
+        e = capi.body_edits(rows)
+        _check(_lib.bx_batch_edit_bodies(self.h, world, e, len(rows)))
 
     def step_noalloc(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
         """n steps through bx_batch_step_noalloc (same results as step; allocates nothing, untimed)."""

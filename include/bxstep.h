@@ -369,9 +369,41 @@ int bx_batch_get_pairs(bx_batch* batch, uint32_t world, bx_pair* out, uint32_t c
 int bx_batch_get_island_contacts(bx_batch* batch, uint32_t world, uint32_t* out, uint32_t capacity,
                                  uint32_t* count);
 
-/* Overwrite body velocities / forces between steps (b3Body::SetLinearVelocity, ApplyForce ...).
-   `bodies` is in creation order with `count` == body_count of the world. */
+/* Set every body's linear and angular velocity between steps, as b3Body::SetLinearVelocity and
+   SetAngularVelocity (body.h:652-684) would for each body in creation order: static bodies are left
+   alone, and a nonzero velocity wakes the body (b3Body::SetAwake(true), body.h:588-607). `bodies` is in
+   creation order with `count` == body_count of the world; only the velocities are read. */
 int bx_batch_set_bodies(bx_batch* batch, uint32_t world, const bx_body_state* bodies, uint32_t count);
+
+/* Body edits between steps (b3Body's setters and force/impulse methods), applied in order. */
+#define BX_EDIT_LINEAR_VELOCITY 1   /* b3Body::SetLinearVelocity(v)              body.h:652-665 */
+#define BX_EDIT_ANGULAR_VELOCITY 2  /* b3Body::SetAngularVelocity(v)             body.h:672-685 */
+#define BX_EDIT_FORCE 3             /* b3Body::ApplyForce(v, p, wake)            body.h:727-744 */
+#define BX_EDIT_FORCE_TO_CENTER 4   /* b3Body::ApplyForceToCenter(v, wake)       body.h:746-762 */
+#define BX_EDIT_TORQUE 5            /* b3Body::ApplyTorque(v, wake)              body.h:764-780 */
+#define BX_EDIT_LINEAR_IMPULSE 6    /* b3Body::ApplyLinearImpulse(v, p, wake)    body.h:782-799 */
+#define BX_EDIT_ANGULAR_IMPULSE 7   /* b3Body::ApplyAngularImpulse(v, wake)      body.h:801-817 */
+#define BX_EDIT_TRANSFORM 8         /* b3Body::SetTransform({v, p})              body.cpp:194-214 */
+#define BX_EDIT_AWAKE 9             /* b3Body::SetAwake(wake)                    body.h:588-607 */
+#define BX_EDIT_SLEEPING_ALLOWED 10 /* b3Body::SetSleepingAllowed(wake)          body.h:829-840 */
+#define BX_EDIT_GRAVITY_SCALE 11    /* b3Body::SetGravityScale(v)                body.h:634-640 */
+#define BX_EDIT_LINEAR_DAMPING 12   /* b3Body::SetLinearDamping(v)               body.h:614-617 */
+#define BX_EDIT_ANGULAR_DAMPING 13  /* b3Body::SetAngularDamping(v)              body.h:624-627 */
+
+typedef struct bx_body_edit {
+    uint32_t body;                 /* creation index */
+    uint32_t op;                   /* BX_EDIT_* */
+    uint32_t wake;                 /* `wake` of the Apply* methods; the flag of SetAwake / SetSleepingAllowed */
+    uint32_t reserved;
+    float v[3];                    /* velocity, force, torque, impulse, translation, scale or damping */
+    float p[4];                    /* point of application (x y z), or rotation (x y z w) of SetTransform */
+} bx_body_edit;
+
+/* Apply `count` edits to bodies of `world`, in order, as the b3Body calls named above would between two
+   b3World::Step calls. SetTransform also moves the body's fixture proxies (b3Fixture::Synchronize with
+   the new transform at both ends) and makes the next step begin with the world's new-contact search,
+   as the reference's m_newContacts does. Synchronous. */
+int bx_batch_edit_bodies(bx_batch* batch, uint32_t world, const bx_body_edit* edits, uint32_t count);
 
 /* Per-phase GPU time per step, in milliseconds, averaged over the steps of the last bx_batch_step
    call on the batch's context (its first 1024 steps), from HIP events recorded on the context

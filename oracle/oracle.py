@@ -47,6 +47,7 @@ def load(kind):
         "create": fn("create", P, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc)),
         "destroy": fn("destroy", None, P),
         "step": fn("step", None, P, C.c_float, u, u, u),
+        "edit_bodies": fn("edit_bodies", C.c_int, P, C.POINTER(capi.bx_body_edit), u),
         "get_counts": fn("get_counts", C.c_int, P, C.POINTER(capi.bx_world_counts)),
         "get_bodies": fn("get_bodies", C.c_int, P, C.POINTER(capi.bx_body_state), u),
         "get_contacts": fn("get_contacts", C.c_int, P, C.POINTER(capi.bx_contact_state), u),
@@ -93,6 +94,11 @@ class CpuWorld:
 
     def step(self, dt=1.0 / 60.0, vit=8, pit=2, n=1):
         self.api["step"](self.h, C.c_float(dt), vit, pit, n)
+
+    def edit_bodies(self, rows):
+        """Body edits between steps: (body, op, v, p, wake) tuples, op a capi.BX_EDIT_* value."""
+        if self.api["edit_bodies"](self.h, capi.body_edits(rows), len(rows)) != 0:
+            raise RuntimeError(f"{self.kind}: body edit rejected")
 
     def counts(self):
         c = capi.bx_world_counts()

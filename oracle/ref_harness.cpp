@@ -322,6 +322,41 @@ void BXREF(step)(void* h, float dt, uint32_t vit, uint32_t pit, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) Step(w, scalar(dt), vit, pit);
 }
 
+// Body edits between steps through the reference's own b3Body methods (bx_body_edit, bxstep.h).
+int BXREF(edit_bodies)(void* h, const bx_body_edit* edits, uint32_t count) {
+    RefWorld* w = (RefWorld*)h;
+    for (uint32_t k = 0; k < count; ++k) {
+        const bx_body_edit& e = edits[k];
+        if (e.body >= w->bodies.size()) return -1;
+        b3Body* b = w->bodies[e.body];
+        const b3Vec3 v(scalar(e.v[0]), scalar(e.v[1]), scalar(e.v[2]));
+        const b3Vec3 p(scalar(e.p[0]), scalar(e.p[1]), scalar(e.p[2]));
+        switch (e.op) {
+        case BX_EDIT_LINEAR_VELOCITY: b->SetLinearVelocity(v); break;
+        case BX_EDIT_ANGULAR_VELOCITY: b->SetAngularVelocity(v); break;
+        case BX_EDIT_FORCE: b->ApplyForce(v, p, e.wake != 0); break;
+        case BX_EDIT_FORCE_TO_CENTER: b->ApplyForceToCenter(v, e.wake != 0); break;
+        case BX_EDIT_TORQUE: b->ApplyTorque(v, e.wake != 0); break;
+        case BX_EDIT_LINEAR_IMPULSE: b->ApplyLinearImpulse(v, p, e.wake != 0); break;
+        case BX_EDIT_ANGULAR_IMPULSE: b->ApplyAngularImpulse(v, e.wake != 0); break;
+        case BX_EDIT_TRANSFORM: {
+            b3Transform xf;
+            xf.translation = v;
+            xf.rotation = b3Quat(p, scalar(e.p[3]));
+            b->SetTransform(xf);
+            break;
+        }
+        case BX_EDIT_AWAKE: b->SetAwake(e.wake != 0); break;
+        case BX_EDIT_SLEEPING_ALLOWED: b->SetSleepingAllowed(e.wake != 0); break;
+        case BX_EDIT_GRAVITY_SCALE: b->SetGravityScale(v); break;
+        case BX_EDIT_LINEAR_DAMPING: b->SetLinearDamping(v); break;
+        case BX_EDIT_ANGULAR_DAMPING: b->SetAngularDamping(v); break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
 int BXREF(get_counts)(void* h, bx_world_counts* out) {
     RefWorld* w = (RefWorld*)h;
     memset(out, 0, sizeof(*out));

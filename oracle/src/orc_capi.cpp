@@ -174,6 +174,83 @@ void orc_step(void* h, float dt, uint32_t vit, uint32_t pit, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) w->Step(dt, vit, pit);
 }
 
+// Body edits between steps (bx_body_edit, bxstep.h), restating b3Body's setters and force / impulse
+// methods (body.h:588-840) and b3Body::SetTransform (body.cpp:194-214).
+int orc_edit_bodies(void* h, const bx_body_edit* edits, uint32_t count) {
+    World* w = (World*)h;
+    for (uint32_t k = 0; k < count; ++k) {
+        const bx_body_edit& e = edits[k];
+        if (e.body >= w->bodies.size()) return -1;
+        Body& b = w->bodies[e.body];
+        const Vec3 v(e.v[0], e.v[1], e.v[2]), p(e.p[0], e.p[1], e.p[2]);
+        const bool dynamic = b.type == e_dynamicBody;
+        auto wakeFirst = [&]() {  // the Apply* methods: wake first if asked, act only on an awake body
+            if (e.wake && !b.IsAwake()) b.SetAwake(true);
+            return b.IsAwake();
+        };
+        auto setVelocity = [&](Vec3& dst) {  // body.h:652-685
+            if (b.type == e_staticBody) return;
+            if (Dot(v, v) > scalar(0)) b.SetAwake(true);
+            dst = v;
+        };
+        switch (e.op) {
+        case BX_EDIT_LINEAR_VELOCITY: setVelocity(b.linearVelocity); break;
+        case BX_EDIT_ANGULAR_VELOCITY: setVelocity(b.angularVelocity); break;
+        case BX_EDIT_FORCE:
+            if (dynamic && wakeFirst()) {
+                b.force += v;
+                b.torque += Cross(p - b.sweep.worldCenter, v);
+            }
+            break;
+        case BX_EDIT_FORCE_TO_CENTER:
+            if (dynamic && wakeFirst()) b.force += v;
+            break;
+        case BX_EDIT_TORQUE:
+            if (dynamic && wakeFirst()) b.torque += v;
+            break;
+        case BX_EDIT_LINEAR_IMPULSE:
+            if (dynamic && wakeFirst()) {
+                b.linearVelocity += b.invMass * v;
+                b.angularVelocity += b.worldInvI * Cross(p - b.sweep.worldCenter, v);
+            }
+            break;
+        case BX_EDIT_ANGULAR_IMPULSE:
+            if (dynamic && wakeFirst()) b.angularVelocity += b.worldInvI * v;
+            break;
+        case BX_EDIT_TRANSFORM: {
+            Transform xf;
+            xf.translation = v;
+            xf.rotation = Quat(p, e.p[3]);
+            b.xf = xf;
+            b.sweep.worldCenter = Mul(xf, b.sweep.localCenter);
+            b.sweep.orientation = xf.rotation;
+            b.sweep.worldCenter0 = b.sweep.worldCenter;
+            b.sweep.orientation0 = b.sweep.orientation;
+            b.worldInvI = RotateToFrame(b.invI, xf.rotation);
+            for (u32 f : b.fixtures) w->SynchronizeFixture(f, xf, xf);
+            w->newContacts = true;
+            break;
+        }
+        case BX_EDIT_AWAKE: b.SetAwake(e.wake != 0); break;
+        case BX_EDIT_SLEEPING_ALLOWED:
+            if (e.wake) {
+                b.flags |= e_autoSleepFlag;
+            } else {
+                b.flags &= ~u32(e_autoSleepFlag);
+                b.SetAwake(true);
+            }
+            break;
+        case BX_EDIT_GRAVITY_SCALE:
+            if (b.type != e_staticBody) b.gravityScale = v;
+            break;
+        case BX_EDIT_LINEAR_DAMPING: b.linearDamping = v; break;
+        case BX_EDIT_ANGULAR_DAMPING: b.angularDamping = v; break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
 int orc_get_counts(void* h, bx_world_counts* out) {
     World* w = (World*)h;
     std::memset(out, 0, sizeof(*out));
