@@ -34,6 +34,15 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+# Steps a batch spends timing the velocity-sweep layouts when solver_mode is 0 (three steps first, then
+# each of three layouts twice): the warm-up covers them, so no tuning step lands in the timed region.
+TUNE_STEPS = 9
+# Compulsory bytes of one step (the unit of work is one body-step), from the counts of the step: per
+# awake non-static body 432 B (state read 256 B, written 176 B), per proxy 52 B (fat AABB read and
+# written, moved flag), per broadphase pair 16 B (sorted pair written and read), per convex contact
+# 580 B (manifold read and written, feature cache, header), per mesh contact 1,668 B (three manifolds),
+# per revolute joint 164 B.
+STEP_BYTES = {"awake_body": 432, "proxy": 52, "pair": 16, "contact": 580, "mesh_contact": 1668, "joint": 164}
 # Algorithmic (compulsory) bytes of one launch of each heavy kernel, per unit of work, from the
 # data it must read and write once (bounce_amd/csrc/bx_types.h layouts):
 #  * k_solve_velocity: per island contact the velocity constraint (400 B) read and its impulses
@@ -67,6 +76,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=60)
     p.add_argument("--latency-steps", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--parity-worlds", type=int, default=8,
+                   help="worlds of the benchmarked batch compared with the oracle after timing (0: none)")
     p.add_argument("--serve-worlds", type=int, default=512)
     p.add_argument("--serve-rate", type=float, default=1000.0)
     p.add_argument("--serve-seconds", type=float, default=0.0)
@@ -76,19 +87,63 @@ def parse():
 
 
 def cpu_baseline(args, scene_factory):
-    """Reference Step (oracle/_ref build) on host cores, one world per thread, bounded sample."""
+    """Reference Step (oracle/_ref build) on the host cores this process may run on, one world per
+    thread, bounded sample."""
     try:
         from oracle import oracle as O
     except Exception as e:  # pragma: no cover
         return {"value": None, "error": str(e)}
     kind = "reference" if O.available("ref32") else "port"
     lib = "ref32" if kind == "reference" else "oracle"
-    threads = min(os.cpu_count() or 1, 16)
+    threads = len(os.sched_getaffinity(0))
     sample = scene_factory(args.cpu_worlds)
     secs = O.bench(lib, sample, warmup=0, steps=args.cpu_steps, threads=threads)
     value = sample.body_count * args.cpu_steps / secs if secs > 0 else None
     return {"value": value, "unit": "body-steps/s", "cores": threads, "kind": kind,
-            "sample": f"{args.cpu_worlds} worlds x {args.cpu_steps} steps from step 0, one world per thread"}
+            "sample": f"{args.cpu_worlds} worlds x {args.cpu_steps} steps from step 0, one world per thread on "
+                      f"all {threads} cores this process may run on"}
+
+
+def step_bytes(t):
+    """Compulsory bytes of one step from the batch totals (bx_batch_totals)."""
+    convex = t.contacts - t.mesh_world_contacts
+    return (STEP_BYTES["awake_body"] * t.awake_bodies + STEP_BYTES["proxy"] * t.fixtures + STEP_BYTES["pair"] * t.pairs
+            + STEP_BYTES["contact"] * convex + STEP_BYTES["mesh_contact"] * t.mesh_world_contacts
+            + STEP_BYTES["joint"] * t.joints)
+
+
+class _DeviceWorld:
+    def __init__(self, batch, w):
+        self.b, self.w = batch, w
+
+    def bodies(self):
+        return self.b.bodies(self.w)
+
+    def contacts(self):
+        return self.b.contacts(self.w)
+
+    def pairs(self):
+        return self.b.pairs(self.w)
+
+    def island_contacts(self):
+        return self.b.island_contacts(self.w)
+
+
+def parity(batch, scene, worlds, steps):
+    """The sampled worlds of the benchmarked batch against the oracle stepped as often from the same
+    descriptors: every world must be bit-identical (bodies, contacts, pairs, island order)."""
+    from oracle.oracle import CpuWorld
+    from tests.parity import compare_worlds
+    diffs = {}
+    for w in worlds:
+        ref = CpuWorld("oracle", scene, w)
+        ref.step(n=steps)
+        d = compare_worlds(_DeviceWorld(batch, w), ref, mask_cache=True)
+        if d:
+            diffs[w] = d[:2]
+        ref.close()
+    return {"worlds_compared": len(worlds), "steps": steps, "bit_exact": not diffs,
+            "differing_worlds": {str(k): v for k, v in diffs.items()}}
 
 
 def serve(args):
@@ -137,10 +192,13 @@ def main():
     total_bodies = scene.body_count
     ctx = device.Context(0)
     batch = device.Batch(ctx, scene, solver_mode=args.solver_mode)
-    # warmup
-    for _ in range(args.warmup):
+    # warm-up: at least the layout-tuning steps, so that the timed region runs the chosen layout only
+    warmup = max(args.warmup, TUNE_STEPS) if args.solver_mode == 0 else args.warmup
+    for _ in range(warmup):
         batch.step()
     batch.sync()
+    lanes, tuned_ms = batch.solver_layout()
+    tuned_before_timing = args.solver_mode != 0 or all(t > 0 for t in tuned_ms)
     # timed region: K steps, synchronised on both sides
     t0 = time.perf_counter()
     batch.step(n=args.steps)
@@ -153,23 +211,26 @@ def main():
     kernel_ms = batch.kernel_timings()
     phase_ms = batch.timings()
     dominant = max(KERNEL_BYTES, key=lambda k: kernel_ms.get(k, 0.0))
-    # units of work of one launch at the end of the timed region (a sample of worlds, times the
-    # world count)
-    sample_w = list(range(0, args.worlds, max(1, args.worlds // 16)))
-    counts = [batch.counts(w) for w in sample_w]
-    island_bodies = np.mean([c.awake_count for c in counts]) * args.worlds
-    island_contacts = np.mean([len(batch.island_contacts(w)) for w in sample_w]) * args.worlds
-    contacts = np.mean([c.contact_count for c in counts]) * args.worlds
-    status = int(np.bitwise_or.reduce([c.status for c in counts]))
+    # counts of the last timed step over every world (device counters): the compulsory bytes of a step
+    tot = batch.totals()
+    counts = {"awake_bodies": tot.awake_bodies, "proxies": tot.fixtures, "pairs": tot.pairs,
+              "contacts": tot.contacts, "mesh_world_contacts": tot.mesh_world_contacts, "joints": tot.joints,
+              "islands": tot.islands, "island_contacts": tot.island_contacts, "bodies": tot.bodies}
+    bytes_step = step_bytes(tot)
+    status = int(tot.status_or)
     device_bytes = batch.device_bytes()
-    lanes, tuned_ms = batch.solver_layout()
-    dom_bytes = KERNEL_BYTES[dominant](island_contacts, island_bodies, contacts)
-    achieved = dom_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
+    model_bytes = KERNEL_BYTES[dominant](tot.island_contacts, tot.awake_bodies, tot.contacts)
+    achieved = bytes_step / (kernel_ms[dominant] * 1e-3) / 1e9
     # step latency: single steps, each bracketed by events on the stream
     lat = []
     for _ in range(args.latency_steps):
         batch.step()
         lat.append(batch.timings()[4])
+    batch.sync()
+    par = None
+    if args.parity_worlds > 0:
+        sample = sorted(set(np.linspace(0, args.worlds - 1, args.parity_worlds).astype(int).tolist()))
+        par = parity(batch, scene, sample, warmup + args.steps + args.latency_steps)
     # HBM bytes per launch of the dominant kernel, from the committed FETCH_SIZE / WRITE_SIZE profile
     # of this workload (bounce_amd/profile_summary.py), when one exists for this configuration
     traffic = None
@@ -185,7 +246,7 @@ def main():
     graph = None
     if args.graph:
         gb = device.Batch(ctx, scene, solver_mode=args.solver_mode)
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             gb.step_graph()
         gb.sync()
         g0 = time.perf_counter()
@@ -193,7 +254,7 @@ def main():
         gb.sync()
         g1 = time.perf_counter()
         graph = {"value": total_bodies * args.steps / (g1 - g0), "unit": "body-steps/s",
-                 "ms_per_step": (g1 - g0) / args.steps * 1e3, "steps": args.steps, "warmup": args.warmup}
+                 "ms_per_step": (g1 - g0) / args.steps * 1e3, "steps": args.steps, "warmup": warmup}
         gb.close()
     # plain per-world path: the same kernels, one world per launch sequence
     plain = None
@@ -223,7 +284,7 @@ def main():
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -243,6 +304,7 @@ def main():
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
         "solver_layout": {"islands_per_wavefront": lanes, "layouts": [16, 32, 64],
                           "tuned_k_solve_velocity_ms": [round(t, 3) for t in tuned_ms],
+                          "tuned_before_timing": tuned_before_timing,
                           "note": "solver_mode 0: each layout timed twice on warm-up steps 4-9 (order "
                                   "16 32 64 64 32 16), the one with the smallest sample kept (16 unless "
                                   "another is faster by more than 5%)"},
@@ -256,11 +318,23 @@ def main():
         "kernel_ms": kernel_ms,
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": dom_bytes, "kernel_ms": kernel_ms[dominant]},
+                     "algorithmic_bytes": bytes_step, "kernel_ms": kernel_ms[dominant], "counts": counts,
+                     "bytes_per_body_step": bytes_step / total_bodies,
+                     "note": "algorithmic bytes = compulsory bytes of one whole step from the device counts "
+                             "(432 per awake body, 52 per proxy, 16 per pair, 580 per convex contact, 1668 per "
+                             "mesh contact, 164 per joint), over the dominant kernel's mean time",
+                     "kernel_model": {"bytes": model_bytes, "achieved": model_bytes / (kernel_ms[dominant] * 1e-3) / 1e9,
+                                      "note": "the kernel's own compulsory bytes: 428 per island contact, 48 per "
+                                              "island body"},
+                     "whole_step": {"achieved": bytes_step / (wall / args.steps) / 1e9,
+                                    "frac": bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
         "serving": serving,
         "device_bytes": device_bytes,
         "status": status,
+        "status_worlds": int(tot.status_worlds),
+        "valid": status == 0 and (par is None or par["bit_exact"]) and tuned_before_timing,
+        "parity": par,
     }
     print(json.dumps(out))
 

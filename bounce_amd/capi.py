@@ -85,6 +85,13 @@ class bx_body_state(C.Structure):
                 ("sleep_time", f32), ("flags", u32), ("island_index", u32)]
 
 
+class bx_batch_totals(C.Structure):
+    _fields_ = [("bodies", C.c_uint64), ("awake_bodies", C.c_uint64), ("fixtures", C.c_uint64), ("joints", C.c_uint64),
+                ("contacts", C.c_uint64), ("mesh_world_contacts", C.c_uint64), ("pairs", C.c_uint64),
+                ("islands", C.c_uint64), ("island_contacts", C.c_uint64), ("worlds", u32), ("status_worlds", u32),
+                ("status_or", u32), ("reserved", u32)]
+
+
 class bx_body_edit(C.Structure):
     _fields_ = [("body", u32), ("op", u32), ("wake", u32), ("reserved", u32), ("v", _arr(f32, 3)), ("p", _arr(f32, 4))]
 

@@ -1739,6 +1739,45 @@ int bx_batch_get_counts(bx_batch* b, uint32_t world, bx_world_counts* out) {
     return BX_OK;
 }
 
+int bx_batch_get_totals(bx_batch* b, bx_batch_totals* out) {
+    if (!b || !out) return Fail(BX_E_INVALID, "null argument");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    std::memset(out, 0, sizeof(*out));
+    std::vector<WorldState> ws;
+    int rc = Download(b, ws, b->view.wstate, b->worldCount);
+    if (rc) return rc;
+    // the body ranges of the resident worlds, read in one copy
+    u64 lo = ~u64(0), hi = 0;
+    for (u32 w = 0; w < b->worldCount; ++w) {
+        if (!b->resident[w]) continue;
+        lo = std::min<u64>(lo, b->worlds[w].bodyBase);
+        hi = std::max<u64>(hi, u64(b->worlds[w].bodyBase) + b->worlds[w].bodyCount);
+    }
+    std::vector<Body> bodies;
+    if (hi > lo && (rc = Download(b, bodies, b->view.bodies + lo, size_t(hi - lo)))) return rc;
+    for (u32 w = 0; w < b->worldCount; ++w) {
+        if (!b->resident[w]) continue;
+        const WorldInfo& wi = b->worlds[w];
+        const WorldState& s = ws[w];
+        ++out->worlds;
+        out->bodies += wi.bodyCount;
+        out->fixtures += wi.fixtureCount;
+        out->joints += wi.jointCount;
+        out->contacts += s.contactCount;
+        if (wi.meshTriCap) out->mesh_world_contacts += s.contactCount;
+        out->pairs += s.pairCount;
+        out->islands += s.islandCount;
+        out->island_contacts += std::min(s.islandContactCount, wi.conCap);
+        out->status_or |= s.status;
+        out->status_worlds += s.status ? 1u : 0u;
+        for (u32 i = 0; i < wi.bodyCount; ++i) {
+            const Body& x = bodies[wi.bodyBase - lo + i];
+            out->awake_bodies += (x.type != kStatic && (x.flags & kAwake)) ? 1u : 0u;
+        }
+    }
+    return BX_OK;
+}
+
 int bx_batch_get_bodies(bx_batch* b, uint32_t world, bx_body_state* out, uint32_t capacity) {
     if (!b || !IsResident(b, world) || !out) return Fail(BX_E_INVALID, "bad argument or world not resident");
     const WorldInfo& wi = b->worlds[world];
@@ -2066,7 +2105,7 @@ uint32_t bx_struct_size(const char* name) {
 #define S(T) if (n == #T) return uint32_t(sizeof(T));
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
-    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit)
+    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals)
 #undef S
     return 0;
 }

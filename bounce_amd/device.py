@@ -42,6 +42,7 @@ def load(path=capi.LIB_PATH):
         "bx_batch_step_subset": (C.c_int, [P, C.POINTER(capi.bx_step_cfg), C.POINTER(u), C.POINTER(u), u]),
         "bx_batch_sync": (C.c_int, [P]),
         "bx_batch_get_counts": (C.c_int, [P, u, C.POINTER(capi.bx_world_counts)]),
+        "bx_batch_get_totals": (C.c_int, [P, C.POINTER(capi.bx_batch_totals)]),
         "bx_batch_get_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
         "bx_batch_get_contacts": (C.c_int, [P, u, C.POINTER(capi.bx_contact_state), u]),
         "bx_batch_get_pairs": (C.c_int, [P, u, C.POINTER(capi.bx_pair), u]),
@@ -185,6 +186,12 @@ class Batch:
         c = capi.bx_world_counts()
         _check(_lib.bx_batch_get_counts(self.h, world, C.byref(c)))
         return c
+
+    def totals(self):
+        """bx_batch_totals over every resident world (one read of world states and bodies)."""
+        t = capi.bx_batch_totals()
+        _check(_lib.bx_batch_get_totals(self.h, C.byref(t)))
+        return t
 
     def _body_count(self, world):
         return self.scene.worlds[world].body_count

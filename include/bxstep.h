@@ -282,6 +282,24 @@ typedef struct bx_pool_stats {
     uint32_t reserved[5];
 } bx_pool_stats;
 
+/* Totals over every resident world of a batch or pool after the last step: the counts the step's
+   compulsory-traffic model is written in, and every world's status. */
+typedef struct bx_batch_totals {
+    uint64_t bodies;                   /* all bodies */
+    uint64_t awake_bodies;             /* awake non-static bodies */
+    uint64_t fixtures;                 /* fixtures = broadphase proxies */
+    uint64_t joints;
+    uint64_t contacts;                 /* live contacts (b3World::GetContactCount summed) */
+    uint64_t mesh_world_contacts;      /* of those, contacts of worlds that hold a mesh fixture */
+    uint64_t pairs;                    /* unique pairs of the last pair search */
+    uint64_t islands;                  /* islands solved in the last step */
+    uint64_t island_contacts;          /* contact constraints solved in the last step */
+    uint32_t worlds;                   /* resident worlds */
+    uint32_t status_worlds;            /* worlds whose status is nonzero */
+    uint32_t status_or;                /* OR of every world's status */
+    uint32_t reserved;
+} bx_batch_totals;
+
 /* Library version and build information ("bxstep gfx950 ..."). */
 const char* bx_version(void);
 
@@ -362,6 +380,8 @@ int bx_batch_sync(bx_batch* batch);
 
 /* Read back one world. Arrays must hold the counts given by bx_batch_get_counts. */
 int bx_batch_get_counts(bx_batch* batch, uint32_t world, bx_world_counts* out);
+/* bx_batch_totals of the whole batch or pool, from one read of the world states and bodies. */
+int bx_batch_get_totals(bx_batch* batch, bx_batch_totals* out);
 int bx_batch_get_bodies(bx_batch* batch, uint32_t world, bx_body_state* out, uint32_t capacity);
 int bx_batch_get_contacts(bx_batch* batch, uint32_t world, bx_contact_state* out, uint32_t capacity);
 int bx_batch_get_pairs(bx_batch* batch, uint32_t world, bx_pair* out, uint32_t capacity);
