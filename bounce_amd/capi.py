@@ -18,6 +18,8 @@ BX_BODY_ISLAND = 0x2
 BX_BODY_AUTO_SLEEP = 0x4
 BX_BODY_FIXED_ROT_X, BX_BODY_FIXED_ROT_Y, BX_BODY_FIXED_ROT_Z = 0x8, 0x10, 0x20
 BX_WORLD_SLEEPING = 0x1
+BX_WORLD_EVENTS = 0x8
+BX_EVENT_BEGIN_CONTACT, BX_EVENT_END_CONTACT, BX_EVENT_PRE_SOLVE, BX_EVENT_POST_SOLVE = 1, 2, 3, 4
 (BX_EDIT_LINEAR_VELOCITY, BX_EDIT_ANGULAR_VELOCITY, BX_EDIT_FORCE, BX_EDIT_FORCE_TO_CENTER, BX_EDIT_TORQUE,
  BX_EDIT_LINEAR_IMPULSE, BX_EDIT_ANGULAR_IMPULSE, BX_EDIT_TRANSFORM, BX_EDIT_AWAKE, BX_EDIT_SLEEPING_ALLOWED,
  BX_EDIT_GRAVITY_SCALE, BX_EDIT_LINEAR_DAMPING, BX_EDIT_ANGULAR_DAMPING) = range(1, 14)
@@ -90,6 +92,10 @@ class bx_batch_totals(C.Structure):
                 ("contacts", C.c_uint64), ("mesh_world_contacts", C.c_uint64), ("pairs", C.c_uint64),
                 ("islands", C.c_uint64), ("island_contacts", C.c_uint64), ("worlds", u32), ("status_worlds", u32),
                 ("status_or", u32), ("reserved", u32)]
+
+
+class bx_contact_event(C.Structure):
+    _fields_ = [("type", u32), ("fixture_a", u32), ("fixture_b", u32), ("contact", u32)]
 
 
 class bx_body_edit(C.Structure):

@@ -28,7 +28,6 @@ void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCo
 
 using namespace bx;
 typedef uint8_t u8;
-typedef uint64_t u64;
 
 static thread_local std::string g_error;
 static thread_local const char* g_errorStatic = nullptr;  // set instead of g_error where nothing may allocate
@@ -163,7 +162,7 @@ static const ArraySpec kArrays[] = {
     BX_ARRAY(joints, kSpaceJoint, 1), BX_ARRAY(islandJoints, kSpaceJoint, 1), BX_ARRAY(jointCons, kSpaceJoint, 1),
     BX_ARRAY(jointEdges, kSpaceJoint, 2),
     BX_ARRAY(contacts, kSpaceContact, 1), BX_ARRAY(contactOrder, kSpaceContact, 1),
-    BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1),
+    BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1), BX_ARRAY(events, kSpaceContact, 1),
     BX_ARRAY(edgeSlots, kSpaceContact, 2),
     BX_ARRAY(islandContacts, kSpaceCon, 1), BX_ARRAY(velCons, kSpaceCon, 1), BX_ARRAY(posCons, kSpaceCon, 1),
     BX_ARRAY(meshContacts, kSpaceMesh, 1), BX_ARRAY(meshTriLists, kSpaceMeshTri, 1),
@@ -1578,6 +1577,7 @@ int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* 
     HIP_CHECK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     WorldInfo wi = h.info;
+    wi.filter = nullptr;  // contact filter tables belong to the batch that set them (pools hold none)
     u32 units[kSpaces], base[kSpaces], s;
     WorldUnits(wi, units);
     int rc = Reserve(b, units, base, &s);
@@ -1775,6 +1775,71 @@ int bx_batch_get_totals(bx_batch* b, bx_batch_totals* out) {
             out->awake_bodies += (x.type != kStatic && (x.flags & kAwake)) ? 1u : 0u;
         }
     }
+    return BX_OK;
+}
+
+int bx_batch_set_contact_filter(bx_batch* b, uint32_t world, const uint8_t* collide, const uint8_t* respond) {
+    if (!b || !IsResident(b, world)) return Fail(BX_E_INVALID, "bad batch or world not resident");
+    if (b->pool) return Fail(BX_E_UNSUPPORTED, "contact filter tables are set on batches, not pools");
+    if ((collide == nullptr) != (respond == nullptr)) return Fail(BX_E_INVALID, "set both filter tables or neither");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    WorldInfo& wi = b->worlds[world];
+    if (!collide) {
+        wi.filter = nullptr;
+    } else {
+        if (wi.fixtureCount > 16384) return Fail(BX_E_CAPACITY, "contact filter tables hold at most 16,384 fixtures");
+        const u32 nf = wi.fixtureCount, words = FilterWords(nf);
+        std::vector<u32> bits(size_t(2) * std::max(1u, words), 0u);
+        for (u64 k = 0; k < u64(nf) * nf; ++k) {
+            if (collide[k >> 3] & (1u << (k & 7))) bits[k >> 5] |= 1u << (k & 31);
+            if (respond[k >> 3] & (1u << (k & 7))) bits[words + (k >> 5)] |= 1u << (k & 31);
+        }
+        u32* dev = nullptr;
+        int rc = DevAlloc(b, &dev, bits.size(), bits.data());
+        if (rc) return rc;
+        wi.filter = dev;
+    }
+    HIP_CHECK(hipMemcpy((void*)(b->view.worlds + world), &wi, sizeof(wi), hipMemcpyHostToDevice));
+    return BX_OK;
+}
+
+int bx_batch_get_events(bx_batch* b, uint32_t world, bx_contact_event* out, uint32_t capacity, uint32_t* count) {
+    WorldState ws;
+    int rc = GetState(b, world, ws);
+    if (rc) return rc;
+    const WorldInfo& wi = b->worlds[world];
+    if ((wi.flags & kWorldEvents) == 0) return Fail(BX_E_INVALID, "the world does not record contact events");
+    std::vector<u32> order;
+    std::vector<uint4> visits, entries;
+    if ((rc = Download(b, order, b->view.contactOrder + wi.contactBase, ws.contactCount))) return rc;
+    if ((rc = Download(b, visits, b->view.events + wi.contactBase, std::min(ws.eventCount, wi.contactCap)))) return rc;
+    if ((rc = Download(b, entries, b->view.islandContacts + wi.conBase, std::min(ws.islandContactCount, wi.conCap))))
+        return rc;
+    std::vector<u32> listIndex(wi.contactCap, BX_NULL_INDEX);  // slot -> position in the contact list
+    for (u32 k = 0; k < ws.contactCount; ++k) listIndex[order[k]] = ws.contactCount - 1 - k;
+    u32 total = 0;
+    auto emit = [&](u32 type, u32 fa, u32 fb, u32 contact) {
+        if (total < capacity) out[total] = bx_contact_event{type, fa, fb, contact};
+        ++total;
+    };
+    // b3ContactManager::UpdateContacts visits the world list, newest contact first
+    for (u32 k = u32(visits.size()); k-- > 0;) {
+        const uint4 v = visits[k];
+        const u32 contact = (v.w & kEvDestroyed) ? BX_NULL_INDEX : listIndex[v.x];
+        if (v.w & kEvBegin) emit(BX_EVENT_BEGIN_CONTACT, v.y, v.z, contact);
+        if (v.w & kEvEnd) emit(BX_EVENT_END_CONTACT, v.y, v.z, contact);
+        if (v.w & kEvPreSolve) emit(BX_EVENT_PRE_SOLVE, v.y, v.z, contact);
+    }
+    if (ws.islandContactCount) {  // b3Island::Report: every solved contact, island by island
+        std::vector<Contact> contacts;
+        if ((rc = Download(b, contacts, b->view.contacts + wi.contactBase, wi.contactCap))) return rc;
+        for (const uint4& e : entries) {
+            if (e.x >> 30) continue;  // one PostSolve per contact (a mesh contact has one entry per manifold)
+            const Contact& c = contacts[e.x & 0x3FFFFFFFu];
+            emit(BX_EVENT_POST_SOLVE, c.fixtureA, c.fixtureB, listIndex[e.x & 0x3FFFFFFFu]);
+        }
+    }
+    if (count) *count = total;
     return BX_OK;
 }
 
@@ -2105,7 +2170,7 @@ uint32_t bx_struct_size(const char* name) {
 #define S(T) if (n == #T) return uint32_t(sizeof(T));
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
-    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals)
+    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event)
 #undef S
     return 0;
 }

@@ -17,6 +17,7 @@ namespace bx {
 
 typedef float scalar;
 typedef uint32_t u32;
+typedef uint64_t u64;
 
 // include/bounce/common/common.h:73-123, same literals evaluated in binary32.
 constexpr scalar kEpsilon = 1.19209290e-07f;             // FLT_EPSILON

@@ -247,7 +247,25 @@ struct WorldInfo {
     u32 meshTriBase;                   // mesh contact triangle lists (contactCap x meshTriCap)
     u32 meshTriCap;                    // triangles a mesh contact can hold (0: no mesh fixture)
     u32 meshScratchBase;               // clustering stripes (kMeshLanes x MeshLaneStride(meshTriCap))
+    u32 pad0;
+    // contact filter tables (b3ContactFilter) or null: bit i * fixtureCount + j of the first table is
+    // ShouldCollide(fixture i, fixture j), of the second (FilterWords(fixtureCount) words later)
+    // ShouldRespond(i, j)
+    const u32* filter;
 };
+
+BX_HD u32 FilterWords(u32 nf) { return u32((u64(nf) * nf + 31) / 32); }
+BX_HD bool FilterBit(const u32* table, u32 nf, u32 i, u32 j) {
+    const u32 bit = i * nf + j;
+    return ((table[bit >> 5] >> (bit & 31u)) & 1u) != 0;
+}
+
+// World flag: record contact events (bx_batch_get_events).
+constexpr u32 kWorldEvents = 0x8;
+// Contact event bits of one visited contact in b3ContactManager::UpdateContacts: destroyed (b3Contact
+// removed; EndContact when it was touching), BeginContact, EndContact, PreSolve (contact.cpp:268-284,
+// contact_manager.cpp:230-237).
+constexpr u32 kEvDestroyed = 1, kEvBegin = 2, kEvEnd = 4, kEvPreSolve = 8;
 
 // Per-world mutable counters.
 struct WorldState {
@@ -263,6 +281,8 @@ struct WorldState {
     scalar inv_dt0;
     u32 touchingCount;
     u32 anyAsleep;
+    u32 eventCount;       // contacts visited by the last contact update (entries of BatchView::events)
+    u32 pad;
 };
 
 // Words of island-builder scratch a world needs: per body flags, fixture range, joint range,
@@ -349,6 +369,7 @@ struct BatchView {
     u32* contactOrder;        // per world: live slots, creation order
     u32* freeSlots;           // per world: free slot stack
     u32* contactAlive;        // per slot: scratch flag for the update pass
+    uint4* events;            // per world: the last contact update's visits {slot, fixtureA, fixtureB, kEv* bits}
     // adjacency (rebuilt each step)
     u32* edgeOffset;          // per fixture + 1 per world (fixtureCount + 1 entries from WorldInfo::edgeBase)
     u32* edgeSlots;           // per world: 2 * contactCap

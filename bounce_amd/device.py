@@ -43,6 +43,8 @@ def load(path=capi.LIB_PATH):
         "bx_batch_sync": (C.c_int, [P]),
         "bx_batch_get_counts": (C.c_int, [P, u, C.POINTER(capi.bx_world_counts)]),
         "bx_batch_get_totals": (C.c_int, [P, C.POINTER(capi.bx_batch_totals)]),
+        "bx_batch_get_events": (C.c_int, [P, u, C.POINTER(capi.bx_contact_event), u, C.POINTER(u)]),
+        "bx_batch_set_contact_filter": (C.c_int, [P, u, C.c_void_p, C.c_void_p]),
         "bx_batch_get_bodies": (C.c_int, [P, u, C.POINTER(capi.bx_body_state), u]),
         "bx_batch_get_contacts": (C.c_int, [P, u, C.POINTER(capi.bx_contact_state), u]),
         "bx_batch_get_pairs": (C.c_int, [P, u, C.POINTER(capi.bx_pair), u]),
@@ -186,6 +188,20 @@ class Batch:
         c = capi.bx_world_counts()
         _check(_lib.bx_batch_get_counts(self.h, world, C.byref(c)))
         return c
+
+    def events(self, world=0):
+        """Listener events of the world's last step (BX_WORLD_EVENTS): (type, fixture_a, fixture_b, contact)."""
+        cnt = C.c_uint32(0)
+        _check(_lib.bx_batch_get_events(self.h, world, None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=np.dtype(capi.bx_contact_event))
+        _check(_lib.bx_batch_get_events(self.h, world, out.ctypes.data_as(C.POINTER(capi.bx_contact_event)),
+                                        cnt.value, C.byref(cnt)))
+        return out
+
+    def set_contact_filter(self, world, collide, respond):
+        """Filter tables (uint8 arrays of n*n bits, bx_batch_set_contact_filter) or None for no filter."""
+        _check(_lib.bx_batch_set_contact_filter(self.h, world, None if collide is None else collide.ctypes.data,
+                                                None if respond is None else respond.ctypes.data))
 
     def totals(self):
         """bx_batch_totals over every resident world (one read of world states and bodies)."""

@@ -68,6 +68,7 @@ extern "C" {
 #define BX_WORLD_SLEEPING 0x0001u      /* b3World::SetSleeping(true); default off (world.cpp:46) */
 #define BX_WORLD_NO_WARM_START 0x0002u /* b3World::SetWarmStart(false); default on */
 #define BX_WORLD_NO_CLEAR_FORCES 0x0004u
+#define BX_WORLD_EVENTS 0x0008u        /* record the contact listener's events of every step (bx_batch_get_events) */
 
 /* b3JointType (include/bounce/dynamics/joints/joint.h) */
 #define BX_JOINT_REVOLUTE 4u
@@ -380,6 +381,37 @@ int bx_batch_sync(bx_batch* batch);
 
 /* Read back one world. Arrays must hold the counts given by bx_batch_get_counts. */
 int bx_batch_get_counts(bx_batch* batch, uint32_t world, bx_world_counts* out);
+/* Contact listener events of the last step (b3ContactListener, world_callbacks.h), in the order the
+   reference calls them: BeginContact / EndContact / PreSolve as b3ContactManager::UpdateContacts visits
+   the contacts (contact_manager.cpp:165-237, contact.cpp:268-284; EndContact also for a touching contact
+   it destroys), then PostSolve for every solved contact, island by island (island.cpp:309-320). The
+   replay runs after the step, so a listener reading a contact in BeginContact or PreSolve sees its
+   state after the step, and changes it makes there do not reach the solver. */
+#define BX_EVENT_BEGIN_CONTACT 1
+#define BX_EVENT_END_CONTACT 2
+#define BX_EVENT_PRE_SOLVE 3
+#define BX_EVENT_POST_SOLVE 4
+typedef struct bx_contact_event {
+    uint32_t type;                 /* BX_EVENT_* */
+    uint32_t fixture_a;            /* creation indices of b3Contact::GetFixtureA() / GetFixtureB() */
+    uint32_t fixture_b;
+    uint32_t contact;              /* position in bx_batch_get_contacts' list after the step, BX_NULL_INDEX
+                                      for a contact the step destroyed */
+} bx_contact_event;
+/* Events of the last step of a world created with BX_WORLD_EVENTS; `count` receives the total. */
+int bx_batch_get_events(bx_batch* batch, uint32_t world, bx_contact_event* out, uint32_t capacity,
+                        uint32_t* count);
+
+/* A contact filter (b3ContactFilter, world_callbacks.h; b3World::SetContactFilter, world.h:77-81) as two
+   tables over the world's fixtures in creation order: bit i * n + j (n = fixture count, bit k at
+   byte k / 8, bit k % 8) of `collide` is ShouldCollide(fixture i, fixture j), of `respond`
+   ShouldRespond(fixture i, fixture j). The device consults them where the reference calls the filter:
+   when a pair becomes a contact (contact_manager.cpp:86-92), when a contact is updated
+   (contact_manager.cpp:189-199) and when the island search meets a touching contact (world.cpp:351-357),
+   so a filter whose answers depend only on the two fixtures is reproduced exactly. Null tables remove
+   the filter. Batches only; at most 16,384 fixtures. */
+int bx_batch_set_contact_filter(bx_batch* batch, uint32_t world, const uint8_t* collide, const uint8_t* respond);
+
 /* bx_batch_totals of the whole batch or pool, from one read of the world states and bodies. */
 int bx_batch_get_totals(bx_batch* batch, bx_batch_totals* out);
 int bx_batch_get_bodies(bx_batch* batch, uint32_t world, bx_body_state* out, uint32_t capacity);

@@ -48,6 +48,8 @@ def load(kind):
         "destroy": fn("destroy", None, P),
         "step": fn("step", None, P, C.c_float, u, u, u),
         "edit_bodies": fn("edit_bodies", C.c_int, P, C.POINTER(capi.bx_body_edit), u),
+        "set_contact_filter": fn("set_contact_filter", C.c_int, P, C.c_void_p, C.c_void_p),
+        "get_events": fn("get_events", C.c_int, P, C.POINTER(capi.bx_contact_event), u, C.POINTER(u)),
         "get_counts": fn("get_counts", C.c_int, P, C.POINTER(capi.bx_world_counts)),
         "get_bodies": fn("get_bodies", C.c_int, P, C.POINTER(capi.bx_body_state), u),
         "get_contacts": fn("get_contacts", C.c_int, P, C.POINTER(capi.bx_contact_state), u),
@@ -99,6 +101,20 @@ class CpuWorld:
         """Body edits between steps: (body, op, v, p, wake) tuples, op a capi.BX_EDIT_* value."""
         if self.api["edit_bodies"](self.h, capi.body_edits(rows), len(rows)) != 0:
             raise RuntimeError(f"{self.kind}: body edit rejected")
+
+    def set_contact_filter(self, collide, respond):
+        """Filter tables as bx_batch_set_contact_filter takes them (uint8 arrays of n*n bits), or None."""
+        self._filter = (collide, respond)
+        self.api["set_contact_filter"](self.h, None if collide is None else collide.ctypes.data,
+                                       None if respond is None else respond.ctypes.data)
+
+    def events(self):
+        """The listener events of the last step: structured array (type, fixture_a, fixture_b, contact)."""
+        cnt = C.c_uint32(0)
+        self.api["get_events"](self.h, None, 0, C.byref(cnt))
+        out = np.zeros(cnt.value, dtype=np.dtype(capi.bx_contact_event))
+        self.api["get_events"](self.h, out.ctypes.data_as(C.POINTER(capi.bx_contact_event)), cnt.value, C.byref(cnt))
+        return out
 
     def counts(self):
         c = capi.bx_world_counts()

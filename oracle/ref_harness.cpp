@@ -61,12 +61,35 @@ struct RefMesh {
     b3Mesh mesh;
 };
 
+// Records the island order (PostSolve) and every listener call of a step. EndContact is called from
+// b3ContactManager::Destroy while the contact is still flagged touching, and from b3Contact::Update after
+// the flag was cleared, which tells the two apart.
 struct IslandRecorder : public b3ContactListener {
+    struct Event { uint32_t type; const b3Fixture* a; const b3Fixture* b; b3Contact* contact; };
     std::vector<b3Contact*> order;
-    void BeginContact(b3Contact*) override {}
-    void EndContact(b3Contact*) override {}
-    void PreSolve(b3Contact*) override {}
-    void PostSolve(b3Contact* c) override { order.push_back(c); }
+    std::vector<Event> events;
+    void BeginContact(b3Contact* c) override { events.push_back(Event{1u, c->GetFixtureA(), c->GetFixtureB(), c}); }
+    void EndContact(b3Contact* c) override {
+        events.push_back(Event{2u, c->GetFixtureA(), c->GetFixtureB(), c->IsTouching() ? nullptr : c});
+    }
+    void PreSolve(b3Contact* c) override { events.push_back(Event{3u, c->GetFixtureA(), c->GetFixtureB(), c}); }
+    void PostSolve(b3Contact* c) override {
+        order.push_back(c);
+        events.push_back(Event{4u, c->GetFixtureA(), c->GetFixtureB(), c});
+    }
+};
+
+// A b3ContactFilter answering from two bit tables over the fixtures in creation order.
+struct TableFilter : public b3ContactFilter {
+    std::unordered_map<const b3Fixture*, uint32_t> index;
+    std::vector<uint8_t> collide, respond;
+    size_t n = 0;
+    bool Bit(const std::vector<uint8_t>& t, const b3Fixture* a, const b3Fixture* b) const {
+        const size_t k = size_t(index.at(a)) * n + index.at(b);
+        return (t[k >> 3] >> (k & 7)) & 1u;
+    }
+    bool ShouldCollide(b3Fixture* a, b3Fixture* b) override { return Bit(collide, a, b); }
+    bool ShouldRespond(b3Fixture* a, b3Fixture* b) override { return Bit(respond, a, b); }
 };
 
 struct RefWorld {
@@ -78,6 +101,7 @@ struct RefWorld {
     std::vector<b3Joint*> joints;
     IslandRecorder recorder;
     std::vector<b3Contact*> lastIslandOrder;
+    TableFilter filter;
 };
 
 b3Vec3 V3(const float* p) { return b3Vec3(scalar(p[0]), scalar(p[1]), scalar(p[2])); }
@@ -253,6 +277,7 @@ RefWorld* CreateWorld(const bx_shape_library* lib, const bx_world_desc* wd) {
 
 void Step(RefWorld* w, scalar dt, uint32 vit, uint32 pit) {
     w->recorder.order.clear();
+    w->recorder.events.clear();
     w->world.Step(dt, vit, pit);
     w->lastIslandOrder = w->recorder.order;
 }
@@ -315,6 +340,42 @@ void BXREF(destroy)(void* h) {
     delete w;
     for (RefHull* p : hulls) delete p;
     for (RefMesh* p : meshes) delete p;
+}
+
+int BXREF(set_contact_filter)(void* h, const uint8_t* collide, const uint8_t* respond) {
+    RefWorld* w = (RefWorld*)h;
+    if (!collide) {
+        w->world.SetContactFilter(nullptr);
+        return 0;
+    }
+    TableFilter& f = w->filter;
+    f.n = w->fixtures.size();
+    f.index.clear();
+    for (size_t i = 0; i < w->fixtures.size(); ++i) f.index[w->fixtures[i]] = uint32_t(i);
+    const size_t bytes = (f.n * f.n + 7) / 8;
+    f.collide.assign(collide, collide + bytes);
+    if (respond) f.respond.assign(respond, respond + bytes);
+    else f.respond.assign(bytes, 0xFF);
+    w->world.SetContactFilter(&f);
+    return 0;
+}
+
+int BXREF(get_events)(void* h, bx_contact_event* out, uint32_t cap, uint32_t* count) {
+    RefWorld* w = (RefWorld*)h;
+    std::unordered_map<const b3Fixture*, uint32_t> fi;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) fi[w->fixtures[i]] = uint32_t(i);
+    std::unordered_map<const b3Contact*, uint32_t> ci;
+    uint32_t k = 0;
+    for (b3Contact* c = w->world.m_contactManager.m_contactList; c; c = c->m_next) ci[c] = k++;
+    uint32_t n = 0;
+    for (const IslandRecorder::Event& e : w->recorder.events) {
+        uint32_t idx = BX_NULL_INDEX;
+        if (e.contact && ci.count(e.contact)) idx = ci[e.contact];
+        if (n < cap) out[n] = bx_contact_event{e.type, fi[e.a], fi[e.b], idx};
+        ++n;
+    }
+    if (count) *count = n;
+    return 0;
 }
 
 void BXREF(step)(void* h, float dt, uint32_t vit, uint32_t pit, uint32_t n) {

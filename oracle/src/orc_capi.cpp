@@ -43,6 +43,7 @@ static World* Create(const bx_shape_library* lib, const bx_world_desc* wd) {
     w->sleeping = (wd->flags & BX_WORLD_SLEEPING) != 0;
     w->warmStarting = (wd->flags & BX_WORLD_NO_WARM_START) == 0;
     w->clearForces = (wd->flags & BX_WORLD_NO_CLEAR_FORCES) == 0;
+    w->recordEvents = (wd->flags & BX_WORLD_EVENTS) != 0;
     if (lib) {
         w->hulls.resize(lib->hull_count);
         for (u32 i = 0; i < lib->hull_count; ++i)
@@ -248,6 +249,33 @@ int orc_edit_bodies(void* h, const bx_body_edit* edits, uint32_t count) {
         default: return -1;
         }
     }
+    return 0;
+}
+
+// The filter as two bit tables over the fixtures (bx_batch_set_contact_filter's layout); null removes it.
+int orc_set_contact_filter(void* h, const uint8_t* collide, const uint8_t* respond) {
+    World* w = (World*)h;
+    const size_t bytes = (w->fixtures.size() * w->fixtures.size() + 7) / 8;
+    w->filterCollide.assign(collide, collide ? collide + bytes : collide);
+    w->filterRespond.assign(respond, respond ? respond + bytes : respond);
+    if (collide && !respond) w->filterRespond.assign(bytes, 0xFF);
+    return 0;
+}
+
+// Events of the last step with contacts as positions in the world contact list after the step.
+int orc_get_events(void* h, bx_contact_event* out, uint32_t cap, uint32_t* count) {
+    World* w = (World*)h;
+    uint32_t n = 0;
+    for (const World::Event& e : w->events) {
+        uint32_t idx = BX_NULL_INDEX;
+        if (e.contact) {
+            auto it = std::find(w->contactList.begin(), w->contactList.end(), e.contact);
+            if (it != w->contactList.end()) idx = uint32_t(it - w->contactList.begin());
+        }
+        if (n < cap) out[n] = bx_contact_event{e.type, e.fixtureA, e.fixtureB, idx};
+        ++n;
+    }
+    if (count) *count = n;
     return 0;
 }
 

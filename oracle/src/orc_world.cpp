@@ -172,6 +172,7 @@ void World::AddPair(u32 fA, u32 fB) {  // contact_manager.cpp:38-141
         }
     }
     if (!ShouldCollide(bA, bB)) return;
+    if (!FilterAllows(filterCollide, fA, fB)) return;  // b3ContactFilter::ShouldCollide (contact_manager.cpp:86-92)
     int reg = RegisterLookup(fixtures[fA].shape.type, fixtures[fB].shape.type);
     if (reg == 0) return;
     if (reg < 0) std::swap(fA, fB);
@@ -206,6 +207,7 @@ void World::FindNewContacts() {  // contact_manager.cpp:153-163, broad_phase.h:1
 }
 
 void World::DestroyContact(Contact* c) {  // contact_manager.cpp:229-289, contact.cpp:108-131
+    if (recordEvents && (c->flags & 2u)) events.push_back(Event{2u, c->fixtureA, c->fixtureB, nullptr});
     contactList.erase(std::find(contactList.begin(), contactList.end(), c));
     auto& la = fixtures[c->fixtureA].contacts;
     la.erase(std::find(la.begin(), la.end(), c));
@@ -283,6 +285,11 @@ void World::UpdateContact(Contact* c) {  // contact.cpp:183-285
         if (touching != wasTouching) { bA.SetAwake(true); bB.SetAwake(true); }
     }
     if (touching) c->flags |= 2u; else c->flags &= ~2u;
+    if (recordEvents) {  // contact.cpp:268-284
+        if (!wasTouching && touching) events.push_back(Event{1u, c->fixtureA, c->fixtureB, c});
+        if (wasTouching && !touching) events.push_back(Event{2u, c->fixtureA, c->fixtureB, c});
+        if (!sensor && touching) events.push_back(Event{3u, c->fixtureA, c->fixtureB, c});
+    }
 }
 
 void World::UpdateContacts() {  // contact_manager.cpp:165-227
@@ -290,6 +297,7 @@ void World::UpdateContacts() {  // contact_manager.cpp:165-227
     for (Contact* c : list) {
         u32 bA = fixtures[c->fixtureA].body, bB = fixtures[c->fixtureB].body;
         if (!ShouldCollide(bA, bB)) { DestroyContact(c); continue; }
+        if (!FilterAllows(filterCollide, c->fixtureA, c->fixtureB)) { DestroyContact(c); continue; }
         bool activeA = bodies[bA].IsAwake() && bodies[bA].type != e_staticBody;
         bool activeB = bodies[bB].IsAwake() && bodies[bB].type != e_staticBody;
         if (!activeA && !activeB) continue;
@@ -692,6 +700,8 @@ void World::SolveIsland(const TimeStep& step, std::vector<u32>& ib, std::vector<
         b.SynchronizeTransform();
     }
     for (Contact* c : ic) lastIslandOrder.push_back(c);
+    if (recordEvents)  // b3Island::Report (island.cpp:309-320)
+        for (Contact* c : ic) events.push_back(Event{4u, c->fixtureA, c->fixtureB, c});
     if (sleeping) {  // 7. sleep
         scalar minSleepTime = kMaxScalar;
         for (u32 i = 0; i < nb; ++i) {
@@ -746,6 +756,7 @@ void World::Solve(const TimeStep& step) {  // world.cpp:253-450
                     if (c->flags & 1u) continue;
                     if ((c->flags & 2u) == 0) continue;
                     if (fixtures[c->fixtureA].isSensor || fixtures[c->fixtureB].isSensor) continue;
+                    if (!FilterAllows(filterRespond, c->fixtureA, c->fixtureB)) continue;  // world.cpp:351-357
                     ic.push_back(c);
                     c->flags |= 1u;
                     u32 otherF = c->fixtureA == f ? c->fixtureB : c->fixtureA;
@@ -782,6 +793,7 @@ void World::Solve(const TimeStep& step) {  // world.cpp:253-450
 }
 
 void World::Step(scalar dt, u32 vit, u32 pit) {  // world.cpp:187-242
+    events.clear();
     if (newContacts) { FindNewContacts(); newContacts = false; }
     TimeStep step;
     step.dt = dt;

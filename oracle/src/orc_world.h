@@ -132,6 +132,18 @@ struct World {
     std::vector<Contact*> contactList;
     std::vector<Contact*> lastIslandOrder;
     u32 lastIslandCount = 0;
+    // contact listener events of the last step (b3ContactListener, world_callbacks.h); `contact` is
+    // null for a contact the step destroyed
+    struct Event { u32 type, fixtureA, fixtureB; Contact* contact; };
+    bool recordEvents = false;
+    std::vector<Event> events;
+    // contact filter tables (b3ContactFilter): bit i * n + j over fixtures in creation order; empty = none
+    std::vector<uint8_t> filterCollide, filterRespond;
+    bool FilterAllows(const std::vector<uint8_t>& t, u32 i, u32 j) const {
+        if (t.empty()) return true;
+        const size_t k = size_t(i) * fixtures.size() + j;
+        return (t[k >> 3] >> (k & 7)) & 1u;
+    }
 
     ~World();
     u32 AllocateNode();
