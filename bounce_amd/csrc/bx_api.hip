@@ -19,6 +19,7 @@
 #include "bx_shapes.h"
 #include "bx_joint.h"
 
+
 namespace bx {
 struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIterations; };
 void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
@@ -162,7 +163,10 @@ static const ArraySpec kArrays[] = {
     BX_ARRAY(joints, kSpaceJoint, 1), BX_ARRAY(islandJoints, kSpaceJoint, 1), BX_ARRAY(jointCons, kSpaceJoint, 1),
     BX_ARRAY(jointEdges, kSpaceJoint, 2),
     BX_ARRAY(contacts, kSpaceContact, 1), BX_ARRAY(contactOrder, kSpaceContact, 1),
-    BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1), BX_ARRAY(events, kSpaceContact, 1),
+    BX_ARRAY(freeSlots, kSpaceContact, 1), BX_ARRAY(contactAlive, kSpaceContact, 1),
+#if BX_DEVICE_CALLBACKS
+    BX_ARRAY(events, kSpaceContact, 1),
+#endif
     BX_ARRAY(edgeSlots, kSpaceContact, 2),
     BX_ARRAY(islandContacts, kSpaceCon, 1), BX_ARRAY(velCons, kSpaceCon, 1), BX_ARRAY(posCons, kSpaceCon, 1),
     BX_ARRAY(meshContacts, kSpaceMesh, 1), BX_ARRAY(meshTriLists, kSpaceMeshTri, 1),
@@ -686,6 +690,10 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
     wi.fixtureCount = d.fixture_count;
     wi.jointCount = d.joint_count;
     wi.flags = d.flags;
+#if !BX_DEVICE_CALLBACKS
+    if (d.flags & BX_WORLD_EVENTS)
+        return Fail(BX_E_UNSUPPORTED, "contact event recording (BX_WORLD_EVENTS) is not enabled in this build");
+#endif
     wi.gravity = V3(d.gravity);
     wi.serialIslands = 0;
     if ((d.body_count && !d.bodies) || (d.fixture_count && !d.fixtures) || (d.joint_count && !d.joints))
@@ -1577,7 +1585,9 @@ int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* 
     HIP_CHECK(hipSetDevice(b->ctx->device));
     hipStream_t st = b->ctx->stream;
     WorldInfo wi = h.info;
+#if BX_DEVICE_CALLBACKS
     wi.filter = nullptr;  // contact filter tables belong to the batch that set them (pools hold none)
+#endif
     u32 units[kSpaces], base[kSpaces], s;
     WorldUnits(wi, units);
     int rc = Reserve(b, units, base, &s);
@@ -1782,6 +1792,9 @@ int bx_batch_set_contact_filter(bx_batch* b, uint32_t world, const uint8_t* coll
     if (!b || !IsResident(b, world)) return Fail(BX_E_INVALID, "bad batch or world not resident");
     if (b->pool) return Fail(BX_E_UNSUPPORTED, "contact filter tables are set on batches, not pools");
     if ((collide == nullptr) != (respond == nullptr)) return Fail(BX_E_INVALID, "set both filter tables or neither");
+#if !BX_DEVICE_CALLBACKS
+    return Fail(BX_E_UNSUPPORTED, "contact filter tables are not enabled in this build");
+#else
     HIP_CHECK(hipSetDevice(b->ctx->device));
     WorldInfo& wi = b->worlds[world];
     if (!collide) {
@@ -1801,9 +1814,15 @@ int bx_batch_set_contact_filter(bx_batch* b, uint32_t world, const uint8_t* coll
     }
     HIP_CHECK(hipMemcpy((void*)(b->view.worlds + world), &wi, sizeof(wi), hipMemcpyHostToDevice));
     return BX_OK;
+#endif
 }
 
 int bx_batch_get_events(bx_batch* b, uint32_t world, bx_contact_event* out, uint32_t capacity, uint32_t* count) {
+#if !BX_DEVICE_CALLBACKS
+    (void)out, (void)capacity, (void)count;
+    if (!b || !IsResident(b, world)) return Fail(BX_E_INVALID, "bad batch or world not resident");
+    return Fail(BX_E_UNSUPPORTED, "contact event recording is not enabled in this build");
+#else
     WorldState ws;
     int rc = GetState(b, world, ws);
     if (rc) return rc;
@@ -1841,6 +1860,7 @@ int bx_batch_get_events(bx_batch* b, uint32_t world, bx_contact_event* out, uint
     }
     if (count) *count = total;
     return BX_OK;
+#endif
 }
 
 int bx_batch_get_bodies(bx_batch* b, uint32_t world, bx_body_state* out, uint32_t capacity) {

@@ -164,8 +164,10 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
                         if ((cf & 2u) == 0) continue;
                         u32 fa = cfa[k], fbb = cfb[k];
                         if ((fbody[fa] | fbody[fbb]) & 0x80000000u) continue;  // sensors
+#if BX_DEVICE_CALLBACKS
                         // b3ContactFilter::ShouldRespond (world.cpp:351-357)
                         if (wi.filter && !FilterBit(wi.filter + FilterWords(nf), nf, fa, fbb)) continue;
+#endif
                         // one constraint entry per manifold (a mesh contact has up to three,
                         // solved one after the other as b3ContactSolver does; a touching convex
                         // contact has one)

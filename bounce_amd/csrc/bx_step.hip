@@ -191,8 +191,11 @@ __device__ __noinline__ bool UpdateMeshManifolds(const BatchView& B, const World
 // (contact.cpp:183-285) for one contact. Returns false if the contact is destroyed. kMesh compiles
 // the mesh contact path in (launches over batches that hold a mesh fixture).
 template <bool kMesh>
-__device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot, bool applyWakes,
-                                 u32& events) {
+__device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldState& ws, u32 slot, bool applyWakes
+#if BX_DEVICE_CALLBACKS
+                                 , u32& events  // kEv* bits of this visit
+#endif
+) {
     Contact& c = B.contacts[wi.contactBase + slot];
     const Fixture fA = B.fixtures[wi.fixtureBase + c.fixtureA];
     const Fixture fB = B.fixtures[wi.fixtureBase + c.fixtureB];
@@ -201,10 +204,13 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     bool sensor = fA.isSensor || fB.isSensor;
     const bool isMesh = kMesh && (fA.shapeType == kMeshShape || fB.shapeType == kMeshShape);
     FaceScratch scratch;  // hull clipping buffers of every narrowphase below
-    const bool wasTouching = (c.flags & 2u) != 0;
+#if BX_DEVICE_CALLBACKS
     events = 0;
+#endif
     auto destroy = [&]() {
-        events = kEvDestroyed | (wasTouching ? kEvEnd : 0u);  // contact_manager.cpp:230-237
+#if BX_DEVICE_CALLBACKS
+        events = kEvDestroyed | ((c.flags & 2u) ? kEvEnd : 0u);  // contact_manager.cpp:230-237
+#endif
         if (applyWakes && !sensor) {
             for (u32 j = 0; j < c.manifoldCount; ++j) {
                 if (ContactManifold(B, wi, c, slot, j).pointCount > 0) {
@@ -217,13 +223,16 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
         return false;
     };
     if (!BodiesShouldCollide(B, wi, fA.body, fB.body)) return destroy();
+#if BX_DEVICE_CALLBACKS
     // b3ContactFilter::ShouldCollide (contact_manager.cpp:189-199)
     if (wi.filter && !FilterBit(wi.filter, wi.fixtureCount, c.fixtureA, c.fixtureB)) return destroy();
+#endif
     bool activeA = (bA.flags & kAwake) && bA.type != kStatic;
     bool activeB = (bB.flags & kAwake) && bB.type != kStatic;
     if (!activeA && !activeB) return true;
     if (!TestOverlap(B.fatAABB[wi.fixtureBase + c.fixtureA], B.fatAABB[wi.fixtureBase + c.fixtureB])) return destroy();
 
+    bool wasTouching = (c.flags & 2u) != 0;
     bool touching = false;
     if (sensor && isMesh) {
         touching = MeshContactTestOverlap(B, wi, slot, c, Transform(bA.xfT, bA.q), Transform(bB.xfT, bB.q));
@@ -316,10 +325,12 @@ __device__ bool UpdateOneContact(const BatchView& B, const WorldInfo& wi, WorldS
     }
     if (touching) c.flags |= 2u;
     else c.flags &= ~2u;
+#if BX_DEVICE_CALLBACKS
     // the listener calls of b3Contact::Update (contact.cpp:268-284)
     if (!wasTouching && touching) events |= kEvBegin;
     if (wasTouching && !touching) events |= kEvEnd;
     if (!sensor && touching) events |= kEvPreSolve;
+#endif
     return true;
 }
 
@@ -343,6 +354,7 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
         if (b.type != kStatic && (b.flags & kAwake) == 0) asleep = 1;
     }
     asleep = __syncthreads_or(asleep);
+#if BX_DEVICE_CALLBACKS
     const bool record = (wi.flags & kWorldEvents) != 0;
     auto update = [&](u32 i) {
         u32 ev = 0;
@@ -359,6 +371,14 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
         for (u32 k = n; k-- > 0;) update(k);
     }
     if (record && threadIdx.x == 0) ws.eventCount = n;
+#else
+    if (!asleep) {
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            alive[i] = UpdateOneContact<kMesh>(B, wi, ws, order[i], true) ? 1u : 0u;
+    } else if (threadIdx.x == 0) {
+        for (u32 k = n; k-- > 0;) alive[k] = UpdateOneContact<kMesh>(B, wi, ws, order[k], true) ? 1u : 0u;
+    }
+#endif
     __syncthreads();
     // stable compaction of the order list; destroyed slots go back on the free stack
     u32 kept = 0, freed = 0;
@@ -522,8 +542,10 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
                 }
             }
             if (keep && !BodiesShouldCollide(B, wi, FA.body, FB.body)) keep = 0;
+#if BX_DEVICE_CALLBACKS
             // b3ContactFilter::ShouldCollide (contact_manager.cpp:86-92), fixtures in pair order
             if (keep && wi.filter && !FilterBit(wi.filter, nf, fa, fb)) keep = 0;
+#endif
             if (keep) {
                 int reg = RegisterLookup(FA.shapeType, FB.shapeType);
                 if (reg == 0) keep = 0;

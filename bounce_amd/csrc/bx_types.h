@@ -11,6 +11,14 @@
 #pragma once
 #include "bx_math.h"
 
+// Contact listener events (BX_WORLD_EVENTS) and contact filter tables in the kernels and their host
+// side: compiled in when BX_DEVICE_CALLBACKS is nonzero (the host build of these kernels sets it). The
+// GPU library leaves them out: a device test of event recording ended in a GPU memory fault whose
+// cause has not been found, and without them its kernels are those that run clean on the GPU.
+#ifndef BX_DEVICE_CALLBACKS
+#define BX_DEVICE_CALLBACKS 0
+#endif
+
 namespace bx {
 
 struct HalfEdge { u32 origin, twin, face, prev, next; };
@@ -247,11 +255,13 @@ struct WorldInfo {
     u32 meshTriBase;                   // mesh contact triangle lists (contactCap x meshTriCap)
     u32 meshTriCap;                    // triangles a mesh contact can hold (0: no mesh fixture)
     u32 meshScratchBase;               // clustering stripes (kMeshLanes x MeshLaneStride(meshTriCap))
+#if BX_DEVICE_CALLBACKS
     u32 pad0;
     // contact filter tables (b3ContactFilter) or null: bit i * fixtureCount + j of the first table is
     // ShouldCollide(fixture i, fixture j), of the second (FilterWords(fixtureCount) words later)
     // ShouldRespond(i, j)
     const u32* filter;
+#endif
 };
 
 BX_HD u32 FilterWords(u32 nf) { return u32((u64(nf) * nf + 31) / 32); }
@@ -281,8 +291,10 @@ struct WorldState {
     scalar inv_dt0;
     u32 touchingCount;
     u32 anyAsleep;
+#if BX_DEVICE_CALLBACKS
     u32 eventCount;       // contacts visited by the last contact update (entries of BatchView::events)
     u32 pad;
+#endif
 };
 
 // Words of island-builder scratch a world needs: per body flags, fixture range, joint range,
@@ -369,7 +381,9 @@ struct BatchView {
     u32* contactOrder;        // per world: live slots, creation order
     u32* freeSlots;           // per world: free slot stack
     u32* contactAlive;        // per slot: scratch flag for the update pass
+#if BX_DEVICE_CALLBACKS
     uint4* events;            // per world: the last contact update's visits {slot, fixtureA, fixtureB, kEv* bits}
+#endif
     // adjacency (rebuilt each step)
     u32* edgeOffset;          // per fixture + 1 per world (fixtureCount + 1 entries from WorldInfo::edgeBase)
     u32* edgeSlots;           // per world: 2 * contactCap

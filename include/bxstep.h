@@ -68,7 +68,9 @@ extern "C" {
 #define BX_WORLD_SLEEPING 0x0001u      /* b3World::SetSleeping(true); default off (world.cpp:46) */
 #define BX_WORLD_NO_WARM_START 0x0002u /* b3World::SetWarmStart(false); default on */
 #define BX_WORLD_NO_CLEAR_FORCES 0x0004u
-#define BX_WORLD_EVENTS 0x0008u        /* record the contact listener's events of every step (bx_batch_get_events) */
+#define BX_WORLD_EVENTS 0x0008u        /* record the contact listener's events of every step (bx_batch_get_events);
+                                          compiled in only with -DBX_DEVICE_CALLBACKS=1: the GPU library as
+                                          built refuses such a world with BX_E_UNSUPPORTED */
 
 /* b3JointType (include/bounce/dynamics/joints/joint.h) */
 #define BX_JOINT_REVOLUTE 4u
@@ -398,7 +400,8 @@ typedef struct bx_contact_event {
     uint32_t contact;              /* position in bx_batch_get_contacts' list after the step, BX_NULL_INDEX
                                       for a contact the step destroyed */
 } bx_contact_event;
-/* Events of the last step of a world created with BX_WORLD_EVENTS; `count` receives the total. */
+/* Events of the last step of a world created with BX_WORLD_EVENTS; `count` receives the total.
+   Returns BX_E_UNSUPPORTED in a library built without -DBX_DEVICE_CALLBACKS=1 (the GPU library). */
 int bx_batch_get_events(bx_batch* batch, uint32_t world, bx_contact_event* out, uint32_t capacity,
                         uint32_t* count);
 
@@ -409,7 +412,8 @@ int bx_batch_get_events(bx_batch* batch, uint32_t world, bx_contact_event* out, 
    when a pair becomes a contact (contact_manager.cpp:86-92), when a contact is updated
    (contact_manager.cpp:189-199) and when the island search meets a touching contact (world.cpp:351-357),
    so a filter whose answers depend only on the two fixtures is reproduced exactly. Null tables remove
-   the filter. Batches only; at most 16,384 fixtures. */
+   the filter. Batches only; at most 16,384 fixtures. Returns BX_E_UNSUPPORTED in a library built
+   without -DBX_DEVICE_CALLBACKS=1 (the GPU library). */
 int bx_batch_set_contact_filter(bx_batch* batch, uint32_t world, const uint8_t* collide, const uint8_t* respond);
 
 /* bx_batch_totals of the whole batch or pool, from one read of the world states and bodies. */

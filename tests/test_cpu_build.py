@@ -80,6 +80,30 @@ print("probe read done", flush=True)
     assert past.returncode != 0 and "use-after-poison" in past.stderr, past.stderr[-2000:]
 
 
+def test_cpu_build_events_and_filter(tmp_path):
+    """The device tests of contact events and the filter (tests/test_gpu_events.py) on the host build,
+    which compiles event recording in, under AddressSanitizer with the per-world check: the event
+    sequence and the world state equal the oracle's at every step, without and with the filter tables,
+    and the filter alone over the mixed drop."""
+    _make("asan")
+    script = tmp_path / "events.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {os.path.dirname(HERE)!r})
+from bounce_amd import device
+device.load({os.path.join(BUILD, "_out", "libbxstep_asan.so")!r})
+from tests import test_gpu_events as t
+ctx = device.Context(0)
+for filtered in (False, True):
+    t.test_events_device_equal_oracle(ctx, filtered)
+t.test_contact_filter_device_equal_oracle(ctx)
+print("events and filter equal to the oracle", flush=True)
+""")
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
+
+
 @pytest.mark.slow
 def test_cpu_build_memory_sanitizer_mixed_worlds(tmp_path):
     _make("msan")
