@@ -35,8 +35,10 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 # Steps a batch spends timing the velocity-sweep layouts when solver_mode is 0 (three steps first, then
-# each of three layouts twice): the warm-up covers them, so no tuning step lands in the timed region.
-TUNE_STEPS = 9
+# each of five layouts twice): the warm-up covers them, so no tuning step lands in the timed region.
+TUNE_STEPS = 13
+# Velocity-sweep layouts (solver_mode 1..5): islands per wavefront and lanes per island.
+LAYOUTS = [(16, 1), (32, 1), (64, 1), (4, 8), (8, 8)]
 # Compulsory bytes of one step (the unit of work is one body-step), from the counts of the step: per
 # awake non-static body 432 B (state read 256 B, written 176 B), per proxy 52 B (fat AABB read and
 # written, moved flag), per broadphase pair 16 B (sorted pair written and read), per convex contact
@@ -302,12 +304,14 @@ def main():
                     f"20..2000 per world; spheres, capsules, 8-16 point hulls, a 10-link motor chain), "
                     f"dt=1/60, 8/2 iterations, sleep off"),
                    "worlds": args.worlds, "bodies": total_bodies, "parallelism": "worlds x islands"},
-        "solver_layout": {"islands_per_wavefront": lanes, "layouts": [16, 32, 64],
+        "solver_layout": {"islands_per_wavefront": lanes,
+                          "lanes_per_island": dict(LAYOUTS).get(lanes),
+                          "layouts": [{"islands_per_wavefront": i, "lanes_per_island": n} for i, n in LAYOUTS],
                           "tuned_k_solve_velocity_ms": [round(t, 3) for t in tuned_ms],
                           "tuned_before_timing": tuned_before_timing,
-                          "note": "solver_mode 0: each layout timed twice on warm-up steps 4-9 (order "
-                                  "16 32 64 64 32 16), the one with the smallest sample kept (16 unless "
-                                  "another is faster by more than 5%)"},
+                          "note": "solver_mode 0: each layout timed twice on warm-up steps 4-13 (order "
+                                  "1 2 3 4 5 5 4 3 2 1), the one with the smallest sample kept (layout 1 "
+                                  "unless another is faster by more than 5%)"},
         "graph_replay": graph,
         "plain_per_world": plain,
         "batched_over_plain": value / plain["value"] if plain else None,

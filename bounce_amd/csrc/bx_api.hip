@@ -250,7 +250,7 @@ static const u32 kListRing = 4;
 // lose to one timed later; with the first layout sampled first and last, such a start cannot move
 // the choice away from it, while a layout that is faster by more than the early slowdown still wins.
 static const u32 kTuneSkip = 3;
-static const u32 kTuneOrder[2 * kSolveLayouts] = {0, 1, 2, 2, 1, 0};
+static const u32 kTuneOrder[2 * kSolveLayouts] = {0, 1, 2, 3, 4, 4, 3, 2, 1, 0};
 static const u32 kTuneSteps = kTuneSkip + 2 * kSolveLayouts;
 // Layouts within a few percent of each other are a tie in one or two samples; the first layout is
 // kept unless another is faster by more than this fraction.
@@ -1333,11 +1333,11 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     V.islandLdsWords = lds;
     V.meshWorlds = 0;
     for (u32 w = 0; w < world_count; ++w) V.meshWorlds += b->worlds[w].meshTriCap ? 1u : 0u;
-    // velocity-sweep layout: fixed by solver_mode 1..3, else timed on the first steps (StepAll)
+    // velocity-sweep layout: fixed by solver_mode 1..5, else timed on the first steps (StepAll)
     const u32 mode = opts ? opts->solver_mode : 0u;
     if (mode > kSolveLayouts) {
         bx_batch_destroy(b);
-        return Fail(BX_E_INVALID, "solver_mode must be 0 (tuned) or 1..3");
+        return Fail(BX_E_INVALID, "solver_mode must be 0 (tuned) or 1..5");
     }
     V.solveLayout = mode ? mode - 1 : 0u;
     b->tuneStep = mode ? kTuneSteps : 0u;
@@ -1415,7 +1415,7 @@ int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_po
     b->newContacts = false;
     if (opts->solver_mode > kSolveLayouts) {
         bx_batch_destroy(b);
-        return Fail(BX_E_INVALID, "solver_mode must be 0 (16 islands per wavefront) or 1..3");
+        return Fail(BX_E_INVALID, "solver_mode must be 0 (16 islands per wavefront) or 1..5");
     }
     b->view.solveLayout = opts->solver_mode ? opts->solver_mode - 1 : 0u;
     *out = b;

@@ -1,0 +1,340 @@
+// bx_solve_quad.h -- velocity sweeps with eight lanes per island (component-parallel contact solve).
+//
+// The warm start and PGS sweeps of b3ContactSolver (contact_solver.cpp:272-486) are a serial chain
+// per island: each contact reads the two body velocities the previous contact wrote. Inside one
+// contact, though, almost all of the arithmetic is on 3-vectors: cross products, 3x3 inverse
+// inertia times a vector, dot products, and v +=/-= m * P on body A and body B. Here a group of
+// eight lanes solves one contact: lanes 0-3 of the group hold body A, lanes 4-7 body B, and lane
+// 4*side + c holds component c (x, y, z; lane 3 of each quad is padding that computes along and
+// never stores). Each lane evaluates exactly the binary32 operations the serial solver evaluates
+// for its component, in the same order; the operands it needs from its siblings (the other
+// components of a vector, body A's values on the B lanes) arrive by DPP lane permutes, which move
+// bits and round nothing. Scalars (impulses, clamps, the friction cone) are evaluated on all eight
+// lanes from the same inputs, so every lane holds the same impulse values, and each lane reloads
+// in the next sweep only values it stored itself. The results are bit-identical to the serial
+// solver's: the order of every rounded operation and of every sum is unchanged, and a subtraction
+// a - b is evaluated as a + (-b), which IEEE defines identically (body A's updates use the negated
+// inverse mass, and -1 * X for the inertia term, both exact negations).
+//
+// Body velocities live in an LDS slab per island (kSlab bodies); an island that does not fit, and
+// every joint row, is solved by the group's first lane with the serial code of bx_solve_kernels.h.
+#pragma once
+// (included by bx_step.hip inside namespace bx, after bx_solve_kernels.h)
+
+// DPP moves. BX_DPP_MOV: every lane takes the value of the lane kCtrl names (a permutation inside
+// each quad, where every source lane is valid; the compiler folds it into the instruction that
+// uses it). BX_DPP_UPDATE: lanes of the banks in bankMask take it, the others keep `old`. The host
+// build of the kernels defines both as its own group exchange.
+#ifndef BX_DPP_MOV
+#define BX_DPP_MOV(src, ctrl) __builtin_amdgcn_mov_dpp(src, ctrl, 0xF, 0xF, true)
+#define BX_DPP_UPDATE(old, src, ctrl, bankMask) __builtin_amdgcn_update_dpp(old, src, ctrl, 0xF, bankMask, false)
+#endif
+// Lanes of a group exchange slab values through LDS in program order, which a wavefront's lockstep
+// execution gives on the device; the host build, which runs the lanes one after another, defines
+// BX_GROUP_SYNC as a wait for the group's eight lanes.
+#ifndef BX_GROUP_SYNC
+#define BX_GROUP_SYNC()
+#endif
+template <int kCtrl>
+__device__ __forceinline__ scalar Dpp(scalar x) {
+    return __builtin_bit_cast(scalar, BX_DPP_MOV(__builtin_bit_cast(int, x), kCtrl));
+}
+template <int kCtrl, int kBank>
+__device__ __forceinline__ scalar DppBanks(scalar old, scalar x) {
+    return __builtin_bit_cast(scalar, BX_DPP_UPDATE(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x), kCtrl, kBank));
+}
+constexpr int kDppNext = 0xC9;        // quad_perm [1,2,0,3]: lane c reads component c+1 (mod 3)
+constexpr int kDppNext2 = 0xD2;       // quad_perm [2,0,1,3]: component c+2 (mod 3)
+constexpr int kDppBcastX = 0x00, kDppBcastY = 0x55, kDppBcastZ = 0xAA;
+constexpr int kDppFromA = 0x114;      // row_shr:4 into the B banks: body A's value on the B lanes
+constexpr int kDppToA = 0x104;        // row_shl:4 into the A banks: body B's value on the A lanes
+constexpr int kBanksA = 0x5, kBanksB = 0xA;
+
+__device__ __forceinline__ scalar QNext(scalar x) { return Dpp<kDppNext>(x); }
+__device__ __forceinline__ scalar QNext2(scalar x) { return Dpp<kDppNext2>(x); }
+// b3Dot over the quad's three components, ((x + y) + z), on every lane of the quad
+__device__ __forceinline__ scalar QSum(scalar p) {
+    const scalar s = Dpp<kDppBcastX>(p) + Dpp<kDppBcastY>(p);
+    return s + Dpp<kDppBcastZ>(p);
+}
+// component c of Cross(a, b) = a[c+1] * b[c+2] - a[c+2] * b[c+1], with b's permuted components given
+__device__ __forceinline__ scalar QCrossVar(scalar a, scalar bn, scalar bnn) { return QNext(a) * bnn - QNext2(a) * bn; }
+// component c of Cross(r, p) for a lane-constant r (rn = r[c+1], rnn = r[c+2]) and a varying p
+__device__ __forceinline__ scalar QCrossConst(scalar rn, scalar rnn, scalar p) { return rn * QNext2(p) - rnn * QNext(p); }
+// component c of A * v = v.x * A.x + v.y * A.y + v.z * A.z (row c of A is i0, i1, i2)
+__device__ __forceinline__ scalar QMatVec(scalar i0, scalar i1, scalar i2, scalar v) {
+    const scalar s = Dpp<kDppBcastX>(v) * i0 + Dpp<kDppBcastY>(v) * i1;
+    return s + Dpp<kDppBcastZ>(v) * i2;
+}
+__device__ __forceinline__ scalar FromA(scalar x) { return DppBanks<kDppFromA, kBanksB>(x, x); }
+__device__ __forceinline__ scalar ToA(scalar x) { return DppBanks<kDppToA, kBanksA>(x, x); }
+
+// A lane's role in its group.
+struct QuadLane {
+    u32 side;   // 0: body A, 1: body B
+    u32 c;      // component 0..2 (3: padding lane)
+    u32 cc;     // component this lane reads (the padding lane reads z)
+    u32 cn, cnn;
+    scalar sign;  // -1 on body A, +1 on body B
+};
+
+// Float offsets of the VelCon fields a lane reads.
+constexpr u32 kVcInvMass = offsetof(VelCon, invMassA) / 4;
+constexpr u32 kVcFriction = offsetof(VelCon, friction) / 4;
+constexpr u32 kVcInvIA = offsetof(VelCon, invIA) / 4, kVcInvIB = offsetof(VelCon, invIB) / 4;
+constexpr u32 kVcRA = offsetof(VelCon, rA) / 4, kVcRB = offsetof(VelCon, rB) / 4;
+constexpr u32 kVcNormal = offsetof(VelCon, normal) / 4;
+constexpr u32 kVcT1 = offsetof(VelCon, tangent1) / 4, kVcT2 = offsetof(VelCon, tangent2) / 4;
+constexpr u32 kVcSpeeds = offsetof(VelCon, tangentSpeed1) / 4;  // tangentSpeed1/2, motorSpeed, motorMass
+constexpr u32 kVcTangentMass = offsetof(VelCon, tangentMass) / 4;
+constexpr u32 kVcImpulses = offsetof(VelCon, tangentImpulse) / 4;  // tangentImpulse x, y, motorImpulse
+constexpr u32 kVcPoints = offsetof(VelCon, points) / 4;
+constexpr u32 kVcPointStride = sizeof(VCPoint) / 4;
+constexpr u32 kVcpRA = offsetof(VCPoint, rA) / 4, kVcpRB = offsetof(VCPoint, rB) / 4;
+constexpr u32 kVcpNormal = offsetof(VCPoint, normal) / 4, kVcpNormalMass = offsetof(VCPoint, normalMass) / 4;
+constexpr u32 kVcpNormalImpulse = offsetof(VCPoint, normalImpulse) / 4;
+constexpr u32 kVcpBias = offsetof(VCPoint, velocityBias) / 4;
+static_assert(offsetof(VelCon, tangentSpeed2) == offsetof(VelCon, tangentSpeed1) + 4 &&
+                  offsetof(VelCon, motorSpeed) == offsetof(VelCon, tangentSpeed1) + 8 &&
+                  offsetof(VelCon, motorMass) == offsetof(VelCon, tangentSpeed1) + 12 &&
+                  offsetof(VelCon, motorImpulse) == offsetof(VelCon, tangentImpulse) + 8,
+              "VelCon field order the quad loader assumes");
+
+// One lane's share of a VelCon.
+struct QCon {
+    u32 ia, ib, pc;
+    scalar m;              // inverse mass of the lane's body, negated on body A
+    scalar i0, i1, i2;     // row cc of the body's inverse inertia
+    scalar friction;
+    scalar rn, rnn;        // the contact's arm (rA or rB) at components cn, cnn
+    scalar n, t1, t2;      // normal and tangents at component cc
+    scalar ts1, ts2, ms, mm;
+    scalar kxx, kxy, kyx, kyy;  // tangentMass columns
+    scalar tix, tiy, mi;   // tangent and motor impulses
+    scalar prn[kMaxManifoldPoints], prnn[kMaxManifoldPoints], pn[kMaxManifoldPoints];
+    scalar pm[kMaxManifoldPoints], pi[kMaxManifoldPoints], pb[kMaxManifoldPoints];
+};
+
+__device__ __forceinline__ void QLoad(QCon& q, const VelCon* src, const QuadLane& L) {
+    const scalar* f = reinterpret_cast<const scalar*>(src);
+    const u32* u = reinterpret_cast<const u32*>(src);
+    q.ia = u[0];
+    q.ib = u[1];
+    q.pc = u[2];
+    const scalar m = f[kVcInvMass + L.side];
+    q.m = L.side ? m : -m;
+    const u32 I = L.side ? kVcInvIB : kVcInvIA;
+    q.i0 = f[I + L.cc];
+    q.i1 = f[I + 3 + L.cc];
+    q.i2 = f[I + 6 + L.cc];
+    q.friction = f[kVcFriction];
+    const u32 r = L.side ? kVcRB : kVcRA;
+    q.rn = f[r + L.cn];
+    q.rnn = f[r + L.cnn];
+    q.n = f[kVcNormal + L.cc];
+    q.t1 = f[kVcT1 + L.cc];
+    q.t2 = f[kVcT2 + L.cc];
+    q.ts1 = f[kVcSpeeds];
+    q.ts2 = f[kVcSpeeds + 1];
+    q.ms = f[kVcSpeeds + 2];
+    q.mm = f[kVcSpeeds + 3];
+    q.kxx = f[kVcTangentMass];
+    q.kxy = f[kVcTangentMass + 1];
+    q.kyx = f[kVcTangentMass + 2];
+    q.kyy = f[kVcTangentMass + 3];
+    q.tix = f[kVcImpulses];
+    q.tiy = f[kVcImpulses + 1];
+    q.mi = f[kVcImpulses + 2];
+    const u32 pr = L.side ? kVcpRB : kVcpRA;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        const scalar* p = f + kVcPoints + k * kVcPointStride;
+        q.prn[k] = p[pr + L.cn];
+        q.prnn[k] = p[pr + L.cnn];
+        q.pn[k] = p[kVcpNormal + L.cc];
+        q.pm[k] = p[kVcpNormalMass];
+        q.pi[k] = p[kVcpNormalImpulse];
+        q.pb[k] = p[kVcpBias];
+    }
+}
+
+// Body A's value of x on the B lanes, combined as b - a_from_A.
+__device__ __forceinline__ scalar SubA(scalar b, scalar x) { return b - FromA(x); }
+
+// v, w: the lane's component of its body's velocities. contact_solver.cpp:272-326.
+__device__ __forceinline__ void QWarmStart(const QCon& q, scalar& v, scalar& w, const QuadLane& L) {
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= q.pc) break;
+        const scalar P = q.pi[k] * q.pn[k];
+        v = v + q.m * P;
+        w = w + L.sign * QMatVec(q.i0, q.i1, q.i2, QCrossConst(q.prn[k], q.prnn[k], P));
+    }
+    if (q.pc > 0) {
+        const scalar P12 = q.tix * q.t1 + q.tiy * q.t2;
+        const scalar P3 = q.mi * q.n;
+        v = v + q.m * P12;
+        w = w + L.sign * QMatVec(q.i0, q.i1, q.i2, QCrossConst(q.rn, q.rnn, P12) + P3);
+    }
+}
+
+// contact_solver.cpp:328-486; the impulses in q are updated on every lane.
+__device__ __forceinline__ void QSolve(QCon& q, scalar& v, scalar& w, const QuadLane& L) {
+    scalar normalImpulse = 0;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= q.pc) break;
+        // dv = vB + Cross(wB, rB) - vA - Cross(wA, rA), on the B lanes
+        const scalar cr = QCrossVar(w, q.prn[k], q.prnn[k]);
+        const scalar dv = SubA(SubA(v + cr, v), cr);
+        const scalar Cdot = ToA(QSum(q.pn[k] * dv));
+        scalar impulse = -q.pm[k] * (Cdot - q.pb[k]);
+        const scalar oldImpulse = q.pi[k];
+        q.pi[k] = Max(q.pi[k] + impulse, scalar(0));
+        impulse = q.pi[k] - oldImpulse;
+        const scalar P = impulse * q.pn[k];
+        v = v + q.m * P;
+        w = w + L.sign * QMatVec(q.i0, q.i1, q.i2, QCrossConst(q.prn[k], q.prnn[k], P));
+        normalImpulse += q.pi[k];
+    }
+    if (q.pc > 0) {
+        {
+            const scalar cr = QCrossVar(w, q.rn, q.rnn);
+            const scalar dv = SubA(SubA(v + cr, v), cr);
+            const scalar cx = ToA(QSum(dv * q.t1) - q.ts1);
+            const scalar cy = ToA(QSum(dv * q.t2) - q.ts2);
+            // tangentMass * -Cdot = (-cx) * K.x + (-cy) * K.y
+            scalar ix = -cx * q.kxx + -cy * q.kyx;
+            scalar iy = -cx * q.kxy + -cy * q.kyy;
+            const scalar ox = q.tix, oy = q.tiy;
+            q.tix += ix;
+            q.tiy += iy;
+            const scalar maxImpulse = q.friction * normalImpulse;
+            if (q.tix * q.tix + q.tiy * q.tiy > maxImpulse * maxImpulse) {
+                const scalar len = Sqrt(q.tix * q.tix + q.tiy * q.tiy);  // b3Vec2::Normalize
+                if (len > kEpsilon) {
+                    q.tix /= len;
+                    q.tiy /= len;
+                }
+                q.tix *= maxImpulse;
+                q.tiy *= maxImpulse;
+            }
+            ix = q.tix - ox;
+            iy = q.tiy - oy;
+            const scalar P = ix * q.t1 + iy * q.t2;
+            v = v + q.m * P;
+            w = w + L.sign * QMatVec(q.i0, q.i1, q.i2, QCrossConst(q.rn, q.rnn, P));
+        }
+        {
+            const scalar Cdot = ToA(QSum(q.n * SubA(w, w)) - q.ms);
+            scalar impulse = -q.mm * Cdot;
+            const scalar oldImpulse = q.mi;
+            const scalar maxImpulse = q.friction * normalImpulse;
+            q.mi = Clamp(q.mi + impulse, -maxImpulse, maxImpulse);
+            impulse = q.mi - oldImpulse;
+            const scalar P = impulse * q.n;
+            w = w + L.sign * QMatVec(q.i0, q.i1, q.i2, P);
+        }
+    }
+}
+
+// Writes the impulses a sweep accumulated (every lane stores the same values).
+__device__ __forceinline__ void QStoreImpulses(const QCon& q, VelCon* dst) {
+    scalar* f = reinterpret_cast<scalar*>(dst);
+    f[kVcImpulses] = q.tix;
+    f[kVcImpulses + 1] = q.tiy;
+    f[kVcImpulses + 2] = q.mi;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k)
+        if (k < q.pc) f[kVcPoints + k * kVcPointStride + kVcpNormalImpulse] = q.pi[k];
+}
+
+// One contact of a sweep on the group's slab (6 floats per body: v, w).
+template <bool kSolve>
+__device__ __forceinline__ void QStep(QCon& cur, QCon& ahead, const VelCon* next, VelCon* self, scalar* slab,
+                                      const QuadLane& L) {
+    BX_GROUP_SYNC();  // the previous contact's velocities are in the slab
+    const u32 b = (L.side ? cur.ib : cur.ia) * 6u + L.cc;
+    scalar v = slab[b], w = slab[b + 3];
+    QLoad(ahead, next, L);
+    if (kSolve) QSolve(cur, v, w, L);
+    else QWarmStart(cur, v, w, L);
+    if (L.c < 3) {
+        slab[b] = v;
+        slab[b + 3] = w;
+    }
+    if (kSolve) QStoreImpulses(cur, self);
+}
+
+template <bool kSolve>
+__device__ __forceinline__ void QSweep(VelCon* vcs, u32 nc, scalar* slab, const QuadLane& L) {
+    QCon a, b;
+    QLoad(a, vcs, L);
+    for (u32 k = 0;;) {
+        QStep<kSolve>(a, b, vcs + Min(k + 1, nc - 1), vcs + k, slab, L);
+        if (++k == nc) break;
+        QStep<kSolve>(b, a, vcs + Min(k + 1, nc - 1), vcs + k, slab, L);
+        if (++k == nc) break;
+    }
+}
+
+// kGroups islands per 64-lane workgroup, eight lanes each; velocities of islands of at most kSlab
+// bodies in LDS.
+template <u32 kGroups, u32 kSlab>
+__global__ void __launch_bounds__(64) k_solve_velocity_quad(BatchView B, StepParams sp) {
+    static_assert(kGroups * 8 <= 64, "eight lanes per island");
+    __shared__ scalar slabs[kGroups][kSlab * 6];
+    const u32 lane = threadIdx.x;
+    const u32 g = lane >> 3;
+    if (g >= kGroups) return;
+    QuadLane L;
+    L.side = (lane >> 2) & 1u;
+    L.c = lane & 3u;
+    L.cc = L.c < 3 ? L.c : 2u;
+    L.cn = L.cc == 2 ? 0u : L.cc + 1;
+    L.cnn = L.cn == 2 ? 0u : L.cn + 1;
+    L.sign = L.side ? scalar(1) : scalar(-1);
+    const u32 inGroup = lane & 7u;
+    scalar* slab = slabs[g];
+    const u32 count = *B.islandListCount;
+    for (u32 i = blockIdx.x * kGroups + g; i < count; i += gridDim.x * kGroups) {
+        const uint2 e = B.islandList[i];
+        BX_WORLD_HOOK(B, e.x);
+        const WorldInfo wi = B.worlds[e.x];
+        const WorldState& ws = B.wstate[e.x];
+        const bool warm = (wi.flags & 0x2u) == 0;
+        const u32 first = e.y == kNull ? 0u : e.y;
+        const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
+        for (u32 k = first; k < last; ++k) {
+            const IslandInfo isl = B.islands[wi.bodyBase + k];
+            VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;
+            VelCon* vcs = B.velCons + wi.conBase + isl.contactStart;
+            JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
+            const u32 nc = isl.contactCount, nj = isl.jointCount;
+            if (isl.bodyCount > kSlab) {  // the serial sweeps on the velocities in global memory
+                if (inGroup == 0)
+                    VelocitySweeps<false>(vcs, nc, jcs, nj, gvel, sp.velocityIterations, warm, sp.dt);
+                continue;
+            }
+            const scalar* gf = reinterpret_cast<const scalar*>(gvel);
+            BX_GROUP_SYNC();  // the previous island's velocities have left the slab
+            for (u32 t = inGroup; t < isl.bodyCount * 6u; t += 8) slab[t] = gf[t];
+            BX_GROUP_SYNC();
+            VelBody* vb = reinterpret_cast<VelBody*>(slab);
+            // island.cpp:154-181: contact warm start, joint warm start, then per iteration all
+            // joints followed by all contacts
+            if (warm && nc > 0) QSweep<false>(vcs, nc, slab, L);
+            if (warm && nj > 0 && inGroup == 0)
+                for (u32 j = 0; j < nj; ++j) WarmStartJoint(jcs[j], vb);
+            for (u32 it = 0; it < sp.velocityIterations && (nc > 0 || nj > 0); ++it) {
+                if (nj > 0 && inGroup == 0)
+                    for (u32 j = 0; j < nj; ++j) SolveJointVelocity(jcs[j], vb, sp.dt);
+                if (nc > 0) QSweep<true>(vcs, nc, slab, L);
+            }
+            BX_GROUP_SYNC();
+            scalar* gout = reinterpret_cast<scalar*>(gvel);
+            for (u32 t = inGroup; t < isl.bodyCount * 6u; t += 8) gout[t] = slab[t];
+        }
+    }
+}
+

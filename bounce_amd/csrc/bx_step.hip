@@ -407,6 +407,7 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
 
 #include "bx_islands.h"
 #include "bx_solve_kernels.h"
+#include "bx_solve_quad.h"
 
 // ================================================================== fixture sync + clear forces
 __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
@@ -626,9 +627,11 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         if (B.meshWorlds) hipLaunchKernelGGL(k_islands<true>, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
         else hipLaunchKernelGGL(k_islands<false>, grid, dim3(kBlock), B.islandLdsWords * sizeof(u32), stream, B);
         mark(3);
-        // one lane per island; islands <= bodies of the launched worlds
+        // islands <= bodies of the launched worlds
         static_assert(kSolveLayoutLanes[0] == 16 && kSolveLayoutSlab[0] == 256 && kSolveLayoutLanes[1] == 32 &&
-                          kSolveLayoutSlab[1] == 128 && kSolveLayoutLanes[2] == 64 && kSolveLayoutSlab[2] == 64,
+                          kSolveLayoutSlab[1] == 128 && kSolveLayoutLanes[2] == 64 && kSolveLayoutSlab[2] == 64 &&
+                          kSolveLayoutLanes[3] == 4 && kSolveLayoutSlab[3] == 320 && kSolveLayoutLanes[4] == 8 &&
+                          kSolveLayoutSlab[4] == 224,
                       "launched layouts");
         const u32 layout = std::min(B.solveLayout, kSolveLayouts - 1);
         const u32 lanes = kSolveLayoutLanes[layout];
@@ -640,7 +643,9 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         mark(5);
         if (layout == 0) hipLaunchKernelGGL((k_solve_velocity<16, 256>), dim3(vblocks), dim3(64), 0, stream, B, sp);
         else if (layout == 1) hipLaunchKernelGGL((k_solve_velocity<32, 128>), dim3(vblocks), dim3(64), 0, stream, B, sp);
-        else hipLaunchKernelGGL((k_solve_velocity<64, 64>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        else if (layout == 2) hipLaunchKernelGGL((k_solve_velocity<64, 64>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        else if (layout == 3) hipLaunchKernelGGL((k_solve_velocity_quad<4, 320>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        else hipLaunchKernelGGL((k_solve_velocity_quad<8, 224>), dim3(vblocks), dim3(64), 0, stream, B, sp);
         mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
         mark(7);

@@ -340,16 +340,18 @@ struct PosCon {
     Vec3 localPointB[kMaxManifoldPoints];
 };
 
-// Layouts of the velocity sweeps (k_solve_velocity): islands per 64-wide workgroup (one lane each)
-// and the largest island whose velocities live in the lane's LDS slab; every layout requests 96 KB
-// of LDS, so one workgroup runs per CU. The layout changes only which lane solves which island,
-// never the order of operations inside an island, so all layouts give identical results. A
-// wavefront's sweep is a serial chain whose instruction count does not depend on how many of its
-// lanes are active: few islands per wavefront spread a small island count over more CUs, many
-// islands per wavefront need fewer wavefronts when islands are many.
-constexpr u32 kSolveLayouts = 3;
-constexpr u32 kSolveLayoutLanes[kSolveLayouts] = {16, 32, 64};
-constexpr u32 kSolveLayoutSlab[kSolveLayouts] = {256, 128, 64};
+// Layouts of the velocity sweeps: islands per 64-wide workgroup and the largest island whose
+// velocities live in LDS. Layouts 0-2 (k_solve_velocity) give each island one lane and request
+// 96 KB of LDS, so one workgroup runs per CU; layouts 3-4 (k_solve_velocity_quad) give each island
+// eight lanes that share each contact's vector arithmetic. The layout changes only which lanes
+// solve which island, never the order of operations inside an island, so all layouts give
+// identical results. A wavefront's sweep is a serial chain whose instruction count does not depend
+// on how many of its lanes are active: few islands per wavefront spread a small island count over
+// more CUs, many islands per wavefront need fewer wavefronts when islands are many.
+constexpr u32 kSolveLayouts = 5;
+constexpr u32 kSolveLayoutLanes[kSolveLayouts] = {16, 32, 64, 4, 8};
+constexpr u32 kSolveLayoutSlab[kSolveLayouts] = {256, 128, 64, 320, 224};
+constexpr bool SolveLayoutQuad(u32 layout) { return layout >= 3; }
 
 // Everything a step kernel needs; all pointers are device pointers.
 struct BatchView {

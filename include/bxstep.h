@@ -255,11 +255,14 @@ typedef struct bx_step_cfg {
 typedef struct bx_batch_opts {
     uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
-    uint32_t solver_mode;              /* velocity-sweep layout, one lane per island: 1, 2, 3 = 16, 32, 64
-                                          islands per wavefront; 0: tuned on the batch's steps 4-9
-                                          (each layout timed twice, order 16 32 64 64 32 16), the
-                                          one with the smallest sample is kept; 16 unless another
-                                          is faster by more than 5% */
+    uint32_t solver_mode;              /* velocity-sweep layout: 1, 2, 3 = 16, 32, 64 islands per
+                                          wavefront, one lane each; 4, 5 = 4, 8 islands per
+                                          wavefront, eight lanes each (the contact's vector
+                                          arithmetic split over components and bodies); 0: tuned
+                                          on the batch's steps 4-13 (each layout timed twice, in
+                                          the order 1 2 3 4 5 5 4 3 2 1), the one with the smallest
+                                          sample is kept; mode 1 unless another is faster by more
+                                          than 5% */
     uint32_t mesh_triangles_per_contact; /* triangles a mesh contact can overlap (0: 64) */
     uint32_t reserved[4];
 } bx_batch_opts;
@@ -270,9 +273,9 @@ typedef struct bx_pool_opts {
     uint32_t max_contacts_per_world;   /* 0: 16 x fixtures + 64 */
     uint32_t max_pairs_per_world;      /* 0: 2 x max_contacts_per_world */
     uint32_t joints_per_1000_bodies;   /* share of the budget kept for joints (0: 50) */
-    uint32_t solver_mode;              /* velocity-sweep layout: 1, 2, 3 = 16, 32, 64 islands per
-                                          wavefront; 0: 16 (pools are not tuned: their world mix
-                                          changes from tick to tick) */
+    uint32_t solver_mode;              /* velocity-sweep layout as bx_batch_opts.solver_mode 1..5;
+                                          0: mode 1 (pools are not tuned: their world mix changes
+                                          from tick to tick) */
     uint32_t reserved[3];
 } bx_pool_opts;
 
@@ -342,7 +345,8 @@ int bx_batch_step_graph(bx_batch* batch, const bx_step_cfg* cfg, uint32_t step_c
 
 /* The velocity-sweep layout in use: islands per wavefront (16 until solver_mode 0 has timed every
    layout), and the time of k_solve_velocity in ms measured for each layout by solver_mode 0 (the
-   smaller of its two samples; 0 before the tuning steps) in tuned_ms[0..count). Returns the number of layouts (3). No counterpart
+   smaller of its two samples; 0 before the tuning steps) in tuned_ms[0..count). Returns the number of layouts (5:
+   16, 32, 64 islands per wavefront with one lane each, 4 and 8 with eight lanes each). No counterpart
    in the reference, whose solver runs one island at a time on the CPU (island.cpp:154-181). */
 int bx_batch_get_solver_layout(bx_batch* batch, uint32_t* islands_per_wavefront, float* tuned_ms, uint32_t count);
 

@@ -56,7 +56,8 @@ void WorldHook(const void* view, uint32_t w) {
 namespace {
 
 const size_t kStackBytes = size_t(128) << 10;  // per workgroup thread; the largest kernel frame is a few KiB
-enum { kReady, kWaiting, kDone };
+enum { kReady, kWaiting, kDone, kWaitGroup };
+const unsigned kGroup = 8;  // threads of a lane group (GroupDpp, GroupSync)
 
 struct Fiber {
     ucontext_t ctx;
@@ -72,6 +73,9 @@ struct Block {
     unsigned alive = 0;
     int orAcc = 0, orResult = 0;
     std::vector<unsigned> xch;
+    // group exchanges: two value buffers alternate by each thread's exchange count, so a thread
+    // that has gone on to the next exchange never overwrites a value its group has yet to read
+    std::vector<unsigned> gx[2], garrive[2], gcount;
     const std::function<void()>* body = nullptr;
     void* schedFake = nullptr;
     const void* schedBottom = nullptr;
@@ -109,6 +113,11 @@ void FiberEntry() {
 void RunBlock(Block& b, const std::function<void()>& body, unsigned n) {
     if (b.fibers.size() < n) b.fibers.resize(n);
     b.xch.assign(n, 0u);
+    for (int p = 0; p < 2; ++p) {
+        b.gx[p].assign(n, 0u);
+        b.garrive[p].assign((n + kGroup - 1) / kGroup, 0u);
+    }
+    b.gcount.assign(n, 0u);
     b.body = &body;
     b.alive = n;
     b.orAcc = b.orResult = 0;
@@ -126,8 +135,10 @@ void RunBlock(Block& b, const std::function<void()>& body, unsigned n) {
     Block* prev = tl_block;
     tl_block = &b;
     while (b.alive > 0) {
+        bool ran = false;
         for (unsigned i = 0; i < n; ++i) {
             if (b.fibers[i].state != kReady) continue;
+            ran = true;
             b.cur = int(i);
             tl_ctx.threadIdx = dim3(i, 0, 0);
 #ifdef BXCPU_ASAN
@@ -139,6 +150,16 @@ void RunBlock(Block& b, const std::function<void()>& body, unsigned n) {
 #endif
         }
         if (b.alive == 0) break;
+        bool ready = false, inGroup = false;
+        for (unsigned i = 0; i < n; ++i) {
+            ready = ready || b.fibers[i].state == kReady;
+            inGroup = inGroup || b.fibers[i].state == kWaitGroup;
+        }
+        if (ready) continue;  // a group exchange woke threads this pass had already passed
+        if (inGroup || !ran) {
+            std::fprintf(stderr, "bxcpu: a lane group waits for threads that never reach its exchange\n");
+            std::abort();
+        }
         // every live thread waits at the barrier: release them (finished threads do not take part,
         // as a wavefront that has ended does not hold a workgroup barrier)
         b.orResult = b.orAcc;
@@ -228,6 +249,54 @@ unsigned ShflUp(unsigned v, unsigned d) {
     Barrier();
     return r;
 }
+
+namespace {
+// Deposits v and returns the eight values of the caller's group once all eight have deposited.
+const unsigned* GroupExchange(unsigned v) {
+    Block* b = tl_block;
+    const unsigned t = unsigned(b->cur), g = t / kGroup;
+    const unsigned par = b->gcount[t]++ & 1u;
+    b->gx[par][t] = v;
+    if (++b->garrive[par][g] == kGroup) {
+        b->garrive[par][g] = 0;
+        for (unsigned i = g * kGroup; i < g * kGroup + kGroup; ++i)
+            if (i != t && b->fibers[i].state == kWaitGroup) b->fibers[i].state = kReady;
+    } else {
+        b->fibers[t].state = kWaitGroup;
+        ToScheduler(false);
+    }
+    return &b->gx[par][g * kGroup];
+}
+}  // namespace
+
+unsigned GroupDpp(unsigned old, unsigned src, int ctrl, int bankMask) {
+    const unsigned* vals = GroupExchange(src);
+    const unsigned t = unsigned(tl_block->cur);
+    const unsigned r = t & 15u;  // lane within its DPP row of 16
+    if (((unsigned(bankMask) >> (r >> 2)) & 1u) == 0) return old;
+    unsigned s;
+    if (ctrl >= 0 && ctrl < 0x100) {
+        s = (t & ~3u) | ((unsigned(ctrl) >> (2 * (t & 3u))) & 3u);
+    } else if (ctrl > 0x100 && ctrl < 0x110) {  // row_shl:n
+        const unsigned n = unsigned(ctrl) - 0x100u;
+        if (r + n >= 16) return old;
+        s = t + n;
+    } else if (ctrl > 0x110 && ctrl < 0x120) {  // row_shr:n
+        const unsigned n = unsigned(ctrl) - 0x110u;
+        if (r < n) return old;
+        s = t - n;
+    } else {
+        std::fprintf(stderr, "bxcpu: DPP control 0x%x is not emulated\n", unsigned(ctrl));
+        std::abort();
+    }
+    if (s / kGroup != t / kGroup) {
+        std::fprintf(stderr, "bxcpu: DPP move of thread %u reads thread %u outside its group\n", t, s);
+        std::abort();
+    }
+    return vals[s % kGroup];
+}
+
+void GroupSync() { (void)GroupExchange(0u); }
 
 void Launch(std::function<void()> body, dim3 grid, dim3 block, size_t dynLds, hipStream_t) {
     if (tl_block) {

@@ -63,6 +63,13 @@ void Launch(std::function<void()> body, dim3 grid, dim3 block, size_t dynLds, hi
 // Called by a kernel when its workgroup (or lane) starts work on world w of the batch view at `view`;
 // the address-checking build allows that world's ranges of every per-world array and no other's.
 void WorldHook(const void* view, uint32_t w);
+// Cross-lane moves inside a group of eight consecutive threads (the velocity sweeps' lane groups):
+// the caller's group must reach the same call together, as a wavefront's lanes execute a DPP move
+// together. `ctrl` is a DPP control word (quad_perm, row_shl:n, row_shr:n); lanes outside `bankMask`
+// keep `old`.
+unsigned GroupDpp(unsigned old, unsigned src, int ctrl, int bankMask);
+// Waits until the caller's group of eight threads has reached this call.
+void GroupSync();
 }  // namespace bxcpu
 
 #define threadIdx (bxcpu::tl_ctx.threadIdx)
@@ -70,6 +77,9 @@ void WorldHook(const void* view, uint32_t w);
 #define blockDim (bxcpu::tl_ctx.blockDim)
 #define gridDim (bxcpu::tl_ctx.gridDim)
 #define BX_WORLD_HOOK(view, w) bxcpu::WorldHook(&(view), (w))
+#define BX_DPP_MOV(src, ctrl) int(bxcpu::GroupDpp(0u, unsigned(src), (ctrl), 0xF))
+#define BX_DPP_UPDATE(old, src, ctrl, bankMask) int(bxcpu::GroupDpp(unsigned(old), unsigned(src), (ctrl), (bankMask)))
+#define BX_GROUP_SYNC() bxcpu::GroupSync()
 
 inline void __syncthreads() { bxcpu::Barrier(); }
 inline int __syncthreads_or(int v) { return bxcpu::BarrierOr(v); }
