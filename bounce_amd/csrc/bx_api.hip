@@ -23,7 +23,7 @@
 namespace bx {
 struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIterations; };
 void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
-                bool findFirst, u32 maxIslands, hipStream_t stream, hipEvent_t* ev);
+                bool findFirst, u32 maxIslands, u32 maxWorldBodies, hipStream_t stream, hipEvent_t* ev);
 void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
 }
 
@@ -1085,17 +1085,18 @@ static int StepList(bx_batch* b, const bx_step_cfg* cfg, const u32* worlds, u32 
     if (rc) return rc;
     u32 *h, *d, r;
     if ((rc = NextListRegion(b, &h, &d, &r))) return rc;
-    u32 maxIslands = 0, lds = 0;
+    u32 maxIslands = 0, maxBodies = 0, lds = 0;
     for (u32 i = 0; i < count; ++i) {
         const WorldInfo& wi = b->worlds[worlds[i]];
         h[i] = worlds[i];
         maxIslands += wi.bodyCount;
+        maxBodies = std::max(maxBodies, wi.bodyCount);
         lds = std::max(lds, std::min(wi.islandScratchWords, kIslandLdsBudgetWords));
     }
     HIP_CHECK(hipMemcpyAsync(d, h, count * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
     BatchView v = b->view;
     v.islandLdsWords = lds;
-    LaunchStep(v, d, 0, count, MakeParams(cfg), false, maxIslands, b->ctx->stream, nullptr);
+    LaunchStep(v, d, 0, count, MakeParams(cfg), false, maxIslands, maxBodies, b->ctx->stream, nullptr);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(b->listEv[r], b->ctx->stream));
     b->listUsed[r] = true;
@@ -1105,10 +1106,11 @@ static int StepList(bx_batch* b, const bx_step_cfg* cfg, const u32* worlds, u32 
 // `steps` steps of every world of a (non-pool) batch, the first kMaxTimedSteps timed per kernel.
 static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
     const StepParams sp = MakeParams(cfg);
-    u32 maxIslands = 0;
+    u32 maxIslands = 0, maxBodies = 0;
     bool allNew = true;
     for (u32 i = 0; i < b->worldCount; ++i) {
         maxIslands += b->worlds[i].bodyCount;
+        maxBodies = std::max(maxBodies, b->worlds[i].bodyCount);
         allNew = allNew && b->needsInitialPairs[i];
     }
     bool findFirst = false;
@@ -1134,7 +1136,7 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
             const u32 layout = kTuneOrder[b->tuneStep - kTuneSkip];
             BatchView v = b->view;
             v.solveLayout = layout;
-            LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+            LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, maxBodies, b->ctx->stream, ev);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipEventSynchronize(ev[6]));
             float ms = 0.0f;
@@ -1150,7 +1152,7 @@ static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
             }
         } else {
             if (b->tuneStep < kTuneSkip && sp.dt > scalar(0)) ++b->tuneStep;
-            LaunchStep(b->view, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, b->ctx->stream, ev);
+            LaunchStep(b->view, nullptr, 0, b->worldCount, sp, findFirst, maxIslands, maxBodies, b->ctx->stream, ev);
         }
         findFirst = false;
     }
@@ -1180,8 +1182,11 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
         if (rc || k == steps) return rc;
         steps -= k;
     }
-    u32 maxIslands = 0;
-    for (u32 i = 0; i < b->worldCount; ++i) maxIslands += b->worlds[i].bodyCount;
+    u32 maxIslands = 0, maxBodies = 0;
+    for (u32 i = 0; i < b->worldCount; ++i) {
+        maxIslands += b->worlds[i].bodyCount;
+        maxBodies = std::max(maxBodies, b->worlds[i].bodyCount);
+    }
     if (b->newContacts) {
         int rc = RunInitialPairs(b, nullptr, 0);
         if (rc) return rc;
@@ -1195,7 +1200,7 @@ static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
         hipStream_t st = b->ctx->stream;
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        LaunchStep(b->view, nullptr, 0, b->worldCount, sp, false, maxIslands, st, nullptr);
+        LaunchStep(b->view, nullptr, 0, b->worldCount, sp, false, maxIslands, maxBodies, st, nullptr);
         hipError_t launch = hipGetLastError();
         hipError_t e = hipStreamEndCapture(st, &b->graph);
         if (launch != hipSuccess || e != hipSuccess) {
@@ -1642,9 +1647,11 @@ int bx_batch_step_noalloc(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_cou
     HIP_CHECK_S(hipSetDevice(b->ctx->device));
     if (step_count == 0 || b->worldCount == 0) return BX_OK;
     u64 bodies = 0;
+    u32 maxBodies = 0;
     bool allNew = true;
     for (u32 w = 0; w < b->worldCount; ++w) {
         bodies += b->worlds[w].bodyCount;
+        maxBodies = std::max(maxBodies, b->worlds[w].bodyCount);
         allNew = allNew && b->needsInitialPairs[w];
     }
     if (bodies > kNoAllocIslands)
@@ -1666,7 +1673,7 @@ int bx_batch_step_noalloc(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_cou
     v.islandListCount = reinterpret_cast<u32*>(b->ctx->scratch + kNoAllocListBytes);  // reset by the contact update
     const StepParams sp = MakeParams(cfg);
     for (u32 s = 0; s < step_count; ++s) {
-        LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, u32(bodies), b->ctx->stream, nullptr);
+        LaunchStep(v, nullptr, 0, b->worldCount, sp, findFirst, u32(bodies), maxBodies, b->ctx->stream, nullptr);
         findFirst = false;
     }
     HIP_CHECK_S(hipGetLastError());

@@ -166,10 +166,18 @@ __device__ __forceinline__ void VelocitySweeps(VelCon* vcs, u32 nc, JointCon* jc
     }
 }
 
+// Bodies of the largest island of an island-list entry (islands first..last of one world).
+__device__ __forceinline__ u32 EntryLargestIsland(const BatchView& B, const WorldInfo& wi, u32 first, u32 last) {
+    u32 most = 0;
+    for (u32 k = first; k < last; ++k) most = Max(most, B.islands[wi.bodyBase + k].bodyCount);
+    return most;
+}
+
 // kLanes islands per workgroup of one wavefront (its first kLanes lanes), each island's velocities
-// in an LDS slab of kSlab bodies when they fit.
+// in an LDS slab of kSlab bodies when they fit. Only entries whose largest island has more than
+// minBodies bodies are solved (0: every entry; the eight-lane layouts leave larger islands here).
 template <u32 kLanes, u32 kSlab>
-__global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams sp) {
+__global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams sp, u32 minBodies) {
     static_assert(kLanes <= 64 && kLanes * kSlab * sizeof(VelBody) <= 96 * 1024, "velocity sweep layout");
     __shared__ VelBody slabs[kLanes][kSlab];
     const u32 slot = threadIdx.x;
@@ -183,6 +191,7 @@ __global__ void __launch_bounds__(64) k_solve_velocity(BatchView B, StepParams s
         const bool warm = (wi.flags & 0x2u) == 0;
         const u32 first = e.y == kNull ? 0u : e.y;
         const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
+        if (minBodies && EntryLargestIsland(B, wi, first, last) <= minBodies) continue;
         for (u32 k = first; k < last; ++k) {
             const IslandInfo isl = B.islands[wi.bodyBase + k];
             VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;

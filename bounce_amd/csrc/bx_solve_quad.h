@@ -16,8 +16,10 @@
 // a - b is evaluated as a + (-b), which IEEE defines identically (body A's updates use the negated
 // inverse mass, and -1 * X for the inertia term, both exact negations).
 //
-// Body velocities live in an LDS slab per island (kSlab bodies); an island that does not fit, and
-// every joint row, is solved by the group's first lane with the serial code of bx_solve_kernels.h.
+// Body velocities live in an LDS slab per island (kSlab bodies); every joint row is solved by the
+// group's first lane with the serial code of bx_solve_kernels.h. Islands that do not fit the slab
+// (and the serial worlds' entries holding one) are left to k_solve_velocity, launched after this
+// kernel for those islands only, so that this kernel's register count is that of its own path.
 #pragma once
 // (included by bx_step.hip inside namespace bx, after bx_solve_kernels.h)
 
@@ -249,7 +251,8 @@ __device__ __forceinline__ void QStoreImpulses(const QCon& q, VelCon* dst) {
         if (k < q.pc) f[kVcPoints + k * kVcPointStride + kVcpNormalImpulse] = q.pi[k];
 }
 
-// One contact of a sweep on the group's slab (6 floats per body: v, w).
+// One contact of a sweep on the group's slab (6 floats per body: v, w); the constraint at `next` is
+// requested into `ahead`.
 template <bool kSolve>
 __device__ __forceinline__ void QStep(QCon& cur, QCon& ahead, const VelCon* next, VelCon* self, scalar* slab,
                                       const QuadLane& L) {
@@ -268,12 +271,16 @@ __device__ __forceinline__ void QStep(QCon& cur, QCon& ahead, const VelCon* next
 
 template <bool kSolve>
 __device__ __forceinline__ void QSweep(VelCon* vcs, u32 nc, scalar* slab, const QuadLane& L) {
-    QCon a, b;
+    // three register buffers rotate: the constraint two contacts ahead is requested while one is solved
+    QCon a, b, c;
     QLoad(a, vcs, L);
+    QLoad(b, vcs + Min(1u, nc - 1), L);
     for (u32 k = 0;;) {
-        QStep<kSolve>(a, b, vcs + Min(k + 1, nc - 1), vcs + k, slab, L);
+        QStep<kSolve>(a, c, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
         if (++k == nc) break;
-        QStep<kSolve>(b, a, vcs + Min(k + 1, nc - 1), vcs + k, slab, L);
+        QStep<kSolve>(b, a, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
+        if (++k == nc) break;
+        QStep<kSolve>(c, b, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
         if (++k == nc) break;
     }
 }
@@ -305,17 +312,13 @@ __global__ void __launch_bounds__(64) k_solve_velocity_quad(BatchView B, StepPar
         const bool warm = (wi.flags & 0x2u) == 0;
         const u32 first = e.y == kNull ? 0u : e.y;
         const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
+        if (EntryLargestIsland(B, wi, first, last) > kSlab) continue;  // k_solve_velocity's
         for (u32 k = first; k < last; ++k) {
             const IslandInfo isl = B.islands[wi.bodyBase + k];
             VelBody* gvel = B.velBodies + wi.islandBodyBase + isl.bodyStart;
             VelCon* vcs = B.velCons + wi.conBase + isl.contactStart;
             JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
             const u32 nc = isl.contactCount, nj = isl.jointCount;
-            if (isl.bodyCount > kSlab) {  // the serial sweeps on the velocities in global memory
-                if (inGroup == 0)
-                    VelocitySweeps<false>(vcs, nc, jcs, nj, gvel, sp.velocityIterations, warm, sp.dt);
-                continue;
-            }
             const scalar* gf = reinterpret_cast<const scalar*>(gvel);
             BX_GROUP_SYNC();  // the previous island's velocities have left the slab
             for (u32 t = inGroup; t < isl.bodyCount * 6u; t += 8) slab[t] = gf[t];

@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(kBlock) k_find_pairs(BatchView B, u32 rebuildA
 // Launches one step of worldCount worlds: worlds firstWorld.. when worldList is null, else the
 // worlds listed in the device array worldList (any subset of the batch, in any order).
 void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
-                bool findFirst, u32 maxIslands, hipStream_t stream, hipEvent_t* ev) {
+                bool findFirst, u32 maxIslands, u32 maxWorldBodies, hipStream_t stream, hipEvent_t* ev) {
     BatchView B = view;
     B.worldList = worldList;
     if (!worldList) {
@@ -641,11 +641,16 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         mark(4);
         hipLaunchKernelGGL(k_init_contacts, grid, dim3(kBlock), 0, stream, B, sp);
         mark(5);
-        if (layout == 0) hipLaunchKernelGGL((k_solve_velocity<16, 256>), dim3(vblocks), dim3(64), 0, stream, B, sp);
-        else if (layout == 1) hipLaunchKernelGGL((k_solve_velocity<32, 128>), dim3(vblocks), dim3(64), 0, stream, B, sp);
-        else if (layout == 2) hipLaunchKernelGGL((k_solve_velocity<64, 64>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        if (layout == 0) hipLaunchKernelGGL((k_solve_velocity<16, 256>), dim3(vblocks), dim3(64), 0, stream, B, sp, 0u);
+        else if (layout == 1) hipLaunchKernelGGL((k_solve_velocity<32, 128>), dim3(vblocks), dim3(64), 0, stream, B, sp, 0u);
+        else if (layout == 2) hipLaunchKernelGGL((k_solve_velocity<64, 64>), dim3(vblocks), dim3(64), 0, stream, B, sp, 0u);
         else if (layout == 3) hipLaunchKernelGGL((k_solve_velocity_quad<4, 320>), dim3(vblocks), dim3(64), 0, stream, B, sp);
         else hipLaunchKernelGGL((k_solve_velocity_quad<8, 224>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        // islands above the eight-lane layouts' slab, when a launched world is large enough to hold one
+        if (SolveLayoutQuad(layout) && maxWorldBodies > kSolveLayoutSlab[layout]) {
+            const u32 big = std::max(1u, std::min((maxIslands + 15) / 16, 16384u));
+            hipLaunchKernelGGL((k_solve_velocity<16, 256>), dim3(big), dim3(64), 0, stream, B, sp, kSolveLayoutSlab[layout]);
+        }
         mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
         mark(7);

@@ -25,12 +25,13 @@ def main():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--steps", type=int, default=60)
     p.add_argument("--copy-mib", type=int, default=256)
+    p.add_argument("--solver-mode", type=int, default=0, help="bx_batch_opts.solver_mode (fix it to profile one layout)")
     a = p.parse_args()
     from bounce_amd import device, scenes
 
     scene = scenes.pyramids(a.worlds, levels=a.levels, seed=2)
     ctx = device.Context(0)
-    batch = device.Batch(ctx, scene)
+    batch = device.Batch(ctx, scene, solver_mode=a.solver_mode)
     for _ in range(a.warmup):
         batch.step()
     batch.sync()
@@ -55,6 +56,7 @@ def main():
     if copy_us < 0:
         raise RuntimeError(f"calibration copy failed ({copy_us})")
     print(json.dumps({"warmup": a.warmup, "steps": a.steps, "worlds": a.worlds, "levels": a.levels,
+                      "solver_mode": a.solver_mode,
                       "units": units, "kernel_ms_events": kernel_ms, "copy_bytes_each_way": n,
                       "copy_us": copy_us}), flush=True)
     return 0

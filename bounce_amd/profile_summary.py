@@ -30,7 +30,8 @@ STEP_KERNELS = ["k_update_contacts", "k_islands", "k_integrate_vel", "k_init_con
 def short(name):
     m = re.search(r"bx::(k_\w+)(?:<[^>]*>)?\(", name)
     if m:
-        return m.group(1)
+        # the velocity sweeps' eight-lane kernel fills the velocity-sweep slot of the step
+        return "k_solve_velocity" if m.group(1) == "k_solve_velocity_quad" else m.group(1)
     m = re.search(r"(k_copy|k_fill)\(", name)
     return m.group(1) if m else name
 
