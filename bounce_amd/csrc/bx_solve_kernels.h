@@ -287,7 +287,9 @@ __device__ __forceinline__ bool PositionSweeps(const PosCon* pcs, u32 nc, const 
     return false;
 }
 
-__global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams sp) {
+// A lane per island; only entries whose largest island has more than minBodies bodies (0: every
+// entry; the eight-lane layouts leave larger islands here).
+__global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams sp, u32 minBodies) {
     if (threadIdx.x >= kPosLanes) return;
     const u32 count = *B.islandListCount;
     for (u32 i = blockIdx.x * kPosLanes + threadIdx.x; i < count; i += gridDim.x * kPosLanes) {
@@ -297,6 +299,7 @@ __global__ void __launch_bounds__(64) k_solve_position(BatchView B, StepParams s
         const WorldState& ws = B.wstate[e.x];
         const u32 first = e.y == kNull ? 0u : e.y;
         const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
+        if (minBodies && EntryLargestIsland(B, wi, first, last) <= minBodies) continue;
         for (u32 k = first; k < last; ++k) {
             IslandInfo& isl = B.islands[wi.bodyBase + k];
             isl.positionsSolved = PositionSweeps(B.posCons + wi.conBase + isl.contactStart, isl.contactCount,

@@ -341,3 +341,245 @@ __global__ void __launch_bounds__(64) k_solve_velocity_quad(BatchView B, StepPar
     }
 }
 
+
+// ---- Position sweeps with eight lanes per island (contact_solver.cpp:517-624) ----
+//
+// The NGS position solve of one contact point rotates the local center, normal and points by the
+// bodies' quaternions, solves one scalar impulse, moves the centers, updates the quaternions by
+// their derivative, renormalizes them and rotates the local inverse inertia into the world frame
+// again. The same eight lanes hold each body's vectors by component; a quaternion's vector part sits
+// on lanes 0-2 of the body's quad and its scalar part on lane 3. Each lane evaluates the serial
+// solver's operations for its component in the same order, the inverse inertia's row c on lane c.
+// The island's poses (PosBody: x, q, world inverse inertia) live in an LDS slab; joints are solved
+// by the group's first lane with the serial code.
+
+constexpr int kDppBcastW = 0xFF;  // quad_perm [3,3,3,3]: the quaternion's scalar part
+
+// PosCon float offsets.
+constexpr u32 kPcInvMass = offsetof(PosCon, invMassA) / 4, kPcRadius = offsetof(PosCon, radiusA) / 4;
+constexpr u32 kPcCenterA = offsetof(PosCon, localCenterA) / 4, kPcCenterB = offsetof(PosCon, localCenterB) / 4;
+constexpr u32 kPcInvIA = offsetof(PosCon, localInvIA) / 4, kPcInvIB = offsetof(PosCon, localInvIB) / 4;
+constexpr u32 kPcNormalA = offsetof(PosCon, localNormalA) / 4;
+constexpr u32 kPcPointA = offsetof(PosCon, localPointA) / 4, kPcPointB = offsetof(PosCon, localPointB) / 4;
+static_assert(offsetof(PosCon, invMassB) == offsetof(PosCon, invMassA) + 4 &&
+                  offsetof(PosCon, radiusB) == offsetof(PosCon, radiusA) + 4,
+              "PosCon field order the quad loader assumes");
+// PosBody float offsets: x 0-2, q.v 3-5, q.s 6, inverse inertia columns 7-9, 10-12, 13-15.
+static_assert(sizeof(PosBody) == 64 && offsetof(PosBody, q) == 12 && offsetof(PosBody, I) == 28,
+              "PosBody layout the position slab assumes");
+
+// One lane's share of a PosCon.
+struct PCon {
+    u32 ia, ib, pc;
+    scalar mA, mB, radA, radB;
+    scalar m;                 // the lane's body's inverse mass, negated on body A
+    scalar lc, lcn, lcnn;     // the body's local center at components cc, cn, cnn
+    scalar L[9];              // the body's local inverse inertia (columns x, y, z)
+    // per point: body A's lanes rotate the local normal, body B's its local point (components cc,
+    // cn, cnn), then both quads rotate body A's local point
+    scalar v2[kMaxManifoldPoints], v2n[kMaxManifoldPoints], v2nn[kMaxManifoldPoints];
+    scalar v3[kMaxManifoldPoints], v3n[kMaxManifoldPoints], v3nn[kMaxManifoldPoints];
+};
+
+__device__ __forceinline__ void PLoad(PCon& p, const PosCon* src, const QuadLane& L) {
+    const scalar* f = reinterpret_cast<const scalar*>(src);
+    const u32* u = reinterpret_cast<const u32*>(src);
+    p.ia = u[0];
+    p.ib = u[1];
+    p.pc = u[2];
+    p.mA = f[kPcInvMass];
+    p.mB = f[kPcInvMass + 1];
+    p.radA = f[kPcRadius];
+    p.radB = f[kPcRadius + 1];
+    p.m = L.side ? p.mB : -p.mA;
+    const u32 lc = L.side ? kPcCenterB : kPcCenterA;
+    p.lc = f[lc + L.cc];
+    p.lcn = f[lc + L.cn];
+    p.lcnn = f[lc + L.cnn];
+    const u32 I = L.side ? kPcInvIB : kPcInvIA;
+#pragma unroll
+    for (u32 j = 0; j < 9; ++j) p.L[j] = f[I + j];
+    const u32 v2 = L.side ? kPcPointB : kPcNormalA;
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        p.v2[k] = f[v2 + 3 * k + L.cc];
+        p.v2n[k] = f[v2 + 3 * k + L.cn];
+        p.v2nn[k] = f[v2 + 3 * k + L.cnn];
+        p.v3[k] = f[kPcPointA + 3 * k + L.cc];
+        p.v3n[k] = f[kPcPointA + 3 * k + L.cn];
+        p.v3nn[k] = f[kPcPointA + 3 * k + L.cnn];
+    }
+}
+
+// component c of Mul(q, v) = v + q.s * t + Cross(q.v, t), t = 2 * Cross(q.v, v) (quat.h:270-274), for
+// a lane-constant v (components cc, cn, cnn); q holds the quaternion (q.v on lanes 0-2, q.s on 3)
+__device__ __forceinline__ scalar QRotate(scalar q, scalar qs, scalar v, scalar vn, scalar vnn) {
+    const scalar t = scalar(2) * QCrossVar(q, vn, vnn);
+    return (v + qs * t) + QCrossVar(q, QNext(t), QNext2(t));
+}
+
+// Lane c's row of RotateToFrame(L, q) = R * L * Transpose(R), R = q.GetRotationMatrix() (quat.h,
+// mat33.h): i0, i1, i2 = columns x, y, z of the result at component c.
+__device__ __forceinline__ void QRotateToFrame(const scalar* L, scalar q, const QuadLane& Ln, scalar& i0, scalar& i1,
+                                               scalar& i2) {
+    const scalar x = Dpp<kDppBcastX>(q), y = Dpp<kDppBcastY>(q), z = Dpp<kDppBcastZ>(q), w = Dpp<kDppBcastW>(q);
+    const scalar x2 = x + x, y2 = y + y, z2 = z + z;
+    const scalar xx = x * x2, xy = x * y2, xz = x * z2;
+    const scalar yy = y * y2, yz = y * z2, zz = z * z2;
+    const scalar wx = w * x2, wy = w * y2, wz = w * z2;
+    // R's columns
+    const scalar Rx0 = scalar(1) - (yy + zz), Rx1 = xy + wz, Rx2 = xz - wy;
+    const scalar Ry0 = xy - wz, Ry1 = scalar(1) - (xx + zz), Ry2 = yz + wx;
+    const scalar Rz0 = xz + wy, Rz1 = yz - wx, Rz2 = scalar(1) - (xx + yy);
+    const u32 c = Ln.cc;
+    const scalar rx = c == 0 ? Rx0 : (c == 1 ? Rx1 : Rx2);
+    const scalar ry = c == 0 ? Ry0 : (c == 1 ? Ry1 : Ry2);
+    const scalar rz = c == 0 ? Rz0 : (c == 1 ? Rz1 : Rz2);
+    // (R * L) column j at component c: (L_j.x * R.x + L_j.y * R.y) + L_j.z * R.z
+    const scalar a0 = (L[0] * rx + L[1] * ry) + L[2] * rz;
+    const scalar a1 = (L[3] * rx + L[4] * ry) + L[5] * rz;
+    const scalar a2 = (L[6] * rx + L[7] * ry) + L[8] * rz;
+    // (R L) * Transpose(R) column k at component c: (R.x[k] * RL.x + R.y[k] * RL.y) + R.z[k] * RL.z
+    i0 = (Rx0 * a0 + Ry0 * a1) + Rz0 * a2;
+    i1 = (Rx1 * a0 + Ry1 * a1) + Rz1 * a2;
+    i2 = (Rx2 * a0 + Ry2 * a1) + Rz2 * a2;
+}
+
+// One contact of a position sweep (contact_solver.cpp:524-621); x, q, i0-i2 are the lane's share of
+// its body's pose; returns the running minimum separation (the same on every lane).
+__device__ __forceinline__ scalar QSolvePosition(const PCon& p, scalar& x, scalar& q, scalar& i0, scalar& i1,
+                                                 scalar& i2, scalar minSeparation, const QuadLane& L) {
+#pragma unroll
+    for (u32 k = 0; k < kMaxManifoldPoints; ++k) {
+        if (k >= p.pc) break;
+        const scalar qs = Dpp<kDppBcastW>(q);
+        const scalar xt = x - QRotate(q, qs, p.lc, p.lcn, p.lcnn);  // xfA / xfB translations
+        const scalar r2 = QRotate(q, qs, p.v2[k], p.v2n[k], p.v2nn[k]);  // A: normal; B: Mul(qB, pointB)
+        const scalar c1 = QRotate(q, qs, p.v3[k], p.v3n[k], p.v3nn[k]) + xt;  // A: Mul(xfA, pointA)
+        const scalar c2 = r2 + xt;                                  // B: Mul(xfB, pointB)
+        const scalar normal = FromA(r2);
+        // separation = Dot(c2 - c1, normal) - radiusA - radiusB, on the B lanes
+        const scalar separation = ToA((QSum((c2 - FromA(c1)) * normal) - p.radA) - p.radB);
+        minSeparation = Min(minSeparation, separation);
+        const scalar C = Clamp(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, scalar(0));
+        const scalar r = ToA(c2) - x;                               // rA = point - cA, rB = point - cB
+        const scalar rn = QCrossVar(r, QNext(normal), QNext2(normal));
+        const scalar d = QSum(rn * QMatVec(i0, i1, i2, rn));       // Dot(rn, I * rn) of the lane's body
+        const scalar K = ((p.mA + p.mB) + FromA(d)) + ToA(d);
+        const scalar impulse = K > scalar(0) ? -C / K : scalar(0);
+        const scalar P = impulse * normal;
+        x = x + p.m * P;                                            // cA -= mA * P, cB += mB * P
+        // q -/+= Derivative(q, I * Cross(r, P)) = (0.5 * Quat(omega, 0)) * q, the product
+        // Quat(Cross(a.v, q.v) + a.s * q.v + q.s * a.v, a.s * q.s - Dot(a.v, q.v)) with a = (0.5 omega, 0.5 * 0)
+        const scalar av = scalar(0.5) * QMatVec(i0, i1, i2, QCrossVar(r, QNext(P), QNext2(P)));
+        const scalar as = scalar(0.5) * scalar(0);
+        const scalar dv = (QCrossVar(av, QNext(q), QNext2(q)) + as * q) + qs * av;
+        const scalar ds = as * q - QSum(av * q);
+        q = q + L.sign * (L.c < 3 ? dv : ds);
+        // q.Normalize(): the vector part multiplies by the reciprocal, the scalar part divides
+        const scalar qq = q * q;
+        const scalar len = Sqrt(QSum(qq) + Dpp<kDppBcastW>(qq));
+        if (len > kEpsilon) {
+            const scalar inv = scalar(1) / len;
+            q = L.c < 3 ? q * inv : q / len;
+        }
+        QRotateToFrame(p.L, q, L, i0, i1, i2);
+    }
+    return minSeparation;
+}
+
+// One contact of a position sweep on the group's slab (16 floats per body); the constraint at `next`
+// is requested into `ahead`.
+__device__ __forceinline__ scalar PStep(const PCon& cur, PCon& ahead, const PosCon* next, scalar* slab,
+                                        scalar minSeparation, const QuadLane& L) {
+    BX_GROUP_SYNC();  // the previous contact's poses are in the slab
+    const u32 b = (L.side ? cur.ib : cur.ia) * 16u;
+    scalar x = slab[b + L.cc], q = slab[b + 3 + L.c];
+    scalar i0 = slab[b + 7 + L.cc], i1 = slab[b + 10 + L.cc], i2 = slab[b + 13 + L.cc];
+    PLoad(ahead, next, L);
+    minSeparation = QSolvePosition(cur, x, q, i0, i1, i2, minSeparation, L);
+    if (L.c < 3) {
+        slab[b + L.c] = x;
+        slab[b + 7 + L.c] = i0;
+        slab[b + 10 + L.c] = i1;
+        slab[b + 13 + L.c] = i2;
+    }
+    slab[b + 3 + L.c] = q;
+    return minSeparation;
+}
+
+// kGroups islands per 64-lane workgroup, eight lanes each; poses of islands of at most kSlab bodies in
+// LDS (larger islands, and serial-world entries holding one, are k_solve_position's).
+template <u32 kGroups, u32 kSlab>
+__global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepParams sp) {
+    static_assert(kGroups * 8 <= 64, "eight lanes per island");
+    __shared__ scalar slabs[kGroups][kSlab * 16];
+    __shared__ u32 jointFlags[kGroups];
+    const u32 lane = threadIdx.x;
+    const u32 g = lane >> 3;
+    if (g >= kGroups) return;
+    QuadLane L;
+    L.side = (lane >> 2) & 1u;
+    L.c = lane & 3u;
+    L.cc = L.c < 3 ? L.c : 2u;
+    L.cn = L.cc == 2 ? 0u : L.cc + 1;
+    L.cnn = L.cn == 2 ? 0u : L.cn + 1;
+    L.sign = L.side ? scalar(1) : scalar(-1);
+    const u32 inGroup = lane & 7u;
+    scalar* slab = slabs[g];
+    const u32 count = *B.islandListCount;
+    for (u32 i = blockIdx.x * kGroups + g; i < count; i += gridDim.x * kGroups) {
+        const uint2 e = B.islandList[i];
+        BX_WORLD_HOOK(B, e.x);
+        const WorldInfo wi = B.worlds[e.x];
+        const WorldState& ws = B.wstate[e.x];
+        const u32 first = e.y == kNull ? 0u : e.y;
+        const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
+        if (EntryLargestIsland(B, wi, first, last) > kSlab) continue;  // k_solve_position's
+        for (u32 k = first; k < last; ++k) {
+            IslandInfo& isl = B.islands[wi.bodyBase + k];
+            PosBody* gpos = B.posBodies + wi.islandBodyBase + isl.bodyStart;
+            const PosCon* pcs = B.posCons + wi.conBase + isl.contactStart;
+            const JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
+            const u32 nc = isl.contactCount, nj = isl.jointCount;
+            BX_GROUP_SYNC();  // the previous island's poses have left the slab
+            const scalar* gf = reinterpret_cast<const scalar*>(gpos);
+            for (u32 t = inGroup; t < isl.bodyCount * 16u; t += 8) slab[t] = gf[t];
+            BX_GROUP_SYNC();
+            PosBody* pb = reinterpret_cast<PosBody*>(slab);
+            // island.cpp:231-248 / PositionSweeps: contacts, then joints, until both are solved
+            bool solved = false;
+            for (u32 it = 0; it < sp.positionIterations; ++it) {
+                scalar minSeparation = 0;
+                if (nc > 0) {
+                    PCon a, b;
+                    PLoad(a, pcs, L);
+                    for (u32 c = 0;;) {
+                        minSeparation = PStep(a, b, pcs + Min(c + 1, nc - 1), slab, minSeparation, L);
+                        if (++c == nc) break;
+                        minSeparation = PStep(b, a, pcs + Min(c + 1, nc - 1), slab, minSeparation, L);
+                        if (++c == nc) break;
+                    }
+                }
+                const bool contactsSolved = minSeparation >= scalar(-3) * kLinearSlop;
+                BX_GROUP_SYNC();  // the contacts' poses are in the slab
+                if (inGroup == 0) {
+                    bool jointsSolved = true;  // joint_solver.cpp:55-64
+                    for (u32 j = 0; j < nj; ++j) {
+                        const bool ok = SolveJointPosition(jcs[j], pb);
+                        jointsSolved = jointsSolved && ok;
+                    }
+                    jointFlags[g] = jointsSolved ? 1u : 0u;
+                }
+                BX_GROUP_SYNC();  // the joints' poses and flag are in LDS
+                if (contactsSolved && jointFlags[g] != 0) {
+                    solved = true;
+                    break;
+                }
+            }
+            if (inGroup == 0) isl.positionsSolved = solved ? 1u : 0u;
+            scalar* gout = reinterpret_cast<scalar*>(gpos);
+            for (u32 t = inGroup; t < isl.bodyCount * 16u; t += 8) gout[t] = slab[t];
+        }
+    }
+}

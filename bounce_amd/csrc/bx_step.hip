@@ -654,7 +654,15 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         mark(6);
         hipLaunchKernelGGL(k_mid, grid, dim3(kBlock), 0, stream, B, sp);
         mark(7);
-        hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), kPosSpreadLds, stream, B, sp);
+        if (layout == 3) {
+            hipLaunchKernelGGL((k_solve_position_quad<4, kPosQuadSlab>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        } else if (layout == 4) {
+            hipLaunchKernelGGL((k_solve_position_quad<8, kPosQuadSlab>), dim3(vblocks), dim3(64), 0, stream, B, sp);
+        }
+        if (!SolveLayoutQuad(layout))
+            hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), kPosSpreadLds, stream, B, sp, 0u);
+        else if (maxWorldBodies > kPosQuadSlab)  // islands above the eight-lane layouts' pose slab
+            hipLaunchKernelGGL(k_solve_position, dim3(pblocks), dim3(64), kPosSpreadLds, stream, B, sp, kPosQuadSlab);
         mark(8);
         hipLaunchKernelGGL(k_finish, grid, dim3(kBlock), 0, stream, B, sp);
         mark(9);
