@@ -350,8 +350,8 @@ __global__ void __launch_bounds__(64) k_solve_velocity_quad(BatchView B, StepPar
 // again. The same eight lanes hold each body's vectors by component; a quaternion's vector part sits
 // on lanes 0-2 of the body's quad and its scalar part on lane 3. Each lane evaluates the serial
 // solver's operations for its component in the same order, the inverse inertia's row c on lane c.
-// The island's poses (PosBody: x, q, world inverse inertia) live in an LDS slab; joints are solved
-// by the group's first lane with the serial code.
+// The island's poses (x, q) live in an LDS slab; joints are solved by the group's first lane with the
+// serial code.
 
 constexpr int kDppBcastW = 0xFF;  // quad_perm [3,3,3,3]: the quaternion's scalar part
 
@@ -488,24 +488,71 @@ __device__ __forceinline__ scalar QSolvePosition(const PCon& p, scalar& x, scala
     return minSeparation;
 }
 
-// One contact of a position sweep on the group's slab (16 floats per body); the constraint at `next`
-// is requested into `ahead`.
-__device__ __forceinline__ scalar PStep(const PCon& cur, PCon& ahead, const PosCon* next, scalar* slab,
-                                        scalar minSeparation, const QuadLane& L) {
-    BX_GROUP_SYNC();  // the previous contact's poses are in the slab
-    const u32 b = (L.side ? cur.ib : cur.ia) * 16u;
-    scalar x = slab[b + L.cc], q = slab[b + 3 + L.c];
-    scalar i0 = slab[b + 7 + L.cc], i1 = slab[b + 10 + L.cc], i2 = slab[b + 13 + L.cc];
-    PLoad(ahead, next, L);
-    minSeparation = QSolvePosition(cur, x, q, i0, i1, i2, minSeparation, L);
-    if (L.c < 3) {
-        slab[b + L.c] = x;
-        slab[b + 7 + L.c] = i0;
-        slab[b + 10 + L.c] = i1;
-        slab[b + 13 + L.c] = i2;
+// The pose slab holds 8 words per body: x (0-2), q (3-6) and a flag (7). The world inverse inertia is
+// not stored: after any position update of a body it is RotateToFrame(invI, q) with the body's own
+// local inverse inertia (contact_solver.cpp:580-621, revolute_joint.cpp), and before the first one it
+// is that too for every body the position integration moved (island.cpp:190-229 sets it so); only a
+// static body starts from its stored matrix, which the flag marks until the body is first updated.
+constexpr u32 kPoseWords = 8;
+constexpr u32 kPoseFromGlobal = 1;  // flag: the inverse inertia is the island's stored one
+
+// The lane's row of the body's world inverse inertia at the start of a contact.
+__device__ __forceinline__ void PoseInertia(const scalar* L, scalar q, u32 flag, const PosBody* gpos, u32 body,
+                                            const QuadLane& Ln, scalar& i0, scalar& i1, scalar& i2) {
+    QRotateToFrame(L, q, Ln, i0, i1, i2);
+    if (flag & kPoseFromGlobal) {
+        const Mat33& I = gpos[body].I;
+        i0 = I.x[Ln.cc];
+        i1 = I.y[Ln.cc];
+        i2 = I.z[Ln.cc];
     }
+}
+
+// One contact of a position sweep on the group's slab; the constraint at `next` is requested into
+// `ahead`.
+__device__ __forceinline__ scalar PStep(const PCon& cur, PCon& ahead, const PosCon* next, scalar* slab,
+                                        const PosBody* gpos, scalar minSeparation, const QuadLane& L) {
+    BX_GROUP_SYNC();  // the previous contact's poses are in the slab
+    const u32 body = L.side ? cur.ib : cur.ia;
+    const u32 b = body * kPoseWords;
+    scalar x = slab[b + L.cc], q = slab[b + 3 + L.c];
+    const u32 flag = __builtin_bit_cast(u32, slab[b + 7]);
+    PLoad(ahead, next, L);
+    scalar i0, i1, i2;
+    PoseInertia(cur.L, q, flag, gpos, body, L, i0, i1, i2);
+    minSeparation = QSolvePosition(cur, x, q, i0, i1, i2, minSeparation, L);
+    if (L.c < 3) slab[b + L.c] = x;
     slab[b + 3 + L.c] = q;
+    if (L.c == 0 && cur.pc > 0) slab[b + 7] = __builtin_bit_cast(scalar, 0u);  // updated: I follows q now
     return minSeparation;
+}
+
+// A pose of the slab as the serial joint code reads it.
+struct SlabPose {
+    Vec3 x;
+    Quat q;
+    Mat33 I;
+};
+
+__device__ __forceinline__ SlabPose LoadPose(const scalar* slab, const PosBody* gpos, u32 body, const Mat33& invI) {
+    const scalar* r = slab + body * kPoseWords;
+    SlabPose p;
+    p.x = Vec3(r[0], r[1], r[2]);
+    p.q = Quat(Vec3(r[3], r[4], r[5]), r[6]);
+    p.I = (__builtin_bit_cast(u32, r[7]) & kPoseFromGlobal) ? gpos[body].I : RotateToFrame(invI, p.q);
+    return p;
+}
+
+__device__ __forceinline__ void StorePose(scalar* slab, u32 body, const SlabPose& p) {
+    scalar* r = slab + body * kPoseWords;
+    r[0] = p.x.x;
+    r[1] = p.x.y;
+    r[2] = p.x.z;
+    r[3] = p.q.v.x;
+    r[4] = p.q.v.y;
+    r[5] = p.q.v.z;
+    r[6] = p.q.s;
+    r[7] = __builtin_bit_cast(scalar, 0u);
 }
 
 // kGroups islands per 64-lane workgroup, eight lanes each; poses of islands of at most kSlab bodies in
@@ -513,7 +560,7 @@ __device__ __forceinline__ scalar PStep(const PCon& cur, PCon& ahead, const PosC
 template <u32 kGroups, u32 kSlab>
 __global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepParams sp) {
     static_assert(kGroups * 8 <= 64, "eight lanes per island");
-    __shared__ scalar slabs[kGroups][kSlab * 16];
+    __shared__ scalar slabs[kGroups][kSlab * kPoseWords];
     __shared__ u32 jointFlags[kGroups];
     const u32 lane = threadIdx.x;
     const u32 g = lane >> 3;
@@ -536,17 +583,28 @@ __global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepPar
         const u32 first = e.y == kNull ? 0u : e.y;
         const u32 last = e.y == kNull ? ws.islandCount : e.y + 1;
         if (EntryLargestIsland(B, wi, first, last) > kSlab) continue;  // k_solve_position's
+        const Body* bodies = B.bodies + wi.bodyBase;
         for (u32 k = first; k < last; ++k) {
             IslandInfo& isl = B.islands[wi.bodyBase + k];
             PosBody* gpos = B.posBodies + wi.islandBodyBase + isl.bodyStart;
+            const u32* ib = B.islandBodies + wi.islandBodyBase + isl.bodyStart;
             const PosCon* pcs = B.posCons + wi.conBase + isl.contactStart;
             const JointCon* jcs = B.jointCons + wi.jointBase + isl.jointStart;
             const u32 nc = isl.contactCount, nj = isl.jointCount;
             BX_GROUP_SYNC();  // the previous island's poses have left the slab
-            const scalar* gf = reinterpret_cast<const scalar*>(gpos);
-            for (u32 t = inGroup; t < isl.bodyCount * 16u; t += 8) slab[t] = gf[t];
+            for (u32 t = inGroup; t < isl.bodyCount; t += 8) {
+                const PosBody& p = gpos[t];
+                scalar* r = slab + t * kPoseWords;
+                r[0] = p.x.x;
+                r[1] = p.x.y;
+                r[2] = p.x.z;
+                r[3] = p.q.v.x;
+                r[4] = p.q.v.y;
+                r[5] = p.q.v.z;
+                r[6] = p.q.s;
+                r[7] = __builtin_bit_cast(scalar, bodies[ib[t]].type == kStatic ? kPoseFromGlobal : 0u);
+            }
             BX_GROUP_SYNC();
-            PosBody* pb = reinterpret_cast<PosBody*>(slab);
             // island.cpp:231-248 / PositionSweeps: contacts, then joints, until both are solved
             bool solved = false;
             for (u32 it = 0; it < sp.positionIterations; ++it) {
@@ -555,9 +613,9 @@ __global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepPar
                     PCon a, b;
                     PLoad(a, pcs, L);
                     for (u32 c = 0;;) {
-                        minSeparation = PStep(a, b, pcs + Min(c + 1, nc - 1), slab, minSeparation, L);
+                        minSeparation = PStep(a, b, pcs + Min(c + 1, nc - 1), slab, gpos, minSeparation, L);
                         if (++c == nc) break;
-                        minSeparation = PStep(b, a, pcs + Min(c + 1, nc - 1), slab, minSeparation, L);
+                        minSeparation = PStep(b, a, pcs + Min(c + 1, nc - 1), slab, gpos, minSeparation, L);
                         if (++c == nc) break;
                     }
                 }
@@ -566,7 +624,15 @@ __global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepPar
                 if (inGroup == 0) {
                     bool jointsSolved = true;  // joint_solver.cpp:55-64
                     for (u32 j = 0; j < nj; ++j) {
-                        const bool ok = SolveJointPosition(jcs[j], pb);
+                        JointCon jc = jcs[j];
+                        SlabPose two[2] = {LoadPose(slab, gpos, jc.indexA, jc.localInvIA),
+                                           LoadPose(slab, gpos, jc.indexB, jc.localInvIB)};
+                        const u32 a = jc.indexA, bb = jc.indexB;
+                        jc.indexA = 0;
+                        jc.indexB = 1;
+                        const bool ok = SolveJointPosition(jc, two);
+                        StorePose(slab, a, two[0]);
+                        StorePose(slab, bb, two[1]);
                         jointsSolved = jointsSolved && ok;
                     }
                     jointFlags[g] = jointsSolved ? 1u : 0u;
@@ -578,8 +644,13 @@ __global__ void __launch_bounds__(64) k_solve_position_quad(BatchView B, StepPar
                 }
             }
             if (inGroup == 0) isl.positionsSolved = solved ? 1u : 0u;
-            scalar* gout = reinterpret_cast<scalar*>(gpos);
-            for (u32 t = inGroup; t < isl.bodyCount * 16u; t += 8) gout[t] = slab[t];
+            for (u32 t = inGroup; t < isl.bodyCount; t += 8) {
+                const scalar* r = slab + t * kPoseWords;
+                PosBody& p = gpos[t];
+                p.x = Vec3(r[0], r[1], r[2]);
+                p.q = Quat(Vec3(r[3], r[4], r[5]), r[6]);
+                if ((__builtin_bit_cast(u32, r[7]) & kPoseFromGlobal) == 0) p.I = RotateToFrame(bodies[ib[t]].invI, p.q);
+            }
         }
     }
 }

@@ -352,7 +352,7 @@ constexpr u32 kSolveLayouts = 5;
 constexpr u32 kSolveLayoutLanes[kSolveLayouts] = {16, 32, 64, 4, 8};
 constexpr u32 kSolveLayoutSlab[kSolveLayouts] = {256, 128, 64, 320, 224};
 constexpr bool SolveLayoutQuad(u32 layout) { return layout >= 3; }
-// The eight-lane layouts solve positions the same way (k_solve_position_quad), with a 64-byte pose
+// The eight-lane layouts solve positions the same way (k_solve_position_quad), with a 32-byte pose
 // per body in LDS for islands of at most this many bodies.
 constexpr u32 kPosQuadSlab = 224;
 
