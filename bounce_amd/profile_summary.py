@@ -30,8 +30,9 @@ STEP_KERNELS = ["k_update_contacts", "k_islands", "k_integrate_vel", "k_init_con
 def short(name):
     m = re.search(r"bx::(k_\w+)(?:<[^>]*>)?\(", name)
     if m:
-        # the velocity sweeps' eight-lane kernel fills the velocity-sweep slot of the step
-        return "k_solve_velocity" if m.group(1) == "k_solve_velocity_quad" else m.group(1)
+        # the eight-lane sweep kernels fill the velocity- and position-sweep slots of the step
+        return {"k_solve_velocity_quad": "k_solve_velocity",
+                "k_solve_position_quad": "k_solve_position"}.get(m.group(1), m.group(1))
     m = re.search(r"(k_copy|k_fill)\(", name)
     return m.group(1) if m else name
 
