@@ -114,6 +114,18 @@ def body_edits(rows):
     return out
 
 
+class bx_ray_query(C.Structure):
+    _fields_ = [("world", u32), ("mask", u32), ("p1", _arr(f32, 3)), ("p2", _arr(f32, 3))]
+
+
+class bx_ray_hit(C.Structure):
+    _fields_ = [("fixture", u32), ("fraction", f32), ("point", _arr(f32, 3)), ("normal", _arr(f32, 3))]
+
+
+class bx_aabb_query(C.Structure):
+    _fields_ = [("world", u32), ("mask", u32), ("lower", _arr(f32, 3)), ("upper", _arr(f32, 3))]
+
+
 class bx_manifold_point_state(C.Structure):
     _fields_ = [("local_normal1", _arr(f32, 3)), ("local_point1", _arr(f32, 3)),
                 ("local_point2", _arr(f32, 3)), ("normal_impulse", f32), ("triangle_key", u32),

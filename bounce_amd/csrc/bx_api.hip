@@ -9,6 +9,7 @@
 // takes one more for the new parent; moving a proxy frees and retakes the same parent, so ids
 // are fixed at creation (dynamic_tree.cpp:44-106).
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <map>
@@ -25,6 +26,8 @@ struct StepParams { scalar dt, inv_dt; u32 velocityIterations, positionIteration
 void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32 worldCount, const StepParams& sp,
                 bool findFirst, u32 maxIslands, u32 maxWorldBodies, hipStream_t stream, hipEvent_t* ev);
 void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
+void LaunchBuildBvh(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
+void LaunchQueries(const BatchView& view, const QueryParams& qp, hipStream_t stream);
 }
 
 using namespace bx;
@@ -160,6 +163,7 @@ static const ArraySpec kArrays[] = {
     BX_ARRAY(fixtures, kSpaceFixture, 1), BX_ARRAY(fixtureList, kSpaceFixture, 1),
     BX_ARRAY(proxyOrder, kSpaceFixture, 1), BX_ARRAY(fatAABB, kSpaceFixture, 1), BX_ARRAY(moved, kSpaceFixture, 1),
     BX_ARRAY(edgeOffset, kSpaceFixture, 1), BX_ARRAY(pairCounts, kSpaceFixture, 1),
+    BX_ARRAY(bvhLeaves, kSpaceFixture, 1), BX_ARRAY(bvhNodes, kSpaceFixture, 1),
     BX_ARRAY(joints, kSpaceJoint, 1), BX_ARRAY(islandJoints, kSpaceJoint, 1), BX_ARRAY(jointCons, kSpaceJoint, 1),
     BX_ARRAY(jointEdges, kSpaceJoint, 2),
     BX_ARRAY(contacts, kSpaceContact, 1), BX_ARRAY(contactOrder, kSpaceContact, 1),
@@ -288,6 +292,12 @@ struct bx_batch {
     // time of each layout (the smaller of its two samples)
     u32 tuneStep = kTuneSteps;
     float tuneMs[kSolveLayouts] = {};
+    // world queries: the state generation (advanced by every call that moves fat AABBs), the
+    // generation each world's query hierarchy was built at, and the device buffer of the queries
+    u64 stateGen = 1;
+    std::vector<u64> bvhGen;
+    char* queryDev = nullptr;
+    size_t queryBytes = 0;
 };
 
 static bool IsResident(const bx_batch* b, u32 world) { return world < b->worldCount && b->resident[world]; }
@@ -978,6 +988,7 @@ static int PutAsync(bx_batch* b, T* dev, u64 at, const std::vector<T>& host) {
 // Writes a built world into its ranges (bases already in hw.info) and its slot; every other
 // per-world array of the world is cleared first, since a pool reuses ranges.
 static int PlaceWorld(bx_batch* b, u32 slot, const HostWorld& hw) {
+    ++b->stateGen;  // the slot's fat AABBs change
     const WorldInfo& wi = hw.info;
     BatchView& V = b->view;
     hipStream_t st = b->ctx->stream;
@@ -1018,6 +1029,7 @@ static void InitSlots(bx_batch* b, u32 slots) {
     b->proxyIds.assign(slots, {});
     b->resident.assign(slots, 0);
     b->needsInitialPairs.assign(slots, 0);
+    b->bvhGen.assign(slots, 0);
 }
 
 
@@ -1073,6 +1085,7 @@ static int RunInitialPairs(bx_batch* b, const u32* worlds, u32 count) {
 // One step of the listed worlds in one launch sequence (continuous batching: any subset of the
 // resident worlds, each block of the one-workgroup-per-world kernels stepping one listed world).
 static int StepList(bx_batch* b, const bx_step_cfg* cfg, const u32* worlds, u32 count) {
+    ++b->stateGen;
     if (count == 0) return BX_OK;
     if (count > b->worldCount) return Fail(BX_E_INVALID, "more worlds listed than the batch holds");
     std::vector<u8> seen(b->worldCount, 0);
@@ -1105,6 +1118,7 @@ static int StepList(bx_batch* b, const bx_step_cfg* cfg, const u32* worlds, u32 
 
 // `steps` steps of every world of a (non-pool) batch, the first kMaxTimedSteps timed per kernel.
 static int StepAll(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
+    ++b->stateGen;
     const StepParams sp = MakeParams(cfg);
     u32 maxIslands = 0, maxBodies = 0;
     bool allNew = true;
@@ -1175,6 +1189,7 @@ static void DropGraph(bx_batch* b) {
 // own. The step's kernels write no memory through copies or fills, so the graph has kernel nodes
 // only.
 static int StepGraph(bx_batch* b, const bx_step_cfg* cfg, u32 steps) {
+    ++b->stateGen;
     const StepParams sp = MakeParams(cfg);
     if (b->tuneStep < kTuneSteps && sp.dt > scalar(0)) {  // solver_mode 0: time the layouts first
         const u32 k = std::min(steps, kTuneSteps - b->tuneStep);
@@ -1271,6 +1286,7 @@ void bx_batch_destroy(bx_batch* b) {
         if (e) hipEventDestroy(e);
     DropGraph(b);
     if (b->listHost) hipHostFree(b->listHost);
+    if (b->queryDev) hipFree(b->queryDev);
     delete b->library;
     delete b;
 }
@@ -1578,6 +1594,7 @@ int bx_batch_snapshot(bx_batch* b, uint32_t world, void* out, uint64_t capacity,
 }
 
 int bx_pool_restore(bx_batch* b, const void* snapshot, uint64_t size, uint32_t* slot) {
+    if (b) ++b->stateGen;  // a restored world's fat AABBs replace the slot's
     if (!b || !snapshot || !slot) return Fail(BX_E_INVALID, "null argument");
     if (!b->pool) return Fail(BX_E_INVALID, "not a pool");
     SnapshotHeader h;
@@ -1668,6 +1685,7 @@ int bx_batch_step_noalloc(bx_batch* b, const bx_step_cfg* cfg, uint32_t step_cou
             if (rc) return rc;
         }
     }
+    ++b->stateGen;
     BatchView v = b->view;
     v.islandList = reinterpret_cast<uint2*>(b->ctx->scratch);
     v.islandListCount = reinterpret_cast<u32*>(b->ctx->scratch + kNoAllocListBytes);  // reset by the contact update
@@ -2040,6 +2058,164 @@ int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, ui
     return UploadBodies(b, wi, bodies);
 }
 
+// ------------------------------------------------------------------ world queries
+// Builds the query hierarchy (k_build_bvh) of every listed world whose hierarchy predates the
+// batch's current state.
+static int EnsureBvh(bx_batch* b, const std::vector<u32>& worlds) {
+    std::vector<u32> need;
+    for (u32 w : worlds) {
+        if (b->bvhGen[w] == b->stateGen) continue;
+        b->bvhGen[w] = b->stateGen;  // also marks a world listed twice
+        need.push_back(w);
+    }
+    if (need.empty()) return BX_OK;
+    u32 *h, *d, r;
+    int rc = NextListRegion(b, &h, &d, &r);
+    if (rc) return rc;
+    std::memcpy(h, need.data(), need.size() * sizeof(u32));
+    HIP_CHECK(hipMemcpyAsync(d, h, need.size() * sizeof(u32), hipMemcpyHostToDevice, b->ctx->stream));
+    LaunchBuildBvh(b->view, d, u32(need.size()), b->ctx->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(b->listEv[r], b->ctx->stream));
+    b->listUsed[r] = true;
+    return BX_OK;
+}
+
+static size_t Align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Runs `count` queries of one kind (0: single ray, 1: every ray hit, 2: boxes) given as world, mask
+// offset and six coordinates each; outputs go to hits (kinds 0, 1) or fixtures (kind 2) and counts.
+static int RunQueries(bx_batch* b, u32 kind, const u32* world, const u32* maskOff, const float* coords, u32 count,
+                      const uint8_t* maskBytes, u64 maskSize, u32 perQuery, bx_ray_hit* hits, uint32_t* fixtures,
+                      uint32_t* counts) {
+    if (count == 0) return BX_OK;
+    if ((kind != 0 && !counts) || (kind == 0 && !hits) || (kind == 1 && !hits && perQuery) ||
+        (kind == 2 && !fixtures && perQuery))
+        return Fail(BX_E_INVALID, "null output");
+    std::vector<u32> worlds(count);
+    for (u32 i = 0; i < count; ++i) {
+        const u32 w = world[i];
+        if (!IsResident(b, w)) return Fail(BX_E_INVALID, "query names a world that is not resident");
+        if (maskOff[i] != kNull) {
+            if (!maskBytes || u64(maskOff[i]) + b->worlds[w].fixtureCount > maskSize)
+                return Fail(BX_E_INVALID, "query mask lies outside the mask bytes");
+        }
+        for (u32 k = 0; k < 6; ++k)
+            if (!std::isfinite(coords[6 * u64(i) + k])) return Fail(BX_E_INVALID, "query coordinates are not finite");
+        worlds[i] = w;
+    }
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    int rc = EnsureBvh(b, worlds);
+    if (rc) return rc;
+    // device layout: query headers, coordinates, mask bytes, outputs, counts
+    const size_t perOut = kind == 2 ? sizeof(u32) : sizeof(RayHitOut);
+    const size_t outCount = kind == 0 ? size_t(count) : size_t(count) * perQuery;
+    const size_t oQ = 0, oC = Align256(oQ + size_t(count) * sizeof(uint4));
+    const size_t oM = Align256(oC + size_t(count) * 6 * sizeof(float));
+    const size_t oO = Align256(oM + (maskBytes ? size_t(maskSize) : 0));
+    const size_t oN = Align256(oO + std::max<size_t>(1, outCount) * perOut);
+    const size_t total = Align256(oN + size_t(count) * sizeof(u32));
+    if (total > b->queryBytes) {
+        HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+        if (b->queryDev) HIP_CHECK(hipFree(b->queryDev));
+        b->queryDev = nullptr;
+        b->queryBytes = 0;
+        void* p = nullptr;
+        HIP_CHECK(hipMalloc(&p, total));
+        b->queryDev = static_cast<char*>(p);
+        b->queryBytes = total;
+    }
+    std::vector<uint4> heads(count);
+    for (u32 i = 0; i < count; ++i) heads[i] = make_uint4(world[i], maskOff[i], 0u, 0u);
+    hipStream_t st = b->ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(b->queryDev + oQ, heads.data(), heads.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(b->queryDev + oC, coords, size_t(count) * 6 * sizeof(float), hipMemcpyHostToDevice, st));
+    if (maskBytes && maskSize)
+        HIP_CHECK(hipMemcpyAsync(b->queryDev + oM, maskBytes, size_t(maskSize), hipMemcpyHostToDevice, st));
+    QueryParams qp;
+    qp.queries = reinterpret_cast<const uint4*>(b->queryDev + oQ);
+    qp.coords = reinterpret_cast<const float*>(b->queryDev + oC);
+    qp.mask = reinterpret_cast<const uint8_t*>(b->queryDev + oM);
+    qp.count = count;
+    qp.kind = kind;
+    qp.perQuery = perQuery;
+    qp.hits = reinterpret_cast<RayHitOut*>(b->queryDev + oO);
+    qp.fixtures = reinterpret_cast<u32*>(b->queryDev + oO);
+    qp.counts = reinterpret_cast<u32*>(b->queryDev + oN);
+    LaunchQueries(b->view, qp, st);
+    HIP_CHECK(hipGetLastError());
+    if (kind == 2) {
+        if (outCount)
+            HIP_CHECK(hipMemcpyAsync(fixtures, b->queryDev + oO, outCount * sizeof(u32), hipMemcpyDeviceToHost, st));
+    } else if (outCount) {
+        std::vector<RayHitOut> out(outCount);
+        HIP_CHECK(hipMemcpyAsync(out.data(), b->queryDev + oO, outCount * sizeof(RayHitOut), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < outCount; ++i) {
+            hits[i].fixture = out[i].fixture;
+            hits[i].fraction = out[i].fraction;
+            hits[i].point[0] = out[i].point.x;
+            hits[i].point[1] = out[i].point.y;
+            hits[i].point[2] = out[i].point.z;
+            hits[i].normal[0] = out[i].normal.x;
+            hits[i].normal[1] = out[i].normal.y;
+            hits[i].normal[2] = out[i].normal.z;
+        }
+    }
+    if (kind != 0)
+        HIP_CHECK(hipMemcpyAsync(counts, b->queryDev + oN, size_t(count) * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return BX_OK;
+}
+
+// World, mask offset and the six coordinates (p1 p2, or lower upper) of each query record.
+static_assert(offsetof(bx_ray_query, p1) == 8 && offsetof(bx_ray_query, p2) == 20 &&
+                  offsetof(bx_aabb_query, lower) == 8 && offsetof(bx_aabb_query, upper) == 20,
+              "query records: world, mask, then six floats");
+static_assert(sizeof(bx_ray_query) == 32 && sizeof(bx_aabb_query) == 32, "query records are 32 bytes");
+static void SplitQueries(const void* q, u32 count, std::vector<u32>& world, std::vector<u32>& mask,
+                         std::vector<float>& coords) {
+    world.resize(count);
+    mask.resize(count);
+    coords.resize(size_t(count) * 6);
+    const char* p = static_cast<const char*>(q);
+    for (u32 i = 0; i < count; ++i, p += 32) {
+        std::memcpy(&world[i], p, sizeof(u32));
+        std::memcpy(&mask[i], p + 4, sizeof(u32));
+        std::memcpy(&coords[6 * size_t(i)], p + 8, 6 * sizeof(float));
+    }
+}
+
+int bx_batch_raycast_single(bx_batch* b, const bx_ray_query* rays, uint32_t count, const uint8_t* mask_bytes,
+                            uint64_t mask_size, bx_ray_hit* out) {
+    if (!b || (!rays && count) || (!out && count)) return Fail(BX_E_INVALID, "null argument");
+    std::vector<u32> world, mask;
+    std::vector<float> coords;
+    SplitQueries(rays, count, world, mask, coords);
+    return RunQueries(b, 0, world.data(), mask.data(), coords.data(), count, mask_bytes, mask_size, 1, out, nullptr,
+                      nullptr);
+}
+
+int bx_batch_raycast(bx_batch* b, const bx_ray_query* rays, uint32_t count, const uint8_t* mask_bytes,
+                     uint64_t mask_size, bx_ray_hit* hits, uint32_t hits_per_ray, uint32_t* hit_counts) {
+    if (!b || (!rays && count)) return Fail(BX_E_INVALID, "null argument");
+    std::vector<u32> world, mask;
+    std::vector<float> coords;
+    SplitQueries(rays, count, world, mask, coords);
+    return RunQueries(b, 1, world.data(), mask.data(), coords.data(), count, mask_bytes, mask_size, hits_per_ray, hits,
+                      nullptr, hit_counts);
+}
+
+int bx_batch_query_aabb(bx_batch* b, const bx_aabb_query* boxes, uint32_t count, const uint8_t* mask_bytes,
+                        uint64_t mask_size, uint32_t* fixtures, uint32_t per_query, uint32_t* counts) {
+    if (!b || (!boxes && count)) return Fail(BX_E_INVALID, "null argument");
+    std::vector<u32> world, mask;
+    std::vector<float> coords;
+    SplitQueries(boxes, count, world, mask, coords);
+    return RunQueries(b, 2, world.data(), mask.data(), coords.data(), count, mask_bytes, mask_size, per_query, nullptr,
+                      fixtures, counts);
+}
+
 int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits, uint32_t count) {
     if (!b || !IsResident(b, world) || (!edits && count)) return Fail(BX_E_INVALID, "bad argument or world not resident");
     HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -2052,6 +2228,7 @@ int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits,
         transform = transform || edits[k].op == BX_EDIT_TRANSFORM;
     }
     if (count == 0) return BX_OK;
+    if (transform) ++b->stateGen;  // SetTransform moves fat AABBs
     std::vector<Body> bodies;
     int rc = Download(b, bodies, b->view.bodies + wi.bodyBase, wi.bodyCount);
     if (rc) return rc;
@@ -2197,7 +2374,8 @@ uint32_t bx_struct_size(const char* name) {
 #define S(T) if (n == #T) return uint32_t(sizeof(T));
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
-    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event)
+    S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event) S(bx_ray_query)
+    S(bx_ray_hit) S(bx_aabb_query)
 #undef S
     return 0;
 }

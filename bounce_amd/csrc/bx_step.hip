@@ -408,6 +408,7 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
 #include "bx_islands.h"
 #include "bx_solve_kernels.h"
 #include "bx_solve_quad.h"
+#include "bx_query.h"
 
 // ================================================================== fixture sync + clear forces
 __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
@@ -676,6 +677,20 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
         mark(10);
         mark(11);
     }
+}
+
+// The query hierarchy of the worlds listed in the device array worldList (k_build_bvh).
+void LaunchBuildBvh(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream) {
+    BatchView B = view;
+    B.worldList = worldList;
+    B.worldCount = worldCount;
+    hipLaunchKernelGGL(k_build_bvh, dim3(worldCount), dim3(kBlock), 0, stream, B);
+}
+
+// One lane per query (k_query).
+void LaunchQueries(const BatchView& view, const QueryParams& qp, hipStream_t stream) {
+    const u32 blocks = (qp.count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_query, dim3(blocks), dim3(kBlock), 0, stream, view, qp);
 }
 
 // The creation-time pair search (b3World::Step's FindNewContacts after shapes were added) of the

@@ -395,6 +395,9 @@ struct BatchView {
     // pairs
     uint2* pairs;             // per world: pairCap
     u32* pairCounts;          // per fixture (world-local proxy order) scratch
+    // the query hierarchy (bx_query.h), rebuilt after a step before the next query
+    u32* bvhLeaves;           // per fixture: fixture indices sorted by the Morton code of their fat-AABB centre
+    AABB* bvhNodes;           // per fixture: the node levels above them, bottom-up
     // islands
     IslandInfo* islands;      // per world: up to bodyCount islands (base = bodyBase)
     u32* islandBodies;        // per world: islandBodyCap
@@ -416,6 +419,21 @@ struct BatchView {
     u32 islandLdsWords;       // LDS words available to the island builder per world
     u32 meshWorlds;           // nonzero when some world holds a mesh fixture (mesh kernel variants)
     u32 solveLayout;          // layout of the velocity sweeps (index into kSolveLayoutLanes)
+};
+
+// Output of one ray of RayCastSingle / RayCast or one box of QueryAABB, as the C ABI lays it out.
+struct RayHitOut { u32 fixture; scalar fraction; Vec3 point, normal; };
+
+struct QueryParams {
+    const uint4* queries;   // per query: {world, mask offset, -, -} then 6 floats (p1 p2 or lower upper)
+    const float* coords;    // 6 per query
+    const uint8_t* mask;
+    u32 count;
+    u32 kind;               // 0 single ray, 1 all ray hits, 2 boxes
+    u32 perQuery;           // output slots per query (kinds 1 and 2)
+    RayHitOut* hits;        // kinds 0 and 1
+    u32* fixtures;          // kind 2
+    u32* counts;            // kinds 1 and 2
 };
 
 // BX_WORLD_HOOK(view, w) marks where a workgroup or lane starts work on world w; it compiles to

@@ -64,6 +64,12 @@ def load(path=capi.LIB_PATH):
         "bx_pool_get_stats": (C.c_int, [P, C.POINTER(capi.bx_pool_stats)]),
         "bx_batch_snapshot": (C.c_int, [P, u, P, C.c_uint64, C.POINTER(C.c_uint64)]),
         "bx_pool_restore": (C.c_int, [P, P, C.c_uint64, C.POINTER(u)]),
+        "bx_batch_raycast_single": (C.c_int, [P, C.POINTER(capi.bx_ray_query), u, C.c_void_p, C.c_uint64,
+                                              C.POINTER(capi.bx_ray_hit)]),
+        "bx_batch_raycast": (C.c_int, [P, C.POINTER(capi.bx_ray_query), u, C.c_void_p, C.c_uint64,
+                                       C.POINTER(capi.bx_ray_hit), u, C.POINTER(u)]),
+        "bx_batch_query_aabb": (C.c_int, [P, C.POINTER(capi.bx_aabb_query), u, C.c_void_p, C.c_uint64,
+                                          C.POINTER(u), u, C.POINTER(u)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -258,6 +264,56 @@ class Batch:
 
     def device_bytes(self):
         return int(_lib.bx_batch_device_bytes(self.h))
+
+    @staticmethod
+    def _queries(kind, rows):
+        """bx_ray_query / bx_aabb_query records from (world, a[3], b[3], mask_offset) rows."""
+        out = np.zeros(len(rows), dtype=np.dtype(kind))
+        a_name, b_name = out.dtype.names[2], out.dtype.names[3]
+        for i, r in enumerate(rows):
+            out["world"][i] = r[0]
+            out["mask"][i] = capi.BX_NULL_INDEX if len(r) < 4 or r[3] is None else r[3]
+            out[a_name][i] = np.asarray(r[1], dtype=np.float32)
+            out[b_name][i] = np.asarray(r[2], dtype=np.float32)
+        return out
+
+    @staticmethod
+    def _mask_args(mask):
+        if mask is None:
+            return None, 0, None
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        return m.ctypes.data, m.size, m
+
+    def raycast_single(self, rays, mask=None):
+        """b3World::RayCastSingle for each (world, p1, p2[, mask offset]) row: a bx_ray_hit array."""
+        q = self._queries(capi.bx_ray_query, rays)
+        out = np.zeros(len(rays), dtype=np.dtype(capi.bx_ray_hit))
+        mp, ms, _keep = self._mask_args(mask)
+        _check(_lib.bx_batch_raycast_single(self.h, q.ctypes.data_as(C.POINTER(capi.bx_ray_query)), len(rays), mp, ms,
+                                            out.ctypes.data_as(C.POINTER(capi.bx_ray_hit))))
+        return out
+
+    def raycast(self, rays, per_ray=16, mask=None):
+        """b3World::RayCast (a listener returning 1) for each row: (hits [n, per_ray], counts [n])."""
+        q = self._queries(capi.bx_ray_query, rays)
+        hits = np.zeros((len(rays), max(1, per_ray)), dtype=np.dtype(capi.bx_ray_hit))
+        counts = np.zeros(len(rays), dtype=np.uint32)
+        mp, ms, _keep = self._mask_args(mask)
+        _check(_lib.bx_batch_raycast(self.h, q.ctypes.data_as(C.POINTER(capi.bx_ray_query)), len(rays), mp, ms,
+                                     hits.ctypes.data_as(C.POINTER(capi.bx_ray_hit)), per_ray,
+                                     counts.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return hits[:, :per_ray], counts
+
+    def query_aabb(self, boxes, per_query=64, mask=None):
+        """b3World::QueryAABB for each (world, lower, upper[, mask offset]) row: (fixtures, counts)."""
+        q = self._queries(capi.bx_aabb_query, boxes)
+        fx = np.zeros((len(boxes), max(1, per_query)), dtype=np.uint32)
+        counts = np.zeros(len(boxes), dtype=np.uint32)
+        mp, ms, _keep = self._mask_args(mask)
+        _check(_lib.bx_batch_query_aabb(self.h, q.ctypes.data_as(C.POINTER(capi.bx_aabb_query)), len(boxes), mp, ms,
+                                        fx.ctypes.data_as(C.POINTER(C.c_uint32)), per_query,
+                                        counts.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return fx[:, :per_query], counts
 
 
 class PoolFull(BxError):

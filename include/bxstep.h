@@ -465,6 +465,57 @@ typedef struct bx_body_edit {
    as the reference's m_newContacts does. Synchronous. */
 int bx_batch_edit_bodies(bx_batch* batch, uint32_t world, const bx_body_edit* edits, uint32_t count);
 
+/* ---------------------------------------------------------------------------------------------
+ * World queries between steps, many at once over any worlds of a batch or pool: the rays and
+ * boxes of b3World::RayCast, RayCastSingle and QueryAABB (world.h:141-172, world.cpp:452-566,
+ * 852-870) against every fixture's broadphase proxy (its fat AABB) and shape. A ray is a candidate
+ * for a fixture when the fixture's fat AABB passes b3DynamicTree::RayCast's segment test with the
+ * ray's full extent (dynamic_tree.h:201-339); the fixture's shape is then cast as b3Fixture::RayCast
+ * does (sphere_shape.cpp:69-121, capsule_shape.cpp:180-324, hull_shape.cpp:189-281,
+ * triangle_shape.cpp:95-176, mesh_shape.cpp:81-146, the mesh's static tree in stack order). The
+ * device finds candidates with a bounding-volume hierarchy of its own over the fat AABBs, rebuilt
+ * after each step; the reference's dynamic-tree shape does not enter the results, except through
+ * the order of hits at exactly equal fractions (here: lowest fixture index first).
+ * A fixture mask stands for b3RayCastFilter::ShouldRayCast / b3QueryFilter::ShouldReport
+ * (world_callbacks.h): a query whose `mask` is not BX_NULL_INDEX considers fixture f of its world only
+ * when mask_bytes[mask + f] != 0.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_ray_query {
+    uint32_t world;                /* world (pool slot) the ray is cast in */
+    uint32_t mask;                 /* BX_NULL_INDEX or an offset into the call's mask bytes */
+    float p1[3];                   /* b3RayCastInput::p1, p2 (p1 != p2) */
+    float p2[3];
+} bx_ray_query;
+
+typedef struct bx_ray_hit {
+    uint32_t fixture;              /* creation index of the fixture hit, BX_NULL_INDEX for none */
+    float fraction;                /* b3RayCastOutput::fraction */
+    float point[3];                /* (1 - fraction) * p1 + fraction * p2 */
+    float normal[3];               /* b3RayCastOutput::normal */
+} bx_ray_hit;
+
+typedef struct bx_aabb_query {
+    uint32_t world;
+    uint32_t mask;                 /* BX_NULL_INDEX or an offset into the call's mask bytes */
+    float lower[3];                /* b3AABB lowerBound, upperBound */
+    float upper[3];
+} bx_aabb_query;
+
+/* b3World::RayCastSingle for each ray: the hit with the smallest fraction, in out[i] (fixture ==
+   BX_NULL_INDEX when the ray hits nothing). Synchronous. */
+int bx_batch_raycast_single(bx_batch* batch, const bx_ray_query* rays, uint32_t count, const uint8_t* mask_bytes,
+                            uint64_t mask_size, bx_ray_hit* out);
+/* b3World::RayCast with a listener that never clips the ray (ReportFixture returning 1): every hit
+   of ray i in increasing fraction, the first hits_per_ray of them in hits[i * hits_per_ray ..];
+   hit_counts[i] receives the number of hits. Synchronous. */
+int bx_batch_raycast(bx_batch* batch, const bx_ray_query* rays, uint32_t count, const uint8_t* mask_bytes,
+                     uint64_t mask_size, bx_ray_hit* hits, uint32_t hits_per_ray, uint32_t* hit_counts);
+/* b3World::QueryAABB: the fixtures whose fat AABB overlaps box i (b3TestOverlap, aabb.h:347-367),
+   in creation order, the first per_query of them in fixtures[i * per_query ..]; counts[i]
+   receives their number. Synchronous. */
+int bx_batch_query_aabb(bx_batch* batch, const bx_aabb_query* boxes, uint32_t count, const uint8_t* mask_bytes,
+                        uint64_t mask_size, uint32_t* fixtures, uint32_t per_query, uint32_t* counts);
+
 /* Per-phase GPU time per step, in milliseconds, averaged over the steps of the last bx_batch_step
    call on the batch's context (its first 1024 steps), from HIP events recorded on the context
    stream between the kernels: [0] update contacts, [1] islands, [2] island solve, [3] fixture

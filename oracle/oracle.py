@@ -58,6 +58,12 @@ def load(kind):
         "get_proxy_ids": fn("get_proxy_ids", C.c_int, P, C.POINTER(u), u),
         "get_fat_aabbs": fn("get_fat_aabbs", C.c_int, P, C.POINTER(C.c_float), u),
         "get_mass": fn("get_mass", C.c_int, P, C.POINTER(C.c_float), u),
+        "raycast": fn("raycast", C.c_int, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p,
+                      C.POINTER(capi.bx_ray_hit), u, C.POINTER(u)),
+        "raycast_single": fn("raycast_single", C.c_int, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p,
+                             C.POINTER(capi.bx_ray_hit)),
+        "query_aabb": fn("query_aabb", C.c_int, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p,
+                         C.POINTER(u), u, C.POINTER(u)),
         "bench": fn("bench", C.c_double, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc), u,
                     C.c_float, u, u, u, u, u),
     }
@@ -163,6 +169,45 @@ class CpuWorld:
         out = np.zeros((n, 6), dtype=np.float32)
         self.api["get_fat_aabbs"](self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
         return out
+
+    @staticmethod
+    def _vec(v):
+        return (C.c_float * 3)(*[float(x) for x in v])
+
+    @staticmethod
+    def _mask(mask):
+        return None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+
+    def raycast(self, p1, p2, mask=None):
+        """Every hit of the segment p1..p2 (b3World::RayCast with a listener returning 1), as a
+        structured bx_ray_hit array in increasing fraction (ties by fixture index)."""
+        m = self._mask(mask)
+        mp = None if m is None else m.ctypes.data
+        cnt = C.c_uint32(0)
+        self.api["raycast"](self.h, self._vec(p1), self._vec(p2), mp, None, 0, C.byref(cnt))
+        out = np.zeros(cnt.value, dtype=np.dtype(capi.bx_ray_hit))
+        self.api["raycast"](self.h, self._vec(p1), self._vec(p2), mp,
+                            out.ctypes.data_as(C.POINTER(capi.bx_ray_hit)), cnt.value, C.byref(cnt))
+        return out
+
+    def raycast_single(self, p1, p2, mask=None):
+        """b3World::RayCastSingle: one bx_ray_hit record (fixture BX_NULL_INDEX for no hit)."""
+        m = self._mask(mask)
+        out = np.zeros(1, dtype=np.dtype(capi.bx_ray_hit))
+        self.api["raycast_single"](self.h, self._vec(p1), self._vec(p2), None if m is None else m.ctypes.data,
+                                   out.ctypes.data_as(C.POINTER(capi.bx_ray_hit)))
+        return out[0]
+
+    def query_aabb(self, lower, upper, mask=None):
+        """b3World::QueryAABB: fixture indices whose fat AABB overlaps the box, ascending."""
+        m = self._mask(mask)
+        mp = None if m is None else m.ctypes.data
+        cnt = C.c_uint32(0)
+        self.api["query_aabb"](self.h, self._vec(lower), self._vec(upper), mp, None, 0, C.byref(cnt))
+        out = np.zeros(max(1, cnt.value), dtype=np.uint32)
+        self.api["query_aabb"](self.h, self._vec(lower), self._vec(upper), mp,
+                               out.ctypes.data_as(C.POINTER(C.c_uint32)), cnt.value, C.byref(cnt))
+        return out[: cnt.value]
 
     def mass(self):
         n = self.counts().body_count

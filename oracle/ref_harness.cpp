@@ -565,6 +565,88 @@ int BXREF(get_mass)(void* h, float* out, uint32_t cap) {
     return int(n);
 }
 
+// World queries through the reference's own b3World::RayCast / RayCastSingle / QueryAABB. The ray
+// listener never clips (returns 1), so every hit is reported; hits and query results are sorted
+// here (fraction, then fixture creation index) since the dynamic tree's visit order is not part of
+// what is compared. Filters answer from an optional byte per fixture.
+struct MaskRayFilter : public b3RayCastFilter {
+    const uint8_t* mask = nullptr;
+    bool ShouldRayCast(b3Fixture* f) override { return !mask || mask[FixtureIndex(f)] != 0; }
+};
+struct CollectRays : public b3RayCastListener {
+    std::vector<bx_ray_hit> hits;
+    scalar ReportFixture(b3Fixture* f, const b3Vec3& point, const b3Vec3& normal, scalar fraction) override {
+        bx_ray_hit h;
+        h.fixture = FixtureIndex(f);
+        h.fraction = float(fraction);
+        for (int a = 0; a < 3; ++a) {
+            h.point[a] = float(point[a]);
+            h.normal[a] = float(normal[a]);
+        }
+        hits.push_back(h);
+        return scalar(1);
+    }
+};
+struct MaskQueryFilter : public b3QueryFilter {
+    const uint8_t* mask = nullptr;
+    bool ShouldReport(b3Fixture* f) override { return !mask || mask[FixtureIndex(f)] != 0; }
+};
+struct CollectFixtures : public b3QueryListener {
+    std::vector<uint32_t> fixtures;
+    bool ReportFixture(b3Fixture* f) override {
+        fixtures.push_back(FixtureIndex(f));
+        return true;
+    }
+};
+
+int BXREF(raycast)(void* h, const float* p1, const float* p2, const uint8_t* mask, bx_ray_hit* out, uint32_t cap,
+                   uint32_t* count) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    MaskRayFilter filter;
+    filter.mask = mask;
+    CollectRays rays;
+    w->world.RayCast(&rays, &filter, V3(p1), V3(p2));
+    std::stable_sort(rays.hits.begin(), rays.hits.end(), [](const bx_ray_hit& a, const bx_ray_hit& b) {
+        return a.fraction < b.fraction || (a.fraction == b.fraction && a.fixture < b.fixture);
+    });
+    *count = uint32_t(rays.hits.size());
+    for (uint32_t i = 0; i < cap && i < rays.hits.size(); ++i) out[i] = rays.hits[i];
+    return 0;
+}
+
+int BXREF(raycast_single)(void* h, const float* p1, const float* p2, const uint8_t* mask, bx_ray_hit* out) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    MaskRayFilter filter;
+    filter.mask = mask;
+    b3RayCastSingleOutput o;
+    std::memset(out, 0, sizeof(*out));
+    out->fixture = BX_NULL_INDEX;
+    if (!w->world.RayCastSingle(&o, &filter, V3(p1), V3(p2))) return 0;
+    out->fixture = FixtureIndex(o.fixture);
+    out->fraction = float(o.fraction);
+    for (int a = 0; a < 3; ++a) {
+        out->point[a] = float(o.point[a]);
+        out->normal[a] = float(o.normal[a]);
+    }
+    return 1;
+}
+
+int BXREF(query_aabb)(void* h, const float* lower, const float* upper, const uint8_t* mask, uint32_t* out,
+                      uint32_t cap, uint32_t* count) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    MaskQueryFilter filter;
+    filter.mask = mask;
+    CollectFixtures q;
+    b3AABB box;
+    box.lowerBound = V3(lower);
+    box.upperBound = V3(upper);
+    w->world.QueryAABB(&q, &filter, box);
+    std::sort(q.fixtures.begin(), q.fixtures.end());
+    *count = uint32_t(q.fixtures.size());
+    for (uint32_t i = 0; i < cap && i < q.fixtures.size(); ++i) out[i] = q.fixtures[i];
+    return 0;
+}
+
 // b3BoxHull(ex, ey, ez) data in bx_hull_desc layout: 8 vertices, 24 edges, 6 faces.
 void BXREF(box_hull)(float ex, float ey, float ez, float* vertices, uint32_t* edges,
                      uint32_t* faces, float* planes, float* centroid) {

@@ -14,6 +14,9 @@ namespace orc {
 void CreateJoint(World& w, const bx_joint_desc& jd);
 bool BuildMesh(Mesh& m, const bx_mesh_desc& d);
 void InitTriangleShape(Shape& s, const Vec3& a, const Vec3& b, const Vec3& c);
+struct RayHit { u32 fixture; scalar fraction; Vec3 point, normal; };
+void RayCastAll(const World& w, const Vec3& p1, const Vec3& p2, const uint8_t* mask, std::vector<RayHit>& hits);
+void QueryAABB(const World& w, const AABB& aabb, const uint8_t* mask, std::vector<u32>& out);
 }
 
 using namespace orc;
@@ -378,6 +381,49 @@ int orc_get_mass(void* h, float* out, uint32_t cap) {
             for (int r = 0; r < 3; ++r) { o[5 + 3 * c + r] = b.I[c][r]; o[14 + 3 * c + r] = b.invI[c][r]; }
     }
     return int(n);
+}
+
+// World queries (orc_query.cpp), same contract as the reference harness's entries.
+int orc_raycast(void* h, const float* p1, const float* p2, const uint8_t* mask, bx_ray_hit* out, uint32_t cap,
+                uint32_t* count) {
+    World* w = (World*)h;
+    std::vector<RayHit> hits;
+    RayCastAll(*w, V3(p1), V3(p2), mask, hits);
+    *count = u32(hits.size());
+    for (u32 i = 0; i < cap && i < hits.size(); ++i) {
+        out[i].fixture = hits[i].fixture;
+        out[i].fraction = hits[i].fraction;
+        for (int a = 0; a < 3; ++a) {
+            out[i].point[a] = hits[i].point[a];
+            out[i].normal[a] = hits[i].normal[a];
+        }
+    }
+    return 0;
+}
+
+int orc_raycast_single(void* h, const float* p1, const float* p2, const uint8_t* mask, bx_ray_hit* out) {
+    uint32_t n = 0;
+    std::memset(out, 0, sizeof(*out));
+    out->fixture = BX_NULL_INDEX;
+    orc_raycast(h, p1, p2, mask, out, 1, &n);
+    if (n == 0) {
+        out->fixture = BX_NULL_INDEX;
+        return 0;
+    }
+    return 1;
+}
+
+int orc_query_aabb(void* h, const float* lower, const float* upper, const uint8_t* mask, uint32_t* out, uint32_t cap,
+                   uint32_t* count) {
+    World* w = (World*)h;
+    AABB box;
+    box.lowerBound = V3(lower);
+    box.upperBound = V3(upper);
+    std::vector<u32> found;
+    QueryAABB(*w, box, mask, found);
+    *count = u32(found.size());
+    for (u32 i = 0; i < cap && i < found.size(); ++i) out[i] = found[i];
+    return 0;
 }
 
 // CPU baseline with the restatement: same contract as the reference harness's bench entry.

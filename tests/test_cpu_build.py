@@ -104,6 +104,30 @@ print("events and filter equal to the oracle", flush=True)
     assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
 
 
+def test_cpu_build_world_queries(tmp_path):
+    """The device tests of the world queries (tests/test_gpu_queries.py) on the host build under
+    AddressSanitizer with the per-world check: the hierarchy build (LDS and global-memory key sorts),
+    the stackless walk and every shape's ray cast touch only the queried world's ranges, and every
+    result equals the oracle's."""
+    _make("asan")
+    script = tmp_path / "queries.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {os.path.dirname(HERE)!r})
+from bounce_amd import device
+device.load({os.path.join(BUILD, "_out", "libbxstep_asan.so")!r})
+from tests import test_gpu_queries as t
+ctx = device.Context(0)
+for name in dir(t):
+    if name.startswith("test_"):
+        getattr(t, name)(ctx)
+print("queries equal to the oracle", flush=True)
+""")
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
+
+
 @pytest.mark.slow
 def test_cpu_build_memory_sanitizer_mixed_worlds(tmp_path):
     _make("msan")
