@@ -27,6 +27,8 @@ BX_WORLD_NO_WARM_START = 0x2
 BX_WORLD_NO_CLEAR_FORCES = 0x4
 BX_JOINT_REVOLUTE = 4
 BX_NULL_INDEX = 0xFFFFFFFF
+(BX_TOPO_CREATE_BODY, BX_TOPO_CREATE_FIXTURE, BX_TOPO_CREATE_JOINT, BX_TOPO_DESTROY_BODY, BX_TOPO_DESTROY_FIXTURE,
+ BX_TOPO_DESTROY_JOINT) = range(1, 7)
 
 
 def _arr(t, n):
@@ -111,6 +113,18 @@ def body_edits(rows):
         out[i].body, out[i].op, out[i].wake = body, op, int(r[4]) if len(r) > 4 else 1
         out[i].v[:] = [float(x) for x in v]
         out[i].p[:] = [float(x) for x in (list(p) + [0.0] * 4)[:4]]
+    return out
+
+
+class bx_topology_op(C.Structure):
+    _fields_ = [("op", u32), ("index", u32)]
+
+
+def topology_ops(rows):
+    """bx_topology_op records from (op, index) pairs."""
+    out = (bx_topology_op * max(1, len(rows)))()
+    for i, (op, index) in enumerate(rows):
+        out[i].op, out[i].index = op, index
     return out
 
 

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <map>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 #include "../../include/bxstep.h"
@@ -260,6 +261,18 @@ static const u32 kTuneSteps = kTuneSkip + 2 * kSolveLayouts;
 // kept unless another is faster by more than this fraction.
 static const float kTuneMargin = 0.05f;
 
+// b3DynamicTree's node pool as far as it fixes proxy ids (dynamic_tree.cpp:44-136): a LIFO free list
+// over a sequential pool. CreateProxy takes the proxy's node and, when the tree already has a leaf,
+// InsertLeaf takes a parent node; DestroyProxy's RemoveLeaf frees the leaf's parent (when the leaf is
+// not the root) and then the proxy's node. Moving a proxy frees and takes back the same parent, so
+// it leaves the pool unchanged. The ids of freed parents depend on the tree's shape and are not
+// tracked (kNull entries); a proxy always takes a node freed by a proxy or a fresh one.
+struct NodePool {
+    std::vector<u32> freeStack;  // top last
+    u32 nextNode = 0;
+    u32 leafCount = 0;
+};
+
 struct bx_batch {
     bx_context* ctx = nullptr;
     u32 worldCount = 0;  // world slots
@@ -267,6 +280,7 @@ struct bx_batch {
     std::vector<void*> allocations;
     std::vector<WorldInfo> worlds;
     std::vector<std::vector<u32>> proxyIds;  // per world: proxy id of each fixture
+    std::vector<NodePool> nodePools;  // per world: the dynamic tree's node pool (proxy ids of new fixtures)
     std::vector<u8> resident;                // per slot: holds a world
     std::vector<u8> needsInitialPairs;       // per slot: creation-time pair search still to run
     bool newContacts = true;                 // some resident world still needs its initial pair search
@@ -680,8 +694,36 @@ static bool HasMesh(const bx_world_desc& d) {
     return false;
 }
 
-// One world built on the host from its descriptor. Every index in these arrays is world-local;
-// `info` holds sizes, capacities and flags (its bases are set when the world is placed).
+static u32 TakeNode(NodePool& p) {
+    if (p.freeStack.empty()) return p.nextNode++;
+    const u32 n = p.freeStack.back();
+    p.freeStack.pop_back();
+    return n;
+}
+
+// The proxy id b3DynamicTree::CreateProxy returns; false if it would be an untracked node.
+static bool PoolCreateProxy(NodePool& p, u32* proxy) {
+    *proxy = TakeNode(p);
+    if (*proxy == kNull) return false;
+    if (p.leafCount > 0) (void)TakeNode(p);  // InsertLeaf's new parent
+    ++p.leafCount;
+    return true;
+}
+
+static void PoolDestroyProxy(NodePool& p, u32 proxy) {
+    if (p.leafCount > 1) p.freeStack.push_back(kNull);  // RemoveLeaf frees the parent first
+    p.freeStack.push_back(proxy);
+    --p.leafCount;
+}
+
+// A pair-list entry that holds a proxy id rather than a fixture index: the fixture was destroyed
+// after the pair search that reported it (only the host reads such entries; the next search
+// rewrites the list).
+static const u32 kPairProxyBit = 0x80000000u;
+
+// One world built on the host from its descriptor, or taken from a resident world for a topology
+// edit. Every index in these arrays is world-local; `info` holds sizes, capacities and flags (its
+// bases are set when the world is placed).
 struct HostWorld {
     WorldInfo info;
     WorldState state;
@@ -691,7 +733,198 @@ struct HostWorld {
     std::vector<AABB> fat;
     std::vector<Joint> joints;
     std::vector<JointEdge> jointEdges;
+    NodePool nodes;
+    // State carried over from a resident world: the fixtures' moved flags, the live contacts in list
+    // order with their mesh records and triangle lists, the last pair search's pairs, and whether the
+    // next step begins with the new-contact search.
+    bool carried = false;
+    std::vector<u32> moved;
+    std::vector<Contact> contacts;
+    std::vector<MeshContact> meshContacts;
+    std::vector<u32> meshTris;  // contacts.size() x info.meshTriCap
+    std::vector<uint2> pairs;
+    bool needsInitialPairs = true;
 };
+
+// b3Body::b3Body (body.cpp:25-103) from a descriptor; mass data follows from the fixtures.
+static int MakeBody(const bx_body_desc& bd, Body& x) {
+    if (bd.type > 2) return Fail(BX_E_INVALID, "bad body type");
+    std::memset(&x, 0, sizeof(x));
+    x.type = bd.type;
+    x.flags = 0;
+    if (bd.flags & BX_BODY_AWAKE) x.flags |= kAwake;
+    if (bd.flags & BX_BODY_AUTO_SLEEP) x.flags |= kAutoSleep;
+    if (x.type == kDynamic) {
+        x.mass = 1;
+        x.invMass = 1;
+        x.flags |= bd.flags & (kFixedX | kFixedY | kFixedZ);
+    }
+    x.v = V3(bd.linear_velocity);
+    x.w = V3(bd.angular_velocity);
+    x.localCenter.SetZero();
+    x.x = V3(bd.position);
+    x.q = Q4(bd.orientation);
+    x.x0 = x.x;
+    x.q0 = x.q;
+    x.xfT = x.x;
+    x.linearDamping = V3(bd.linear_damping);
+    x.angularDamping = V3(bd.angular_damping);
+    x.gravityScale = V3(bd.gravity_scale);
+    x.linTol = bd.linear_sleep_tolerance;
+    x.angTol = bd.angular_sleep_tolerance;
+    x.islandIndex = 0;
+    return BX_OK;
+}
+
+// b3Fixture::Create (fixture.cpp:43-56) from a descriptor (the owning body is checked by the caller).
+static int MakeFixture(const bx_fixture_desc& fd, const HostLibrary& L, Fixture& f) {
+    std::memset(&f, 0, sizeof(f));
+    f.body = fd.body;
+    f.shapeType = fd.shape_type;
+    f.shapeIndex = fd.shape_index;
+    f.isSensor = fd.is_sensor ? 1u : 0u;
+    f.density = fd.density;
+    f.friction = fd.friction;
+    f.restitution = fd.restitution;
+    f.v1 = V3(fd.vertex1);
+    f.v2 = V3(fd.vertex2);
+    f.v3 = V3(fd.vertex3);
+    switch (fd.shape_type) {
+    case BX_SHAPE_HULL:
+        if (fd.shape_index >= L.info.size()) return Fail(BX_E_INVALID, "hull index out of range");
+        f.radius = kHullRadius;
+        break;
+    case BX_SHAPE_SPHERE:
+    case BX_SHAPE_CAPSULE:
+        f.radius = fd.radius;
+        break;
+    case BX_SHAPE_TRIANGLE:  // b3TriangleShape::Set: hull skin radius, no edge wings
+        f.radius = kHullRadius;
+        break;
+    case BX_SHAPE_MESH:  // b3MeshShape: the mesh by index, its scale in v1, hull skin radius
+        if (fd.shape_index >= L.meshes.size()) return Fail(BX_E_INVALID, "mesh index out of range");
+        if (fd.mesh_scale[0] == 0.0f || fd.mesh_scale[1] == 0.0f || fd.mesh_scale[2] == 0.0f)
+            return Fail(BX_E_INVALID, "mesh scale must be nonzero");
+        f.radius = kHullRadius;
+        f.v1 = V3(fd.mesh_scale);
+        f.v2.SetZero();
+        f.v3.SetZero();
+        break;
+    default:
+        return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");
+    }
+    return BX_OK;
+}
+
+// b3Joint::Create of a revolute joint (revolute_joint.cpp:22-76) between two of `bodies`, the frames
+// of b3RevoluteJointDef::Initialize taken from the bodies' current transforms.
+static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, Joint& j) {
+    if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= bodies.size() || jd.body_b >= bodies.size() ||
+        jd.body_a == jd.body_b)
+        return Fail(BX_E_INVALID, "bad joint descriptor");
+    std::memset(&j, 0, sizeof(j));
+    j.bodyA = jd.body_a;
+    j.bodyB = jd.body_b;
+    j.collideConnected = jd.collide_connected ? 1u : 0u;
+    if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
+        Mat33 rotation;
+        rotation.z = V3(jd.axis);
+        ComputeBasis(rotation.z, rotation.x, rotation.y);
+        Quat q = Mat33Quat(rotation);
+        const Body& A = bodies[jd.body_a];
+        const Body& Bd = bodies[jd.body_b];
+        Vec3 anchor = V3(jd.anchor);
+        j.localAnchorA = MulT(Transform(A.xfT, A.q), anchor);
+        j.localRotationA = MulC(A.q, q);
+        j.localAnchorB = MulT(Transform(Bd.xfT, Bd.q), anchor);
+        j.localRotationB = MulC(Bd.q, q);
+        j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
+    } else {
+        j.localAnchorA = V3(jd.local_anchor_a);
+        j.localRotationA = Q4(jd.local_rotation_a);
+        j.localAnchorB = V3(jd.local_anchor_b);
+        j.localRotationB = Q4(jd.local_rotation_b);
+        j.referenceRotation = Q4(jd.reference_rotation);
+    }
+    j.enableMotor = jd.enable_motor ? 1u : 0u;
+    j.enableLimit = jd.enable_limit ? 1u : 0u;
+    j.lowerAngle = jd.lower_angle;
+    j.upperAngle = jd.upper_angle;
+    j.motorSpeed = jd.motor_speed;
+    j.maxMotorTorque = jd.max_motor_torque;
+    return BX_OK;
+}
+
+// b3Body::ResetMass (body.cpp:265-376) of a body whose fixture list (list order) is `list`.
+static void ResetMassOf(Body& x, const std::vector<Fixture>& fixtures, const std::vector<u32>& list, const HostLibrary& L) {
+    const u32 fb = x.fixtureBegin, fc = x.fixtureCount;
+    x.fixtureBegin = 0;
+    x.fixtureCount = u32(list.size());
+    ResetMass(x, fixtures, list, 0, L);
+    x.fixtureBegin = fb;
+    x.fixtureCount = fc;
+}
+
+// The fat AABB b3DynamicTree::CreateProxy stores for a fixture at transform xf.
+static AABB CreationFatAABB(const Fixture& f, const Transform& xf, const HostLibrary& L) {
+    AABB aabb;
+    HostAABB(f, xf, L, aabb);
+    Vec3 r(kAabbExtension, kAabbExtension, kAabbExtension);
+    AABB fa;
+    fa.lowerBound = aabb.lowerBound - r;
+    fa.upperBound = aabb.upperBound + r;
+    return fa;
+}
+
+// Sorted proxy order, the serial-island flag, capacities, the contact slot stack and the counters
+// of a world whose bodies, fixtures, joints (and carried contacts) are final.
+static int FinishWorld(HostWorld& hw, bool hasMesh, const bx_batch_opts* opts) {
+    WorldInfo& wi = hw.info;
+    const std::vector<u32>& pid = hw.proxyIds;
+    hw.proxyOrder.resize(wi.fixtureCount);
+    for (u32 i = 0; i < wi.fixtureCount; ++i) hw.proxyOrder[i] = i;
+    std::sort(hw.proxyOrder.begin(), hw.proxyOrder.end(), [&](u32 a, u32 c) { return pid[a] < pid[c]; });
+    // Static bodies join every island they touch; the position solver renormalizes their
+    // orientation once per contact point. If that is not a fixed point of the binary32
+    // normalization the outcome depends on island order, so such worlds solve islands in order.
+    wi.serialIslands = 0;
+    for (const Body& x : hw.bodies) {
+        if (x.type != kStatic) continue;
+        Quat q = x.q;
+        Quat n1 = q;
+        n1.Normalize();
+        if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
+    }
+    const u32 oldTriCap = wi.meshTriCap;
+    int rc = SetCapacities(wi, hasMesh, opts);
+    if (rc) return rc;
+    const u32 cap = wi.contactCap;
+    const u32 live = u32(hw.contacts.size());
+    if (live > cap) return Fail(BX_E_CAPACITY, "the world's live contacts exceed its contact capacity");
+    hw.freeSlots.resize(cap - live);  // slots [live, cap) free, slot `live` on top
+    for (u32 k = 0; k < cap - live; ++k) hw.freeSlots[k] = cap - 1 - k;
+    if (!hw.carried) {
+        std::memset(&hw.state, 0, sizeof(hw.state));
+        hw.state.newContacts = 1;
+        hw.moved.assign(wi.fixtureCount, 1u);  // CreateProxy buffers every new proxy
+    } else {
+        // the contacts keep their list order in slots [0, live); the last step's islands are not
+        // carried (the next step builds its own)
+        if (wi.meshTriCap != oldTriCap) {
+            hw.meshContacts.assign(wi.meshTriCap ? live : 0, MeshContact{});
+            hw.meshTris.assign(size_t(live) * wi.meshTriCap, 0u);
+            if (oldTriCap) return Fail(BX_E_UNSUPPORTED, "mesh triangle capacity changed under live mesh contacts");
+        }
+        if (hw.pairs.size() > wi.pairCap) hw.pairs.resize(wi.pairCap);
+        hw.state.contactCount = live;
+        hw.state.freeCount = cap - live;
+        hw.state.pairCount = u32(hw.pairs.size());
+        hw.state.islandCount = hw.state.islandBodyCount = hw.state.islandContactCount = hw.state.islandJointCount = 0;
+        hw.state.newContacts = hw.needsInitialPairs ? 1u : 0u;
+    }
+    hw.state.freeCount = cap - live;
+    return BX_OK;
+}
 
 static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_batch_opts* opts, HostWorld& hw) {
     WorldInfo& wi = hw.info;
@@ -714,81 +947,20 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         return Fail(BX_E_INVALID, "world too large");
     // bodies (b3Body::b3Body, body.cpp:25-103)
     std::vector<Body>& bodies = hw.bodies;
-    bodies.reserve(d.body_count);
+    bodies.resize(d.body_count);
     for (u32 i = 0; i < d.body_count; ++i) {
-        const bx_body_desc& bd = d.bodies[i];
-        if (bd.type > 2) return Fail(BX_E_INVALID, "bad body type");
-        Body x;
-        std::memset(&x, 0, sizeof(x));
-        x.type = bd.type;
-        x.flags = 0;
-        if (bd.flags & BX_BODY_AWAKE) x.flags |= kAwake;
-        if (bd.flags & BX_BODY_AUTO_SLEEP) x.flags |= kAutoSleep;
-        if (x.type == kDynamic) {
-            x.mass = 1;
-            x.invMass = 1;
-            x.flags |= bd.flags & (kFixedX | kFixedY | kFixedZ);
-        }
-        x.v = V3(bd.linear_velocity);
-        x.w = V3(bd.angular_velocity);
-        x.localCenter.SetZero();
-        x.x = V3(bd.position);
-        x.q = Q4(bd.orientation);
-        x.x0 = x.x;
-        x.q0 = x.q;
-        x.xfT = x.x;
-        x.linearDamping = V3(bd.linear_damping);
-        x.angularDamping = V3(bd.angular_damping);
-        x.gravityScale = V3(bd.gravity_scale);
-        x.linTol = bd.linear_sleep_tolerance;
-        x.angTol = bd.angular_sleep_tolerance;
-        x.islandIndex = 0;
-        bodies.push_back(x);
+        int rc = MakeBody(d.bodies[i], bodies[i]);
+        if (rc) return rc;
     }
     // fixtures, grouped per body in list order (newest first)
     std::vector<Fixture>& fixtures = hw.fixtures;
+    fixtures.resize(d.fixture_count);
     std::vector<std::vector<u32>> perBody(d.body_count);
     for (u32 i = 0; i < d.fixture_count; ++i) {
-        const bx_fixture_desc& fd = d.fixtures[i];
-        if (fd.body >= d.body_count) return Fail(BX_E_INVALID, "fixture body out of range");
-        Fixture f;
-        std::memset(&f, 0, sizeof(f));
-        f.body = fd.body;
-        f.shapeType = fd.shape_type;
-        f.shapeIndex = fd.shape_index;
-        f.isSensor = fd.is_sensor ? 1u : 0u;
-        f.density = fd.density;
-        f.friction = fd.friction;
-        f.restitution = fd.restitution;
-        f.v1 = V3(fd.vertex1);
-        f.v2 = V3(fd.vertex2);
-        f.v3 = V3(fd.vertex3);
-        switch (fd.shape_type) {
-        case BX_SHAPE_HULL:
-            if (fd.shape_index >= L.info.size()) return Fail(BX_E_INVALID, "hull index out of range");
-            f.radius = kHullRadius;
-            break;
-        case BX_SHAPE_SPHERE:
-        case BX_SHAPE_CAPSULE:
-            f.radius = fd.radius;
-            break;
-        case BX_SHAPE_TRIANGLE:  // b3TriangleShape::Set: hull skin radius, no edge wings
-            f.radius = kHullRadius;
-            break;
-        case BX_SHAPE_MESH:  // b3MeshShape: the mesh by index, its scale in v1, hull skin radius
-            if (fd.shape_index >= L.meshes.size()) return Fail(BX_E_INVALID, "mesh index out of range");
-            if (fd.mesh_scale[0] == 0.0f || fd.mesh_scale[1] == 0.0f || fd.mesh_scale[2] == 0.0f)
-                return Fail(BX_E_INVALID, "mesh scale must be nonzero");
-            f.radius = kHullRadius;
-            f.v1 = V3(fd.mesh_scale);
-            f.v2.SetZero();
-            f.v3.SetZero();
-            break;
-        default:
-            return Fail(BX_E_UNSUPPORTED, "shape type not supported by the device path yet");
-        }
-        fixtures.push_back(f);
-        perBody[fd.body].insert(perBody[fd.body].begin(), i);
+        if (d.fixtures[i].body >= d.body_count) return Fail(BX_E_INVALID, "fixture body out of range");
+        int rc = MakeFixture(d.fixtures[i], L, fixtures[i]);
+        if (rc) return rc;
+        perBody[d.fixtures[i].body].insert(perBody[d.fixtures[i].body].begin(), i);
     }
     u32 cursor = 0;
     for (u32 i = 0; i < d.body_count; ++i) {
@@ -801,79 +973,31 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
     // proxies, fat AABBs and mass, in fixture creation order (b3Body::CreateFixture)
     std::vector<u32>& pid = hw.proxyIds;
     pid.resize(d.fixture_count);
-    u32 nextNode = 0, leaves = 0;
     const std::vector<Fixture> local(fixtures);
     for (u32 i = 0; i < d.fixture_count; ++i) {
         Fixture& f = fixtures[i];
         Body& x = bodies[f.body];
-        AABB aabb;
-        HostAABB(f, Transform(x.xfT, x.q), L, aabb);
-        Vec3 r(kAabbExtension, kAabbExtension, kAabbExtension);
-        AABB fa;
-        fa.lowerBound = aabb.lowerBound - r;
-        fa.upperBound = aabb.upperBound + r;
-        hw.fat.push_back(fa);
-        pid[i] = nextNode++;
-        if (leaves > 0) ++nextNode;  // InsertLeaf allocates the new parent
-        ++leaves;
+        hw.fat.push_back(CreationFatAABB(f, Transform(x.xfT, x.q), L));
+        (void)PoolCreateProxy(hw.nodes, &pid[i]);  // a fresh pool: every node is tracked
         f.proxyId = pid[i];
         if (f.density > scalar(0)) {
             // ResetMass sees the fixtures created so far: restrict the list to them.
-            Body tmp = x;
             std::vector<u32> visible;
             for (u32 k = 0; k < x.fixtureCount; ++k) {
                 u32 fi = hw.fixtureList[x.fixtureBegin + k];
                 if (fi <= i) visible.push_back(fi);
             }
-            tmp.fixtureBegin = 0;
-            tmp.fixtureCount = u32(visible.size());
-            ResetMass(tmp, local, visible, 0, L);
-            u32 fb = x.fixtureBegin, fc = x.fixtureCount;
-            x = tmp;
-            x.fixtureBegin = fb;
-            x.fixtureCount = fc;
+            ResetMassOf(x, local, visible, L);
         }
     }
     // revolute joints (b3World::CreateJoint after the bodies' fixtures; joint_manager.cpp:30-72
     // prepends each joint's edges to both bodies' joint lists)
     std::vector<std::vector<JointEdge>> edgesOf(d.body_count);
+    hw.joints.resize(d.joint_count);
     for (u32 i = 0; i < d.joint_count; ++i) {
         const bx_joint_desc& jd = d.joints[i];
-        if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= d.body_count || jd.body_b >= d.body_count ||
-            jd.body_a == jd.body_b)
-            return Fail(BX_E_INVALID, "bad joint descriptor");
-        Joint j;
-        std::memset(&j, 0, sizeof(j));
-        j.bodyA = jd.body_a;
-        j.bodyB = jd.body_b;
-        j.collideConnected = jd.collide_connected ? 1u : 0u;
-        if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
-            Mat33 rotation;
-            rotation.z = V3(jd.axis);
-            ComputeBasis(rotation.z, rotation.x, rotation.y);
-            Quat q = Mat33Quat(rotation);
-            const Body& A = bodies[jd.body_a];
-            const Body& Bd = bodies[jd.body_b];
-            Vec3 anchor = V3(jd.anchor);
-            j.localAnchorA = MulT(Transform(A.xfT, A.q), anchor);
-            j.localRotationA = MulC(A.q, q);
-            j.localAnchorB = MulT(Transform(Bd.xfT, Bd.q), anchor);
-            j.localRotationB = MulC(Bd.q, q);
-            j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
-        } else {
-            j.localAnchorA = V3(jd.local_anchor_a);
-            j.localRotationA = Q4(jd.local_rotation_a);
-            j.localAnchorB = V3(jd.local_anchor_b);
-            j.localRotationB = Q4(jd.local_rotation_b);
-            j.referenceRotation = Q4(jd.reference_rotation);
-        }
-        j.enableMotor = jd.enable_motor ? 1u : 0u;
-        j.enableLimit = jd.enable_limit ? 1u : 0u;
-        j.lowerAngle = jd.lower_angle;
-        j.upperAngle = jd.upper_angle;
-        j.motorSpeed = jd.motor_speed;
-        j.maxMotorTorque = jd.max_motor_torque;
-        hw.joints.push_back(j);
+        int rc = MakeJoint(jd, bodies, hw.joints[i]);
+        if (rc) return rc;
         edgesOf[jd.body_a].insert(edgesOf[jd.body_a].begin(), JointEdge{i, jd.body_b});
         edgesOf[jd.body_b].insert(edgesOf[jd.body_b].begin(), JointEdge{i, jd.body_a});
     }
@@ -885,30 +1009,7 @@ static int BuildWorld(const bx_world_desc& d, const HostLibrary& L, const bx_bat
         for (const JointEdge& e : edgesOf[i]) hw.jointEdges.push_back(e);
         ecursor += x.jointCount;
     }
-    hw.proxyOrder.resize(d.fixture_count);
-    for (u32 i = 0; i < d.fixture_count; ++i) hw.proxyOrder[i] = i;
-    std::sort(hw.proxyOrder.begin(), hw.proxyOrder.end(), [&](u32 a, u32 c) { return pid[a] < pid[c]; });
-    // Static bodies join every island they touch; the position solver renormalizes their
-    // orientation once per contact point. If that is not a fixed point of the binary32
-    // normalization the outcome depends on island order, so such worlds solve islands in order.
-    for (u32 i = 0; i < d.body_count; ++i) {
-        const Body& x = bodies[i];
-        if (x.type != kStatic) continue;
-        Quat q = x.q;
-        Quat n1 = q;
-        n1.Normalize();
-        if (std::memcmp(&n1, &q, sizeof(Quat)) != 0) wi.serialIslands = 1;
-    }
-    // capacities
-    int rc = SetCapacities(wi, HasMesh(d), opts);
-    if (rc) return rc;
-    const u32 cap = wi.contactCap;
-    hw.freeSlots.resize(cap);
-    for (u32 k = 0; k < cap; ++k) hw.freeSlots[k] = u32(cap) - 1 - k;
-    std::memset(&hw.state, 0, sizeof(hw.state));
-    hw.state.freeCount = u32(cap);
-    hw.state.newContacts = 1;
-    return BX_OK;
+    return FinishWorld(hw, HasMesh(d), opts);
 }
 
 // Island-builder LDS words for a launch over worlds with the given full-capacity needs: worlds
@@ -1027,6 +1128,7 @@ static void InitSlots(bx_batch* b, u32 slots) {
     b->worldCount = slots;
     b->worlds.assign(slots, WorldInfo{});
     b->proxyIds.assign(slots, {});
+    b->nodePools.assign(slots, NodePool{});
     b->resident.assign(slots, 0);
     b->needsInitialPairs.assign(slots, 0);
     b->bvhGen.assign(slots, 0);
@@ -1291,15 +1393,33 @@ void bx_batch_destroy(bx_batch* b) {
     delete b;
 }
 
-int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_world_desc* worlds,
-                    uint32_t world_count, const bx_batch_opts* opts, bx_batch** out) {
-    if (!ctx || !out || (!worlds && world_count)) return Fail(BX_E_INVALID, "null argument");
+// Uploads the carried state of a placed world (a topology edit): its live contacts in slots
+// [0, live) in list order, their mesh records and triangle lists, and the last pair search's pairs.
+static int PutCarried(bx_batch* b, const HostWorld& hw) {
+    const WorldInfo& wi = hw.info;
+    const u32 live = u32(hw.contacts.size());
+    int rc;
+    if ((rc = PutAsync(b, b->view.contacts, wi.contactBase, hw.contacts))) return rc;
+    std::vector<u32> order(live);
+    for (u32 i = 0; i < live; ++i) order[i] = i;
+    if ((rc = PutAsync(b, b->view.contactOrder, wi.contactBase, order))) return rc;
+    if (wi.meshTriCap) {
+        if ((rc = PutAsync(b, b->view.meshContacts, wi.meshBase, hw.meshContacts))) return rc;
+        if ((rc = PutAsync(b, b->view.meshTriLists, wi.meshTriBase, hw.meshTris))) return rc;
+    }
+    if ((rc = PutAsync(b, b->view.pairs, wi.pairBase, hw.pairs))) return rc;
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    return BX_OK;
+}
+
+// A batch of `world_count` worlds, world w produced by produce(w, hw) (built from a descriptor or
+// taken from a resident world), on the shape library L (owned by the batch from here on).
+static int CreateBatchFrom(bx_context* ctx, HostLibrary* L, const bx_batch_opts* opts, u32 world_count,
+                           const std::function<int(u32, HostWorld&)>& produce, bx_batch** out) {
     HIP_CHECK(hipSetDevice(ctx->device));
     bx_batch* b = new bx_batch();
     b->ctx = ctx;
-    b->library = new HostLibrary();
-    int rc = BuildLibrary(library, *b->library);
-    if (rc) { bx_batch_destroy(b); return rc; }
+    b->library = L;
     if (opts) b->worldOpts = *opts;
     InitSlots(b, world_count);
     // the initialized arrays, concatenated in world order (one unit of padding per world in the
@@ -1311,11 +1431,13 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     std::vector<Joint> joints;
     std::vector<JointEdge> jointEdges;
     std::vector<WorldState> wstate(world_count);
+    std::vector<HostWorld> carried;  // worlds with state to upload after the arrays exist
     u64 next[kSpaces] = {};
     u32 lds = 0;
+    int rc;
     for (u32 w = 0; w < world_count; ++w) {
         HostWorld hw;
-        rc = BuildWorld(worlds[w], *b->library, opts, hw);
+        rc = produce(w, hw);
         if (rc) { bx_batch_destroy(b); return rc; }
         u32 units[kSpaces], base[kSpaces];
         WorldUnits(hw.info, units);
@@ -1334,17 +1456,20 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
         proxyOrder.push_back(0);
         fat.insert(fat.end(), hw.fat.begin(), hw.fat.end());
         fat.push_back(AABB{});
-        moved.insert(moved.end(), hw.info.fixtureCount, 1u);  // CreateProxy buffers every new proxy
+        moved.insert(moved.end(), hw.moved.begin(), hw.moved.end());
         moved.push_back(0);
         joints.insert(joints.end(), hw.joints.begin(), hw.joints.end());
         jointEdges.insert(jointEdges.end(), hw.jointEdges.begin(), hw.jointEdges.end());
         freeSlots.insert(freeSlots.end(), hw.freeSlots.begin(), hw.freeSlots.end());
+        freeSlots.insert(freeSlots.end(), hw.info.contactCap - hw.freeSlots.size(), 0u);  // live slots' units
         wstate[w] = hw.state;
         b->worlds[w] = hw.info;
         b->proxyIds[w] = std::move(hw.proxyIds);
+        b->nodePools[w] = std::move(hw.nodes);
         b->resident[w] = 1;
-        b->needsInitialPairs[w] = 1;
+        b->needsInitialPairs[w] = hw.needsInitialPairs ? 1 : 0;
         lds = std::max(lds, std::min(hw.info.islandScratchWords, kIslandLdsBudgetWords));
+        if (hw.carried && (!hw.contacts.empty() || !hw.pairs.empty())) carried.push_back(std::move(hw));
     }
     for (u32 s = 0; s < kSpaces; ++s) b->spaces[s].Reset(0);
     BatchView& V = b->view;
@@ -1379,9 +1504,24 @@ int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_w
     PUT(V.worlds, b->worlds);
     PUT(V.wstate, wstate);
 #undef PUT
-    b->newContacts = world_count > 0;
+    for (const HostWorld& hw : carried) {  // hw.info holds the world's bases
+        rc = PutCarried(b, hw);
+        if (rc) { bx_batch_destroy(b); return rc; }
+    }
+    b->newContacts = false;
+    for (u32 w = 0; w < world_count; ++w) b->newContacts = b->newContacts || b->needsInitialPairs[w];
     *out = b;
     return BX_OK;
+}
+
+int bx_batch_create(bx_context* ctx, const bx_shape_library* library, const bx_world_desc* worlds,
+                    uint32_t world_count, const bx_batch_opts* opts, bx_batch** out) {
+    if (!ctx || !out || (!worlds && world_count)) return Fail(BX_E_INVALID, "null argument");
+    HostLibrary* L = new HostLibrary();
+    int rc = BuildLibrary(library, *L);
+    if (rc) { delete L; return rc; }
+    return CreateBatchFrom(ctx, L, opts, world_count,
+                           [&](u32 w, HostWorld& hw) { return BuildWorld(worlds[w], *L, opts, hw); }, out);
 }
 
 int bx_pool_create(bx_context* ctx, const bx_shape_library* library, const bx_pool_opts* opts, bx_batch** out) {
@@ -1980,9 +2120,11 @@ int bx_batch_get_pairs(bx_batch* b, uint32_t world, bx_pair* out, uint32_t capac
     if (rc) return rc;
     const std::vector<u32>& pid = b->proxyIds[world];
     u32 n = std::min<u32>(capacity, u32(pairs.size()));
+    // entries of fixtures destroyed since the pair search hold the proxy id itself (kPairProxyBit)
+    auto proxy = [&](u32 x) { return (x & kPairProxyBit) ? (x & ~kPairProxyBit) : pid[x]; };
     for (u32 i = 0; i < n; ++i) {
-        out[i].proxy1 = pid[pairs[i].x];
-        out[i].proxy2 = pid[pairs[i].y];
+        out[i].proxy1 = proxy(pairs[i].x);
+        out[i].proxy2 = proxy(pairs[i].y);
     }
     return int(n);
 }
@@ -2216,6 +2358,310 @@ int bx_batch_query_aabb(bx_batch* b, const bx_aabb_query* boxes, uint32_t count,
                       fixtures, counts);
 }
 
+// ------------------------------------------------------------------ topology edits
+// A resident world's whole state on the host with per-body fixture and joint lists (list order,
+// newest first), as b3World::CreateBody / DestroyBody / CreateJoint / DestroyJoint and
+// b3Body::CreateFixture / DestroyFixture change it.
+struct EditWorld {
+    HostWorld hw;
+    std::vector<std::vector<u32>> fixturesOf;
+    std::vector<std::vector<JointEdge>> jointsOf;
+};
+
+// The state of world w of batch b (synchronous): bodies, fixtures, joints, proxies, the live
+// contacts in list order with their mesh records, and the last pairs.
+static int TakeWorld(bx_batch* b, u32 w, EditWorld& e) {
+    HostWorld& hw = e.hw;
+    const WorldInfo wi = b->worlds[w];
+    const BatchView& V = b->view;
+    hw.info = wi;
+    hw.carried = true;
+    int rc = GetState(b, w, hw.state);
+    if (rc) return rc;
+    if ((rc = Download(b, hw.bodies, V.bodies + wi.bodyBase, wi.bodyCount))) return rc;
+    if ((rc = Download(b, hw.fixtures, V.fixtures + wi.fixtureBase, wi.fixtureCount))) return rc;
+    std::vector<u32> flist;
+    if ((rc = Download(b, flist, V.fixtureList + wi.fixtureListBase, wi.fixtureCount))) return rc;
+    if ((rc = Download(b, hw.fat, V.fatAABB + wi.fixtureBase, wi.fixtureCount))) return rc;
+    if ((rc = Download(b, hw.moved, V.moved + wi.fixtureBase, wi.fixtureCount))) return rc;
+    if ((rc = Download(b, hw.joints, V.joints + wi.jointBase, wi.jointCount))) return rc;
+    std::vector<JointEdge> edges;
+    if ((rc = Download(b, edges, V.jointEdges + wi.jointEdgeBase, 2 * size_t(wi.jointCount)))) return rc;
+    std::vector<u32> order;
+    if ((rc = Download(b, order, V.contactOrder + wi.contactBase, hw.state.contactCount))) return rc;
+    std::vector<Contact> slots;
+    if ((rc = Download(b, slots, V.contacts + wi.contactBase, wi.contactCap))) return rc;
+    std::vector<MeshContact> meshSlots;
+    std::vector<u32> triSlots;
+    if (wi.meshTriCap) {
+        if ((rc = Download(b, meshSlots, V.meshContacts + wi.meshBase, wi.contactCap))) return rc;
+        if ((rc = Download(b, triSlots, V.meshTriLists + wi.meshTriBase, size_t(wi.contactCap) * wi.meshTriCap)))
+            return rc;
+    }
+    if ((rc = Download(b, hw.pairs, V.pairs + wi.pairBase, std::min(hw.state.pairCount, wi.pairCap)))) return rc;
+    for (u32 slot : order) {
+        hw.contacts.push_back(slots[slot]);
+        if (wi.meshTriCap) {
+            hw.meshContacts.push_back(meshSlots[slot]);
+            hw.meshTris.insert(hw.meshTris.end(), triSlots.begin() + size_t(slot) * wi.meshTriCap,
+                               triSlots.begin() + size_t(slot + 1) * wi.meshTriCap);
+        }
+    }
+    hw.proxyIds = b->proxyIds[w];
+    hw.nodes = b->nodePools[w];
+    hw.needsInitialPairs = b->needsInitialPairs[w] != 0;
+    e.fixturesOf.assign(wi.bodyCount, {});
+    e.jointsOf.assign(wi.bodyCount, {});
+    for (u32 i = 0; i < wi.bodyCount; ++i) {
+        const Body& x = hw.bodies[i];
+        e.fixturesOf[i].assign(flist.begin() + x.fixtureBegin, flist.begin() + x.fixtureBegin + x.fixtureCount);
+        e.jointsOf[i].assign(edges.begin() + x.jointBegin, edges.begin() + x.jointBegin + x.jointCount);
+    }
+    return BX_OK;
+}
+
+// Removes the live contacts of fixture f, keeping the others in list order (b3ContactManager::
+// Destroy, contact_manager.cpp:229-290). b3Contact::Destroy (contact.cpp:108-131) wakes both bodies
+// of a contact with a nonempty manifold unless a fixture is a sensor.
+static void DropContactsOf(HostWorld& hw, u32 f) {
+    const u32 tri = hw.info.meshTriCap;
+    u32 kept = 0;
+    for (u32 i = 0; i < hw.contacts.size(); ++i) {
+        const Contact& c = hw.contacts[i];
+        if (c.fixtureA == f || c.fixtureB == f) {
+            const Fixture& fA = hw.fixtures[c.fixtureA];
+            const Fixture& fB = hw.fixtures[c.fixtureB];
+            bool points = false;
+            for (u32 m = 0; m < c.manifoldCount && m < kMaxManifolds; ++m)
+                points = points || (m == 0 ? c.manifold.pointCount : hw.meshContacts[i].extra[m - 1].pointCount) > 0;
+            if (points && !fA.isSensor && !fB.isSensor) {
+                HostSetAwake(hw.bodies[fA.body], true);
+                HostSetAwake(hw.bodies[fB.body], true);
+            }
+            continue;
+        }
+        hw.contacts[kept] = hw.contacts[i];
+        if (tri) {
+            hw.meshContacts[kept] = hw.meshContacts[i];
+            std::copy(hw.meshTris.begin() + size_t(i) * tri, hw.meshTris.begin() + size_t(i + 1) * tri,
+                      hw.meshTris.begin() + size_t(kept) * tri);
+        }
+        ++kept;
+    }
+    hw.contacts.resize(kept);
+    if (tri) {
+        hw.meshContacts.resize(kept);
+        hw.meshTris.resize(size_t(kept) * tri);
+    }
+}
+
+// b3JointManager::Destroy (joint_manager.cpp:75-141): wakes both bodies, unlinks the joint's edges.
+static void EditDestroyJoint(EditWorld& e, u32 j) {
+    HostWorld& hw = e.hw;
+    const Joint J = hw.joints[j];
+    HostSetAwake(hw.bodies[J.bodyA], true);
+    HostSetAwake(hw.bodies[J.bodyB], true);
+    for (auto& list : e.jointsOf) {
+        list.erase(std::remove_if(list.begin(), list.end(), [&](const JointEdge& x) { return x.joint == j; }), list.end());
+        for (JointEdge& x : list)
+            if (x.joint > j) --x.joint;
+    }
+    hw.joints.erase(hw.joints.begin() + j);
+}
+
+// The contacts and the broadphase proxy of fixture f go; fixture indices above f move down.
+static void EditRemoveFixture(EditWorld& e, u32 f) {
+    HostWorld& hw = e.hw;
+    DropContactsOf(hw, f);
+    for (Contact& c : hw.contacts) {
+        if (c.fixtureA > f) --c.fixtureA;
+        if (c.fixtureB > f) --c.fixtureB;
+    }
+    // The pairs of the last search stay as the broadphase's pair buffer keeps them; an entry of the
+    // destroyed fixture keeps its proxy id (kPairProxyBit), the others follow the new indices.
+    const u32 gone = kPairProxyBit | hw.proxyIds[f];
+    for (uint2& p : hw.pairs) {
+        for (u32* x : {&p.x, &p.y}) {
+            if (*x & kPairProxyBit) continue;
+            if (*x == f) *x = gone;
+            else if (*x > f) --*x;
+        }
+    }
+    PoolDestroyProxy(hw.nodes, hw.proxyIds[f]);  // b3BroadPhase::DestroyProxy (also unbuffers its move)
+    const u32 body = hw.fixtures[f].body;
+    auto& own = e.fixturesOf[body];
+    own.erase(std::remove(own.begin(), own.end(), f), own.end());
+    for (auto& list : e.fixturesOf)
+        for (u32& x : list)
+            if (x > f) --x;
+    hw.fixtures.erase(hw.fixtures.begin() + f);
+    hw.proxyIds.erase(hw.proxyIds.begin() + f);
+    hw.fat.erase(hw.fat.begin() + f);
+    hw.moved.erase(hw.moved.begin() + f);
+}
+
+static int EditOp(EditWorld& e, const HostLibrary& L, const bx_world_desc* adds, const bx_topology_op& op) {
+    HostWorld& hw = e.hw;
+    const u32 nb = u32(hw.bodies.size()), nf = u32(hw.fixtures.size()), nj = u32(hw.joints.size());
+    switch (op.op) {
+    case BX_TOPO_CREATE_BODY: {  // b3World::CreateBody (world.cpp:94-110)
+        if (!adds || op.index >= adds->body_count) return Fail(BX_E_INVALID, "body descriptor index out of range");
+        Body x;
+        int rc = MakeBody(adds->bodies[op.index], x);
+        if (rc) return rc;
+        hw.bodies.push_back(x);
+        e.fixturesOf.emplace_back();
+        e.jointsOf.emplace_back();
+        return BX_OK;
+    }
+    case BX_TOPO_CREATE_FIXTURE: {  // b3Body::CreateFixture (body.cpp:105-136)
+        if (!adds || op.index >= adds->fixture_count) return Fail(BX_E_INVALID, "fixture descriptor index out of range");
+        const bx_fixture_desc& fd = adds->fixtures[op.index];
+        if (fd.body >= nb) return Fail(BX_E_INVALID, "fixture body out of range");
+        if (nf >= 0xFFFFu) return Fail(BX_E_INVALID, "world too large");
+        Fixture f;
+        int rc = MakeFixture(fd, L, f);
+        if (rc) return rc;
+        Body& x = hw.bodies[fd.body];
+        u32 proxy;
+        if (!PoolCreateProxy(hw.nodes, &proxy))
+            return Fail(BX_E_UNSUPPORTED, "the new proxy's node id depends on the dynamic tree's shape");
+        f.proxyId = proxy;
+        hw.fat.push_back(CreationFatAABB(f, Transform(x.xfT, x.q), L));
+        hw.moved.push_back(1u);  // BufferMove
+        hw.proxyIds.push_back(proxy);
+        hw.fixtures.push_back(f);
+        e.fixturesOf[fd.body].insert(e.fixturesOf[fd.body].begin(), nf);
+        if (f.density > scalar(0)) ResetMassOf(x, hw.fixtures, e.fixturesOf[fd.body], L);
+        hw.needsInitialPairs = true;  // m_newContacts
+        return BX_OK;
+    }
+    case BX_TOPO_CREATE_JOINT: {  // b3JointManager::Create (joint_manager.cpp:30-73)
+        if (!adds || op.index >= adds->joint_count) return Fail(BX_E_INVALID, "joint descriptor index out of range");
+        if (nj >= 0x7FFFu) return Fail(BX_E_INVALID, "world too large");
+        const bx_joint_desc& jd = adds->joints[op.index];
+        Joint j;
+        int rc = MakeJoint(jd, hw.bodies, j);
+        if (rc) return rc;
+        hw.joints.push_back(j);
+        e.jointsOf[jd.body_a].insert(e.jointsOf[jd.body_a].begin(), JointEdge{nj, jd.body_b});
+        e.jointsOf[jd.body_b].insert(e.jointsOf[jd.body_b].begin(), JointEdge{nj, jd.body_a});
+        return BX_OK;
+    }
+    case BX_TOPO_DESTROY_JOINT:
+        if (op.index >= nj) return Fail(BX_E_INVALID, "joint index out of range");
+        EditDestroyJoint(e, op.index);
+        return BX_OK;
+    case BX_TOPO_DESTROY_FIXTURE: {  // b3Body::DestroyFixture (body.cpp:138-191)
+        if (op.index >= nf) return Fail(BX_E_INVALID, "fixture index out of range");
+        const u32 body = hw.fixtures[op.index].body;
+        EditRemoveFixture(e, op.index);
+        ResetMassOf(hw.bodies[body], hw.fixtures, e.fixturesOf[body], L);
+        return BX_OK;
+    }
+    case BX_TOPO_DESTROY_BODY: {  // b3World::DestroyBody (world.cpp:112-175)
+        const u32 b = op.index;
+        if (b >= nb) return Fail(BX_E_INVALID, "body index out of range");
+        while (!e.jointsOf[b].empty()) EditDestroyJoint(e, e.jointsOf[b].front().joint);
+        while (!e.fixturesOf[b].empty()) EditRemoveFixture(e, e.fixturesOf[b].front());
+        hw.bodies.erase(hw.bodies.begin() + b);
+        e.fixturesOf.erase(e.fixturesOf.begin() + b);
+        e.jointsOf.erase(e.jointsOf.begin() + b);
+        for (Fixture& f : hw.fixtures)
+            if (f.body > b) --f.body;
+        for (Joint& j : hw.joints) {
+            if (j.bodyA > b) --j.bodyA;
+            if (j.bodyB > b) --j.bodyB;
+        }
+        for (auto& list : e.jointsOf)
+            for (JointEdge& x : list)
+                if (x.other > b) --x.other;
+        return BX_OK;
+    }
+    }
+    return Fail(BX_E_INVALID, "unknown topology operation");
+}
+
+// Back to the CSR lists of the device layout, with counts, capacities and the slot stack.
+static int FinishEdit(EditWorld& e, const bx_batch_opts* opts) {
+    HostWorld& hw = e.hw;
+    WorldInfo& wi = hw.info;
+    wi.bodyCount = u32(hw.bodies.size());
+    wi.fixtureCount = u32(hw.fixtures.size());
+    wi.jointCount = u32(hw.joints.size());
+    hw.fixtureList.clear();
+    hw.jointEdges.clear();
+    for (u32 i = 0; i < wi.bodyCount; ++i) {
+        Body& x = hw.bodies[i];
+        x.fixtureBegin = u32(hw.fixtureList.size());
+        x.fixtureCount = u32(e.fixturesOf[i].size());
+        hw.fixtureList.insert(hw.fixtureList.end(), e.fixturesOf[i].begin(), e.fixturesOf[i].end());
+        x.jointBegin = u32(hw.jointEdges.size());
+        x.jointCount = u32(e.jointsOf[i].size());
+        hw.jointEdges.insert(hw.jointEdges.end(), e.jointsOf[i].begin(), e.jointsOf[i].end());
+    }
+    for (u32 i = 0; i < wi.fixtureCount; ++i) hw.fixtures[i].proxyId = hw.proxyIds[i];
+    bool hasMesh = false;
+    for (const Fixture& f : hw.fixtures) hasMesh = hasMesh || f.shapeType == BX_SHAPE_MESH;
+    return FinishWorld(hw, hasMesh, opts);
+}
+
+// Whether the first hulls and meshes of `next` are those of `prev` (indices stay valid).
+static bool PrefixBytes(const void* a, size_t na, const void* b, size_t nb) {
+    return na <= nb && (na == 0 || std::memcmp(a, b, na) == 0);
+}
+#define BX_PREFIX(m) PrefixBytes(prev.m.data(), prev.m.size() * sizeof(prev.m[0]), next.m.data(), next.m.size() * sizeof(next.m[0]))
+static bool LibraryExtends(const HostLibrary& prev, const HostLibrary& next) {
+    return BX_PREFIX(info) && BX_PREFIX(vertices) && BX_PREFIX(edges) && BX_PREFIX(faces) && BX_PREFIX(planes) &&
+           BX_PREFIX(meshes) && BX_PREFIX(meshVertices) && BX_PREFIX(meshTris) && BX_PREFIX(meshNodes);
+}
+#undef BX_PREFIX
+
+int bx_batch_edit_topology(bx_batch* b, uint32_t world, const bx_shape_library* library, const bx_world_desc* adds,
+                           const bx_topology_op* ops, uint32_t op_count) {
+    if (!b || (!ops && op_count)) return Fail(BX_E_INVALID, "null argument");
+    if (b->pool) return Fail(BX_E_UNSUPPORTED, "topology edits apply to a batch, not a pool");
+    if (!IsResident(b, world)) return Fail(BX_E_INVALID, "world not resident");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    HostLibrary* L = new HostLibrary();
+    int rc = library ? BuildLibrary(library, *L) : BX_OK;
+    if (!library) *L = *b->library;
+    if (!rc && !LibraryExtends(*b->library, *L))
+        rc = Fail(BX_E_INVALID, "the new shape library must begin with the batch's hulls and meshes");
+    if (rc) { delete L; return rc; }
+    // the edited world first, so that a failing operation leaves the batch unchanged
+    EditWorld edited;
+    rc = TakeWorld(b, world, edited);
+    for (u32 k = 0; !rc && k < op_count; ++k) rc = EditOp(edited, *L, adds, ops[k]);
+    if (!rc) rc = FinishEdit(edited, &b->worldOpts);
+    if (rc) { delete L; return rc; }
+    bx_batch_opts opts = b->worldOpts;
+    opts.solver_mode = b->view.solveLayout + 1;  // the batch's current layout (tuned or fixed)
+    bx_batch* nb = nullptr;
+    rc = CreateBatchFrom(b->ctx, L, &opts, b->worldCount, [&](u32 w, HostWorld& hw) -> int {
+        if (w == world) {
+            hw = std::move(edited.hw);
+            return BX_OK;
+        }
+        EditWorld other;
+        int r = TakeWorld(b, w, other);
+        if (!r) r = FinishEdit(other, &b->worldOpts);
+        if (!r) hw = std::move(other.hw);
+        return r;
+    }, &nb);
+    if (rc) return rc;
+    // the new arrays replace the old ones under the same handle; the velocity-sweep tuning state and
+    // the options carry over
+    nb->worldOpts = b->worldOpts;
+    nb->tuneStep = b->tuneStep;
+    std::memcpy(nb->tuneMs, b->tuneMs, sizeof(b->tuneMs));
+    std::swap(*b, *nb);
+    bx_batch_destroy(nb);
+    ++b->stateGen;
+    return BX_OK;
+}
+
 int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits, uint32_t count) {
     if (!b || !IsResident(b, world) || (!edits && count)) return Fail(BX_E_INVALID, "bad argument or world not resident");
     HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -2375,7 +2821,7 @@ uint32_t bx_struct_size(const char* name) {
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
     S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event) S(bx_ray_query)
-    S(bx_ray_hit) S(bx_aabb_query)
+    S(bx_ray_hit) S(bx_aabb_query) S(bx_topology_op)
 #undef S
     return 0;
 }

@@ -64,6 +64,8 @@ def load(path=capi.LIB_PATH):
         "bx_pool_get_stats": (C.c_int, [P, C.POINTER(capi.bx_pool_stats)]),
         "bx_batch_snapshot": (C.c_int, [P, u, P, C.c_uint64, C.POINTER(C.c_uint64)]),
         "bx_pool_restore": (C.c_int, [P, P, C.c_uint64, C.POINTER(u)]),
+        "bx_batch_edit_topology": (C.c_int, [P, u, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc),
+                                             C.POINTER(capi.bx_topology_op), u]),
         "bx_batch_raycast_single": (C.c_int, [P, C.POINTER(capi.bx_ray_query), u, C.c_void_p, C.c_uint64,
                                               C.POINTER(capi.bx_ray_hit)]),
         "bx_batch_raycast": (C.c_int, [P, C.POINTER(capi.bx_ray_query), u, C.c_void_p, C.c_uint64,
@@ -216,7 +218,17 @@ class Batch:
         return t
 
     def _body_count(self, world):
-        return self.scene.worlds[world].body_count
+        return self.counts(world).body_count
+
+    def edit_topology(self, world, ops, adds, library_scene=None):
+        """Topology operations ((op, index) pairs, capi.BX_TOPO_*) on one world, descriptors from the
+        scenes.World `adds`, an optionally grown shape library from `library_scene`."""
+        lib = library_scene.library() if library_scene is not None else None
+        desc = adds.desc()
+        _check(_lib.bx_batch_edit_topology(self.h, world, None if lib is None else C.byref(lib), C.byref(desc),
+                                           capi.topology_ops(ops), len(ops)))
+        if library_scene is not None:
+            self._lib_keep = (lib, library_scene)
 
     def bodies(self, world=0):
         n = self._body_count(world)

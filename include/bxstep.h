@@ -466,6 +466,49 @@ typedef struct bx_body_edit {
 int bx_batch_edit_bodies(bx_batch* batch, uint32_t world, const bx_body_edit* edits, uint32_t count);
 
 /* ---------------------------------------------------------------------------------------------
+ * Topology changes between steps, keeping the world's state: contacts with their manifolds,
+ * impulses and feature caches, joint impulses, proxies and their fat AABBs, sleep state and the
+ * previous step's inverse time step all carry over, as they do in the reference.
+ *   BX_TOPO_CREATE_BODY     b3World::CreateBody (world.cpp:94-110): adds->bodies[index] becomes the
+ *                           newest body (creation index = the world's body count at that point)
+ *   BX_TOPO_CREATE_FIXTURE  b3Body::CreateFixture (body.cpp:105-136): adds->fixtures[index], whose
+ *                           `body` is a creation index at that point; new proxy, fat AABB from the
+ *                           body's current transform, ResetMass when the density is positive, and
+ *                           the next step begins with the new-contact search
+ *   BX_TOPO_CREATE_JOINT    b3World::CreateJoint (joint_manager.cpp:30-73): adds->joints[index]
+ *                           (Initialize frames from the bodies' current transforms); no body wakes
+ *   BX_TOPO_DESTROY_JOINT   b3World::DestroyJoint (joint_manager.cpp:75-141): wakes both bodies
+ *   BX_TOPO_DESTROY_FIXTURE b3Body::DestroyFixture (body.cpp:138-191): its contacts and proxy go,
+ *                           then ResetMass
+ *   BX_TOPO_DESTROY_BODY    b3World::DestroyBody (world.cpp:112-175): its joints (waking both
+ *                           bodies of each), its fixtures' contacts and proxies, then the body
+ * For a destroy, `index` is the creation index at that point; later creation indices move down
+ * by one, as they do in a list that loses an element. Operations apply in order. Proxy ids follow
+ * b3DynamicTree's node free list (dynamic_tree.cpp:44-136), so pair order and contact creation
+ * order match the reference; BX_E_UNSUPPORTED when a new proxy would take a node whose id depends
+ * on the dynamic tree's shape (an internal node freed by a destroy).
+ * ------------------------------------------------------------------------------------------- */
+#define BX_TOPO_CREATE_BODY 1
+#define BX_TOPO_CREATE_FIXTURE 2
+#define BX_TOPO_CREATE_JOINT 3
+#define BX_TOPO_DESTROY_BODY 4
+#define BX_TOPO_DESTROY_FIXTURE 5
+#define BX_TOPO_DESTROY_JOINT 6
+
+typedef struct bx_topology_op {
+    uint32_t op;                   /* BX_TOPO_* */
+    uint32_t index;                /* descriptor index in `adds` (create) or creation index (destroy) */
+} bx_topology_op;
+
+/* Apply `op_count` topology operations to world `world` of a batch (not a pool). The batch keeps its
+   handle; its device arrays are rebuilt with the world's new capacities and every other world
+   carried over unchanged. `library` (or NULL to keep the batch's) must begin with the batch's
+   hulls and meshes, so existing indices stay valid. Synchronous; on an error the batch is
+   unchanged. */
+int bx_batch_edit_topology(bx_batch* batch, uint32_t world, const bx_shape_library* library,
+                           const bx_world_desc* adds, const bx_topology_op* ops, uint32_t op_count);
+
+/* ---------------------------------------------------------------------------------------------
  * World queries between steps, many at once over any worlds of a batch or pool: the rays and
  * boxes of b3World::RayCast, RayCastSingle and QueryAABB (world.h:141-172, world.cpp:452-566,
  * 852-870) against every fixture's broadphase proxy (its fat AABB) and shape. A ray is a candidate

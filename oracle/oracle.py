@@ -48,6 +48,8 @@ def load(kind):
         "destroy": fn("destroy", None, P),
         "step": fn("step", None, P, C.c_float, u, u, u),
         "edit_bodies": fn("edit_bodies", C.c_int, P, C.POINTER(capi.bx_body_edit), u),
+        "edit_topology": fn("edit_topology", C.c_int, P, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc),
+                            C.POINTER(capi.bx_topology_op), u),
         "set_contact_filter": fn("set_contact_filter", C.c_int, P, C.c_void_p, C.c_void_p),
         "get_events": fn("get_events", C.c_int, P, C.POINTER(capi.bx_contact_event), u, C.POINTER(u)),
         "get_counts": fn("get_counts", C.c_int, P, C.POINTER(capi.bx_world_counts)),
@@ -107,6 +109,16 @@ class CpuWorld:
         """Body edits between steps: (body, op, v, p, wake) tuples, op a capi.BX_EDIT_* value."""
         if self.api["edit_bodies"](self.h, capi.body_edits(rows), len(rows)) != 0:
             raise RuntimeError(f"{self.kind}: body edit rejected")
+
+    def edit_topology(self, ops, adds, library_scene=None):
+        """Topology operations ((op, index) pairs, capi.BX_TOPO_*) with descriptors from the scenes.World
+        `adds`; `library_scene` supplies a grown shape library (its first hulls and meshes unchanged)."""
+        lib = library_scene.library() if library_scene is not None else None
+        self._keep = (lib, adds, library_scene)
+        desc = adds.desc()
+        if self.api["edit_topology"](self.h, None if lib is None else C.byref(lib), C.byref(desc),
+                                     capi.topology_ops(ops), len(ops)) != 0:
+            raise RuntimeError(f"{self.kind}: topology edit rejected")
 
     def set_contact_filter(self, collide, respond):
         """Filter tables as bx_batch_set_contact_filter takes them (uint8 arrays of n*n bits), or None."""

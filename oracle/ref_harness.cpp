@@ -162,6 +162,111 @@ void InitRegistersOnce() {
     lock.clear();
 }
 
+void AddBody(RefWorld* w, const bx_body_desc& bd) {
+    b3BodyDef def;
+    def.type = b3BodyType(bd.type);
+    def.awake = (bd.flags & BX_BODY_AWAKE) != 0;
+    def.allowSleep = (bd.flags & BX_BODY_AUTO_SLEEP) != 0;
+    def.fixedRotationX = (bd.flags & BX_BODY_FIXED_ROT_X) != 0;
+    def.fixedRotationY = (bd.flags & BX_BODY_FIXED_ROT_Y) != 0;
+    def.fixedRotationZ = (bd.flags & BX_BODY_FIXED_ROT_Z) != 0;
+    def.userData = (void*)(uintptr_t)w->bodies.size();
+    def.position = V3(bd.position);
+    def.orientation = Q4(bd.orientation);
+    def.linearVelocity = V3(bd.linear_velocity);
+    def.angularVelocity = V3(bd.angular_velocity);
+    def.linearDamping = V3(bd.linear_damping);
+    def.angularDamping = V3(bd.angular_damping);
+    def.gravityScale = V3(bd.gravity_scale);
+    def.linearSleepTolerance = scalar(bd.linear_sleep_tolerance);
+    def.angularSleepTolerance = scalar(bd.angular_sleep_tolerance);
+    w->bodies.push_back(w->world.CreateBody(def));
+}
+
+bool AddFixture(RefWorld* w, const bx_fixture_desc& fd) {
+    b3FixtureDef def;
+    def.userData = (void*)(uintptr_t)w->fixtures.size();
+    def.isSensor = fd.is_sensor != 0;
+    def.density = scalar(fd.density);
+    def.friction = scalar(fd.friction);
+    def.restitution = scalar(fd.restitution);
+    b3SphereShape sphere;
+    b3CapsuleShape capsule;
+    b3TriangleShape triangle;
+    b3HullShape hull;
+    b3MeshShape mesh;
+    switch (fd.shape_type) {
+    case BX_SHAPE_SPHERE:
+        sphere.m_center = V3(fd.vertex1);
+        sphere.m_radius = scalar(fd.radius);
+        def.shape = &sphere;
+        break;
+    case BX_SHAPE_CAPSULE:
+        capsule.m_vertex1 = V3(fd.vertex1);
+        capsule.m_vertex2 = V3(fd.vertex2);
+        capsule.m_radius = scalar(fd.radius);
+        def.shape = &capsule;
+        break;
+    case BX_SHAPE_TRIANGLE:
+        triangle.Set(V3(fd.vertex1), V3(fd.vertex2), V3(fd.vertex3));
+        def.shape = &triangle;
+        break;
+    case BX_SHAPE_HULL:
+        hull.m_hull = &w->hulls[fd.shape_index]->hull;
+        def.shape = &hull;
+        break;
+    case BX_SHAPE_MESH:
+        mesh.m_mesh = &w->meshes[fd.shape_index]->mesh;
+        mesh.m_scale = V3(fd.mesh_scale);
+        def.shape = &mesh;
+        break;
+    default:
+        return false;
+    }
+    w->fixtures.push_back(w->bodies[fd.body]->CreateFixture(def));
+    return true;
+}
+
+bool AddJoint(RefWorld* w, const bx_joint_desc& jd) {
+    if (jd.type != BX_JOINT_REVOLUTE) return false;
+    b3RevoluteJointDef def;
+    b3Body* a = w->bodies[jd.body_a];
+    b3Body* b = w->bodies[jd.body_b];
+    if (jd.use_initialize) {
+        def.Initialize(a, b, V3(jd.axis), V3(jd.anchor), scalar(jd.lower_angle), scalar(jd.upper_angle));
+    } else {
+        def.bodyA = a;
+        def.bodyB = b;
+        def.localAnchorA = V3(jd.local_anchor_a);
+        def.localRotationA = Q4(jd.local_rotation_a);
+        def.localAnchorB = V3(jd.local_anchor_b);
+        def.localRotationB = Q4(jd.local_rotation_b);
+        def.referenceRotation = Q4(jd.reference_rotation);
+        def.lowerAngle = scalar(jd.lower_angle);
+        def.upperAngle = scalar(jd.upper_angle);
+    }
+    def.collideConnected = jd.collide_connected != 0;
+    def.enableLimit = jd.enable_limit != 0;
+    def.enableMotor = jd.enable_motor != 0;
+    def.motorSpeed = scalar(jd.motor_speed);
+    def.maxMotorTorque = scalar(jd.max_motor_torque);
+    w->joints.push_back(w->world.CreateJoint(def));
+    return true;
+}
+
+void AddLibrary(RefWorld* w, const bx_shape_library* lib) {
+    for (uint32 i = uint32(w->hulls.size()); lib && i < lib->hull_count; ++i) {
+        RefHull* h = new RefHull();
+        BuildHull(h, lib->hulls[i]);
+        w->hulls.push_back(h);
+    }
+    for (uint32 i = uint32(w->meshes.size()); lib && i < lib->mesh_count; ++i) {
+        RefMesh* m = new RefMesh();
+        BuildMesh(m, lib->meshes[i]);
+        w->meshes.push_back(m);
+    }
+}
+
 RefWorld* CreateWorld(const bx_shape_library* lib, const bx_world_desc* wd) {
     InitRegistersOnce();
     RefWorld* w = new RefWorld();
@@ -171,108 +276,19 @@ RefWorld* CreateWorld(const bx_shape_library* lib, const bx_world_desc* wd) {
     w->world.SetWarmStarting((wd->flags & BX_WORLD_NO_WARM_START) == 0);
     w->world.SetAutoClearForces((wd->flags & BX_WORLD_NO_CLEAR_FORCES) == 0);
     w->world.SetContactListener(&w->recorder);
-    for (uint32 i = 0; lib && i < lib->hull_count; ++i) {
-        RefHull* h = new RefHull();
-        BuildHull(h, lib->hulls[i]);
-        w->hulls.push_back(h);
-    }
-    for (uint32 i = 0; lib && i < lib->mesh_count; ++i) {
-        RefMesh* m = new RefMesh();
-        BuildMesh(m, lib->meshes[i]);
-        w->meshes.push_back(m);
-    }
-    for (uint32 i = 0; i < wd->body_count; ++i) {
-        const bx_body_desc& bd = wd->bodies[i];
-        b3BodyDef def;
-        def.type = b3BodyType(bd.type);
-        def.awake = (bd.flags & BX_BODY_AWAKE) != 0;
-        def.allowSleep = (bd.flags & BX_BODY_AUTO_SLEEP) != 0;
-        def.fixedRotationX = (bd.flags & BX_BODY_FIXED_ROT_X) != 0;
-        def.fixedRotationY = (bd.flags & BX_BODY_FIXED_ROT_Y) != 0;
-        def.fixedRotationZ = (bd.flags & BX_BODY_FIXED_ROT_Z) != 0;
-        def.userData = (void*)(uintptr_t)i;
-        def.position = V3(bd.position);
-        def.orientation = Q4(bd.orientation);
-        def.linearVelocity = V3(bd.linear_velocity);
-        def.angularVelocity = V3(bd.angular_velocity);
-        def.linearDamping = V3(bd.linear_damping);
-        def.angularDamping = V3(bd.angular_damping);
-        def.gravityScale = V3(bd.gravity_scale);
-        def.linearSleepTolerance = scalar(bd.linear_sleep_tolerance);
-        def.angularSleepTolerance = scalar(bd.angular_sleep_tolerance);
-        w->bodies.push_back(w->world.CreateBody(def));
-    }
-    for (uint32 i = 0; i < wd->fixture_count; ++i) {
-        const bx_fixture_desc& fd = wd->fixtures[i];
-        b3FixtureDef def;
-        def.userData = (void*)(uintptr_t)i;
-        def.isSensor = fd.is_sensor != 0;
-        def.density = scalar(fd.density);
-        def.friction = scalar(fd.friction);
-        def.restitution = scalar(fd.restitution);
-        b3SphereShape sphere;
-        b3CapsuleShape capsule;
-        b3TriangleShape triangle;
-        b3HullShape hull;
-        b3MeshShape mesh;
-        switch (fd.shape_type) {
-        case BX_SHAPE_SPHERE:
-            sphere.m_center = V3(fd.vertex1);
-            sphere.m_radius = scalar(fd.radius);
-            def.shape = &sphere;
-            break;
-        case BX_SHAPE_CAPSULE:
-            capsule.m_vertex1 = V3(fd.vertex1);
-            capsule.m_vertex2 = V3(fd.vertex2);
-            capsule.m_radius = scalar(fd.radius);
-            def.shape = &capsule;
-            break;
-        case BX_SHAPE_TRIANGLE:
-            triangle.Set(V3(fd.vertex1), V3(fd.vertex2), V3(fd.vertex3));
-            def.shape = &triangle;
-            break;
-        case BX_SHAPE_HULL:
-            hull.m_hull = &w->hulls[fd.shape_index]->hull;
-            def.shape = &hull;
-            break;
-        case BX_SHAPE_MESH:
-            mesh.m_mesh = &w->meshes[fd.shape_index]->mesh;
-            mesh.m_scale = V3(fd.mesh_scale);
-            def.shape = &mesh;
-            break;
-        default:
-            delete w;
-            return nullptr;
-        }
-        w->fixtures.push_back(w->bodies[fd.body]->CreateFixture(def));
-    }
-    for (uint32 i = 0; i < wd->joint_count; ++i) {
-        const bx_joint_desc& jd = wd->joints[i];
-        if (jd.type != BX_JOINT_REVOLUTE) { delete w; return nullptr; }
-        b3RevoluteJointDef def;
-        b3Body* a = w->bodies[jd.body_a];
-        b3Body* b = w->bodies[jd.body_b];
-        if (jd.use_initialize) {
-            def.Initialize(a, b, V3(jd.axis), V3(jd.anchor), scalar(jd.lower_angle), scalar(jd.upper_angle));
-        } else {
-            def.bodyA = a;
-            def.bodyB = b;
-            def.localAnchorA = V3(jd.local_anchor_a);
-            def.localRotationA = Q4(jd.local_rotation_a);
-            def.localAnchorB = V3(jd.local_anchor_b);
-            def.localRotationB = Q4(jd.local_rotation_b);
-            def.referenceRotation = Q4(jd.reference_rotation);
-            def.lowerAngle = scalar(jd.lower_angle);
-            def.upperAngle = scalar(jd.upper_angle);
-        }
-        def.collideConnected = jd.collide_connected != 0;
-        def.enableLimit = jd.enable_limit != 0;
-        def.enableMotor = jd.enable_motor != 0;
-        def.motorSpeed = scalar(jd.motor_speed);
-        def.maxMotorTorque = scalar(jd.max_motor_torque);
-        w->joints.push_back(w->world.CreateJoint(def));
-    }
+    AddLibrary(w, lib);
+    for (uint32 i = 0; i < wd->body_count; ++i) AddBody(w, wd->bodies[i]);
+    for (uint32 i = 0; i < wd->fixture_count; ++i)
+        if (!AddFixture(w, wd->fixtures[i])) { delete w; return nullptr; }
+    for (uint32 i = 0; i < wd->joint_count; ++i)
+        if (!AddJoint(w, wd->joints[i])) { delete w; return nullptr; }
     return w;
+}
+
+// Creation indices in the user data of every body and fixture, after a destroy moved them.
+void Renumber(RefWorld* w) {
+    for (size_t i = 0; i < w->bodies.size(); ++i) w->bodies[i]->m_userData = (void*)(uintptr_t)i;
+    for (size_t i = 0; i < w->fixtures.size(); ++i) w->fixtures[i]->m_userData = (void*)(uintptr_t)i;
 }
 
 void Step(RefWorld* w, scalar dt, uint32 vit, uint32 pit) {
@@ -563,6 +579,50 @@ int BXREF(get_mass)(void* h, float* out, uint32_t cap) {
             }
     }
     return int(n);
+}
+
+// Topology changes through the reference's own b3World / b3Body calls, in order (the operations of
+// bx_batch_edit_topology).
+int BXREF(edit_topology)(void* h, const bx_shape_library* lib, const bx_world_desc* adds, const bx_topology_op* ops,
+                         uint32_t count) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    AddLibrary(w, lib);
+    for (uint32_t k = 0; k < count; ++k) {
+        const bx_topology_op& op = ops[k];
+        switch (op.op) {
+        case BX_TOPO_CREATE_BODY: AddBody(w, adds->bodies[op.index]); break;
+        case BX_TOPO_CREATE_FIXTURE:
+            if (!AddFixture(w, adds->fixtures[op.index])) return -1;
+            break;
+        case BX_TOPO_CREATE_JOINT:
+            if (!AddJoint(w, adds->joints[op.index])) return -1;
+            break;
+        case BX_TOPO_DESTROY_JOINT:
+            w->world.DestroyJoint(w->joints[op.index]);
+            w->joints.erase(w->joints.begin() + op.index);
+            break;
+        case BX_TOPO_DESTROY_FIXTURE: {
+            b3Fixture* f = w->fixtures[op.index];
+            f->GetBody()->DestroyFixture(f);
+            w->fixtures.erase(w->fixtures.begin() + op.index);
+            break;
+        }
+        case BX_TOPO_DESTROY_BODY: {
+            b3Body* b = w->bodies[op.index];
+            for (size_t j = w->joints.size(); j-- > 0;)
+                if (w->joints[j]->GetBodyA() == b || w->joints[j]->GetBodyB() == b) w->joints.erase(w->joints.begin() + j);
+            for (size_t f = w->fixtures.size(); f-- > 0;)
+                if (w->fixtures[f]->GetBody() == b) w->fixtures.erase(w->fixtures.begin() + f);
+            w->world.DestroyBody(b);
+            w->bodies.erase(w->bodies.begin() + op.index);
+            break;
+        }
+        default: return -1;
+        }
+        Renumber(w);
+    }
+    w->lastIslandOrder.clear();
+    return 0;
 }
 
 // World queries through the reference's own b3World::RayCast / RayCastSingle / QueryAABB. The ray

@@ -40,6 +40,106 @@ static bool BuildHull(Hull& h, const bx_hull_desc& d) {
     return true;
 }
 
+static void AddLibrary(World* w, const bx_shape_library* lib, bool* ok) {
+    // hulls and meshes are held by value and fixtures point at them: grow only before any fixture
+    // refers to the vectors' storage again (reserve keeps the pointers of existing entries)
+    if (!lib) return;
+    if (lib->hull_count > w->hulls.size()) {
+        std::vector<Hull> grown(lib->hull_count);
+        for (u32 i = 0; i < w->hulls.size(); ++i) grown[i] = w->hulls[i];
+        for (u32 i = u32(w->hulls.size()); i < lib->hull_count; ++i)
+            if (!BuildHull(grown[i], lib->hulls[i])) *ok = false;
+        w->hulls.swap(grown);
+        for (Fixture& f : w->fixtures)
+            if (f.shape.type == e_hull) f.shape.hull = &w->hulls[f.shape.hullIndex];
+    }
+    if (lib->mesh_count > w->meshes.size()) {
+        std::vector<Mesh> grown(lib->mesh_count);
+        for (u32 i = 0; i < w->meshes.size(); ++i) grown[i] = w->meshes[i];
+        for (u32 i = u32(w->meshes.size()); i < lib->mesh_count; ++i)
+            if (!BuildMesh(grown[i], lib->meshes[i])) *ok = false;
+        w->meshes.swap(grown);
+        for (Fixture& f : w->fixtures)
+            if (f.shape.type == e_mesh) f.shape.mesh = &w->meshes[f.shape.meshIndex];
+    }
+}
+
+static void AddBody(World* w, const bx_body_desc& d) {  // b3Body::b3Body (body.cpp:25-103)
+    w->bodies.emplace_back();
+    Body& b = w->bodies.back();
+    b.type = d.type;
+    b.flags = 0;
+    if (d.flags & BX_BODY_AWAKE) b.flags |= e_awakeFlag;
+    if (d.flags & BX_BODY_AUTO_SLEEP) b.flags |= e_autoSleepFlag;
+    if (b.type == e_dynamicBody) {
+        b.mass = 1; b.invMass = 1;
+        b.flags |= d.flags & (BX_BODY_FIXED_ROT_X | BX_BODY_FIXED_ROT_Y | BX_BODY_FIXED_ROT_Z);
+    } else {
+        b.mass = 0; b.invMass = 0;
+    }
+    b.I.SetZero(); b.invI.SetZero(); b.worldInvI.SetZero();
+    b.force.SetZero(); b.torque.SetZero();
+    b.linearVelocity = V3(d.linear_velocity);
+    b.angularVelocity = V3(d.angular_velocity);
+    b.sweep.localCenter.SetZero();
+    b.sweep.worldCenter = V3(d.position);
+    b.sweep.orientation = Q4(d.orientation);
+    b.sweep.worldCenter0 = V3(d.position);
+    b.sweep.orientation0 = Q4(d.orientation);
+    b.sweep.t0 = 0;
+    b.xf.translation = b.sweep.worldCenter;
+    b.xf.rotation = b.sweep.orientation;
+    b.linearDamping = V3(d.linear_damping);
+    b.angularDamping = V3(d.angular_damping);
+    b.gravityScale = V3(d.gravity_scale);
+    b.linearSleepTolerance = d.linear_sleep_tolerance;
+    b.angularSleepTolerance = d.angular_sleep_tolerance;
+    b.sleepTime = 0;
+}
+
+// b3Body::CreateFixture (body.cpp:105-136, fixture.cpp:43-56).
+static bool AddFixture(World* w, const bx_fixture_desc& d) {
+    if (d.body >= w->bodies.size()) return false;
+    const u32 i = u32(w->fixtures.size());
+    w->fixtures.emplace_back();
+    Fixture& f = w->fixtures.back();
+    f.body = d.body;
+    f.friction = d.friction;
+    f.restitution = d.restitution;
+    f.isSensor = d.is_sensor != 0;
+    f.density = d.density;
+    Shape& s = f.shape;
+    s.type = d.shape_type;
+    switch (d.shape_type) {
+    case BX_SHAPE_SPHERE: s.v1 = V3(d.vertex1); s.radius = d.radius; break;
+    case BX_SHAPE_CAPSULE: s.v1 = V3(d.vertex1); s.v2 = V3(d.vertex2); s.radius = d.radius; break;
+    case BX_SHAPE_TRIANGLE: InitTriangleShape(s, V3(d.vertex1), V3(d.vertex2), V3(d.vertex3)); break;
+    case BX_SHAPE_HULL:
+        if (d.shape_index >= w->hulls.size()) return false;
+        s.hull = &w->hulls[d.shape_index];
+        s.hullIndex = d.shape_index;
+        s.radius = kHullRadius;
+        break;
+    case BX_SHAPE_MESH:
+        if (d.shape_index >= w->meshes.size()) return false;
+        s.mesh = &w->meshes[d.shape_index];
+        s.meshIndex = d.shape_index;
+        s.scale = V3(d.mesh_scale);
+        s.radius = kHullRadius;
+        break;
+    default: return false;
+    }
+    Body& b = w->bodies[d.body];
+    AABB aabb;
+    ComputeAABB(s, b.xf, &aabb);
+    f.proxyId = w->CreateProxy(aabb, i);
+    if (f.proxyId == kNullU32) return false;  // a node whose id depends on the tree's shape
+    b.fixtures.insert(b.fixtures.begin(), i);
+    if (f.density > scalar(0)) w->ResetMass(d.body);
+    w->newContacts = true;
+    return true;
+}
+
 static World* Create(const bx_shape_library* lib, const bx_world_desc* wd) {
     World* w = new World();
     w->gravity = V3(wd->gravity);
@@ -47,86 +147,80 @@ static World* Create(const bx_shape_library* lib, const bx_world_desc* wd) {
     w->warmStarting = (wd->flags & BX_WORLD_NO_WARM_START) == 0;
     w->clearForces = (wd->flags & BX_WORLD_NO_CLEAR_FORCES) == 0;
     w->recordEvents = (wd->flags & BX_WORLD_EVENTS) != 0;
-    if (lib) {
-        w->hulls.resize(lib->hull_count);
-        for (u32 i = 0; i < lib->hull_count; ++i)
-            if (!BuildHull(w->hulls[i], lib->hulls[i])) { delete w; return nullptr; }
-        w->meshes.resize(lib->mesh_count);
-        for (u32 i = 0; i < lib->mesh_count; ++i)
-            if (!BuildMesh(w->meshes[i], lib->meshes[i])) { delete w; return nullptr; }
-    }
-    w->bodies.resize(wd->body_count);
-    for (u32 i = 0; i < wd->body_count; ++i) {
-        const bx_body_desc& d = wd->bodies[i];
-        Body& b = w->bodies[i];
-        b.type = d.type;
-        b.flags = 0;
-        if (d.flags & BX_BODY_AWAKE) b.flags |= e_awakeFlag;
-        if (d.flags & BX_BODY_AUTO_SLEEP) b.flags |= e_autoSleepFlag;
-        if (b.type == e_dynamicBody) {
-            b.mass = 1; b.invMass = 1;
-            b.flags |= d.flags & (BX_BODY_FIXED_ROT_X | BX_BODY_FIXED_ROT_Y | BX_BODY_FIXED_ROT_Z);
-        } else {
-            b.mass = 0; b.invMass = 0;
-        }
-        b.I.SetZero(); b.invI.SetZero(); b.worldInvI.SetZero();
-        b.force.SetZero(); b.torque.SetZero();
-        b.linearVelocity = V3(d.linear_velocity);
-        b.angularVelocity = V3(d.angular_velocity);
-        b.sweep.localCenter.SetZero();
-        b.sweep.worldCenter = V3(d.position);
-        b.sweep.orientation = Q4(d.orientation);
-        b.sweep.worldCenter0 = V3(d.position);
-        b.sweep.orientation0 = Q4(d.orientation);
-        b.sweep.t0 = 0;
-        b.xf.translation = b.sweep.worldCenter;
-        b.xf.rotation = b.sweep.orientation;
-        b.linearDamping = V3(d.linear_damping);
-        b.angularDamping = V3(d.angular_damping);
-        b.gravityScale = V3(d.gravity_scale);
-        b.linearSleepTolerance = d.linear_sleep_tolerance;
-        b.angularSleepTolerance = d.angular_sleep_tolerance;
-        b.sleepTime = 0;
-    }
-    w->fixtures.resize(wd->fixture_count);
-    for (u32 i = 0; i < wd->fixture_count; ++i) {
-        const bx_fixture_desc& d = wd->fixtures[i];
-        Fixture& f = w->fixtures[i];
-        if (d.body >= wd->body_count) { delete w; return nullptr; }
-        f.body = d.body;
-        f.friction = d.friction;
-        f.restitution = d.restitution;
-        f.isSensor = d.is_sensor != 0;
-        f.density = d.density;
-        Shape& s = f.shape;
-        s.type = d.shape_type;
-        switch (d.shape_type) {
-        case BX_SHAPE_SPHERE: s.v1 = V3(d.vertex1); s.radius = d.radius; break;
-        case BX_SHAPE_CAPSULE: s.v1 = V3(d.vertex1); s.v2 = V3(d.vertex2); s.radius = d.radius; break;
-        case BX_SHAPE_TRIANGLE: InitTriangleShape(s, V3(d.vertex1), V3(d.vertex2), V3(d.vertex3)); break;
-        case BX_SHAPE_HULL:
-            if (d.shape_index >= w->hulls.size()) { delete w; return nullptr; }
-            s.hull = &w->hulls[d.shape_index];
-            s.radius = kHullRadius;
-            break;
-        case BX_SHAPE_MESH:
-            if (d.shape_index >= w->meshes.size()) { delete w; return nullptr; }
-            s.mesh = &w->meshes[d.shape_index];
-            s.scale = V3(d.mesh_scale);
-            s.radius = kHullRadius;
-            break;
-        default: delete w; return nullptr;
-        }
-        Body& b = w->bodies[d.body];
-        AABB aabb;
-        ComputeAABB(s, b.xf, &aabb);
-        f.proxyId = w->CreateProxy(aabb, i);
-        b.fixtures.insert(b.fixtures.begin(), i);
-        if (f.density > scalar(0)) w->ResetMass(d.body);
-        w->newContacts = true;
-    }
+    bool ok = true;
+    AddLibrary(w, lib, &ok);
+    if (!ok) { delete w; return nullptr; }
+    for (u32 i = 0; i < wd->body_count; ++i) AddBody(w, wd->bodies[i]);
+    for (u32 i = 0; i < wd->fixture_count; ++i)
+        if (!AddFixture(w, wd->fixtures[i])) { delete w; return nullptr; }
     for (u32 i = 0; i < wd->joint_count; ++i) CreateJoint(*w, wd->joints[i]);
     return w;
+}
+
+// ------------------------------------------------------------------ topology changes
+// b3JointManager::Destroy (joint_manager.cpp:75-141): both bodies wake, the edges go.
+static void DestroyJoint(World* w, u32 j) {
+    const Joint J = w->joints[j];
+    w->bodies[J.bodyA].SetAwake(true);
+    w->bodies[J.bodyB].SetAwake(true);
+    for (Body& b : w->bodies) {
+        b.joints.erase(std::remove_if(b.joints.begin(), b.joints.end(), [&](const JointEdge& e) { return e.joint == j; }),
+                       b.joints.end());
+        for (JointEdge& e : b.joints)
+            if (e.joint > j) --e.joint;
+    }
+    w->joints.erase(w->joints.begin() + j);
+}
+
+// The fixture's contacts (b3ContactManager::Destroy) and proxy (b3BroadPhase::DestroyProxy) go;
+// fixture indices above it move down.
+static void RemoveFixture(World* w, u32 f) {
+    while (!w->fixtures[f].contacts.empty()) w->DestroyContact(w->fixtures[f].contacts.front());
+    w->DestroyProxy(w->fixtures[f].proxyId);
+    Body& b = w->bodies[w->fixtures[f].body];
+    b.fixtures.erase(std::find(b.fixtures.begin(), b.fixtures.end(), f));
+    w->fixtures.erase(w->fixtures.begin() + f);
+    for (Body& x : w->bodies)
+        for (u32& k : x.fixtures)
+            if (k > f) --k;
+    for (Contact* c : w->contactList) {
+        if (c->fixtureA > f) --c->fixtureA;
+        if (c->fixtureB > f) --c->fixtureB;
+    }
+    for (u32& k : w->proxyFixture)
+        if (k != kNullU32 && k > f) --k;
+}
+
+static int EditTopology(World* w, const bx_world_desc* adds, const bx_topology_op& op) {
+    switch (op.op) {
+    case BX_TOPO_CREATE_BODY: AddBody(w, adds->bodies[op.index]); return 0;
+    case BX_TOPO_CREATE_FIXTURE: return AddFixture(w, adds->fixtures[op.index]) ? 0 : -1;
+    case BX_TOPO_CREATE_JOINT: CreateJoint(*w, adds->joints[op.index]); return 0;
+    case BX_TOPO_DESTROY_JOINT: DestroyJoint(w, op.index); return 0;
+    case BX_TOPO_DESTROY_FIXTURE: {  // b3Body::DestroyFixture (body.cpp:138-191)
+        const u32 body = w->fixtures[op.index].body;
+        RemoveFixture(w, op.index);
+        w->ResetMass(body);
+        return 0;
+    }
+    case BX_TOPO_DESTROY_BODY: {  // b3World::DestroyBody (world.cpp:112-175)
+        const u32 b = op.index;
+        while (!w->bodies[b].joints.empty()) DestroyJoint(w, w->bodies[b].joints.front().joint);
+        while (!w->bodies[b].fixtures.empty()) RemoveFixture(w, w->bodies[b].fixtures.front());
+        w->bodies.erase(w->bodies.begin() + b);
+        for (Fixture& f : w->fixtures)
+            if (f.body > b) --f.body;
+        for (Joint& j : w->joints) {
+            if (j.bodyA > b) --j.bodyA;
+            if (j.bodyB > b) --j.bodyB;
+        }
+        for (Body& x : w->bodies)
+            for (JointEdge& e : x.joints)
+                if (e.other > b) --e.other;
+        return 0;
+    }
+    }
+    return -1;
 }
 
 static void FillContact(const World& w, const Contact* c, bx_contact_state* out) {
@@ -381,6 +475,18 @@ int orc_get_mass(void* h, float* out, uint32_t cap) {
             for (int r = 0; r < 3; ++r) { o[5 + 3 * c + r] = b.I[c][r]; o[14 + 3 * c + r] = b.invI[c][r]; }
     }
     return int(n);
+}
+
+int orc_edit_topology(void* h, const bx_shape_library* lib, const bx_world_desc* adds, const bx_topology_op* ops,
+                      uint32_t count) {
+    World* w = (World*)h;
+    bool ok = true;
+    AddLibrary(w, lib, &ok);
+    if (!ok) return -1;
+    for (uint32_t k = 0; k < count; ++k)
+        if (EditTopology(w, adds, ops[k])) return -1;
+    w->lastIslandOrder.clear();
+    return 0;
 }
 
 // World queries (orc_query.cpp), same contract as the reference harness's entries.

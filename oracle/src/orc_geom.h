@@ -73,6 +73,7 @@ struct Shape {
     Vec3 e1, e2, e3;
     const Hull* hull = nullptr;
     const Mesh* mesh = nullptr;
+    u32 hullIndex = 0, meshIndex = 0;  // library indices of hull and mesh
     Vec3 scale;                      // mesh scale
 };
 

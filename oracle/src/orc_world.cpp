@@ -39,6 +39,7 @@ u32 World::AllocateNode() {  // dynamic_tree.cpp:44-79: LIFO free list over a se
 
 u32 World::CreateProxy(const AABB& aabb, u32 fixture) {  // dynamic_tree.cpp:108-128, broad_phase.cpp:78-84
     u32 proxy = AllocateNode();
+    if (proxy == kNullU32) return kNullU32;  // an untracked node: the id depends on the tree's shape
     if (leafCount > 0) AllocateNode();  // InsertLeaf allocates the new parent
     ++leafCount;
     if (fatAABB.size() <= proxy) { fatAABB.resize(proxy + 1); proxyFixture.resize(proxy + 1, kNullU32); }
@@ -48,6 +49,18 @@ u32 World::CreateProxy(const AABB& aabb, u32 fixture) {  // dynamic_tree.cpp:108
     proxyFixture[proxy] = fixture;
     moveBuffer.push_back(proxy);
     return proxy;
+}
+
+// b3BroadPhase::DestroyProxy (broad_phase.cpp:87-92): unbuffer the move, then RemoveLeaf frees the
+// leaf's parent when the leaf is not the root (its id depends on the tree's shape and is not
+// tracked) and FreeNode frees the proxy's node (dynamic_tree.cpp:130-137, 333-373).
+void World::DestroyProxy(u32 proxy) {
+    for (u32& m : moveBuffer)
+        if (m == proxy) m = kNullU32;
+    if (leafCount > 1) freeStack.push_back(kNullU32);
+    freeStack.push_back(proxy);
+    --leafCount;
+    proxyFixture[proxy] = kNullU32;
 }
 
 void World::MoveProxy(u32 proxy, const AABB& aabb, const Vec3& displacement) {  // dynamic_tree.cpp:139-207

@@ -124,7 +124,8 @@ struct World {
     std::vector<AABB> fatAABB;       // indexed by proxy id
     std::vector<u32> proxyFixture;   // proxy id -> fixture, kNullU32 for internal nodes
     std::vector<u32> moveBuffer;
-    std::vector<u32> freeStack;      // freed node ids (LIFO), then sequential ids from nextNode
+    std::vector<u32> freeStack;      // freed node ids (LIFO; kNullU32: an internal node whose id is not
+                                     // tracked), then sequential ids from nextNode
     u32 nextNode = 0;
     u32 leafCount = 0;
     std::vector<std::pair<u32, u32>> lastPairs;  // unique sorted pairs of the last FindPairs
@@ -148,6 +149,7 @@ struct World {
     ~World();
     u32 AllocateNode();
     u32 CreateProxy(const AABB& aabb, u32 fixture);
+    void DestroyProxy(u32 proxy);
     void MoveProxy(u32 proxy, const AABB& aabb, const Vec3& displacement);
     void SynchronizeFixture(u32 f, const Transform& xf1, const Transform& xf2);
     void SynchronizeFixtures(u32 b);

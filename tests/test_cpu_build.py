@@ -128,6 +128,30 @@ print("queries equal to the oracle", flush=True)
     assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
 
 
+def test_cpu_build_topology_edits(tmp_path):
+    """The device tests of topology changes (tests/test_gpu_topology.py) on the host build under
+    AddressSanitizer with the per-world check: the rebuilt batches carry every world's state and the
+    edited world equals the oracle after every edit and step."""
+    _make("asan")
+    script = tmp_path / "topology.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {os.path.dirname(HERE)!r})
+from bounce_amd import device
+device.load({os.path.join(BUILD, "_out", "libbxstep_asan.so")!r})
+from tests import test_gpu_topology as t, topology_script as TS
+ctx = device.Context(0)
+for name in TS.SCENARIOS:
+    t.test_topology_device_equals_oracle(ctx, name)
+t.test_topology_edit_in_a_batch_keeps_other_worlds(ctx)
+t.test_topology_errors_leave_the_batch_unchanged(ctx)
+print("topology equal to the oracle", flush=True)
+""")
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
+
+
 @pytest.mark.slow
 def test_cpu_build_memory_sanitizer_mixed_worlds(tmp_path):
     _make("msan")
