@@ -116,6 +116,11 @@ def body_edits(rows):
     return out
 
 
+class bx_body_mass(C.Structure):
+    _fields_ = [("mass", f32), ("inv_mass", f32), ("local_center", _arr(f32, 3)), ("inertia", _arr(f32, 9)),
+                ("inv_inertia", _arr(f32, 9)), ("world_inv_inertia", _arr(f32, 9))]
+
+
 class bx_topology_op(C.Structure):
     _fields_ = [("op", u32), ("index", u32)]
 

@@ -2184,6 +2184,36 @@ static int UploadBodies(bx_batch* b, const WorldInfo& wi, const std::vector<Body
     return BX_OK;
 }
 
+int bx_batch_get_mass(bx_batch* b, uint32_t world, bx_body_mass* out, uint32_t capacity) {
+    if (!b || !IsResident(b, world) || (!out && capacity)) return Fail(BX_E_INVALID, "bad argument or world not resident");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    const WorldInfo& wi = b->worlds[world];
+    std::vector<Body> bodies;
+    int rc = Download(b, bodies, b->view.bodies + wi.bodyBase, wi.bodyCount);
+    if (rc) return rc;
+    const u32 n = std::min<u32>(capacity, wi.bodyCount);
+    auto put = [](float* o, const Mat33& m) {
+        const Vec3 c[3] = {m.x, m.y, m.z};
+        for (int k = 0; k < 3; ++k) {
+            o[3 * k + 0] = c[k].x;
+            o[3 * k + 1] = c[k].y;
+            o[3 * k + 2] = c[k].z;
+        }
+    };
+    for (u32 i = 0; i < n; ++i) {
+        const Body& x = bodies[i];
+        out[i].mass = x.mass;
+        out[i].inv_mass = x.invMass;
+        out[i].local_center[0] = x.localCenter.x;
+        out[i].local_center[1] = x.localCenter.y;
+        out[i].local_center[2] = x.localCenter.z;
+        put(out[i].inertia, x.I);
+        put(out[i].inv_inertia, x.invI);
+        put(out[i].world_inv_inertia, x.worldInvI);
+    }
+    return int(n);
+}
+
 int bx_batch_set_bodies(bx_batch* b, uint32_t world, const bx_body_state* in, uint32_t count) {
     if (!b || !IsResident(b, world) || !in) return Fail(BX_E_INVALID, "bad argument or world not resident");
     HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -2821,7 +2851,7 @@ uint32_t bx_struct_size(const char* name) {
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
     S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event) S(bx_ray_query)
-    S(bx_ray_hit) S(bx_aabb_query) S(bx_topology_op)
+    S(bx_ray_hit) S(bx_aabb_query) S(bx_topology_op) S(bx_body_mass)
 #undef S
     return 0;
 }

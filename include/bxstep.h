@@ -429,6 +429,19 @@ int bx_batch_get_pairs(bx_batch* batch, uint32_t world, bx_pair* out, uint32_t c
 int bx_batch_get_island_contacts(bx_batch* batch, uint32_t world, uint32_t* out, uint32_t capacity,
                                  uint32_t* count);
 
+/* Mass data of every body in creation order (b3Body::GetMass, GetInverseMass, GetLocalCenter,
+   GetInertia, GetWorldInverseInertia; body.h:237-246, 193): what ResetMass computed from the
+   fixtures (body.cpp:265-376). Matrices are b3Mat33 columns x, y, z. */
+typedef struct bx_body_mass {
+    float mass;
+    float inv_mass;
+    float local_center[3];         /* b3Sweep::localCenter */
+    float inertia[9];              /* about the centre of mass, body frame */
+    float inv_inertia[9];
+    float world_inv_inertia[9];
+} bx_body_mass;
+int bx_batch_get_mass(bx_batch* batch, uint32_t world, bx_body_mass* out, uint32_t capacity);
+
 /* Set every body's linear and angular velocity between steps, as b3Body::SetLinearVelocity and
    SetAngularVelocity (body.h:652-684) would for each body in creation order: static bodies are left
    alone, and a nonzero velocity wakes the body (b3Body::SetAwake(true), body.h:588-607). `bodies` is in
