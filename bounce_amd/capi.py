@@ -22,7 +22,10 @@ BX_WORLD_EVENTS = 0x8
 BX_EVENT_BEGIN_CONTACT, BX_EVENT_END_CONTACT, BX_EVENT_PRE_SOLVE, BX_EVENT_POST_SOLVE = 1, 2, 3, 4
 (BX_EDIT_LINEAR_VELOCITY, BX_EDIT_ANGULAR_VELOCITY, BX_EDIT_FORCE, BX_EDIT_FORCE_TO_CENTER, BX_EDIT_TORQUE,
  BX_EDIT_LINEAR_IMPULSE, BX_EDIT_ANGULAR_IMPULSE, BX_EDIT_TRANSFORM, BX_EDIT_AWAKE, BX_EDIT_SLEEPING_ALLOWED,
- BX_EDIT_GRAVITY_SCALE, BX_EDIT_LINEAR_DAMPING, BX_EDIT_ANGULAR_DAMPING) = range(1, 14)
+ BX_EDIT_GRAVITY_SCALE, BX_EDIT_LINEAR_DAMPING, BX_EDIT_ANGULAR_DAMPING, BX_EDIT_LINEAR_SLEEP_TOLERANCE,
+ BX_EDIT_ANGULAR_SLEEP_TOLERANCE, BX_EDIT_FIXED_ROTATION) = range(1, 17)
+(BX_EDIT_WORLD_GRAVITY, BX_EDIT_WORLD_SLEEPING, BX_EDIT_WORLD_WARM_STARTING, BX_EDIT_WORLD_AUTO_CLEAR_FORCES,
+ BX_EDIT_WORLD_CLEAR_FORCES) = range(32, 37)
 BX_WORLD_NO_WARM_START = 0x2
 BX_WORLD_NO_CLEAR_FORCES = 0x4
 BX_JOINT_REVOLUTE = 4

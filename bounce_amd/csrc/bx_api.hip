@@ -2695,31 +2695,64 @@ int bx_batch_edit_topology(bx_batch* b, uint32_t world, const bx_shape_library* 
 int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits, uint32_t count) {
     if (!b || !IsResident(b, world) || (!edits && count)) return Fail(BX_E_INVALID, "bad argument or world not resident");
     HIP_CHECK(hipSetDevice(b->ctx->device));
-    const WorldInfo& wi = b->worlds[world];
-    bool transform = false;
+    WorldInfo& wi = b->worlds[world];
+    bool transform = false, fixed = false, worldEdit = false;
     for (u32 k = 0; k < count; ++k) {
-        if (edits[k].body >= wi.bodyCount) return Fail(BX_E_INVALID, "edit names a body out of range");
-        if (edits[k].op < BX_EDIT_LINEAR_VELOCITY || edits[k].op > BX_EDIT_ANGULAR_DAMPING)
+        const u32 op = edits[k].op;
+        const bool worldOp = op >= BX_EDIT_WORLD_GRAVITY && op <= BX_EDIT_WORLD_CLEAR_FORCES;
+        if (!worldOp && (op < BX_EDIT_LINEAR_VELOCITY || op > BX_EDIT_FIXED_ROTATION))
             return Fail(BX_E_INVALID, "unknown body edit");
-        transform = transform || edits[k].op == BX_EDIT_TRANSFORM;
+        if (!worldOp && edits[k].body >= wi.bodyCount) return Fail(BX_E_INVALID, "edit names a body out of range");
+        transform = transform || op == BX_EDIT_TRANSFORM;
+        fixed = fixed || op == BX_EDIT_FIXED_ROTATION;
+        worldEdit = worldEdit || worldOp;
     }
     if (count == 0) return BX_OK;
     if (transform) ++b->stateGen;  // SetTransform moves fat AABBs
     std::vector<Body> bodies;
     int rc = Download(b, bodies, b->view.bodies + wi.bodyBase, wi.bodyCount);
     if (rc) return rc;
-    // SetTransform moves the body's fixture proxies: their fixtures, fat AABBs and moved flags
+    // SetTransform moves the body's fixture proxies: their fixtures, fat AABBs and moved flags;
+    // SetFixedRotation recomputes the mass from the body's fixtures
     std::vector<u32> flist, moved;
     std::vector<Fixture> fixtures;
     std::vector<AABB> fat;
-    if (transform) {
+    if (transform || fixed) {
         if ((rc = Download(b, flist, b->view.fixtureList + wi.fixtureListBase, wi.fixtureCount))) return rc;
         if ((rc = Download(b, fixtures, b->view.fixtures + wi.fixtureBase, wi.fixtureCount))) return rc;
+    }
+    if (transform) {
         if ((rc = Download(b, fat, b->view.fatAABB + wi.fixtureBase, wi.fixtureCount))) return rc;
         if ((rc = Download(b, moved, b->view.moved + wi.fixtureBase, wi.fixtureCount))) return rc;
     }
+    u32 worldFlags = wi.flags;
+    Vec3 gravity = wi.gravity;
     for (u32 k = 0; k < count; ++k) {
         const bx_body_edit& e = edits[k];
+        switch (e.op) {  // world edits
+        case BX_EDIT_WORLD_GRAVITY: gravity = V3(e.v); continue;
+        case BX_EDIT_WORLD_SLEEPING:  // world.cpp:82-92
+            if (e.wake) {
+                worldFlags |= BX_WORLD_SLEEPING;
+            } else {
+                worldFlags &= ~BX_WORLD_SLEEPING;
+                for (Body& x : bodies) HostSetAwake(x, true);
+            }
+            continue;
+        case BX_EDIT_WORLD_WARM_STARTING:
+            worldFlags = e.wake ? (worldFlags & ~BX_WORLD_NO_WARM_START) : (worldFlags | BX_WORLD_NO_WARM_START);
+            continue;
+        case BX_EDIT_WORLD_AUTO_CLEAR_FORCES:
+            worldFlags = e.wake ? (worldFlags & ~BX_WORLD_NO_CLEAR_FORCES) : (worldFlags | BX_WORLD_NO_CLEAR_FORCES);
+            continue;
+        case BX_EDIT_WORLD_CLEAR_FORCES:  // world.cpp:244-251
+            for (Body& x : bodies) {
+                x.force.SetZero();
+                x.torque.SetZero();
+            }
+            continue;
+        default: break;
+        }
         Body& x = bodies[e.body];
         const Vec3 v = V3(e.v), p(e.p[0], e.p[1], e.p[2]);
         const bool dynamic = x.type == kDynamic;
@@ -2785,7 +2818,24 @@ int bx_batch_edit_bodies(bx_batch* b, uint32_t world, const bx_body_edit* edits,
             break;
         case BX_EDIT_LINEAR_DAMPING: x.linearDamping = v; break;
         case BX_EDIT_ANGULAR_DAMPING: x.angularDamping = v; break;
+        case BX_EDIT_LINEAR_SLEEP_TOLERANCE: x.linTol = e.v[0]; break;
+        case BX_EDIT_ANGULAR_SLEEP_TOLERANCE: x.angTol = e.v[0]; break;
+        case BX_EDIT_FIXED_ROTATION: {  // body.cpp:507-548
+            const u32 want = (e.v[0] != 0.0f ? kFixedX : 0u) | (e.v[1] != 0.0f ? kFixedY : 0u) |
+                             (e.v[2] != 0.0f ? kFixedZ : 0u);
+            if ((x.flags & (kFixedX | kFixedY | kFixedZ)) == want) break;
+            x.flags = (x.flags & ~(kFixedX | kFixedY | kFixedZ)) | want;
+            x.w.SetZero();
+            ResetMass(x, fixtures, flist, 0, *b->library);
+            break;
         }
+        }
+    }
+    if (worldEdit) {
+        wi.flags = worldFlags;
+        wi.gravity = gravity;
+        HIP_CHECK(hipMemcpyAsync((void*)(b->view.worlds + world), &wi, sizeof(wi), hipMemcpyHostToDevice,
+                                 b->ctx->stream));
     }
     if (transform) {
         HIP_CHECK(hipMemcpyAsync(b->view.fatAABB + wi.fixtureBase, fat.data(), fat.size() * sizeof(AABB),

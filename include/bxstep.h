@@ -462,11 +462,24 @@ int bx_batch_set_bodies(bx_batch* batch, uint32_t world, const bx_body_state* bo
 #define BX_EDIT_GRAVITY_SCALE 11    /* b3Body::SetGravityScale(v)                body.h:634-640 */
 #define BX_EDIT_LINEAR_DAMPING 12   /* b3Body::SetLinearDamping(v)               body.h:614-617 */
 #define BX_EDIT_ANGULAR_DAMPING 13  /* b3Body::SetAngularDamping(v)              body.h:624-627 */
+#define BX_EDIT_LINEAR_SLEEP_TOLERANCE 14  /* b3Body::SetLinearSleepTolerance(v[0])  body.cpp:550-554 */
+#define BX_EDIT_ANGULAR_SLEEP_TOLERANCE 15 /* b3Body::SetAngularSleepTolerance(v[0]) body.cpp:556-560 */
+#define BX_EDIT_FIXED_ROTATION 16   /* b3Body::SetFixedRotation(v[0] != 0, v[1] != 0, v[2] != 0)
+                                                                                 body.cpp:507-548 */
+/* World edits in the same list (`body` is not read), so that they keep their order among the body
+   edits: */
+#define BX_EDIT_WORLD_GRAVITY 32           /* b3World::SetGravity(v)             world.h:255-258 */
+#define BX_EDIT_WORLD_SLEEPING 33          /* b3World::SetSleeping(wake): clearing it wakes every
+                                              body                               world.cpp:82-92 */
+#define BX_EDIT_WORLD_WARM_STARTING 34     /* b3World::SetWarmStarting(wake)     world.h:95 */
+#define BX_EDIT_WORLD_AUTO_CLEAR_FORCES 35 /* b3World::SetAutoClearForces(wake)  world.h:265-268 */
+#define BX_EDIT_WORLD_CLEAR_FORCES 36      /* b3World::ClearForces()             world.cpp:244-251 */
 
 typedef struct bx_body_edit {
     uint32_t body;                 /* creation index */
     uint32_t op;                   /* BX_EDIT_* */
-    uint32_t wake;                 /* `wake` of the Apply* methods; the flag of SetAwake / SetSleepingAllowed */
+    uint32_t wake;                 /* `wake` of the Apply* methods; the flag of SetAwake, SetSleepingAllowed
+                                      and the world's Set* edits */
     uint32_t reserved;
     float v[3];                    /* velocity, force, torque, impulse, translation, scale or damping */
     float p[4];                    /* point of application (x y z), or rotation (x y z w) of SetTransform */

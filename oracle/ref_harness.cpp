@@ -404,10 +404,18 @@ int BXREF(edit_bodies)(void* h, const bx_body_edit* edits, uint32_t count) {
     RefWorld* w = (RefWorld*)h;
     for (uint32_t k = 0; k < count; ++k) {
         const bx_body_edit& e = edits[k];
-        if (e.body >= w->bodies.size()) return -1;
-        b3Body* b = w->bodies[e.body];
         const b3Vec3 v(scalar(e.v[0]), scalar(e.v[1]), scalar(e.v[2]));
         const b3Vec3 p(scalar(e.p[0]), scalar(e.p[1]), scalar(e.p[2]));
+        switch (e.op) {
+        case BX_EDIT_WORLD_GRAVITY: w->world.SetGravity(v); continue;
+        case BX_EDIT_WORLD_SLEEPING: w->world.SetSleeping(e.wake != 0); continue;
+        case BX_EDIT_WORLD_WARM_STARTING: w->world.SetWarmStarting(e.wake != 0); continue;
+        case BX_EDIT_WORLD_AUTO_CLEAR_FORCES: w->world.SetAutoClearForces(e.wake != 0); continue;
+        case BX_EDIT_WORLD_CLEAR_FORCES: w->world.ClearForces(); continue;
+        default: break;
+        }
+        if (e.body >= w->bodies.size()) return -1;
+        b3Body* b = w->bodies[e.body];
         switch (e.op) {
         case BX_EDIT_LINEAR_VELOCITY: b->SetLinearVelocity(v); break;
         case BX_EDIT_ANGULAR_VELOCITY: b->SetAngularVelocity(v); break;
@@ -428,6 +436,9 @@ int BXREF(edit_bodies)(void* h, const bx_body_edit* edits, uint32_t count) {
         case BX_EDIT_GRAVITY_SCALE: b->SetGravityScale(v); break;
         case BX_EDIT_LINEAR_DAMPING: b->SetLinearDamping(v); break;
         case BX_EDIT_ANGULAR_DAMPING: b->SetAngularDamping(v); break;
+        case BX_EDIT_LINEAR_SLEEP_TOLERANCE: b->SetLinearSleepTolerance(scalar(e.v[0])); break;
+        case BX_EDIT_ANGULAR_SLEEP_TOLERANCE: b->SetAngularSleepTolerance(scalar(e.v[0])); break;
+        case BX_EDIT_FIXED_ROTATION: b->SetFixedRotation(e.v[0] != 0.0f, e.v[1] != 0.0f, e.v[2] != 0.0f); break;
         default: return -1;
         }
     }

@@ -278,9 +278,26 @@ int orc_edit_bodies(void* h, const bx_body_edit* edits, uint32_t count) {
     World* w = (World*)h;
     for (uint32_t k = 0; k < count; ++k) {
         const bx_body_edit& e = edits[k];
+        const Vec3 v(e.v[0], e.v[1], e.v[2]), p(e.p[0], e.p[1], e.p[2]);
+        switch (e.op) {  // world edits (world.h:95-170, world.cpp:82-92, 244-251)
+        case BX_EDIT_WORLD_GRAVITY: w->gravity = v; continue;
+        case BX_EDIT_WORLD_SLEEPING:
+            w->sleeping = e.wake != 0;
+            if (!w->sleeping)
+                for (Body& b : w->bodies) b.SetAwake(true);
+            continue;
+        case BX_EDIT_WORLD_WARM_STARTING: w->warmStarting = e.wake != 0; continue;
+        case BX_EDIT_WORLD_AUTO_CLEAR_FORCES: w->clearForces = e.wake != 0; continue;
+        case BX_EDIT_WORLD_CLEAR_FORCES:
+            for (Body& b : w->bodies) {
+                b.force.SetZero();
+                b.torque.SetZero();
+            }
+            continue;
+        default: break;
+        }
         if (e.body >= w->bodies.size()) return -1;
         Body& b = w->bodies[e.body];
-        const Vec3 v(e.v[0], e.v[1], e.v[2]), p(e.p[0], e.p[1], e.p[2]);
         const bool dynamic = b.type == e_dynamicBody;
         auto wakeFirst = [&]() {  // the Apply* methods: wake first if asked, act only on an awake body
             if (e.wake && !b.IsAwake()) b.SetAwake(true);
@@ -343,6 +360,19 @@ int orc_edit_bodies(void* h, const bx_body_edit* edits, uint32_t count) {
             break;
         case BX_EDIT_LINEAR_DAMPING: b.linearDamping = v; break;
         case BX_EDIT_ANGULAR_DAMPING: b.angularDamping = v; break;
+        case BX_EDIT_LINEAR_SLEEP_TOLERANCE: b.linearSleepTolerance = e.v[0]; break;
+        case BX_EDIT_ANGULAR_SLEEP_TOLERANCE: b.angularSleepTolerance = e.v[0]; break;
+        case BX_EDIT_FIXED_ROTATION: {  // body.cpp:507-548
+            const u32 mask = e_fixedRotationX | e_fixedRotationY | e_fixedRotationZ;
+            const u32 want = (e.v[0] != 0.0f ? u32(e_fixedRotationX) : 0u) |
+                             (e.v[1] != 0.0f ? u32(e_fixedRotationY) : 0u) |
+                             (e.v[2] != 0.0f ? u32(e_fixedRotationZ) : 0u);
+            if ((b.flags & mask) == want) break;
+            b.flags = (b.flags & ~mask) | want;
+            b.angularVelocity.SetZero();
+            w->ResetMass(e.body);
+            break;
+        }
         default: return -1;
         }
     }

@@ -46,6 +46,27 @@ def edits(step, rng_seed=5):
         rows += [(15, E.BX_EDIT_LINEAR_VELOCITY, (0.0, 1.0, 0.0)),
                  (16, E.BX_EDIT_FORCE, (0.0, 100.0, 0.0), (0.0, 0.0, 0.0), 0),
                  (17, E.BX_EDIT_LINEAR_IMPULSE, (0.0, 3.0, 0.0), (0.0, 1.0, 0.0), 0)]
+    if step == 50:
+        rows += [(19, E.BX_EDIT_LINEAR_SLEEP_TOLERANCE, (0.2, 0.0, 0.0)),
+                 (20, E.BX_EDIT_ANGULAR_SLEEP_TOLERANCE, (0.1, 0.0, 0.0)),
+                 (21, E.BX_EDIT_FIXED_ROTATION, (1.0, 0.0, 1.0))]
+    # world edits (the body field is not read)
+    if step == 70:
+        rows += [(0, E.BX_EDIT_WORLD_GRAVITY, (0.5, -9.0, 0.2))]
+    if step == 90:
+        rows += [(0, E.BX_EDIT_WORLD_AUTO_CLEAR_FORCES, (0, 0, 0), None, 0)]  # body 3's force accumulates
+    if step == 95:
+        rows += [(0, E.BX_EDIT_WORLD_CLEAR_FORCES, (0, 0, 0)), (0, E.BX_EDIT_WORLD_AUTO_CLEAR_FORCES, (0, 0, 0), None, 1)]
+    if step == 110:
+        rows += [(0, E.BX_EDIT_WORLD_WARM_STARTING, (0, 0, 0), None, 0)]
+    if step == 115:
+        rows += [(0, E.BX_EDIT_WORLD_WARM_STARTING, (0, 0, 0), None, 1)]
+    if step == 125:
+        rows += [(21, E.BX_EDIT_FIXED_ROTATION, (0.0, 0.0, 0.0))]
+    if step == 152:
+        rows += [(0, E.BX_EDIT_WORLD_SLEEPING, (0, 0, 0), None, 0)]  # wakes every body
+    if step == 156:
+        rows += [(0, E.BX_EDIT_WORLD_SLEEPING, (0, 0, 0), None, 1)]
     if step == 130:
         rows += [(0, E.BX_EDIT_TRANSFORM, (0.0, -0.55, 0.1), (0.0, 0.0, 0.0, 1.0)),  # move the static ground
                  (18, E.BX_EDIT_AWAKE, (0, 0, 0), None, 1)]
