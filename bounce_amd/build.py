@@ -40,6 +40,7 @@ CAL_LIB = os.path.join(LIB_DIR, "libbxcal.so")
 FACADE_LIB = os.path.join(LIB_DIR, "libbounce_amd.so")
 HELLO_BIN = os.path.join(LIB_DIR, "bx_hello_world")
 MESH_BIN = os.path.join(LIB_DIR, "bx_mesh_ground")
+SPAWN_BIN = os.path.join(LIB_DIR, "bx_spawn_apply")
 LIB_DIR_RPATH = "$ORIGIN"
 
 
@@ -58,6 +59,8 @@ def build(verbose=True):
                                     "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
     jobs.append((MESH_BIN, host + ["-o", MESH_BIN + ".tmp", os.path.join(REPO, "tests", "drop_in", "mesh_ground.cpp"),
                                    "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
+    jobs.append((SPAWN_BIN, host + ["-o", SPAWN_BIN + ".tmp", os.path.join(REPO, "tests", "drop_in", "spawn_apply.cpp"),
+                                    "-L", LIB_DIR, "-lbounce_amd", "-lbxstep", "-Wl,-rpath," + LIB_DIR_RPATH]))
     for out, cmd in jobs:
         if verbose:
             print(" ".join(cmd), flush=True)

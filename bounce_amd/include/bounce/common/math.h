@@ -27,6 +27,14 @@ template <class T> inline T b3Clamp(T a, T low, T high) { return b3Max(low, b3Mi
 template <class T> inline T b3Abs(T a) { return a >= T(0) ? a : -a; }
 inline scalar b3Sqrt(scalar x) { return std::sqrt(x); }
 
+struct b3Vec2 {
+    scalar x, y;
+    b3Vec2() {}
+    b3Vec2(scalar _x, scalar _y) : x(_x), y(_y) {}
+    void SetZero() { x = y = scalar(0); }
+    void Set(scalar _x, scalar _y) { x = _x; y = _y; }
+};
+
 struct b3Vec3 {
     scalar x, y, z;
     b3Vec3() {}
@@ -75,6 +83,21 @@ struct b3Mat33 {
     void SetIdentity() { x.Set(1, 0, 0); y.Set(0, 1, 0); z.Set(0, 0, 1); }
 };
 inline b3Vec3 operator*(const b3Mat33& A, const b3Vec3& v) { return v.x * A.x + v.y * A.y + v.z * A.z; }
+inline b3Vec3 b3Mul(const b3Mat33& A, const b3Vec3& v) { return A * v; }
+inline b3Mat33 operator*(const b3Mat33& A, const b3Mat33& B) { return b3Mat33(A * B.x, A * B.y, A * B.z); }
+inline b3Mat33 b3Transpose(const b3Mat33& A) {
+    return b3Mat33(b3Vec3(A.x.x, A.y.x, A.z.x), b3Vec3(A.x.y, A.y.y, A.z.y), b3Vec3(A.x.z, A.y.z, A.z.z));
+}
+
+// A unit vector orthogonal to v (vec3.h b3Perp).
+inline b3Vec3 b3Perp(const b3Vec3& v) {
+    static const scalar kSqrtInv3 = std::sqrt(scalar(1) / scalar(3));
+    b3Vec3 u;
+    if (b3Abs(v.x) >= kSqrtInv3) u.Set(v.y, -v.x, scalar(0));
+    else u.Set(scalar(0), v.z, -v.y);
+    u.Normalize();
+    return u;
+}
 
 // Quaternion (x, y, z) = v, w = s.
 struct b3Quat {
@@ -126,6 +149,11 @@ inline b3Vec3 b3Mul(const b3Quat& q, const b3Vec3& v) {  // v + 2 s (q x v) + 2 
     return v + q.s * t + b3Cross(q.v, t);
 }
 inline b3Vec3 b3MulC(const b3Quat& q, const b3Vec3& v) { return b3Mul(b3Conjugate(q), v); }
+// R I R^T with R the rotation matrix of q (quat.h b3RotateToFrame).
+inline b3Mat33 b3RotateToFrame(const b3Mat33& I, const b3Quat& q) {
+    b3Mat33 R = q.GetRotationMatrix();
+    return R * I * b3Transpose(R);
+}
 
 struct b3Transform {
     b3Vec3 translation;

@@ -2,11 +2,15 @@
 
 CPU: programs written against the reference's API compile and link against the drop-in headers
 (the reference's own examples/hello_world/main.cpp when the reference tree is present), and the
-drop-in b3BoxHull builds the reference's box table bit for bit. GPU: a hello-world program and a
-program dropping a sphere, a capsule and a box on a b3Mesh heightfield, built against the drop-in
-API, step on the device and print the oracle's poses on every step.
+drop-in b3BoxHull builds the reference's box table bit for bit. The programs, linked with the host build
+of the device library (tests/cpu_build), print exactly the oracle's lines; the oracle's replay of the
+program that changes its world while it runs equals the reference build's. GPU: the same programs step
+on the device and print the oracle's lines: a hello world, a sphere, a capsule and a box on a b3Mesh
+heightfield, and a world whose bodies, fixtures and joints are created and destroyed between steps while
+forces push its bodies every step (tests/drop_in/spawn_apply.cpp), with its contact lists and queries.
 """
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -20,6 +24,9 @@ LIB = os.path.join(REPO, "bounce_amd", "lib")
 REF_HELLO = "/root/reference/examples/hello_world/main.cpp"
 HELLO_BIN = os.path.join(LIB, "bx_hello_world")
 MESH_BIN = os.path.join(LIB, "bx_mesh_ground")
+SPAWN_BIN = os.path.join(LIB, "bx_spawn_apply")
+CPU_BUILD = os.path.join(REPO, "tests", "cpu_build")
+CPU_LIB = os.path.join(CPU_BUILD, "_out", "libbxstep_cpu.so")
 
 
 def _have_libs():
@@ -122,3 +129,71 @@ def test_mesh_program_on_device():
     assert len(got) == len(want) == 360
     for g, w in zip(got, want):
         assert g == w
+
+
+def _program_lines(exe, prefix="step "):
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    return out.stdout.splitlines()
+
+
+def expected_hello_lines():
+    from oracle.oracle import CpuWorld
+    w = CpuWorld("oracle", scenes.hello_world())
+    lines = []
+    for i in range(60):
+        w.step()
+        b = w.bodies()[1]
+        vals = tuple(float(x) for x in b["position"]) + tuple(float(x) for x in b["orientation"])
+        lines.append("step %d position = %.9g %.9g %.9g orientation = %.9g %.9g %.9g %.9g" % ((i + 1,) + vals))
+    return lines
+
+
+def _compare(got, want):
+    assert len(got) == len(want), (len(got), len(want))
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert g == w, f"line {i}:\n got  {g}\n want {w}"
+
+
+@pytest.mark.skipif(not shutil.which("make") or not os.path.exists("/opt/rocm/llvm/bin/clang++"),
+                    reason="the host build of the device library needs the ROCm clang++")
+@pytest.mark.parametrize("program", ["hello_world", "mesh_ground", "spawn_apply"])
+def test_drop_in_programs_on_the_host_build(tmp_path, program):
+    """The drop-in programs and the facade, linked with the host build of the kernels and the C ABI."""
+    subprocess.run(["make", "-C", CPU_BUILD, "all"], check=True, capture_output=True)
+    exe = str(tmp_path / program)
+    cmd = ["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-fno-fast-math", "-I", INC, "-I",
+           os.path.join(REPO, "include"), os.path.join(REPO, "tests", "drop_in", program + ".cpp"),
+           os.path.join(REPO, "bounce_amd", "facade", "world.cpp"), "-o", exe, CPU_LIB,
+           "-Wl,-rpath," + os.path.dirname(CPU_LIB)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    lines = _program_lines(exe)
+    if program == "hello_world":
+        _compare([l for l in lines if l.startswith("step ")], expected_hello_lines())
+    elif program == "mesh_ground":
+        _compare([l for l in lines if l.startswith("step ")], expected_mesh_lines())
+    else:
+        from tests.drop_in_replica import spawn_apply_lines
+        _compare(lines, spawn_apply_lines())
+
+
+def test_spawn_apply_replay_oracle_equals_reference():
+    """The oracle's replay of the program that changes its world while it runs (creations and destroys
+    between steps, forces every step, transforms, fixed rotation, gravity, sleeping, contact lists and
+    queries) prints what the reference build prints."""
+    from oracle import oracle as O
+    if not O.available("ref32"):
+        pytest.skip("reference build (make -C oracle ref) not present")
+    from tests.drop_in_replica import spawn_apply_lines
+    _compare(spawn_apply_lines("oracle"), spawn_apply_lines("ref32"))
+
+
+@pytest.mark.gpu
+def test_spawn_apply_program_on_device():
+    """A world changed while it runs, through the drop-in API on the device: every body on every step, the
+    contact list every ten steps and the final queries equal the oracle's replay of the same calls."""
+    from tests.drop_in_replica import spawn_apply_lines
+    assert os.path.exists(SPAWN_BIN), "bx_spawn_apply is built by bounce_amd/build.py"
+    print("[drop-in] bx_spawn_apply", flush=True)
+    _compare(_program_lines(SPAWN_BIN), spawn_apply_lines())
