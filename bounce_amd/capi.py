@@ -136,6 +136,42 @@ def topology_ops(rows):
     return out
 
 
+class bx_cast_shape(C.Structure):
+    _fields_ = [("shape_type", u32), ("shape_index", u32), ("radius", f32), ("vertex1", _arr(f32, 3)),
+                ("vertex2", _arr(f32, 3)), ("vertex3", _arr(f32, 3))]
+
+
+class bx_sweep(C.Structure):
+    _fields_ = [("local_center", _arr(f32, 3)), ("center0", _arr(f32, 3)), ("center", _arr(f32, 3)),
+                ("orientation0", _arr(f32, 4)), ("orientation", _arr(f32, 4)), ("t0", f32)]
+
+
+BX_TOI_SWEEP, BX_TOI_TRANSLATION, BX_TOI_AABB = 0, 1, 2
+BX_TOI_UNKNOWN, BX_TOI_FAILED, BX_TOI_OVERLAPPED, BX_TOI_TOUCHING, BX_TOI_SEPARATED = 0, 1, 2, 3, 4
+
+
+class bx_toi_query(C.Structure):
+    _fields_ = [("kind", u32), ("max_iterations", u32), ("t_max", f32), ("reserved", u32),
+                ("shape1", bx_cast_shape), ("shape2", bx_cast_shape), ("sweep1", bx_sweep), ("sweep2", bx_sweep),
+                ("position1", _arr(f32, 3)), ("orientation1", _arr(f32, 4)), ("position2", _arr(f32, 3)),
+                ("orientation2", _arr(f32, 4)), ("displacement1", _arr(f32, 3)), ("displacement2", _arr(f32, 3)),
+                ("lower1", _arr(f32, 3)), ("upper1", _arr(f32, 3)), ("lower2", _arr(f32, 3)), ("upper2", _arr(f32, 3))]
+
+
+class bx_toi_result(C.Structure):
+    _fields_ = [("state", u32), ("t", f32), ("iterations", u32)]
+
+
+class bx_shape_cast_query(C.Structure):
+    _fields_ = [("world", u32), ("mask", u32), ("shape", bx_cast_shape), ("position", _arr(f32, 3)),
+                ("orientation", _arr(f32, 4)), ("displacement", _arr(f32, 3))]
+
+
+class bx_shape_hit(C.Structure):
+    _fields_ = [("fixture", u32), ("child", u32), ("fraction", f32), ("point", _arr(f32, 3)),
+                ("normal", _arr(f32, 3))]
+
+
 class bx_ray_query(C.Structure):
     _fields_ = [("world", u32), ("mask", u32), ("p1", _arr(f32, 3)), ("p2", _arr(f32, 3))]
 

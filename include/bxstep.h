@@ -585,6 +585,106 @@ int bx_batch_raycast(bx_batch* batch, const bx_ray_query* rays, uint32_t count, 
 int bx_batch_query_aabb(bx_batch* batch, const bx_aabb_query* boxes, uint32_t count, const uint8_t* mask_bytes,
                         uint64_t mask_size, uint32_t* fixtures, uint32_t per_query, uint32_t* counts);
 
+/* ---------------------------------------------------------------------------------------------
+ * Time of impact and shape casts (time_of_impact.cpp, world.cpp:568-850).
+ * A cast shape is a convex shape as b3GJKProxy::Set takes it (gjk.cpp:37-95): a sphere (vertex1 the
+ * centre), a capsule (vertex1, vertex2), a triangle (vertex1..3) or a hull of the batch's shape library,
+ * with its radius. GJK inside the time of impact carries its simplex cache between iterations
+ * (gjk.cpp:651-879), as the reference's does.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct bx_cast_shape {
+    uint32_t shape_type;           /* BX_SHAPE_SPHERE, BX_SHAPE_CAPSULE, BX_SHAPE_TRIANGLE or BX_SHAPE_HULL */
+    uint32_t shape_index;          /* BX_SHAPE_HULL: index into the batch's shape library */
+    float radius;                  /* b3Shape::m_radius */
+    float vertex1[3];
+    float vertex2[3];
+    float vertex3[3];
+} bx_cast_shape;
+
+typedef struct bx_sweep {          /* b3Sweep (sweep.h:27-63) */
+    float local_center[3];
+    float center0[3];              /* worldCenter0 */
+    float center[3];               /* worldCenter */
+    float orientation0[4];         /* x y z w */
+    float orientation[4];
+    float t0;                      /* not read by the time of impact, as in the reference */
+} bx_sweep;
+
+#define BX_TOI_SWEEP 0u            /* b3TimeOfImpact(const b3TOIInput&)           time_of_impact.cpp:465-643 */
+#define BX_TOI_TRANSLATION 1u      /* b3TimeOfImpact(xf1, proxy1, d1, xf2, proxy2, d2, maxIterations)
+                                                                                  time_of_impact.cpp:647-758 */
+#define BX_TOI_AABB 2u             /* b3TimeOfImpact(aabb1, d1, aabb2, d2)        time_of_impact.cpp:764-901 */
+
+typedef struct bx_toi_query {
+    uint32_t kind;                 /* BX_TOI_* */
+    uint32_t max_iterations;       /* BX_TOI_TRANSLATION: maxIterations (the reference's default is 20) */
+    float t_max;                   /* BX_TOI_SWEEP: b3TOIInput::tMax */
+    uint32_t reserved;
+    bx_cast_shape shape1;          /* BX_TOI_SWEEP, BX_TOI_TRANSLATION */
+    bx_cast_shape shape2;
+    bx_sweep sweep1;               /* BX_TOI_SWEEP */
+    bx_sweep sweep2;
+    float position1[3];            /* BX_TOI_TRANSLATION: xf1, xf2 (orientation x y z w) */
+    float orientation1[4];
+    float position2[3];
+    float orientation2[4];
+    float displacement1[3];        /* BX_TOI_TRANSLATION and BX_TOI_AABB: d1, d2 */
+    float displacement2[3];
+    float lower1[3];               /* BX_TOI_AABB: aabb1, aabb2 */
+    float upper1[3];
+    float lower2[3];
+    float upper2[3];
+} bx_toi_query;
+
+#define BX_TOI_UNKNOWN 0u          /* b3TOIOutput::State (time_of_impact.h:37-51) */
+#define BX_TOI_FAILED 1u
+#define BX_TOI_OVERLAPPED 2u
+#define BX_TOI_TOUCHING 3u
+#define BX_TOI_SEPARATED 4u
+
+typedef struct bx_toi_result {
+    uint32_t state;                /* BX_TOI_UNKNOWN .. BX_TOI_SEPARATED */
+    float t;
+    uint32_t iterations;
+} bx_toi_result;
+
+/* `count` time-of-impact problems, one per lane, hull shapes from the batch's library. Synchronous. */
+int bx_batch_time_of_impact(bx_batch* batch, const bx_toi_query* queries, uint32_t count, bx_toi_result* out);
+
+/* b3World::ShapeCast / ShapeCastSingle (world.cpp:568-850) in world `world`: `shape` at `position`,
+   `orientation` moved by `displacement`. Candidates are the fixtures whose fat AABB overlaps the swept
+   AABB (b3TestOverlap with the shape's AABBs at both ends combined); a convex fixture is cast as one
+   b3TimeOfImpact of the translating shape against the still fixture, a mesh fixture once per triangle
+   its static tree reports for the swept AABB in the mesh frame. A hit is a cast that ends touching;
+   its point and normal come from GJK at the hit (point on the fixture's surface, normal towards the
+   shape). */
+typedef struct bx_shape_cast_query {
+    uint32_t world;
+    uint32_t mask;                 /* BX_NULL_INDEX or an offset into the call's mask bytes (b3ShapeCastFilter) */
+    bx_cast_shape shape;
+    float position[3];
+    float orientation[4];
+    float displacement[3];
+} bx_shape_cast_query;
+
+typedef struct bx_shape_hit {
+    uint32_t fixture;              /* creation index, BX_NULL_INDEX for none */
+    uint32_t child;                /* mesh triangle index; 0 for a convex fixture */
+    float fraction;
+    float point[3];
+    float normal[3];
+} bx_shape_hit;
+
+/* b3World::ShapeCastSingle for each cast: the touching hit of smallest fraction (ties: lowest fixture,
+   then triangle). Synchronous. */
+int bx_batch_shape_cast_single(bx_batch* batch, const bx_shape_cast_query* casts, uint32_t count,
+                               const uint8_t* mask_bytes, uint64_t mask_size, bx_shape_hit* out);
+/* b3World::ShapeCast with a listener that never clips (ReportFixture returning 1): every hit of cast i
+   in increasing (fraction, fixture, triangle), the first hits_per_cast of them in
+   hits[i * hits_per_cast ..]; hit_counts[i] receives their number. Synchronous. */
+int bx_batch_shape_cast(bx_batch* batch, const bx_shape_cast_query* casts, uint32_t count, const uint8_t* mask_bytes,
+                        uint64_t mask_size, bx_shape_hit* hits, uint32_t hits_per_cast, uint32_t* hit_counts);
+
 /* Per-phase GPU time per step, in milliseconds, averaged over the steps of the last bx_batch_step
    call on the batch's context (its first 1024 steps), from HIP events recorded on the context
    stream between the kernels: [0] update contacts, [1] islands, [2] island solve, [3] fixture

@@ -66,6 +66,12 @@ def load(kind):
                              C.POINTER(capi.bx_ray_hit)),
         "query_aabb": fn("query_aabb", C.c_int, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p,
                          C.POINTER(u), u, C.POINTER(u)),
+        "time_of_impact": fn("time_of_impact", C.c_int, C.POINTER(capi.bx_shape_library),
+                             C.POINTER(capi.bx_toi_query), u, C.POINTER(capi.bx_toi_result)),
+        "shape_cast": fn("shape_cast", C.c_int, P, C.POINTER(capi.bx_shape_cast_query), C.c_void_p,
+                         C.POINTER(capi.bx_shape_hit), u, C.POINTER(u)),
+        "shape_cast_single": fn("shape_cast_single", C.c_int, P, C.POINTER(capi.bx_shape_cast_query), C.c_void_p,
+                                C.POINTER(capi.bx_shape_hit)),
         "bench": fn("bench", C.c_double, C.POINTER(capi.bx_shape_library), C.POINTER(capi.bx_world_desc), u,
                     C.c_float, u, u, u, u, u),
     }
@@ -221,11 +227,45 @@ class CpuWorld:
                                out.ctypes.data_as(C.POINTER(C.c_uint32)), cnt.value, C.byref(cnt))
         return out[: cnt.value]
 
+    def shape_cast(self, query, mask=None):
+        """b3World::ShapeCast with a listener returning 1 for one bx_shape_cast_query: every hit as a
+        structured bx_shape_hit array in increasing fraction (the oracle orders ties by fixture, then
+        triangle; the reference build by fixture, then point)."""
+        m = self._mask(mask)
+        mp = None if m is None else m.ctypes.data
+        cnt = C.c_uint32(0)
+        if self.api["shape_cast"](self.h, C.byref(query), mp, None, 0, C.byref(cnt)) != 0:
+            raise RuntimeError(f"{self.kind}: shape cast rejected")
+        out = np.zeros(cnt.value, dtype=np.dtype(capi.bx_shape_hit))
+        self.api["shape_cast"](self.h, C.byref(query), mp, out.ctypes.data_as(C.POINTER(capi.bx_shape_hit)),
+                               cnt.value, C.byref(cnt))
+        return out
+
+    def shape_cast_single(self, query, mask=None):
+        """b3World::ShapeCastSingle: one bx_shape_hit record (fixture BX_NULL_INDEX for no hit)."""
+        m = self._mask(mask)
+        out = np.zeros(1, dtype=np.dtype(capi.bx_shape_hit))
+        if self.api["shape_cast_single"](self.h, C.byref(query), None if m is None else m.ctypes.data,
+                                         out.ctypes.data_as(C.POINTER(capi.bx_shape_hit))) != 0:
+            raise RuntimeError(f"{self.kind}: shape cast rejected")
+        return out[0]
+
     def mass(self):
         n = self.counts().body_count
         out = np.zeros((n, 23), dtype=np.float32)
         self.api["get_mass"](self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
         return out
+
+
+def time_of_impact(kind, scene, queries):
+    """The b3TimeOfImpact problems `queries` (a ctypes array of capi.bx_toi_query) over the hull library of
+    `scene`; returns a structured bx_toi_result array."""
+    api = load(kind)
+    lib = scene.library()
+    out = (capi.bx_toi_result * max(1, len(queries)))()
+    if api["time_of_impact"](C.byref(lib), queries, len(queries), out) != 0:
+        raise RuntimeError(f"{kind}: time of impact rejected")
+    return np.ctypeslib.as_array(out)[: len(queries)].copy()
 
 
 def bench(kind, scene, dt=1.0 / 60.0, vit=8, pit=2, warmup=0, steps=10, threads=1):

@@ -36,6 +36,7 @@
 #include <bounce/dynamics/joints/joint.h>
 #include <bounce/bounce.h>
 #include <bounce/dynamics/contacts/hull_contact.h>
+#include <bounce/collision/time_of_impact.h>
 #undef private
 #undef protected
 
@@ -700,6 +701,169 @@ int BXREF(raycast_single)(void* h, const float* p1, const float* p2, const uint8
         out->normal[a] = float(o.normal[a]);
     }
     return 1;
+}
+
+// Shape casts through the reference's own b3World::ShapeCast / ShapeCastSingle, and time-of-impact
+// problems through its b3TimeOfImpact. A cast shape is built from a bx_cast_shape; a hull comes from the
+// world's library (shape casts) or from the given library (time of impact).
+struct CastShapeHolder {
+    b3SphereShape sphere;
+    b3CapsuleShape capsule;
+    b3TriangleShape triangle;
+    b3HullShape hull;
+    RefHull ownHull;
+    const b3Shape* shape = nullptr;
+};
+
+static bool MakeCastShape(const bx_cast_shape& c, RefWorld* w, const bx_shape_library* lib, CastShapeHolder& out) {
+    switch (c.shape_type) {
+    case BX_SHAPE_SPHERE:
+        out.sphere.m_center = V3(c.vertex1);
+        out.sphere.m_radius = scalar(c.radius);
+        out.shape = &out.sphere;
+        return true;
+    case BX_SHAPE_CAPSULE:
+        out.capsule.m_vertex1 = V3(c.vertex1);
+        out.capsule.m_vertex2 = V3(c.vertex2);
+        out.capsule.m_radius = scalar(c.radius);
+        out.shape = &out.capsule;
+        return true;
+    case BX_SHAPE_TRIANGLE:
+        out.triangle.Set(V3(c.vertex1), V3(c.vertex2), V3(c.vertex3));
+        out.triangle.m_radius = scalar(c.radius);
+        out.shape = &out.triangle;
+        return true;
+    case BX_SHAPE_HULL:
+        if (w) {
+            if (c.shape_index >= w->hulls.size()) return false;
+            out.hull.m_hull = &w->hulls[c.shape_index]->hull;
+        } else {
+            if (!lib || c.shape_index >= lib->hull_count) return false;
+            BuildHull(&out.ownHull, lib->hulls[c.shape_index]);
+            out.hull.m_hull = &out.ownHull.hull;
+        }
+        out.hull.m_radius = scalar(c.radius);
+        out.shape = &out.hull;
+        return true;
+    }
+    return false;
+}
+
+static b3Transform XF(const float* p, const float* q) {
+    b3Transform xf;
+    xf.translation = V3(p);
+    xf.rotation = b3Quat(b3Vec3(scalar(q[0]), scalar(q[1]), scalar(q[2])), scalar(q[3]));
+    return xf;
+}
+
+static b3Sweep SW(const bx_sweep& s) {
+    b3Sweep w;
+    w.localCenter = V3(s.local_center);
+    w.worldCenter0 = V3(s.center0);
+    w.worldCenter = V3(s.center);
+    w.orientation0 = b3Quat(b3Vec3(scalar(s.orientation0[0]), scalar(s.orientation0[1]), scalar(s.orientation0[2])),
+                            scalar(s.orientation0[3]));
+    w.orientation = b3Quat(b3Vec3(scalar(s.orientation[0]), scalar(s.orientation[1]), scalar(s.orientation[2])),
+                           scalar(s.orientation[3]));
+    w.t0 = scalar(s.t0);
+    return w;
+}
+
+int BXREF(time_of_impact)(const bx_shape_library* lib, const bx_toi_query* q, uint32_t count, bx_toi_result* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const bx_toi_query& x = q[i];
+        b3TOIOutput o;
+        if (x.kind == BX_TOI_AABB) {
+            b3AABB a, b;
+            a.lowerBound = V3(x.lower1);
+            a.upperBound = V3(x.upper1);
+            b.lowerBound = V3(x.lower2);
+            b.upperBound = V3(x.upper2);
+            o = b3TimeOfImpact(a, V3(x.displacement1), b, V3(x.displacement2));
+        } else {
+            CastShapeHolder s1, s2;
+            if (!MakeCastShape(x.shape1, nullptr, lib, s1) || !MakeCastShape(x.shape2, nullptr, lib, s2)) return -1;
+            b3GJKProxy p1(s1.shape, 0), p2(s2.shape, 0);
+            if (x.kind == BX_TOI_SWEEP) {
+                b3TOIInput in;
+                in.proxy1 = p1;
+                in.proxy2 = p2;
+                in.sweep1 = SW(x.sweep1);
+                in.sweep2 = SW(x.sweep2);
+                in.tMax = scalar(x.t_max);
+                o = b3TimeOfImpact(in);
+            } else if (x.kind == BX_TOI_TRANSLATION) {
+                o = b3TimeOfImpact(XF(x.position1, x.orientation1), p1, V3(x.displacement1),
+                                   XF(x.position2, x.orientation2), p2, V3(x.displacement2), x.max_iterations);
+            } else {
+                return -1;
+            }
+        }
+        out[i].state = uint32_t(o.state);
+        out[i].t = float(o.t);
+        out[i].iterations = o.iterations;
+    }
+    return 0;
+}
+
+struct MaskShapeFilter : public b3ShapeCastFilter {
+    const uint8_t* mask = nullptr;
+    bool ShouldShapeCast(b3Fixture* f) override { return !mask || mask[FixtureIndex(f)] != 0; }
+};
+struct CollectShapeHits : public b3ShapeCastListener {
+    std::vector<bx_shape_hit> hits;
+    scalar ReportFixture(b3Fixture* f, const b3Vec3& point, const b3Vec3& normal, scalar fraction) override {
+        bx_shape_hit h;
+        h.fixture = FixtureIndex(f);
+        h.child = 0;  // the listener is not told the mesh triangle
+        h.fraction = float(fraction);
+        for (int a = 0; a < 3; ++a) {
+            h.point[a] = float(point[a]);
+            h.normal[a] = float(normal[a]);
+        }
+        hits.push_back(h);
+        return scalar(1);
+    }
+};
+
+// Hits sorted by (fraction, fixture, point, normal), as the tree's visit order is not compared.
+int BXREF(shape_cast)(void* h, const bx_shape_cast_query* q, const uint8_t* mask, bx_shape_hit* out, uint32_t cap,
+                      uint32_t* count) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    CastShapeHolder s;
+    if (!MakeCastShape(q->shape, w, nullptr, s)) return -1;
+    MaskShapeFilter filter;
+    filter.mask = mask;
+    CollectShapeHits hits;
+    w->world.ShapeCast(&hits, &filter, s.shape, XF(q->position, q->orientation), V3(q->displacement));
+    std::stable_sort(hits.hits.begin(), hits.hits.end(), [](const bx_shape_hit& a, const bx_shape_hit& b) {
+        if (a.fraction != b.fraction) return a.fraction < b.fraction;
+        if (a.fixture != b.fixture) return a.fixture < b.fixture;
+        return std::memcmp(a.point, b.point, sizeof(float) * 6) < 0;
+    });
+    *count = uint32_t(hits.hits.size());
+    for (uint32_t i = 0; i < cap && i < hits.hits.size(); ++i) out[i] = hits.hits[i];
+    return 0;
+}
+
+int BXREF(shape_cast_single)(void* h, const bx_shape_cast_query* q, const uint8_t* mask, bx_shape_hit* out) {
+    RefWorld* w = static_cast<RefWorld*>(h);
+    CastShapeHolder s;
+    std::memset(out, 0, sizeof(*out));
+    out->fixture = BX_NULL_INDEX;
+    if (!MakeCastShape(q->shape, w, nullptr, s)) return -1;
+    MaskShapeFilter filter;
+    filter.mask = mask;
+    b3ShapeCastSingleOutput o;
+    if (!w->world.ShapeCastSingle(&o, &filter, s.shape, XF(q->position, q->orientation), V3(q->displacement)))
+        return 0;
+    out->fixture = FixtureIndex(o.fixture);
+    out->fraction = float(o.fraction);
+    for (int a = 0; a < 3; ++a) {
+        out->point[a] = float(o.point[a]);
+        out->normal[a] = float(o.normal[a]);
+    }
+    return 0;
 }
 
 int BXREF(query_aabb)(void* h, const float* lower, const float* upper, const uint8_t* mask, uint32_t* out,

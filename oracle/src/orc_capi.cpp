@@ -17,6 +17,9 @@ void InitTriangleShape(Shape& s, const Vec3& a, const Vec3& b, const Vec3& c);
 struct RayHit { u32 fixture; scalar fraction; Vec3 point, normal; };
 void RayCastAll(const World& w, const Vec3& p1, const Vec3& p2, const uint8_t* mask, std::vector<RayHit>& hits);
 void QueryAABB(const World& w, const AABB& aabb, const uint8_t* mask, std::vector<u32>& out);
+struct ShapeHit { u32 fixture, child; scalar fraction; Vec3 point, normal; };
+void ShapeCastAll(const World& w, const Shape& shape, const Transform& xf, const Vec3& d, const uint8_t* mask,
+                  std::vector<ShapeHit>& hits);
 }
 
 using namespace orc;
@@ -559,6 +562,117 @@ int orc_query_aabb(void* h, const float* lower, const float* upper, const uint8_
     QueryAABB(*w, box, mask, found);
     *count = u32(found.size());
     for (u32 i = 0; i < cap && i < found.size(); ++i) out[i] = found[i];
+    return 0;
+}
+
+// ------------------------------------------------------------------ time of impact and shape casts
+// A cast shape (bx_cast_shape) as a Shape; a hull names one of `hulls`. False when the type or the hull
+// index is out of range.
+static bool CastShape(const bx_cast_shape& c, const std::vector<Hull>* hulls, const bx_shape_library* lib,
+                      Shape& s, GJKProxy& proxy) {
+    s = Shape();
+    s.type = c.shape_type;
+    s.radius = c.radius;
+    s.v1 = V3(c.vertex1);
+    s.v2 = V3(c.vertex2);
+    s.v3 = V3(c.vertex3);
+    s.hasE1 = s.hasE2 = s.hasE3 = false;
+    if (c.shape_type == BX_SHAPE_HULL) {
+        if (hulls) {
+            if (c.shape_index >= hulls->size()) return false;
+            s.hull = &(*hulls)[c.shape_index];
+            s.hullIndex = c.shape_index;
+        } else {  // a library hull's vertices as the proxy (gjk.cpp:63-70)
+            if (!lib || c.shape_index >= lib->hull_count) return false;
+            const bx_hull_desc& d = lib->hulls[c.shape_index];
+            proxy.vertices = reinterpret_cast<const Vec3*>(d.vertices);
+            proxy.count = d.vertex_count;
+            proxy.radius = c.radius;
+            return true;
+        }
+    } else if (c.shape_type > BX_SHAPE_HULL || c.shape_type == BX_SHAPE_MESH) {
+        return false;
+    }
+    proxy.Set(s);
+    return true;
+}
+
+static Transform XF(const float* p, const float* q) {
+    return Transform(V3(p), Quat(Vec3(q[0], q[1], q[2]), q[3]));
+}
+
+static Sweep SW(const bx_sweep& s) {
+    Sweep w;
+    w.localCenter = V3(s.local_center);
+    w.worldCenter0 = V3(s.center0);
+    w.worldCenter = V3(s.center);
+    w.orientation0 = Quat(Vec3(s.orientation0[0], s.orientation0[1], s.orientation0[2]), s.orientation0[3]);
+    w.orientation = Quat(Vec3(s.orientation[0], s.orientation[1], s.orientation[2]), s.orientation[3]);
+    w.t0 = s.t0;
+    return w;
+}
+
+// The three b3TimeOfImpact forms (bx_toi_query, bxstep.h) over the hulls of `lib`.
+int orc_time_of_impact(const bx_shape_library* lib, const bx_toi_query* q, uint32_t count, bx_toi_result* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const bx_toi_query& x = q[i];
+        TOIOutput o;
+        if (x.kind == BX_TOI_AABB) {
+            AABB a, b;
+            a.lowerBound = V3(x.lower1);
+            a.upperBound = V3(x.upper1);
+            b.lowerBound = V3(x.lower2);
+            b.upperBound = V3(x.upper2);
+            o = TimeOfImpact(a, V3(x.displacement1), b, V3(x.displacement2));
+        } else {
+            Shape s1, s2;
+            GJKProxy p1, p2;
+            if (!CastShape(x.shape1, nullptr, lib, s1, p1) || !CastShape(x.shape2, nullptr, lib, s2, p2)) return -1;
+            if (x.kind == BX_TOI_SWEEP) {
+                o = TimeOfImpact(p1, SW(x.sweep1), p2, SW(x.sweep2), x.t_max);
+            } else if (x.kind == BX_TOI_TRANSLATION) {
+                o = TimeOfImpact(XF(x.position1, x.orientation1), p1, V3(x.displacement1),
+                                 XF(x.position2, x.orientation2), p2, V3(x.displacement2), x.max_iterations);
+            } else {
+                return -1;
+            }
+        }
+        out[i].state = o.state;
+        out[i].t = o.t;
+        out[i].iterations = o.iterations;
+    }
+    return 0;
+}
+
+// b3World::ShapeCast with a listener returning 1: every hit, in increasing (fraction, fixture, triangle).
+int orc_shape_cast(void* h, const bx_shape_cast_query* q, const uint8_t* mask, bx_shape_hit* out, uint32_t cap,
+                   uint32_t* count) {
+    World* w = (World*)h;
+    Shape s;
+    GJKProxy unused;
+    if (!CastShape(q->shape, &w->hulls, nullptr, s, unused)) return -1;
+    std::vector<ShapeHit> hits;
+    ShapeCastAll(*w, s, XF(q->position, q->orientation), V3(q->displacement), mask, hits);
+    *count = u32(hits.size());
+    for (u32 i = 0; i < cap && i < hits.size(); ++i) {
+        out[i].fixture = hits[i].fixture;
+        out[i].child = hits[i].child;
+        out[i].fraction = hits[i].fraction;
+        for (int a = 0; a < 3; ++a) {
+            out[i].point[a] = hits[i].point[a];
+            out[i].normal[a] = hits[i].normal[a];
+        }
+    }
+    return 0;
+}
+
+// b3World::ShapeCastSingle: the first hit of that order (fixture BX_NULL_INDEX for none).
+int orc_shape_cast_single(void* h, const bx_shape_cast_query* q, const uint8_t* mask, bx_shape_hit* out) {
+    uint32_t n = 0;
+    std::memset(out, 0, sizeof(*out));
+    int rc = orc_shape_cast(h, q, mask, out, 1, &n);
+    if (rc) return rc;
+    if (n == 0) out->fixture = BX_NULL_INDEX;
     return 0;
 }
 

@@ -329,17 +329,56 @@ struct Simplex {
 
 }  // namespace
 
+// b3Simplex::GetMetric (gjk.cpp:881-917): length, area or volume of the simplex.
+static scalar SimplexMetric(const Simplex& s) {
+    switch (s.count) {
+    case 2: return Distance(s.v[0].point, s.v[1].point);
+    case 3: return Length(Cross(s.v[1].point - s.v[0].point, s.v[2].point - s.v[0].point));
+    case 4: {
+        scalar det = Det(s.v[1].point - s.v[0].point, s.v[2].point - s.v[0].point, s.v[3].point - s.v[0].point);
+        return Sign(det) * det;
+    }
+    default: return scalar(0);
+    }
+}
+
 GJKOutput GJK(const Transform& xf1, const GJKProxy& proxy1, const Transform& xf2, const GJKProxy& proxy2,
-              bool applyRadius) {  // gjk.cpp:651-785 with an empty simplex cache (gjk.cpp:911-917)
+              bool applyRadius) {  // an empty simplex cache (gjk.cpp:911-917)
+    SimplexCache cache;
+    cache.count = 0;
+    return GJK(xf1, proxy1, xf2, proxy2, applyRadius, &cache);
+}
+
+GJKOutput GJK(const Transform& xf1, const GJKProxy& proxy1, const Transform& xf2, const GJKProxy& proxy2,
+              bool applyRadius, SimplexCache* cache) {  // gjk.cpp:651-812
     Simplex simplex;
-    // an empty cache starts from the first vertex of each proxy (gjk.cpp:829-841)
-    simplex.count = 1;
-    simplex.v[0].point1 = Mul(xf1, proxy1.vertices[0]);
-    simplex.v[0].point2 = Mul(xf2, proxy2.vertices[0]);
-    simplex.v[0].point = simplex.v[0].point2 - simplex.v[0].point1;
-    simplex.v[0].weight = scalar(1);
-    simplex.v[0].index1 = 0;
-    simplex.v[0].index2 = 0;
+    // b3Simplex::ReadCache (gjk.cpp:817-868): the cached vertices at the new transforms, dropped when
+    // their metric changed by more than a factor of two; an empty simplex starts from the first vertex
+    // of each proxy
+    simplex.count = cache->count;
+    for (u32 i = 0; i < simplex.count; ++i) {
+        SimplexVertex& v = simplex.v[i];
+        v.index1 = cache->index1[i];
+        v.index2 = cache->index2[i];
+        v.point1 = xf1 * proxy1.vertices[v.index1];
+        v.point2 = xf2 * proxy2.vertices[v.index2];
+        v.point = v.point2 - v.point1;
+        v.weight = scalar(0);
+    }
+    if (simplex.count > 1) {
+        scalar metric1 = cache->metric;
+        scalar metric2 = SimplexMetric(simplex);
+        if (metric2 < scalar(0.5) * metric1 || scalar(2) * metric1 < metric2 || metric2 < kEpsilon) simplex.count = 0;
+    }
+    if (simplex.count == 0) {
+        simplex.count = 1;
+        simplex.v[0].point1 = Mul(xf1, proxy1.vertices[0]);
+        simplex.v[0].point2 = Mul(xf2, proxy2.vertices[0]);
+        simplex.v[0].point = simplex.v[0].point2 - simplex.v[0].point1;
+        simplex.v[0].weight = scalar(1);
+        simplex.v[0].index1 = 0;
+        simplex.v[0].index2 = 0;
+    }
     u32 save1[4], save2[4];
     const Vec3 origin(scalar(0), scalar(0), scalar(0));
     const u32 kMaxIters = 20;
@@ -381,6 +420,13 @@ GJKOutput GJK(const Transform& xf1, const GJKProxy& proxy1, const Transform& xf2
     for (u32 i = 0; i < simplex.count; ++i) {
         out.simplexIndex1[i] = simplex.v[i].index1;
         out.simplexIndex2[i] = simplex.v[i].index2;
+    }
+    // b3Simplex::WriteCache (gjk.cpp:870-879)
+    cache->metric = SimplexMetric(simplex);
+    cache->count = simplex.count;
+    for (u32 i = 0; i < simplex.count; ++i) {
+        cache->index1[i] = simplex.v[i].index1;
+        cache->index2[i] = simplex.v[i].index2;
     }
     if (applyRadius) {
         scalar r1 = proxy1.radius, r2 = proxy2.radius;

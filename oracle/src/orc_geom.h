@@ -63,6 +63,9 @@ struct Mesh {
     u32 root = kNullU32;
 };
 
+// b3StaticTree::QueryAABB (static_tree.h:215-255): the triangles whose leaf box overlaps `aabb`, in stack order
+void QueryMeshTree(const Mesh& m, const AABB& aabb, std::vector<u32>& out);
+
 enum ShapeType : u32 { e_sphere = 0, e_capsule = 1, e_triangle = 2, e_hull = 3, e_mesh = 4 };
 
 struct Shape {
@@ -166,6 +169,19 @@ struct GJKProxy {
 // simplexCount / simplexIndex1: the final simplex (the cache b3GJK writes, gjk.cpp:781, 868-876)
 struct GJKOutput { Vec3 point1, point2; scalar distance; u32 iterations; u32 simplexCount; u32 simplexIndex1[4]; u32 simplexIndex2[4]; };
 GJKOutput GJK(const Transform& xf1, const GJKProxy& p1, const Transform& xf2, const GJKProxy& p2, bool applyRadius);
+// b3SimplexCache (gjk.h): the final simplex of one call, read back by the next
+struct SimplexCache { scalar metric; u32 count; u32 index1[4]; u32 index2[4]; };
+GJKOutput GJK(const Transform& xf1, const GJKProxy& p1, const Transform& xf2, const GJKProxy& p2, bool applyRadius,
+              SimplexCache* cache);
+
+// Time of impact (time_of_impact.cpp). States are b3TOIOutput::State's (time_of_impact.h:37-51).
+enum TOIState : u32 { e_toiUnknown = 0, e_toiFailed = 1, e_toiOverlapped = 2, e_toiTouching = 3, e_toiSeparated = 4 };
+struct TOIOutput { u32 state; scalar t; u32 iterations; };
+TOIOutput TimeOfImpact(const GJKProxy& proxyA, const Sweep& sweepA, const GJKProxy& proxyB, const Sweep& sweepB,
+                       scalar tMax);
+TOIOutput TimeOfImpact(const Transform& xf1, const GJKProxy& proxy1, const Vec3& d1, const Transform& xf2,
+                       const GJKProxy& proxy2, const Vec3& d2, u32 maxIterations);
+TOIOutput TimeOfImpact(const AABB& aabb1, const Vec3& d1, const AABB& aabb2, const Vec3& d2);
 
 // Narrowphase (collide/*.cpp)
 void CollideHulls(Manifold& m, const Transform& xf1, const Shape& s1, const Transform& xf2, const Shape& s2,

@@ -5,6 +5,8 @@
 //   static_tree.h:150-290      b3StaticTree::RayCast (the same test; mesh child triangles)
 //   sphere_shape.cpp:69-121, capsule_shape.cpp:180-324, hull_shape.cpp:189-281,
 //   triangle_shape.cpp:95-176, mesh_shape.cpp:81-146   b3Shape::RayCast of each shape type
+//   world.cpp:568-850          b3World::ShapeCast, ShapeCastSingle (b3TimeOfImpact of the translating
+//                              shape against each candidate fixture or mesh triangle)
 // The dynamic tree's node shapes are not reproduced: a leaf (fixture proxy) is a candidate exactly
 // when its fat AABB passes the segment test with the ray's full extent, which is what the tree
 // reports for a listener that does not clip the ray (RayCastSingle's wrapper, or a listener
@@ -280,6 +282,97 @@ void QueryAABB(const World& w, const AABB& aabb, const uint8_t* mask, std::vecto
     out.clear();
     for (u32 f = 0; f < w.fixtures.size(); ++f)
         if (TestOverlap(w.fatAABB[w.fixtures[f].proxyId], aabb) && (!mask || mask[f])) out.push_back(f);
+}
+
+// ------------------------------------------------------------------ shape casts
+struct ShapeHit { u32 fixture, child; scalar fraction; Vec3 point, normal; };
+
+// The world query wrapper's Evaluate (world.cpp:703-716): the closest points at the hit; the point on
+// the fixture's surface and the normal from the fixture towards the cast shape.
+static void EvaluateHit(Vec3* point, Vec3* normal, const Transform& xA, const GJKProxy& pA, const Transform& xB,
+                        const GJKProxy& pB) {
+    GJKOutput q = GJK(xA, pA, xB, pB, false);
+    Vec3 n = Normalize(q.point1 - q.point2);
+    *point = q.point2 + pB.radius * n;
+    *normal = n;
+}
+
+// b3GJKProxy of child triangle `tri` of a mesh shape (gjk.cpp:71-86): its scaled vertices.
+static void MeshTriangleProxy(GJKProxy& p, const Shape& mesh, u32 tri) {
+    const Mesh& m = *mesh.mesh;
+    const MeshTriangle& t = m.triangles[tri];
+    p.buffer[0] = Mul(mesh.scale, m.vertices[t.v1]);
+    p.buffer[1] = Mul(mesh.scale, m.vertices[t.v2]);
+    p.buffer[2] = Mul(mesh.scale, m.vertices[t.v3]);
+    p.vertices = p.buffer;
+    p.count = 3;
+    p.radius = mesh.radius;
+}
+
+static AABB ScaleAABB(const AABB& a, const Vec3& s) {  // b3AABB::Scale (aabb.h:287-294)
+    Vec3 lo = Mul(s, a.lowerBound), hi = Mul(s, a.upperBound);
+    AABB r;
+    r.lowerBound = Min(lo, hi);
+    r.upperBound = Max(lo, hi);
+    return r;
+}
+
+// Every touching cast of `shape` at `xf` moved by `d` against the fixtures (ShouldShapeCast(fixture) when
+// mask is given), as a listener that never clips sees them: one per convex fixture, one per reported
+// mesh triangle; in increasing (fraction, fixture, triangle).
+void ShapeCastAll(const World& w, const Shape& shape, const Transform& xf, const Vec3& d, const uint8_t* mask,
+                  std::vector<ShapeHit>& hits) {
+    hits.clear();
+    GJKProxy proxyA;
+    proxyA.Set(shape);
+    Transform xf2 = xf;
+    xf2.translation += d;
+    AABB a1, a2;
+    ComputeAABB(shape, xf, &a1);
+    ComputeAABB(shape, xf2, &a2);
+    const AABB aabb = Combine(a1, a2);
+    const Vec3 zero(scalar(0), scalar(0), scalar(0));
+    auto cast = [&](u32 f, u32 child, const Transform& xfB, const GJKProxy& proxyB) {
+        TOIOutput toi = TimeOfImpact(xf, proxyA, d, xfB, proxyB, zero, 20);
+        if (toi.state != e_toiTouching) return;
+        ShapeHit h;
+        h.fixture = f;
+        h.child = child;
+        h.fraction = toi.t;
+        Transform xt;
+        xt.rotation = xf.rotation;
+        xt.translation = xf.translation + toi.t * d;
+        EvaluateHit(&h.point, &h.normal, xt, proxyA, xfB, proxyB);
+        hits.push_back(h);
+    };
+    for (u32 f = 0; f < w.fixtures.size(); ++f) {
+        if (!TestOverlap(w.fatAABB[w.fixtures[f].proxyId], aabb) || (mask && !mask[f])) continue;
+        const Fixture& fx = w.fixtures[f];
+        const Transform& xfB = w.bodies[fx.body].xf;
+        if (fx.shape.type == e_mesh) {
+            const Vec3 s = fx.shape.scale;
+            const Vec3 inv(scalar(1) / s.x, scalar(1) / s.y, scalar(1) / s.z);
+            AABB b1, b2;
+            ComputeAABB(shape, MulT(xfB, xf), &b1);
+            ComputeAABB(shape, MulT(xfB, xf2), &b2);
+            std::vector<u32> tris;
+            QueryMeshTree(*fx.shape.mesh, Combine(ScaleAABB(b1, inv), ScaleAABB(b2, inv)), tris);
+            for (u32 t : tris) {
+                GJKProxy proxyB;
+                MeshTriangleProxy(proxyB, fx.shape, t);
+                cast(f, t, xfB, proxyB);
+            }
+        } else {
+            GJKProxy proxyB;
+            proxyB.Set(fx.shape);
+            cast(f, 0, xfB, proxyB);
+        }
+    }
+    std::stable_sort(hits.begin(), hits.end(), [](const ShapeHit& a, const ShapeHit& b) {
+        if (a.fraction != b.fraction) return a.fraction < b.fraction;
+        if (a.fixture != b.fixture) return a.fixture < b.fixture;
+        return a.child < b.child;
+    });
 }
 
 }  // namespace orc
