@@ -29,6 +29,9 @@ void LaunchStep(const BatchView& view, const u32* worldList, u32 firstWorld, u32
 void LaunchInitialPairs(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
 void LaunchBuildBvh(const BatchView& view, const u32* worldList, u32 worldCount, hipStream_t stream);
 void LaunchQueries(const BatchView& view, const QueryParams& qp, hipStream_t stream);
+void LaunchTimeOfImpact(const BatchView& view, const ToiQueryD* queries, u32 count, ToiResultD* out,
+                        hipStream_t stream);
+void LaunchShapeCasts(const BatchView& view, const CastParams& cp, hipStream_t stream);
 }
 
 using namespace bx;
@@ -2257,6 +2260,20 @@ static size_t Align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Runs `count` queries of one kind (0: single ray, 1: every ray hit, 2: boxes) given as world, mask
 // offset and six coordinates each; outputs go to hits (kinds 0, 1) or fixtures (kind 2) and counts.
+// The batch's query buffer grown to at least `total` bytes (contents not kept).
+static int QueryScratch(bx_batch* b, size_t total) {
+    if (total <= b->queryBytes) return BX_OK;
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    if (b->queryDev) HIP_CHECK(hipFree(b->queryDev));
+    b->queryDev = nullptr;
+    b->queryBytes = 0;
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, total));
+    b->queryDev = static_cast<char*>(p);
+    b->queryBytes = total;
+    return BX_OK;
+}
+
 static int RunQueries(bx_batch* b, u32 kind, const u32* world, const u32* maskOff, const float* coords, u32 count,
                       const uint8_t* maskBytes, u64 maskSize, u32 perQuery, bx_ray_hit* hits, uint32_t* fixtures,
                       uint32_t* counts) {
@@ -2287,16 +2304,7 @@ static int RunQueries(bx_batch* b, u32 kind, const u32* world, const u32* maskOf
     const size_t oO = Align256(oM + (maskBytes ? size_t(maskSize) : 0));
     const size_t oN = Align256(oO + std::max<size_t>(1, outCount) * perOut);
     const size_t total = Align256(oN + size_t(count) * sizeof(u32));
-    if (total > b->queryBytes) {
-        HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
-        if (b->queryDev) HIP_CHECK(hipFree(b->queryDev));
-        b->queryDev = nullptr;
-        b->queryBytes = 0;
-        void* p = nullptr;
-        HIP_CHECK(hipMalloc(&p, total));
-        b->queryDev = static_cast<char*>(p);
-        b->queryBytes = total;
-    }
+    if ((rc = QueryScratch(b, total))) return rc;
     std::vector<uint4> heads(count);
     for (u32 i = 0; i < count; ++i) heads[i] = make_uint4(world[i], maskOff[i], 0u, 0u);
     hipStream_t st = b->ctx->stream;
@@ -2386,6 +2394,195 @@ int bx_batch_query_aabb(bx_batch* b, const bx_aabb_query* boxes, uint32_t count,
     SplitQueries(boxes, count, world, mask, coords);
     return RunQueries(b, 2, world.data(), mask.data(), coords.data(), count, mask_bytes, mask_size, per_query, nullptr,
                       fixtures, counts);
+}
+
+// ------------------------------------------------------------------ time of impact and shape casts
+static bool Finite(const float* v, u32 n) {
+    for (u32 i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// A cast shape checked against the library and converted; false with the reason in `why`.
+static bool ToCastShape(const bx_batch* b, const bx_cast_shape& c, CastShapeD& out, const char** why) {
+    if (c.shape_type > BX_SHAPE_HULL || c.shape_type == BX_SHAPE_MESH) {
+        *why = "a cast shape must be a sphere, capsule, triangle or hull";
+        return false;
+    }
+    if (c.shape_type == BX_SHAPE_HULL && c.shape_index >= b->library->info.size()) {
+        *why = "a cast hull index is outside the shape library";
+        return false;
+    }
+    if (!std::isfinite(c.radius) || c.radius < 0.0f || !Finite(c.vertex1, 3) || !Finite(c.vertex2, 3) ||
+        !Finite(c.vertex3, 3)) {
+        *why = "a cast shape has a negative or non-finite radius or vertex";
+        return false;
+    }
+    out.type = c.shape_type;
+    out.index = c.shape_type == BX_SHAPE_HULL ? c.shape_index : 0u;
+    out.radius = c.radius;
+    out.v1 = V3(c.vertex1);
+    out.v2 = V3(c.vertex2);
+    out.v3 = V3(c.vertex3);
+    return true;
+}
+
+static Transform XF(const float* p, const float* q) { return Transform(V3(p), Quat(V3(q), q[3])); }
+
+static SweepD ToSweep(const bx_sweep& s) {
+    SweepD w;
+    w.localCenter = V3(s.local_center);
+    w.c0 = V3(s.center0);
+    w.c = V3(s.center);
+    w.q0 = Quat(V3(s.orientation0), s.orientation0[3]);
+    w.q = Quat(V3(s.orientation), s.orientation[3]);
+    return w;
+}
+
+int bx_batch_time_of_impact(bx_batch* b, const bx_toi_query* queries, uint32_t count, bx_toi_result* out) {
+    if (!b || ((!queries || !out) && count)) return Fail(BX_E_INVALID, "null argument");
+    if (count == 0) return BX_OK;
+    std::vector<ToiQueryD> qs(count);
+    for (u32 i = 0; i < count; ++i) {
+        const bx_toi_query& x = queries[i];
+        ToiQueryD& q = qs[i];
+        std::memset(static_cast<void*>(&q), 0, sizeof(q));
+        q.kind = x.kind;
+        q.maxIterations = x.max_iterations;
+        q.tMax = x.t_max;
+        const char* why = nullptr;
+        if (x.kind == BX_TOI_AABB) {
+            if (!Finite(x.lower1, 3) || !Finite(x.upper1, 3) || !Finite(x.lower2, 3) || !Finite(x.upper2, 3) ||
+                !Finite(x.displacement1, 3) || !Finite(x.displacement2, 3))
+                return Fail(BX_E_INVALID, "time of impact: non-finite box or displacement");
+            q.a1.lowerBound = V3(x.lower1);
+            q.a1.upperBound = V3(x.upper1);
+            q.a2.lowerBound = V3(x.lower2);
+            q.a2.upperBound = V3(x.upper2);
+            q.d1 = V3(x.displacement1);
+            q.d2 = V3(x.displacement2);
+            continue;
+        }
+        if (x.kind != BX_TOI_SWEEP && x.kind != BX_TOI_TRANSLATION) return Fail(BX_E_INVALID, "time of impact: unknown kind");
+        if (!ToCastShape(b, x.shape1, q.s1, &why) || !ToCastShape(b, x.shape2, q.s2, &why)) return Fail(BX_E_INVALID, why);
+        if (x.kind == BX_TOI_SWEEP) {
+            const bx_sweep* sw[2] = {&x.sweep1, &x.sweep2};
+            for (const bx_sweep* w : sw)
+                if (!Finite(w->local_center, 3) || !Finite(w->center0, 3) || !Finite(w->center, 3) ||
+                    !Finite(w->orientation0, 4) || !Finite(w->orientation, 4))
+                    return Fail(BX_E_INVALID, "time of impact: non-finite sweep");
+            if (!std::isfinite(x.t_max)) return Fail(BX_E_INVALID, "time of impact: non-finite tMax");
+            q.w1 = ToSweep(x.sweep1);
+            q.w2 = ToSweep(x.sweep2);
+        } else {
+            // conservative advancement stops only at its iteration cap in the worst case
+            if (x.max_iterations == 0 || x.max_iterations > 1000)
+                return Fail(BX_E_INVALID, "time of impact: max_iterations must be 1..1000");
+            if (!Finite(x.position1, 3) || !Finite(x.orientation1, 4) || !Finite(x.position2, 3) ||
+                !Finite(x.orientation2, 4) || !Finite(x.displacement1, 3) || !Finite(x.displacement2, 3))
+                return Fail(BX_E_INVALID, "time of impact: non-finite transform or displacement");
+            q.xf1 = XF(x.position1, x.orientation1);
+            q.xf2 = XF(x.position2, x.orientation2);
+            q.d1 = V3(x.displacement1);
+            q.d2 = V3(x.displacement2);
+        }
+    }
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    const size_t oR = Align256(size_t(count) * sizeof(ToiQueryD));
+    int rc = QueryScratch(b, oR + size_t(count) * sizeof(ToiResultD));
+    if (rc) return rc;
+    hipStream_t st = b->ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(b->queryDev, qs.data(), qs.size() * sizeof(ToiQueryD), hipMemcpyHostToDevice, st));
+    LaunchTimeOfImpact(b->view, reinterpret_cast<const ToiQueryD*>(b->queryDev), count,
+                       reinterpret_cast<ToiResultD*>(b->queryDev + oR), st);
+    HIP_CHECK(hipGetLastError());
+    std::vector<ToiResultD> res(count);
+    HIP_CHECK(hipMemcpyAsync(res.data(), b->queryDev + oR, res.size() * sizeof(ToiResultD), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (u32 i = 0; i < count; ++i) {
+        out[i].state = res[i].state;
+        out[i].t = res[i].t;
+        out[i].iterations = res[i].iterations;
+    }
+    return BX_OK;
+}
+
+static int RunShapeCasts(bx_batch* b, const bx_shape_cast_query* casts, u32 count, const uint8_t* maskBytes,
+                         u64 maskSize, u32 single, u32 perQuery, bx_shape_hit* hits, uint32_t* counts) {
+    if (!b || (!casts && count)) return Fail(BX_E_INVALID, "null argument");
+    if (count == 0) return BX_OK;
+    if ((single && !hits) || (!single && (!counts || (!hits && perQuery)))) return Fail(BX_E_INVALID, "null output");
+    std::vector<ShapeCastD> cs(count);
+    std::vector<u32> worlds(count);
+    for (u32 i = 0; i < count; ++i) {
+        const bx_shape_cast_query& x = casts[i];
+        if (!IsResident(b, x.world)) return Fail(BX_E_INVALID, "shape cast names a world that is not resident");
+        if (x.mask != kNull && (!maskBytes || u64(x.mask) + b->worlds[x.world].fixtureCount > maskSize))
+            return Fail(BX_E_INVALID, "shape cast mask lies outside the mask bytes");
+        if (!Finite(x.position, 3) || !Finite(x.orientation, 4) || !Finite(x.displacement, 3))
+            return Fail(BX_E_INVALID, "shape cast transform or displacement is not finite");
+        ShapeCastD& c = cs[i];
+        std::memset(static_cast<void*>(&c), 0, sizeof(c));
+        const char* why = nullptr;
+        if (!ToCastShape(b, x.shape, c.shape, &why)) return Fail(BX_E_INVALID, why);
+        c.world = x.world;
+        c.mask = x.mask;
+        c.xf = XF(x.position, x.orientation);
+        c.d = V3(x.displacement);
+        worlds[i] = x.world;
+    }
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    int rc = EnsureBvh(b, worlds);
+    if (rc) return rc;
+    const size_t outCount = single ? size_t(count) : size_t(count) * perQuery;
+    const size_t oM = Align256(size_t(count) * sizeof(ShapeCastD));
+    const size_t oO = Align256(oM + (maskBytes ? size_t(maskSize) : 0));
+    const size_t oN = Align256(oO + std::max<size_t>(1, outCount) * sizeof(ShapeHitOut));
+    if ((rc = QueryScratch(b, Align256(oN + size_t(count) * sizeof(u32))))) return rc;
+    hipStream_t st = b->ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(b->queryDev, cs.data(), cs.size() * sizeof(ShapeCastD), hipMemcpyHostToDevice, st));
+    if (maskBytes && maskSize)
+        HIP_CHECK(hipMemcpyAsync(b->queryDev + oM, maskBytes, size_t(maskSize), hipMemcpyHostToDevice, st));
+    CastParams cp;
+    cp.casts = reinterpret_cast<const ShapeCastD*>(b->queryDev);
+    cp.mask = reinterpret_cast<const uint8_t*>(b->queryDev + oM);
+    cp.count = count;
+    cp.single = single;
+    cp.perQuery = perQuery;
+    cp.hits = reinterpret_cast<ShapeHitOut*>(b->queryDev + oO);
+    cp.counts = reinterpret_cast<u32*>(b->queryDev + oN);
+    LaunchShapeCasts(b->view, cp, st);
+    HIP_CHECK(hipGetLastError());
+    if (outCount) {
+        std::vector<ShapeHitOut> out(outCount);
+        HIP_CHECK(hipMemcpyAsync(out.data(), b->queryDev + oO, outCount * sizeof(ShapeHitOut), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < outCount; ++i) {
+            hits[i].fixture = out[i].fixture;
+            hits[i].child = out[i].child;
+            hits[i].fraction = out[i].fraction;
+            hits[i].point[0] = out[i].point.x;
+            hits[i].point[1] = out[i].point.y;
+            hits[i].point[2] = out[i].point.z;
+            hits[i].normal[0] = out[i].normal.x;
+            hits[i].normal[1] = out[i].normal.y;
+            hits[i].normal[2] = out[i].normal.z;
+        }
+    }
+    if (!single)
+        HIP_CHECK(hipMemcpyAsync(counts, b->queryDev + oN, size_t(count) * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return BX_OK;
+}
+
+int bx_batch_shape_cast_single(bx_batch* b, const bx_shape_cast_query* casts, uint32_t count,
+                               const uint8_t* mask_bytes, uint64_t mask_size, bx_shape_hit* out) {
+    return RunShapeCasts(b, casts, count, mask_bytes, mask_size, 1u, 1u, out, nullptr);
+}
+
+int bx_batch_shape_cast(bx_batch* b, const bx_shape_cast_query* casts, uint32_t count, const uint8_t* mask_bytes,
+                        uint64_t mask_size, bx_shape_hit* hits, uint32_t hits_per_cast, uint32_t* hit_counts) {
+    return RunShapeCasts(b, casts, count, mask_bytes, mask_size, 0u, hits_per_cast, hits, hit_counts);
 }
 
 // ------------------------------------------------------------------ topology edits
@@ -2901,7 +3098,8 @@ uint32_t bx_struct_size(const char* name) {
     S(bx_hull_desc) S(bx_mesh_desc) S(bx_shape_library) S(bx_body_desc) S(bx_fixture_desc) S(bx_joint_desc)
     S(bx_world_desc) S(bx_body_state) S(bx_manifold_point_state) S(bx_manifold_state) S(bx_contact_state)
     S(bx_pair) S(bx_world_counts) S(bx_step_cfg) S(bx_batch_opts) S(bx_pool_opts) S(bx_pool_stats) S(bx_body_edit) S(bx_batch_totals) S(bx_contact_event) S(bx_ray_query)
-    S(bx_ray_hit) S(bx_aabb_query) S(bx_topology_op) S(bx_body_mass)
+    S(bx_ray_hit) S(bx_aabb_query) S(bx_topology_op) S(bx_body_mass) S(bx_cast_shape) S(bx_sweep) S(bx_toi_query)
+    S(bx_toi_result) S(bx_shape_cast_query) S(bx_shape_hit)
 #undef S
     return 0;
 }

@@ -409,6 +409,7 @@ __global__ void __launch_bounds__(kBlock, BX_UC_BLOCKS_PER_CU) k_update_contacts
 #include "bx_solve_kernels.h"
 #include "bx_solve_quad.h"
 #include "bx_query.h"
+#include "bx_toi.h"
 
 // ================================================================== fixture sync + clear forces
 __global__ void __launch_bounds__(kBlock) k_sync(BatchView B, StepParams sp) {
@@ -691,6 +692,18 @@ void LaunchBuildBvh(const BatchView& view, const u32* worldList, u32 worldCount,
 void LaunchQueries(const BatchView& view, const QueryParams& qp, hipStream_t stream) {
     const u32 blocks = (qp.count + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_query, dim3(blocks), dim3(kBlock), 0, stream, view, qp);
+}
+
+// One lane per time-of-impact problem (k_toi) and per shape cast (k_shape_cast).
+void LaunchTimeOfImpact(const BatchView& view, const ToiQueryD* queries, u32 count, ToiResultD* out,
+                        hipStream_t stream) {
+    const u32 blocks = (count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_toi, dim3(blocks), dim3(kBlock), 0, stream, view, queries, count, out);
+}
+
+void LaunchShapeCasts(const BatchView& view, const CastParams& cp, hipStream_t stream) {
+    const u32 blocks = (cp.count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_shape_cast, dim3(blocks), dim3(kBlock), 0, stream, view, cp);
 }
 
 // The creation-time pair search (b3World::Step's FindNewContacts after shapes were added) of the

@@ -436,6 +436,41 @@ struct QueryParams {
     u32* counts;            // kinds 1 and 2
 };
 
+// A convex shape as b3GJKProxy::Set takes it (gjk.cpp:37-95): sphere (v1 the centre), capsule (v1, v2),
+// triangle (v1..v3) or hull `index` of the library.
+struct CastShapeD { u32 type, index; scalar radius; Vec3 v1, v2, v3; };
+// b3Sweep (sweep.h:27-63) without t0, which the time of impact does not read.
+struct SweepD { Vec3 localCenter, c0, c; Quat q0, q; };
+
+// One b3TimeOfImpact problem: kind 0 two sweeps up to tMax, 1 two translations (conservative
+// advancement, at most maxIterations), 2 two moving boxes.
+struct ToiQueryD {
+    u32 kind, maxIterations;
+    scalar tMax;
+    CastShapeD s1, s2;
+    SweepD w1, w2;
+    Transform xf1, xf2;
+    Vec3 d1, d2;
+    AABB a1, a2;
+};
+// b3TOIOutput (time_of_impact.h:37-51).
+struct ToiResultD { u32 state; scalar t; u32 iterations; };
+
+// One b3World::ShapeCast of `shape` at `xf` moved by `d` in world `world`; `mask` is kNull or an offset
+// into the call's mask bytes.
+struct ShapeCastD { u32 world, mask; CastShapeD shape; Transform xf; Vec3 d; };
+struct ShapeHitOut { u32 fixture, child; scalar fraction; Vec3 point, normal; };
+
+struct CastParams {
+    const ShapeCastD* casts;
+    const uint8_t* mask;
+    u32 count;
+    u32 single;             // 1: the first hit only (ShapeCastSingle); 0: every hit, perQuery kept
+    u32 perQuery;
+    ShapeHitOut* hits;
+    u32* counts;            // single == 0
+};
+
 // BX_WORLD_HOOK(view, w) marks where a workgroup or lane starts work on world w; it compiles to
 // nothing in the device library (the CPU test build uses it to check that a world's kernels touch
 // only that world's ranges).

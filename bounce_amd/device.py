@@ -72,6 +72,11 @@ def load(path=capi.LIB_PATH):
                                        C.POINTER(capi.bx_ray_hit), u, C.POINTER(u)]),
         "bx_batch_query_aabb": (C.c_int, [P, C.POINTER(capi.bx_aabb_query), u, C.c_void_p, C.c_uint64,
                                           C.POINTER(u), u, C.POINTER(u)]),
+        "bx_batch_time_of_impact": (C.c_int, [P, C.POINTER(capi.bx_toi_query), u, C.POINTER(capi.bx_toi_result)]),
+        "bx_batch_shape_cast_single": (C.c_int, [P, C.POINTER(capi.bx_shape_cast_query), u, C.c_void_p, C.c_uint64,
+                                                 C.POINTER(capi.bx_shape_hit)]),
+        "bx_batch_shape_cast": (C.c_int, [P, C.POINTER(capi.bx_shape_cast_query), u, C.c_void_p, C.c_uint64,
+                                          C.POINTER(capi.bx_shape_hit), u, C.POINTER(u)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -326,6 +331,31 @@ class Batch:
                                         fx.ctypes.data_as(C.POINTER(C.c_uint32)), per_query,
                                         counts.ctypes.data_as(C.POINTER(C.c_uint32))))
         return fx[:, :per_query], counts
+
+    def time_of_impact(self, queries):
+        """b3TimeOfImpact for each bx_toi_query of a ctypes array (hulls from this batch's library): a
+        structured bx_toi_result array."""
+        out = (capi.bx_toi_result * max(1, len(queries)))()
+        _check(_lib.bx_batch_time_of_impact(self.h, queries, len(queries), out))
+        return np.ctypeslib.as_array(out)[: len(queries)].copy()
+
+    def shape_cast_single(self, casts, mask=None):
+        """b3World::ShapeCastSingle for each bx_shape_cast_query of a ctypes array: a bx_shape_hit array."""
+        out = np.zeros(len(casts), dtype=np.dtype(capi.bx_shape_hit))
+        mp, ms, _keep = self._mask_args(mask)
+        _check(_lib.bx_batch_shape_cast_single(self.h, casts, len(casts), mp, ms,
+                                               out.ctypes.data_as(C.POINTER(capi.bx_shape_hit))))
+        return out
+
+    def shape_cast(self, casts, per_cast=16, mask=None):
+        """b3World::ShapeCast (a listener returning 1) for each cast: (hits [n, per_cast], counts [n])."""
+        hits = np.zeros((len(casts), max(1, per_cast)), dtype=np.dtype(capi.bx_shape_hit))
+        counts = np.zeros(len(casts), dtype=np.uint32)
+        mp, ms, _keep = self._mask_args(mask)
+        _check(_lib.bx_batch_shape_cast(self.h, casts, len(casts), mp, ms,
+                                        hits.ctypes.data_as(C.POINTER(capi.bx_shape_hit)), per_cast,
+                                        counts.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return hits[:, :per_cast], counts
 
 
 class PoolFull(BxError):

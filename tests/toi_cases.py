@@ -13,7 +13,8 @@ F = np.float32
 
 
 def library():
-    """A scene whose hull library the problems index: a unit box, a flat box and four random hulls."""
+    """A scene whose hull library the problems index (a unit box, a flat box and four random hulls), with
+    one world of a static box so that a batch can be made of it."""
     s = scenes.SceneSet("toi")
     s.add_hull(hull_lib.box_hull(0.5, 0.5, 0.5))
     s.add_hull(hull_lib.box_hull(1.5, 0.2, 0.8))
@@ -23,6 +24,9 @@ def library():
         pts /= np.linalg.norm(pts, axis=1, keepdims=True)
         pts *= rng.uniform(0.4, 1.0, size=(len(pts), 1))
         s.add_hull(hull_lib.convex_hull((0.8 * pts).astype(F)))
+    f = scenes.fixture_array(1)
+    f["shape_type"] = capi.BX_SHAPE_HULL
+    s.worlds.append(scenes.World(scenes.body_array(1), f))
     return s
 
 
