@@ -966,3 +966,127 @@ void b3World::QueryAABB(b3QueryListener* listener, b3QueryFilter* filter, const 
     }
     if (rc < 0) D.error = bx_last_error();
 }
+
+// ------------------------------------------------------------------ shape casts
+bool b3World::CastShapeOf(const b3Shape* shape, void* out) {
+    bx_cast_shape& c = *static_cast<bx_cast_shape*>(out);
+    std::memset(&c, 0, sizeof(c));
+    if (!shape) return false;
+    c.shape_type = uint32(shape->GetType());
+    c.radius = shape->m_radius;
+    switch (shape->GetType()) {
+    case b3Shape::e_sphere: Copy3(c.vertex1, static_cast<const b3SphereShape*>(shape)->m_center); return true;
+    case b3Shape::e_capsule:
+        Copy3(c.vertex1, static_cast<const b3CapsuleShape*>(shape)->m_vertex1);
+        Copy3(c.vertex2, static_cast<const b3CapsuleShape*>(shape)->m_vertex2);
+        return true;
+    case b3Shape::e_triangle: {
+        const b3TriangleShape* t = static_cast<const b3TriangleShape*>(shape);
+        Copy3(c.vertex1, t->m_vertex1);
+        Copy3(c.vertex2, t->m_vertex2);
+        Copy3(c.vertex3, t->m_vertex3);
+        return true;
+    }
+    case b3Shape::e_hull:
+        if (!static_cast<const b3HullShape*>(shape)->m_hull) return false;
+        c.shape_index = HullIndex(static_cast<const b3HullShape*>(shape)->m_hull);
+        return true;
+    default: return false;  // the reference casts convex shapes only (world.cpp:755-760)
+    }
+}
+
+int b3World::FlushLibrary() {
+    b3WorldDevice& D = *m_device;
+    int rc = Flush();
+    if (rc != BX_OK || !D.batch || (D.hulls.size() <= D.hullsOnDevice && D.meshes.size() <= D.meshesOnDevice))
+        return rc;
+    bx_world_desc none;
+    std::memset(&none, 0, sizeof(none));
+    rc = bx_batch_edit_topology(D.batch, 0, &D.Library(), &none, nullptr, 0);
+    if (rc == BX_OK) {
+        D.hullsOnDevice = uint32(D.hulls.size());
+        D.meshesOnDevice = uint32(D.meshes.size());
+    }
+    return rc;
+}
+
+void b3World::ShapeCast(b3ShapeCastListener* listener, b3ShapeCastFilter* filter, const b3Shape* shape,
+                        const b3Transform& xf, const b3Vec3& displacement) const {
+    b3World* self = const_cast<b3World*>(this);
+    b3WorldDevice& D = *m_device;
+    bx_shape_cast_query q;
+    std::memset(&q, 0, sizeof(q));
+    if (!self->CastShapeOf(shape, &q.shape)) return;
+    int rc = self->FlushLibrary();
+    const uint32 nf = uint32(m_fixtures.size());
+    if (rc == BX_OK && D.batch && nf) {
+        q.world = 0;
+        q.mask = BX_NULL_INDEX;
+        Copy3(q.position, xf.translation);
+        Copy4(q.orientation, xf.rotation);
+        Copy3(q.displacement, displacement);
+        std::vector<uint8_t> mask;
+        if (filter) {
+            mask.resize(nf);
+            for (uint32 i = 0; i < nf; ++i) mask[i] = filter->ShouldShapeCast(m_fixtures[i]) ? 1 : 0;
+            q.mask = 0;
+        }
+        // a mesh fixture is hit once per triangle: a second call when the first capacity was short
+        uint32 cap = 4 * nf + 16, count = 0;
+        std::vector<bx_shape_hit> hits(cap);
+        rc = bx_batch_shape_cast(D.batch, &q, 1, mask.empty() ? nullptr : mask.data(), mask.size(), hits.data(), cap,
+                                 &count);
+        if (rc == BX_OK && count > cap) {
+            cap = count;
+            hits.resize(cap);
+            rc = bx_batch_shape_cast(D.batch, &q, 1, mask.empty() ? nullptr : mask.data(), mask.size(), hits.data(),
+                                     cap, &count);
+        }
+        if (rc == BX_OK) {
+            scalar maxFraction = scalar(1);  // as the world's cast wrapper (world.cpp:660-695)
+            for (uint32 i = 0; i < std::min(count, cap); ++i) {
+                const bx_shape_hit& h = hits[i];
+                if (h.fraction > maxFraction) continue;
+                maxFraction = listener->ReportFixture(m_fixtures[h.fixture], V3(h.point), V3(h.normal), h.fraction);
+                if (maxFraction == scalar(0)) break;
+            }
+        }
+    }
+    if (rc < 0) D.error = bx_last_error();
+}
+
+bool b3World::ShapeCastSingle(b3ShapeCastSingleOutput* output, b3ShapeCastFilter* filter, const b3Shape* shape,
+                              const b3Transform& xf, const b3Vec3& displacement) const {
+    b3World* self = const_cast<b3World*>(this);
+    b3WorldDevice& D = *m_device;
+    bx_shape_cast_query q;
+    std::memset(&q, 0, sizeof(q));
+    if (!self->CastShapeOf(shape, &q.shape)) return false;
+    int rc = self->FlushLibrary();
+    const uint32 nf = uint32(m_fixtures.size());
+    bool found = false;
+    if (rc == BX_OK && D.batch && nf) {
+        q.world = 0;
+        q.mask = BX_NULL_INDEX;
+        Copy3(q.position, xf.translation);
+        Copy4(q.orientation, xf.rotation);
+        Copy3(q.displacement, displacement);
+        std::vector<uint8_t> mask;
+        if (filter) {
+            mask.resize(nf);
+            for (uint32 i = 0; i < nf; ++i) mask[i] = filter->ShouldShapeCast(m_fixtures[i]) ? 1 : 0;
+            q.mask = 0;
+        }
+        bx_shape_hit h;
+        rc = bx_batch_shape_cast_single(D.batch, &q, 1, mask.empty() ? nullptr : mask.data(), mask.size(), &h);
+        if (rc == BX_OK && h.fixture != BX_NULL_INDEX) {
+            output->fixture = m_fixtures[h.fixture];
+            output->point = V3(h.point);
+            output->normal = V3(h.normal);
+            output->fraction = h.fraction;
+            found = true;
+        }
+    }
+    if (rc < 0) D.error = bx_last_error();
+    return found;
+}

@@ -372,6 +372,28 @@ public:
     virtual bool ShouldRayCast(b3Fixture* fixture) = 0;
 };
 
+class b3ShapeCastListener {
+public:
+    virtual ~b3ShapeCastListener() {}
+    // A touching cast; the return value becomes the cast's maximum fraction (world.cpp:684-690): 0 ends
+    // the cast, and later casts beyond it are not reported.
+    virtual scalar ReportFixture(b3Fixture* fixture, const b3Vec3& point, const b3Vec3& normal, scalar fraction) = 0;
+};
+
+class b3ShapeCastFilter {
+public:
+    virtual ~b3ShapeCastFilter() {}
+    virtual bool ShouldShapeCast(b3Fixture* fixture) = 0;
+};
+
+// Output of b3World::ShapeCastSingle.
+struct b3ShapeCastSingleOutput {
+    b3Fixture* fixture;
+    b3Vec3 point;
+    b3Vec3 normal;
+    scalar fraction;
+};
+
 // Output of b3World::RayCastSingle.
 struct b3RayCastSingleOutput {
     b3Fixture* fixture;
@@ -416,6 +438,13 @@ public:
     bool RayCastSingle(b3RayCastSingleOutput* output, b3RayCastFilter* filter, const b3Vec3& point1,
                        const b3Vec3& point2) const;
     void QueryAABB(b3QueryListener* listener, b3QueryFilter* filter, const b3AABB& aabb) const;
+    // Shape casts (world.cpp:568-850) of a sphere, capsule, triangle or hull at `xf` moved by
+    // `displacement`: every touching cast against a convex fixture or a mesh triangle, reported in
+    // increasing fraction as RayCast reports hits.
+    void ShapeCast(b3ShapeCastListener* listener, b3ShapeCastFilter* filter, const b3Shape* shape,
+                   const b3Transform& xf, const b3Vec3& displacement) const;
+    bool ShapeCastSingle(b3ShapeCastSingleOutput* output, b3ShapeCastFilter* filter, const b3Shape* shape,
+                         const b3Transform& xf, const b3Vec3& displacement) const;
 
     b3Body* GetBodyList() { return m_bodyList; }
     const b3Body* GetBodyList() const { return m_bodyList; }
@@ -443,6 +472,8 @@ private:
     void Renumber();
     void RemoveJoint(b3Joint* joint);
     uint32 HullIndex(const b3Hull* hull);
+    bool CastShapeOf(const b3Shape* shape, void* cast);  // a bx_cast_shape
+    int FlushLibrary();  // Flush, then send hulls the device world has not seen (cast shapes)
     uint32 MeshIndex(const b3Mesh* mesh);
 
     b3Vec3 m_gravity;

@@ -60,6 +60,17 @@ struct NotGround : public b3RayCastFilter {
     bool ShouldRayCast(b3Fixture* f) override { return Id(f) != 100; }
 };
 
+struct AllCasts : public b3ShapeCastListener {
+    scalar ReportFixture(b3Fixture* f, const b3Vec3& p, const b3Vec3& n, scalar t) override {
+        printf("cast hit %d %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", Id(f), p.x, p.y, p.z, n.x, n.y, n.z, t);
+        return scalar(1);
+    }
+};
+
+struct NotGroundCast : public b3ShapeCastFilter {
+    bool ShouldShapeCast(b3Fixture* f) override { return Id(f) != 100; }
+};
+
 struct Collect : public b3QueryListener {
     bool ReportFixture(b3Fixture* f) override {
         printf(" %d", Id(f));
@@ -238,6 +249,41 @@ int main() {
     printf("query");
     world.QueryAABB(&collect, nullptr, box);
     printf("\n");
+    // shape casts: a sphere dropped through the pile, a box of a hull the world has not seen swept
+    // across it, a capsule that ignores the ground
+    b3SphereShape probe;
+    probe.m_center.SetZero();
+    probe.m_radius = 0.4f;
+    b3ShapeCastSingleOutput so;
+    if (world.ShapeCastSingle(&so, nullptr, &probe, b3Transform(b3Vec3(0.2f, 12.0f, 0.1f), b3QuatIdentity()),
+                              b3Vec3(0.0f, -14.0f, 0.0f)))
+        printf("cast single %d %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", Id(so.fixture), so.point.x, so.point.y,
+               so.point.z, so.normal.x, so.normal.y, so.normal.z, so.fraction);
+    else
+        printf("cast single none\n");
+    b3BoxHull castHull(0.3f, 0.2f, 0.4f);
+    b3HullShape castBox;
+    castBox.m_hull = &castHull;
+    AllCasts casts;
+    printf("cast box\n");
+    world.ShapeCast(&casts, nullptr, &castBox,
+                    b3Transform(b3Vec3(-6.0f, 2.5f, 0.2f), b3Quat(b3Vec3(0.0f, 0.0f, 0.38268343f), 0.9238795f)),
+                    b3Vec3(12.0f, -1.0f, 0.0f));
+    b3CapsuleShape castCapsule;
+    castCapsule.m_vertex1.Set(-0.4f, 0.0f, 0.0f);
+    castCapsule.m_vertex2.Set(0.4f, 0.0f, 0.0f);
+    castCapsule.m_radius = 0.2f;
+    NotGroundCast notGroundCast;
+    printf("cast capsule without ground\n");
+    world.ShapeCast(&casts, &notGroundCast, &castCapsule, b3Transform(b3Vec3(1.0f, 9.0f, -0.5f), b3QuatIdentity()),
+                    b3Vec3(-1.5f, -10.0f, 0.8f));
+    // one more step after the library grew
+    world.Step(1.0f / 60.0f, 8, 2);
+    for (b3Body* b = world.GetBodyList(); b; b = b->GetNext()) {
+        const b3Vec3 p = b->GetPosition();
+        const b3Quat q = b->GetOrientation();
+        printf("last body %d p %.9g %.9g %.9g q %.9g %.9g %.9g %.9g\n", Id(b), p.x, p.y, p.z, q.v.x, q.v.y, q.v.z, q.s);
+    }
     printf("bodies %u joints %u contacts %u\n", world.GetBodyCount(), world.GetJointCount(), world.GetContactCount());
     return 0;
 }

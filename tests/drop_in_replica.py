@@ -297,5 +297,35 @@ def spawn_apply_lines(kind="oracle"):
         best, frac, reports, maxf = fid[h["fixture"]], h["fraction"], reports + 1, h["fraction"]
     out.append("closest %d %s reports %d" % (best, _g(frac), reports))
     out.append("query" + "".join(" %d" % fid[i] for i in R.world.query_aabb((-2.0, 0.0, -2.0), (2.0, 4.0, 2.0))))
+
+    def cast(shape_type, pos, q, disp, radius=0.0, v1=(0, 0, 0), v2=(0, 0, 0), index=0):
+        c = T.bx_shape_cast_query()
+        c.world, c.mask = 0, T.BX_NULL_INDEX
+        c.shape.shape_type, c.shape.shape_index, c.shape.radius = shape_type, index, F(radius)
+        c.shape.vertex1[:], c.shape.vertex2[:] = list(v1), list(v2)
+        c.position[:], c.orientation[:], c.displacement[:] = list(pos), list(q), list(disp)
+        return c
+
+    def cast_text(h):
+        return "%d %s" % (fid[h["fixture"]], " ".join(_g(x) for x in list(h["point"]) + list(h["normal"])
+                                                      + [h["fraction"]]))
+
+    h = R.world.shape_cast_single(cast(T.BX_SHAPE_SPHERE, (0.2, 12.0, 0.1), (0, 0, 0, 1), (0.0, -14.0, 0.0), 0.4))
+    out.append("cast single none" if h["fixture"] == T.BX_NULL_INDEX else "cast single " + cast_text(h))
+    grown = R._hull((0.3, 0.2, 0.4))  # the facade sends the new hull before casting it
+    R.world.edit_topology([], scenes.World(scenes.body_array(0), scenes.fixture_array(0)), R.scene)
+    out.append("cast box")
+    out += ["cast hit " + cast_text(h) for h in R.world.shape_cast(
+        cast(T.BX_SHAPE_HULL, (-6.0, 2.5, 0.2), (0.0, 0.0, 0.38268343, 0.9238795), (12.0, -1.0, 0.0), index=grown))]
+    out.append("cast capsule without ground")
+    out += ["cast hit " + cast_text(h) for h in R.world.shape_cast(
+        cast(T.BX_SHAPE_CAPSULE, (1.0, 9.0, -0.5), (0, 0, 0, 1), (-1.5, -10.0, 0.8), 0.2, (-0.4, 0, 0), (0.4, 0, 0)),
+        mask=not_ground)]
+    R.step()
+    st = R.world.bodies()
+    for bid in reversed(R.bodies):
+        s = st[R.bodies.index(bid)]
+        out.append("last body %d p %s %s %s q %s %s %s %s" % ((bid,) + tuple(
+            _g(x) for x in list(s["position"]) + list(s["orientation"]))))
     out.append("bodies %d joints %d contacts %d" % (len(R.bodies), len(R.joints), R.world.counts().contact_count))
     return out
