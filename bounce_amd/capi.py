@@ -28,7 +28,8 @@ BX_EVENT_BEGIN_CONTACT, BX_EVENT_END_CONTACT, BX_EVENT_PRE_SOLVE, BX_EVENT_POST_
  BX_EDIT_WORLD_CLEAR_FORCES) = range(32, 37)
 BX_WORLD_NO_WARM_START = 0x2
 BX_WORLD_NO_CLEAR_FORCES = 0x4
-BX_JOINT_REVOLUTE = 4
+(BX_JOINT_MOUSE, BX_JOINT_SPRING, BX_JOINT_WELD, BX_JOINT_REVOLUTE, BX_JOINT_SPHERE, BX_JOINT_CONE,
+ BX_JOINT_FRICTION, BX_JOINT_MOTOR, BX_JOINT_PRISMATIC, BX_JOINT_WHEEL) = range(1, 11)
 BX_NULL_INDEX = 0xFFFFFFFF
 (BX_TOPO_CREATE_BODY, BX_TOPO_CREATE_FIXTURE, BX_TOPO_CREATE_JOINT, BX_TOPO_DESTROY_BODY, BX_TOPO_DESTROY_FIXTURE,
  BX_TOPO_DESTROY_JOINT) = range(1, 7)
@@ -76,7 +77,10 @@ class bx_joint_desc(C.Structure):
                 ("local_anchor_b", _arr(f32, 3)), ("local_rotation_b", _arr(f32, 4)),
                 ("reference_rotation", _arr(f32, 4)), ("enable_limit", u32), ("lower_angle", f32),
                 ("upper_angle", f32), ("enable_motor", u32), ("motor_speed", f32),
-                ("max_motor_torque", f32)]
+                ("max_motor_torque", f32), ("anchor_b", _arr(f32, 3)), ("frequency_hz", f32),
+                ("damping_ratio", f32), ("length", f32), ("max_force", f32), ("max_torque", f32),
+                ("correction_factor", f32), ("target", _arr(f32, 3)), ("linear_offset", _arr(f32, 3)),
+                ("angular_offset", _arr(f32, 4))]
 
 
 class bx_world_desc(C.Structure):

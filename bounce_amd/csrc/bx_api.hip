@@ -819,16 +819,56 @@ static int MakeFixture(const bx_fixture_desc& fd, const HostLibrary& L, Fixture&
     return BX_OK;
 }
 
-// b3Joint::Create of a revolute joint (revolute_joint.cpp:22-76) between two of `bodies`, the frames
-// of b3RevoluteJointDef::Initialize taken from the bodies' current transforms.
+// b3Joint::Create (joint.cpp:33-100) between two of `bodies`: the joint's constructor, with the frames
+// of its def's Initialize taken from the bodies' current transforms when asked.
 static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, Joint& j) {
-    if (jd.type != BX_JOINT_REVOLUTE || jd.body_a >= bodies.size() || jd.body_b >= bodies.size() ||
-        jd.body_a == jd.body_b)
+    if (jd.type == BX_JOINT_CONE || jd.type == BX_JOINT_PRISMATIC || jd.type == BX_JOINT_WHEEL)
+        return Fail(BX_E_UNSUPPORTED, "cone, prismatic and wheel joints are not supported by the device path");
+    if (jd.type < BX_JOINT_MOUSE || jd.type > BX_JOINT_MOTOR || jd.body_a >= bodies.size() ||
+        jd.body_b >= bodies.size() || jd.body_a == jd.body_b)
         return Fail(BX_E_INVALID, "bad joint descriptor");
     std::memset(&j, 0, sizeof(j));
+    j.type = jd.type;
     j.bodyA = jd.body_a;
     j.bodyB = jd.body_b;
     j.collideConnected = jd.collide_connected ? 1u : 0u;
+    j.frequencyHz = jd.frequency_hz;
+    j.dampingRatio = jd.damping_ratio;
+    j.length = jd.length;
+    j.maxForce = jd.max_force;
+    j.maxTorque = jd.max_torque;
+    j.correctionFactor = jd.correction_factor;
+    j.target = V3(jd.target);
+    j.linearOffset = V3(jd.linear_offset);
+    j.angularOffset = Q4(jd.angular_offset);
+    const Body& A0 = bodies[jd.body_a];
+    const Body& B0 = bodies[jd.body_b];
+    const Transform xfA(A0.xfT, A0.q), xfB(B0.xfT, B0.q);
+    if (jd.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:22-31: the target is B's anchor at creation
+        j.localAnchorB = MulT(xfB, j.target);
+        j.localRotationA = j.localRotationB = j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
+        return BX_OK;
+    }
+    if (jd.type != BX_JOINT_REVOLUTE) {
+        j.localRotationA = Q4(jd.local_rotation_a);
+        j.localRotationB = Q4(jd.local_rotation_b);
+        j.referenceRotation = Q4(jd.reference_rotation);
+        if (!jd.use_initialize) {
+            j.localAnchorA = V3(jd.local_anchor_a);
+            j.localAnchorB = V3(jd.local_anchor_b);
+        } else if (jd.type == BX_JOINT_MOTOR) {  // motor_joint.cpp:23-31
+            j.linearOffset = MulT(xfA, B0.xfT);
+            j.angularOffset = MulC(A0.q, B0.q);
+        } else {  // sphere_joint.cpp:23-29, weld_joint.cpp:23-33, spring_joint.cpp:23-30, friction_joint.cpp:23-29
+            const Vec3 anchor = V3(jd.anchor);
+            const Vec3 anchorB = jd.type == BX_JOINT_SPRING ? V3(jd.anchor_b) : anchor;
+            j.localAnchorA = MulT(xfA, anchor);
+            j.localAnchorB = MulT(xfB, anchorB);
+            if (jd.type == BX_JOINT_WELD) j.referenceRotation = Conjugate(A0.q) * B0.q;
+            if (jd.type == BX_JOINT_SPRING) j.length = Distance(anchor, anchorB);
+        }
+        return BX_OK;
+    }
     if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
         Mat33 rotation;
         rotation.z = V3(jd.axis);

@@ -64,8 +64,10 @@ BX_HD scalar JointAngle(scalar s, scalar c) { return scalar(::atan2(double(s), d
 
 // revolute_joint.cpp:78-231. `pos` is the island's position array, `indexA/B` the joint bodies'
 // island-local indices in this island.
-BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
-                        const PosBody* pos, scalar dtRatio, bool warm) {
+BX_HD void InitRevoluteCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
+                           const PosBody* pos, scalar dtRatio, bool warm) {
+    jc.angularImpulse3.SetZero();
+    jc.impulse1 = scalar(0);
     jc.indexA = indexA;
     jc.indexB = indexB;
     jc.mA = A.invMass;
@@ -177,7 +179,7 @@ BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u3
 
 // revolute_joint.cpp:233-277
 template <class V>
-BX_HD void WarmStartJoint(const JointCon& jc, V* vel) {
+BX_HD void WarmStartRevolute(const JointCon& jc, V* vel) {
     Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
     Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
     if (jc.enableMotor && jc.limitState != kEqualLimits) {
@@ -209,7 +211,7 @@ BX_HD void WarmStartJoint(const JointCon& jc, V* vel) {
 
 // revolute_joint.cpp:279-367
 template <class V>
-BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
+BX_HD void SolveRevoluteVelocity(JointCon& jc, V* vel, scalar dt) {
     Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w;
     Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
     Mat33 iA = jc.iA, iB = jc.iB;
@@ -271,7 +273,7 @@ BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
 
 // revolute_joint.cpp:369-531. `P` is an array of PosBody.
 template <class P>
-BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
+BX_HD bool SolveRevolutePosition(const JointCon& jc, P* pos) {
     Vec3 xA = pos[jc.indexA].x;
     Quat qA = pos[jc.indexA].q;
     Vec3 xB = pos[jc.indexB].x;
@@ -371,6 +373,365 @@ BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
     pos[jc.indexA].I = iA;
     pos[jc.indexB].I = iB;
     return linearError <= kLinearSlop && limitError <= kAngularSlop && angularError <= kCosSlop;
+}
+
+// ================================================================== the other joint types
+// b3ComputeSoftConstraintCoefficients (weld_joint.cpp:64-81, mouse_joint.cpp:35-52): gamma and bias of
+// a soft constraint of stiffness and damping from `frequencyHz` and `dampingRatio` on mass m.
+BX_HD void SoftCoefficients(scalar& gamma, scalar& bias, scalar frequencyHz, scalar dampingRatio, scalar m, scalar C,
+                            scalar h) {
+    scalar omega = scalar(2) * kPi * frequencyHz;
+    scalar k = omega * omega * m;
+    scalar d = scalar(2) * dampingRatio * omega * m;
+    gamma = h * (d + h * k);
+    gamma = gamma != scalar(0) ? scalar(1) / gamma : scalar(0);
+    bias = gamma * h * k * C;
+}
+
+// M + [rA]x iA [rA]x^T + [rB]x iB [rB]x^T: the point constraint's effective mass.
+BX_HD Mat33 PointMass(scalar mA, scalar mB, const Mat33& iA, const Mat33& iB, const Vec3& rA, const Vec3& rB) {
+    Mat33 RA = Skew(rA);
+    Mat33 RAT = Transpose(RA);
+    Mat33 RB = Skew(rB);
+    Mat33 RBT = Transpose(RB);
+    Mat33 M = Diagonal(mA + mB, mA + mB, mA + mB);
+    return M + RA * iA * RAT + RB * iB * RBT;
+}
+
+// InitializeVelocityConstraints of sphere_joint.cpp:37-79, weld_joint.cpp:83-177,
+// spring_joint.cpp:105-173, mouse_joint.cpp:54-100, friction_joint.cpp:39-86, motor_joint.cpp:41-113.
+BX_HD void InitOtherCon(JointCon& jc, const Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
+                        const PosBody* pos, scalar dt, scalar invDt, scalar dtRatio, bool warm) {
+    jc.indexA = indexA;
+    jc.indexB = indexB;
+    jc.mA = A.invMass;
+    jc.mB = B.invMass;
+    jc.localCenterA = A.localCenter;
+    jc.localCenterB = B.localCenter;
+    jc.localInvIA = A.invI;
+    jc.localInvIB = B.invI;
+    jc.iA = pos[indexA].I;
+    jc.iB = pos[indexB].I;
+    jc.localAnchorA = j.localAnchorA;
+    jc.localAnchorB = j.localAnchorB;
+    jc.frequencyHz = j.frequencyHz;
+    jc.length = j.length;
+    jc.maxForce = j.maxForce;
+    jc.maxTorque = j.maxTorque;
+    jc.correctionFactor = j.correctionFactor;
+    jc.invDt = invDt;
+    // the revolute-only members the impulse store-back reads
+    jc.motorImpulse = j.motorImpulse;
+    jc.limitImpulse = j.limitImpulse;
+    jc.angularImpulse = j.angularImpulse;
+    const Vec3 xA = pos[indexA].x, xB = pos[indexB].x;
+    const Quat qA = pos[indexA].q, qB = pos[indexB].q;
+    const scalar h = dt;
+    switch (j.type) {
+    case kJointSphere:
+    case kJointFriction:
+        jc.rA = Mul(qA, j.localAnchorA - jc.localCenterA);
+        jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
+        jc.linearMass = PointMass(jc.mA, jc.mB, jc.iA, jc.iB, jc.rA, jc.rB);
+        jc.angularMass3 = jc.iA + jc.iB;
+        break;
+    case kJointWeld: {
+        jc.rA = Mul(qA, j.localAnchorA - jc.localCenterA);
+        jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
+        jc.linearMass = PointMass(jc.mA, jc.mB, jc.iA, jc.iB, jc.rA, jc.rB);
+        Quat q1 = Conjugate(qA) * qB;  // weld_joint.cpp:132-148
+        if (Dot(q1, j.referenceRotation) < scalar(0)) q1 = -q1;
+        Quat q = scalar(2) * (j.referenceRotation - q1) * Conjugate(q1);
+        Vec3 C = Mul(qA, -q.v);
+        if (j.frequencyHz > scalar(0)) {
+            Mat33 invM = jc.iA + jc.iB;
+            Mat33 m = Inverse(invM);
+            Vec3 gamma, bias;
+            SoftCoefficients(gamma.x, bias.x, j.frequencyHz, j.dampingRatio, m.x.x, C.x, h);
+            SoftCoefficients(gamma.y, bias.y, j.frequencyHz, j.dampingRatio, m.y.y, C.y, h);
+            SoftCoefficients(gamma.z, bias.z, j.frequencyHz, j.dampingRatio, m.z.z, C.z, h);
+            jc.gamma3 = Diagonal(gamma.x, gamma.y, gamma.z);
+            jc.bias3 = bias;
+            invM = invM + jc.gamma3;
+            jc.angularMass3 = Inverse(invM);
+        } else {
+            jc.gamma3 = Diagonal(scalar(0), scalar(0), scalar(0));
+            jc.angularMass3 = Inverse(jc.iA + jc.iB);
+            jc.bias3 = j.correctionFactor * invDt * C;
+        }
+        break;
+    }
+    case kJointSpring: {
+        jc.rA = Mul(qA, j.localAnchorA - jc.localCenterA);
+        jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
+        jc.n = xB + jc.rB - xA - jc.rA;
+        scalar length = Length(jc.n);
+        if (length > kLinearSlop) jc.n /= length;
+        else jc.n.SetZero();
+        Vec3 rnA = Cross(jc.rA, jc.n);
+        Vec3 rnB = Cross(jc.rB, jc.n);
+        scalar mass = jc.mA + jc.mB + Dot(jc.iA * rnA, rnA) + Dot(jc.iB * rnB, rnB);
+        jc.mass1 = mass > scalar(0) ? scalar(1) / mass : scalar(0);
+        if (j.frequencyHz > scalar(0)) {
+            scalar C = length - j.length;
+            scalar omega = scalar(2) * kPi * j.frequencyHz;
+            scalar d = scalar(2) * jc.mass1 * j.dampingRatio * omega;
+            scalar k = jc.mass1 * omega * omega;
+            jc.gamma1 = h * (d + h * k);
+            jc.gamma1 = jc.gamma1 != scalar(0) ? scalar(1) / jc.gamma1 : scalar(0);
+            jc.bias1 = jc.gamma1 * h * k * C;
+            mass += jc.gamma1;
+            jc.mass1 = mass != scalar(0) ? scalar(1) / mass : scalar(0);
+        } else {
+            jc.bias1 = scalar(0);
+            jc.gamma1 = scalar(0);
+        }
+        break;
+    }
+    case kJointMouse: {
+        jc.rB = Mul(qB, j.localAnchorB - jc.localCenterB);
+        Mat33 M = Diagonal(jc.mB, jc.mB, jc.mB);
+        Mat33 RB = Skew(jc.rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 invM = M + RB * jc.iB * RBT;
+        Mat33 m = Inverse(invM);
+        Vec3 C = xB + jc.rB - j.target;
+        Vec3 gamma, bias;
+        SoftCoefficients(gamma.x, bias.x, j.frequencyHz, j.dampingRatio, m.x.x, C.x, h);
+        SoftCoefficients(gamma.y, bias.y, j.frequencyHz, j.dampingRatio, m.y.y, C.y, h);
+        SoftCoefficients(gamma.z, bias.z, j.frequencyHz, j.dampingRatio, m.z.z, C.z, h);
+        jc.gamma3 = Diagonal(gamma.x, gamma.y, gamma.z);
+        jc.bias3 = bias;
+        invM = invM + jc.gamma3;
+        jc.linearMass = Inverse(invM);
+        break;
+    }
+    case kJointMotor: {
+        jc.rA = Mul(qA, j.linearOffset - jc.localCenterA);
+        jc.rB = Mul(qB, -jc.localCenterB);
+        jc.linearMass = PointMass(jc.mA, jc.mB, jc.iA, jc.iB, jc.rA, jc.rB);
+        jc.linearError = xB + jc.rB - xA - jc.rA;
+        Quat q1 = Conjugate(qA) * qB;
+        if (Dot(q1, j.angularOffset) < scalar(0)) q1 = -q1;
+        Quat qw = invDt * scalar(2) * (j.angularOffset - q1) * Conjugate(q1);
+        jc.angularVelocity = Mul(qA, qw.v);
+        jc.angularVelocity *= j.correctionFactor;
+        jc.angularMass3 = jc.iA + jc.iB;
+        break;
+    }
+    default: break;
+    }
+    Vec3 linearImpulse = j.linearImpulse, angularImpulse3 = j.angularImpulse3;
+    scalar impulse1 = j.impulse1;
+    if (warm) {
+        linearImpulse *= dtRatio;
+        angularImpulse3 *= dtRatio;
+        impulse1 *= dtRatio;
+    } else {
+        linearImpulse.SetZero();
+        angularImpulse3.SetZero();
+        impulse1 = scalar(0);
+    }
+    jc.linearImpulse = linearImpulse;
+    jc.angularImpulse3 = angularImpulse3;
+    jc.impulse1 = impulse1;
+}
+
+// A point impulse P at rA / rB on the two bodies.
+BX_HD void ApplyPoint(const JointCon& jc, const Vec3& P, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
+    vA -= jc.mA * P;
+    wA -= jc.iA * Cross(jc.rA, P);
+    vB += jc.mB * P;
+    wB += jc.iB * Cross(jc.rB, P);
+}
+
+// An accumulated impulse clamped to |impulse| <= maxImpulse; returns the applied change
+// (friction_joint.cpp:118-133, motor_joint.cpp:160-175).
+BX_HD Vec3 ClampedDelta(Vec3& accumulated, const Vec3& impulse, scalar maxImpulse) {
+    Vec3 old = accumulated;
+    accumulated += impulse;
+    if (LengthSquared(accumulated) > maxImpulse * maxImpulse) {
+        accumulated.Normalize();
+        accumulated *= maxImpulse;
+    }
+    return accumulated - old;
+}
+
+template <class V>
+BX_HD void WarmStartOther(const JointCon& jc, V* vel) {
+    if (jc.type == kJointMouse) {  // mouse_joint.cpp:102-106
+        vel[jc.indexB].v += jc.mB * jc.linearImpulse;
+        vel[jc.indexB].w += jc.iB * Cross(jc.rB, jc.linearImpulse);
+        return;
+    }
+    if (jc.type == kJointSpring) {  // spring_joint.cpp:175-183, in place
+        Vec3 P = jc.impulse1 * jc.n;
+        vel[jc.indexA].v -= jc.mA * P;
+        vel[jc.indexA].w -= jc.iA * Cross(jc.rA, P);
+        vel[jc.indexB].v += jc.mB * P;
+        vel[jc.indexB].w += jc.iB * Cross(jc.rB, P);
+        return;
+    }
+    Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w, vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
+    if (jc.type == kJointFriction) {  // friction_joint.cpp:88-107
+        Vec3 P = jc.linearImpulse;
+        vA -= jc.mA * P;
+        wA -= jc.iA * (Cross(jc.rA, P) + jc.angularImpulse3);
+        vB += jc.mB * P;
+        wB += jc.iB * (Cross(jc.rB, P) + jc.angularImpulse3);
+    } else {  // sphere_joint.cpp:81-97; the weld and motor joints add their angular impulse
+        ApplyPoint(jc, jc.linearImpulse, vA, wA, vB, wB);
+        if (jc.type == kJointWeld || jc.type == kJointMotor) {
+            wA -= jc.iA * jc.angularImpulse3;
+            wB += jc.iB * jc.angularImpulse3;
+        }
+    }
+    vel[jc.indexA].v = vA;
+    vel[jc.indexA].w = wA;
+    vel[jc.indexB].v = vB;
+    vel[jc.indexB].w = wB;
+}
+
+template <class V>
+BX_HD void SolveOtherVelocity(JointCon& jc, V* vel, scalar dt) {
+    const scalar h = dt;
+    if (jc.type == kJointMouse) {  // mouse_joint.cpp:108-131
+        Vec3 vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
+        Vec3 Cdot = vB + Cross(wB, jc.rB);
+        Vec3 impulse = -jc.linearMass * (Cdot + jc.bias3 + jc.gamma3 * jc.linearImpulse);
+        Vec3 oldImpulse = jc.linearImpulse;
+        jc.linearImpulse += impulse;
+        scalar maxImpulse = h * jc.maxForce;
+        if (Dot(jc.linearImpulse, jc.linearImpulse) > maxImpulse * maxImpulse)
+            jc.linearImpulse *= maxImpulse / Length(jc.linearImpulse);
+        impulse = jc.linearImpulse - oldImpulse;
+        vB += jc.mB * impulse;
+        wB += jc.iB * Cross(jc.rB, impulse);
+        vel[jc.indexB].v = vB;
+        vel[jc.indexB].w = wB;
+        return;
+    }
+    Vec3 vA = vel[jc.indexA].v, wA = vel[jc.indexA].w, vB = vel[jc.indexB].v, wB = vel[jc.indexB].w;
+    if (jc.type == kJointSphere || jc.type == kJointWeld) {  // sphere_joint.cpp:99-122, weld_joint.cpp:207-246
+        Vec3 Cdot = vB + Cross(wB, jc.rB) - vA - Cross(wA, jc.rA);
+        Vec3 impulse = jc.linearMass.Solve(-Cdot);
+        jc.linearImpulse += impulse;
+        ApplyPoint(jc, impulse, vA, wA, vB, wB);
+        if (jc.type == kJointWeld) {
+            Vec3 dw = wB - wA;
+            Vec3 ai = -jc.angularMass3 * (dw + jc.bias3 + jc.gamma3 * jc.angularImpulse3);
+            jc.angularImpulse3 += ai;
+            wA -= jc.iA * ai;
+            wB += jc.iB * ai;
+        }
+    } else if (jc.type == kJointSpring) {  // spring_joint.cpp:185-212
+        Vec3 dv = vB + Cross(wB, jc.rB) - vA - Cross(wA, jc.rA);
+        scalar Cdot = Dot(jc.n, dv);
+        scalar impulse = -jc.mass1 * (Cdot + jc.bias1 + jc.gamma1 * jc.impulse1);
+        jc.impulse1 += impulse;
+        ApplyPoint(jc, impulse * jc.n, vA, wA, vB, wB);
+    } else if (jc.type == kJointFriction) {  // friction_joint.cpp:109-158: angular, then linear
+        Vec3 ai = ClampedDelta(jc.angularImpulse3, jc.angularMass3.Solve(-(wB - wA)), h * jc.maxTorque);
+        wA -= jc.iA * ai;
+        wB += jc.iB * ai;
+        Vec3 Cdot = vB + Cross(wB, jc.rB) - vA - Cross(wA, jc.rA);
+        Vec3 li = ClampedDelta(jc.linearImpulse, jc.linearMass.Solve(-Cdot), h * jc.maxForce);
+        ApplyPoint(jc, li, vA, wA, vB, wB);
+    } else if (jc.type == kJointMotor) {  // motor_joint.cpp:143-200: linear, then angular
+        Vec3 Cdot = vB + Cross(wB, jc.rB) - vA - Cross(wA, jc.rA) + jc.invDt * jc.correctionFactor * jc.linearError;
+        Vec3 li = ClampedDelta(jc.linearImpulse, jc.linearMass.Solve(-Cdot), h * jc.maxForce);
+        ApplyPoint(jc, li, vA, wA, vB, wB);
+        Vec3 ai = ClampedDelta(jc.angularImpulse3, jc.angularMass3.Solve(-((wB - wA) - jc.angularVelocity)),
+                               h * jc.maxTorque);
+        wA -= jc.iA * ai;
+        wB += jc.iB * ai;
+    }
+    vel[jc.indexA].v = vA;
+    vel[jc.indexA].w = wA;
+    vel[jc.indexB].v = vB;
+    vel[jc.indexB].w = wB;
+}
+
+// SolvePositionConstraints of the sphere and weld joints (sphere_joint.cpp:124-165,
+// weld_joint.cpp:248-296) and the rigid spring (spring_joint.cpp:214-271, orientations not
+// renormalized); the mouse, friction and motor joints and a soft spring have none.
+template <class P>
+BX_HD bool SolveOtherPosition(const JointCon& jc, P* pos) {
+    if (jc.type == kJointMouse || jc.type == kJointFriction || jc.type == kJointMotor ||
+        (jc.type == kJointSpring && jc.frequencyHz > scalar(0)))
+        return true;
+    Vec3 xA = pos[jc.indexA].x, xB = pos[jc.indexB].x;
+    Quat qA = pos[jc.indexA].q, qB = pos[jc.indexB].q;
+    Mat33 iA = pos[jc.indexA].I, iB = pos[jc.indexB].I;
+    const scalar mA = jc.mA, mB = jc.mB;
+    bool solved;
+    Vec3 rA = Mul(qA, jc.localAnchorA - jc.localCenterA);
+    Vec3 rB = Mul(qB, jc.localAnchorB - jc.localCenterB);
+    if (jc.type == kJointSpring) {
+        Vec3 n = xB + rB - xA - rA;
+        scalar length = Length(n);
+        scalar C = length - jc.length;
+        C = Clamp(C, -kMaxLinearCorrection, kMaxLinearCorrection);
+        n.Normalize();
+        Vec3 rnA = Cross(rA, n);
+        Vec3 rnB = Cross(rB, n);
+        scalar kMass = mA + mB + Dot(iA * rnA, rnA) + Dot(iB * rnB, rnB);
+        scalar mass = kMass > scalar(0) ? scalar(1) / kMass : scalar(0);
+        scalar lambda = -mass * C;
+        Vec3 impulse = lambda * n;
+        xA -= mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        iA = RotateToFrame(jc.localInvIA, qA);
+        xB += mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        iB = RotateToFrame(jc.localInvIB, qB);
+        solved = Abs(C) < kLinearSlop;
+    } else {
+        Mat33 mass = PointMass(mA, mB, iA, iB, rA, rB);
+        Vec3 C = xB + rB - xA - rA;
+        Vec3 impulse = mass.Solve(-C);
+        xA -= mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        qA.Normalize();
+        iA = RotateToFrame(jc.localInvIA, qA);
+        xB += mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        qB.Normalize();
+        iB = RotateToFrame(jc.localInvIB, qB);
+        solved = Length(C) <= kLinearSlop;
+    }
+    pos[jc.indexA].x = xA;
+    pos[jc.indexA].q = qA;
+    pos[jc.indexB].x = xB;
+    pos[jc.indexB].q = qB;
+    pos[jc.indexA].I = iA;
+    pos[jc.indexB].I = iB;
+    return solved;
+}
+
+// ================================================================== dispatch by joint type
+BX_HD void InitJointCon(JointCon& jc, Joint& j, const Body& A, const Body& B, u32 indexA, u32 indexB,
+                        const PosBody* pos, scalar dt, scalar invDt, scalar dtRatio, bool warm) {
+    jc.type = j.type;
+    if (j.type == kJointRevolute) InitRevoluteCon(jc, j, A, B, indexA, indexB, pos, dtRatio, warm);
+    else InitOtherCon(jc, j, A, B, indexA, indexB, pos, dt, invDt, dtRatio, warm);
+}
+
+template <class V>
+BX_HD void WarmStartJoint(const JointCon& jc, V* vel) {
+    if (jc.type == kJointRevolute) WarmStartRevolute(jc, vel);
+    else WarmStartOther(jc, vel);
+}
+
+template <class V>
+BX_HD void SolveJointVelocity(JointCon& jc, V* vel, scalar dt) {
+    if (jc.type == kJointRevolute) SolveRevoluteVelocity(jc, vel, dt);
+    else SolveOtherVelocity(jc, vel, dt);
+}
+
+template <class P>
+BX_HD bool SolveJointPosition(const JointCon& jc, P* pos) {
+    return jc.type == kJointRevolute ? SolveRevolutePosition(jc, pos) : SolveOtherPosition(jc, pos);
 }
 
 }  // namespace bx

@@ -165,6 +165,7 @@ struct Mat33 {
 BX_HD Mat33 operator+(const Mat33& A, const Mat33& B) { return Mat33(A.x + B.x, A.y + B.y, A.z + B.z); }
 BX_HD Mat33 operator-(const Mat33& A, const Mat33& B) { return Mat33(A.x - B.x, A.y - B.y, A.z - B.z); }
 BX_HD Mat33 operator*(scalar s, const Mat33& A) { return Mat33(s * A.x, s * A.y, s * A.z); }
+BX_HD Mat33 operator-(const Mat33& A) { return scalar(-1) * A; }  // mat33.h:123-126
 BX_HD Vec3 operator*(const Mat33& A, const Vec3& v) { return v.x * A.x + v.y * A.y + v.z * A.z; }
 BX_HD Mat33 operator*(const Mat33& A, const Mat33& B) { return Mat33(A * B.x, A * B.y, A * B.z); }
 BX_HD Vec3 MulT(const Mat33& A, const Vec3& v) { return Vec3(Dot(A.x, v), Dot(A.y, v), Dot(A.z, v)); }

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock) k_init_contacts(BatchView B, StepParam
         Joint& j = B.joints[wi.jointBase + e.x];
         JointCon jc;
         InitJointCon(jc, j, bodies[j.bodyA], bodies[j.bodyB], e.y, e.z, B.posBodies + wi.islandBodyBase + e.w,
-                     dtRatio, warm);
+                     sp.dt, sp.inv_dt, dtRatio, warm);
         B.jointCons[wi.jointBase + k] = jc;
     }
 }
@@ -237,6 +237,8 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
         j.limitImpulse = jc.limitImpulse;
         j.linearImpulse = jc.linearImpulse;
         j.angularImpulse = jc.angularImpulse;
+        j.angularImpulse3 = jc.angularImpulse3;
+        j.impulse1 = jc.impulse1;
     }
     const u32 n = Min(ws.islandBodyCount, wi.islandBodyCap);
     const u32* ib = B.islandBodies + wi.islandBodyBase;

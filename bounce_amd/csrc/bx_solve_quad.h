@@ -326,10 +326,14 @@ __global__ void __launch_bounds__(64) k_solve_velocity_quad(BatchView B, StepPar
             VelBody* vb = reinterpret_cast<VelBody*>(slab);
             // island.cpp:154-181: contact warm start, joint warm start, then per iteration all
             // joints followed by all contacts
+            // the group's first lane solves the joints on velocities the other lanes' contact
+            // sweeps wrote; each sweep waits for it before it reads the slab
             if (warm && nc > 0) QSweep<false>(vcs, nc, slab, L);
+            BX_GROUP_SYNC();
             if (warm && nj > 0 && inGroup == 0)
                 for (u32 j = 0; j < nj; ++j) WarmStartJoint(jcs[j], vb);
             for (u32 it = 0; it < sp.velocityIterations && (nc > 0 || nj > 0); ++it) {
+                BX_GROUP_SYNC();
                 if (nj > 0 && inGroup == 0)
                     for (u32 j = 0; j < nj; ++j) SolveJointVelocity(jcs[j], vb, sp.dt);
                 if (nc > 0) QSweep<true>(vcs, nc, slab, L);

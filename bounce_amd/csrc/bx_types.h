@@ -194,8 +194,13 @@ BX_HD u32 MeshLaneStride(u32 triCap) { return kMeshManifoldObs + 4u * triCap + (
 
 struct JointEdge { u32 joint, other; };
 
-// Revolute joint (revolute_joint.h / revolute_joint.cpp): definition and the warm-start state that
-// persists between steps.
+// b3JointType values of the joints the device solves (joint.h:33-46).
+constexpr u32 kJointMouse = 1, kJointSpring = 2, kJointWeld = 3, kJointRevolute = 4, kJointSphere = 5,
+              kJointFriction = 7, kJointMotor = 8;
+
+// A joint (joints/*_joint.h): definition and the warm-start state that persists between steps. The
+// revolute members come first; the other types use the anchors, the reference rotation, the linear
+// impulse and the block after `type`.
 struct Joint {
     u32 bodyA, bodyB;
     u32 collideConnected, islandFlag;
@@ -204,10 +209,16 @@ struct Joint {
     u32 enableLimit, enableMotor;
     scalar lowerAngle, upperAngle, motorSpeed, maxMotorTorque;
     scalar motorImpulse, limitImpulse;
-    Vec3 linearImpulse;
+    Vec3 linearImpulse;                // revolute, sphere, weld, mouse, friction, motor
     Vec2 angularImpulse;
     u32 limitState;
-    u32 pad;
+    u32 type;                          // kJoint*
+    scalar frequencyHz, dampingRatio, length, maxForce, maxTorque, correctionFactor;
+    Vec3 target;                       // mouse: world target
+    Vec3 linearOffset;                 // motor
+    Quat angularOffset;                // motor
+    Vec3 angularImpulse3;              // weld, friction, motor
+    scalar impulse1;                   // spring
 };
 
 // Solver constants of one island joint (b3RevoluteJoint's per-solve members), built in parallel
@@ -232,6 +243,12 @@ struct JointCon {
     Mat33 localInvIA, localInvIB;
     Quat localRotationA, localRotationB;
     scalar lowerAngle, upperAngle;
+    // the other joint types (linearMass, rA, rB and linearImpulse are shared with the revolute joint)
+    u32 type;
+    scalar frequencyHz, length, maxForce, maxTorque, correctionFactor, invDt;
+    Mat33 gamma3, angularMass3;
+    Vec3 bias3, n, linearError, angularVelocity, angularImpulse3;
+    scalar mass1, gamma1, bias1, impulse1;
 };
 
 

@@ -49,10 +49,12 @@ def fixture_array(n):
 
 
 def joint_array(n):
+    """Joint descriptors with the defs' defaults (revolute type; weld and motor correction 0.2)."""
     a = np.zeros(n, dtype=JOINT_DT)
     a["type"] = capi.BX_JOINT_REVOLUTE
-    for k in ("local_rotation_a", "local_rotation_b", "reference_rotation"):
+    for k in ("local_rotation_a", "local_rotation_b", "reference_rotation", "angular_offset"):
         a[k][:, 3] = 1.0
+    a["correction_factor"] = F32(0.2)
     return a
 
 
@@ -614,4 +616,73 @@ def mixed_worlds(n_worlds=16384, seed=8, median=200, sigma=0.8, clip=(20, 2000),
         sizes = np.clip(np.round(rng.lognormal(np.log(median), sigma, n_worlds)), clip[0], clip[1]).astype(int)
     for n in sizes:
         s.worlds.append(mixed_world(rng, int(n), g, link, hull_indices, links, sleeping))
+    return s
+
+
+def joint_zoo(n_worlds=1, seed=21, sleeping=False):
+    """Every joint type of the device path in one world, on a ground box: boxes hung from the ground
+    and from each other by a sphere joint, a rigid and a soft weld, a rigid and a soft spring, a motor
+    joint pulling a box to an offset, a friction joint holding a box sliding on the ground, a mouse
+    joint dragging a box to a target, and a revolute joint; some frames from Initialize, some given.
+    Each world is jittered by its seed."""
+    s = SceneSet("joint_zoo")
+    g = s.add_hull(hull_lib.box_hull(30.0, 0.5, 30.0))
+    b = s.add_hull(hull_lib.box_hull(0.4, 0.3, 0.35))
+    rng = np.random.default_rng(seed)
+    J = capi
+    for _ in range(n_worlds):
+        n = 13
+        bodies = body_array(n)
+        fixtures = fixture_array(n)
+        _ground(bodies, fixtures, g)
+        pos = np.array([[-6, 6, 0], [-6, 4.8, 0.2], [-3, 6, 0], [-3, 4.9, 0.1], [0, 6, 0], [0, 4.6, 0],
+                        [3, 6, 0], [3, 4.4, 0.3], [6, 0.81, 0], [-8, 0.81, 3], [8, 3, -3], [9, 6, 0]],
+                       dtype=F32) + rng.uniform(-0.05, 0.05, (n - 1, 3)).astype(F32)
+        sl = slice(1, n)
+        bodies["type"][sl] = J.BX_BODY_DYNAMIC
+        bodies["position"][sl] = pos
+        bodies["orientation"][sl] = random_orientations(rng, n - 1) if rng.random() < 0.0 else (0, 0, 0, 1)
+        bodies["linear_velocity"][9] = (3.0, 0.0, 1.0)      # slides on the ground against its friction joint
+        bodies["angular_velocity"][9] = (0.0, 4.0, 0.0)
+        fixtures["body"][sl] = np.arange(1, n)
+        fixtures["shape_type"][sl] = J.BX_SHAPE_HULL
+        fixtures["shape_index"][sl] = b
+        fixtures["density"][sl] = 1.0
+        joints = joint_array(11)
+        spec = [
+            # (type, a, b, use_initialize, anchor)
+            (J.BX_JOINT_SPHERE, 0, 1, 1, (-6, 6.6, 0)),
+            (J.BX_JOINT_SPHERE, 1, 2, 0, None),
+            (J.BX_JOINT_WELD, 0, 3, 1, (-3, 6.6, 0)),
+            (J.BX_JOINT_WELD, 3, 4, 1, (-3, 5.45, 0.05)),
+            (J.BX_JOINT_SPRING, 0, 5, 1, (0, 7.5, 0)),
+            (J.BX_JOINT_SPRING, 5, 6, 1, (0, 5.7, 0)),
+            (J.BX_JOINT_MOTOR, 0, 7, 1, None),
+            (J.BX_JOINT_MOTOR, 7, 8, 0, None),
+            (J.BX_JOINT_FRICTION, 0, 10, 1, (-8, 0.5, 3)),
+            (J.BX_JOINT_MOUSE, 0, 11, 0, None),
+            (J.BX_JOINT_REVOLUTE, 11, 12, 1, (8.5, 3, -3)),
+        ]
+        for k, (t, a, bb, init, anchor) in enumerate(spec):
+            j = joints[k]
+            j["type"], j["body_a"], j["body_b"], j["use_initialize"] = t, a, bb, init
+            if anchor is not None:
+                j["anchor"] = anchor
+        joints[1]["local_anchor_a"] = (0.0, -0.5, 0.0)
+        joints[1]["local_anchor_b"] = (0.1, 0.5, 0.0)
+        joints[3]["frequency_hz"], joints[3]["damping_ratio"] = F32(3.0), F32(0.4)
+        joints[4]["anchor_b"] = (0.05, 5.0, 0.02)
+        joints[5]["anchor_b"] = (0.0, 5.0, 0.0)
+        joints[5]["frequency_hz"], joints[5]["damping_ratio"] = F32(2.0), F32(0.3)
+        joints[6]["max_force"], joints[6]["max_torque"] = F32(200.0), F32(50.0)
+        joints[7]["linear_offset"] = (0.5, -1.5, 0.0)
+        joints[7]["angular_offset"] = (0.0, 0.38268343, 0.0, 0.9238795)
+        joints[7]["max_force"], joints[7]["max_torque"], joints[7]["correction_factor"] = F32(80.0), F32(20.0), F32(0.3)
+        joints[8]["max_force"], joints[8]["max_torque"] = F32(5.0), F32(2.0)
+        joints[9]["target"] = (7.0, 5.0, -2.0)
+        joints[9]["max_force"] = F32(300.0)
+        joints[9]["frequency_hz"], joints[9]["damping_ratio"] = F32(6.0), F32(0.8)
+        joints[10]["axis"] = (0.0, 0.0, 1.0)
+        joints[10]["collide_connected"] = 1
+        s.worlds.append(World(bodies, fixtures, joints, flags=capi.BX_WORLD_SLEEPING if sleeping else 0))
     return s

@@ -72,8 +72,18 @@ extern "C" {
                                           compiled in only with -DBX_DEVICE_CALLBACKS=1: the GPU library as
                                           built refuses such a world with BX_E_UNSUPPORTED */
 
-/* b3JointType (include/bounce/dynamics/joints/joint.h) */
+/* b3JointType (include/bounce/dynamics/joints/joint.h:33-46); cone, prismatic and wheel joints are
+   not supported by the device path (BX_E_UNSUPPORTED) */
+#define BX_JOINT_MOUSE 1u
+#define BX_JOINT_SPRING 2u
+#define BX_JOINT_WELD 3u
 #define BX_JOINT_REVOLUTE 4u
+#define BX_JOINT_SPHERE 5u
+#define BX_JOINT_CONE 6u
+#define BX_JOINT_FRICTION 7u
+#define BX_JOINT_MOTOR 8u
+#define BX_JOINT_PRISMATIC 9u
+#define BX_JOINT_WHEEL 10u
 
 #define BX_NULL_INDEX 0xFFFFFFFFu
 #define BX_MAX_MANIFOLD_POINTS 4
@@ -142,14 +152,21 @@ typedef struct bx_fixture_desc {
 } bx_fixture_desc;
 
 typedef struct bx_joint_desc {
-    uint32_t type;                 /* BX_JOINT_REVOLUTE */
+    uint32_t type;                 /* BX_JOINT_* */
     uint32_t body_a;               /* creation indices */
     uint32_t body_b;
     uint32_t collide_connected;
-    /* If use_initialize != 0 the local frames are computed as
-       b3RevoluteJointDef::Initialize(bodyA, bodyB, axis, anchor, lower_angle, upper_angle)
-       does (revolute_joint.cpp) from the bodies' transforms at creation; otherwise the
-       local_* and reference_rotation fields are used as given. */
+    /* If use_initialize != 0 the local frames are computed from the bodies' transforms at creation,
+       as the def's Initialize does:
+         revolute  b3RevoluteJointDef::Initialize(A, B, axis, anchor, lower_angle, upper_angle)
+         sphere    b3SphereJointDef::Initialize(A, B, anchor)         (sphere_joint.cpp:23-29)
+         weld      b3WeldJointDef::Initialize(A, B, anchor)           (weld_joint.cpp:23-33)
+         spring    b3SpringJointDef::Initialize(A, B, anchor, anchor_b) (spring_joint.cpp:23-30)
+         friction  b3FrictionJointDef::Initialize(A, B, anchor)      (friction_joint.cpp:23-29)
+         motor     b3MotorJointDef::Initialize(A, B)                  (motor_joint.cpp:23-31)
+       otherwise the local_*, reference_rotation, length and offsets are used as given. A mouse
+       joint's body-B anchor is always the target in body B's frame at creation
+       (mouse_joint.cpp:22-31). */
     uint32_t use_initialize;
     float axis[3];
     float anchor[3];
@@ -164,6 +181,17 @@ typedef struct bx_joint_desc {
     uint32_t enable_motor;
     float motor_speed;
     float max_motor_torque;
+    /* the other joint types' definitions (b3*JointDef members) */
+    float anchor_b[3];             /* spring Initialize: the anchor on body B */
+    float frequency_hz;            /* weld, spring, mouse */
+    float damping_ratio;           /* weld, spring, mouse */
+    float length;                  /* spring rest length */
+    float max_force;               /* mouse, friction, motor */
+    float max_torque;              /* friction, motor */
+    float correction_factor;       /* weld, motor */
+    float target[3];               /* mouse: the world target */
+    float linear_offset[3];        /* motor: position of B in A's frame */
+    float angular_offset[4];       /* motor: orientation of B in A's frame (x y z w) */
 } bx_joint_desc;
 
 typedef struct bx_world_desc {

@@ -228,8 +228,94 @@ bool AddFixture(RefWorld* w, const bx_fixture_desc& fd) {
     return true;
 }
 
+// The other joint types through their own defs and Initialize (bx_joint_desc, bxstep.h).
+static bool AddOtherJoint(RefWorld* w, const bx_joint_desc& jd) {
+    b3Body* a = w->bodies[jd.body_a];
+    b3Body* b = w->bodies[jd.body_b];
+    b3Joint* j = nullptr;
+    const b3Vec3 anchor = V3(jd.anchor);
+    auto common = [&](b3JointDef& def) {
+        def.bodyA = a;
+        def.bodyB = b;
+        def.collideConnected = jd.collide_connected != 0;
+    };
+    switch (jd.type) {
+    case BX_JOINT_SPHERE: {
+        b3SphereJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor);
+        else { def.localAnchorA = V3(jd.local_anchor_a); def.localAnchorB = V3(jd.local_anchor_b); }
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_WELD: {
+        b3WeldJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor);
+        else {
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.referenceRotation = Q4(jd.reference_rotation);
+        }
+        def.frequencyHz = scalar(jd.frequency_hz);
+        def.dampingRatio = scalar(jd.damping_ratio);
+        def.correctionFactor = scalar(jd.correction_factor);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_SPRING: {
+        b3SpringJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor, V3(jd.anchor_b));
+        else {
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.length = scalar(jd.length);
+        }
+        def.frequencyHz = scalar(jd.frequency_hz);
+        def.dampingRatio = scalar(jd.damping_ratio);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_MOUSE: {
+        b3MouseJointDef def;
+        def.target = V3(jd.target);
+        def.maxForce = scalar(jd.max_force);
+        def.frequencyHz = scalar(jd.frequency_hz);
+        def.dampingRatio = scalar(jd.damping_ratio);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_FRICTION: {
+        b3FrictionJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor);
+        else { def.localAnchorA = V3(jd.local_anchor_a); def.localAnchorB = V3(jd.local_anchor_b); }
+        def.maxForce = scalar(jd.max_force);
+        def.maxTorque = scalar(jd.max_torque);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_MOTOR: {
+        b3MotorJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b);
+        else { def.linearOffset = V3(jd.linear_offset); def.angularOffset = Q4(jd.angular_offset); }
+        def.maxForce = scalar(jd.max_force);
+        def.maxTorque = scalar(jd.max_torque);
+        def.correctionFactor = scalar(jd.correction_factor);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    default: return false;
+    }
+    w->joints.push_back(j);
+    return true;
+}
+
 bool AddJoint(RefWorld* w, const bx_joint_desc& jd) {
-    if (jd.type != BX_JOINT_REVOLUTE) return false;
+    if (jd.type != BX_JOINT_REVOLUTE) return AddOtherJoint(w, jd);
     b3RevoluteJointDef def;
     b3Body* a = w->bodies[jd.body_a];
     b3Body* b = w->bodies[jd.body_b];

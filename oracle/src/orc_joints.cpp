@@ -63,7 +63,42 @@ void CreateJoint(World& w, const bx_joint_desc& d) {
     j.bodyA = d.body_a;
     j.bodyB = d.body_b;
     j.collideConnected = d.collide_connected != 0;
-    if (d.use_initialize) {  // revolute_joint.cpp:22-48
+    j.frequencyHz = d.frequency_hz;
+    j.dampingRatio = d.damping_ratio;
+    j.length = d.length;
+    j.maxForce = d.max_force;
+    j.maxTorque = d.max_torque;
+    j.correctionFactor = d.correction_factor;
+    j.target = V3(d.target);
+    j.linearOffset = V3(d.linear_offset);
+    j.angularOffset = Q4(d.angular_offset);
+    j.angularImpulse3.SetZero();
+    j.impulse1 = 0;
+    const Body& A0 = w.bodies[j.bodyA];
+    const Body& B0 = w.bodies[j.bodyB];
+    if (d.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:22-31: the target is B's anchor at creation
+        j.localAnchorB = MulT(B0.xf, j.target);
+    } else if (d.type != BX_JOINT_REVOLUTE) {
+        if (d.use_initialize) {
+            const Vec3 anchor = V3(d.anchor);
+            if (d.type == BX_JOINT_MOTOR) {  // motor_joint.cpp:23-31
+                j.linearOffset = MulT(A0.xf, B0.xf.translation);
+                j.angularOffset = MulC(A0.sweep.orientation, B0.xf.rotation);
+            } else {  // sphere, weld, spring, friction Initialize
+                j.localAnchorA = MulT(A0.xf, anchor);
+                j.localAnchorB = MulT(B0.xf, d.type == BX_JOINT_SPRING ? V3(d.anchor_b) : anchor);
+                if (d.type == BX_JOINT_WELD) j.referenceRotation = Conjugate(A0.xf.rotation) * B0.xf.rotation;
+                if (d.type == BX_JOINT_SPRING) j.length = Distance(anchor, V3(d.anchor_b));
+            }
+            if (d.type != BX_JOINT_WELD) j.referenceRotation = Q4(d.reference_rotation);
+        } else {
+            j.localAnchorA = V3(d.local_anchor_a);
+            j.localAnchorB = V3(d.local_anchor_b);
+            j.referenceRotation = Q4(d.reference_rotation);
+        }
+        j.localRotationA = Q4(d.local_rotation_a);
+        j.localRotationB = Q4(d.local_rotation_b);
+    } else if (d.use_initialize) {  // revolute_joint.cpp:22-48
         Mat33 rotation;
         rotation.z = V3(d.axis);
         ComputeBasis(rotation.z, rotation.x, rotation.y);
@@ -102,7 +137,7 @@ void CreateJoint(World& w, const bx_joint_desc& d) {
     B.joints.insert(B.joints.begin(), JointEdge{index, j.bodyA});
 }
 
-static void InitJoint(World& w, const TimeStep& step, Joint& j, Position* pos) {  // revolute_joint.cpp:78-231
+static void InitRevolute(World& w, const TimeStep& step, Joint& j, Position* pos) {  // revolute_joint.cpp:78-231
     const Body& A = w.bodies[j.bodyA];
     const Body& B = w.bodies[j.bodyB];
     j.indexA = A.islandIndex;
@@ -190,7 +225,7 @@ static void InitJoint(World& w, const TimeStep& step, Joint& j, Position* pos) {
     }
 }
 
-static void WarmStartJoint(Joint& j, Velocity* vel) {  // revolute_joint.cpp:233-277
+static void WarmStartRevolute(Joint& j, Velocity* vel) {  // revolute_joint.cpp:233-277
     Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w;
     Vec3 vB = vel[j.indexB].v, wB = vel[j.indexB].w;
     if (j.enableMotor && j.limitState != e_equalLimits) {
@@ -220,7 +255,7 @@ static void WarmStartJoint(Joint& j, Velocity* vel) {  // revolute_joint.cpp:233
     vel[j.indexB].w = wB;
 }
 
-static void SolveJointVelocity(const TimeStep& step, Joint& j, Velocity* vel) {  // revolute_joint.cpp:279-367
+static void SolveRevoluteVelocity(const TimeStep& step, Joint& j, Velocity* vel) {  // revolute_joint.cpp:279-367
     Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w;
     Vec3 vB = vel[j.indexB].v, wB = vel[j.indexB].w;
     Mat33 iA = j.iA, iB = j.iB;
@@ -280,7 +315,7 @@ static void SolveJointVelocity(const TimeStep& step, Joint& j, Velocity* vel) { 
     vel[j.indexB].w = wB;
 }
 
-static bool SolveJointPosition(Joint& j, Position* pos) {  // revolute_joint.cpp:369-531
+static bool SolveRevolutePosition(Joint& j, Position* pos) {  // revolute_joint.cpp:369-531
     Vec3 xA = pos[j.indexA].x;
     Quat qA = pos[j.indexA].q;
     Vec3 xB = pos[j.indexB].x;
@@ -380,6 +415,378 @@ static bool SolveJointPosition(Joint& j, Position* pos) {  // revolute_joint.cpp
     pos[j.indexA].I = iA;
     pos[j.indexB].I = iB;
     return linearError <= kLinearSlop && limitError <= kAngularSlop && angularError <= kCosSlop;
+}
+
+// ------------------------------------------------------------------ the other joint types
+// b3ComputeSoftConstraintCoefficients (weld_joint.cpp:64-81, mouse_joint.cpp:35-52): the soft
+// constraint's gamma and bias of a mass-spring-damper at `frequencyHz`.
+static void SoftCoefficients(scalar& gamma, scalar& bias, scalar frequencyHz, scalar dampingRatio, scalar m,
+                             scalar C, scalar h) {
+    scalar omega = scalar(2) * kPi * frequencyHz;
+    scalar k = omega * omega * m;
+    scalar d = scalar(2) * dampingRatio * omega * m;
+    gamma = h * (d + h * k);
+    gamma = gamma != scalar(0) ? scalar(1) / gamma : scalar(0);
+    bias = gamma * h * k * C;
+}
+
+// The point-to-point effective mass shared by the sphere, weld and friction joints and the position
+// solvers: M + [rA]x iA [rA]x^T + [rB]x iB [rB]x^T.
+static Mat33 PointMass(scalar mA, scalar mB, const Mat33& iA, const Mat33& iB, const Vec3& rA, const Vec3& rB) {
+    Mat33 RA = Skew(rA);
+    Mat33 RAT = Transpose(RA);
+    Mat33 RB = Skew(rB);
+    Mat33 RBT = Transpose(RB);
+    Mat33 M = Diagonal(mA + mB);
+    return M + RA * iA * RAT + RB * iB * RBT;
+}
+
+// The relative rotation error of weld_joint.cpp:132-148 / motor_joint.cpp:86-99: 2 (q2 - q1) q1*, with
+// q1 = qA* qB flipped to q2's hemisphere.
+static Quat RotationError(const Quat& qA, const Quat& qB, const Quat& q2) {
+    Quat q1 = Conjugate(qA) * qB;
+    if (Dot(q1, q2) < scalar(0)) q1 = -q1;
+    return scalar(2) * (q2 - q1) * Conjugate(q1);
+}
+
+// InitializeVelocityConstraints of sphere_joint.cpp:37-79, weld_joint.cpp:83-177,
+// spring_joint.cpp:105-173, mouse_joint.cpp:54-100, friction_joint.cpp:39-86, motor_joint.cpp:41-113.
+static void InitOther(World& w, const TimeStep& step, Joint& j, Position* pos) {
+    const Body& A = w.bodies[j.bodyA];
+    const Body& B = w.bodies[j.bodyB];
+    j.indexA = A.islandIndex;
+    j.indexB = B.islandIndex;
+    j.mA = A.invMass;
+    j.mB = B.invMass;
+    j.localCenterA = A.sweep.localCenter;
+    j.localCenterB = B.sweep.localCenter;
+    j.localInvIA = A.invI;
+    j.localInvIB = B.invI;
+    j.iA = pos[j.indexA].I;
+    j.iB = pos[j.indexB].I;
+    const Vec3 xA = pos[j.indexA].x, xB = pos[j.indexB].x;
+    const Quat qA = pos[j.indexA].q, qB = pos[j.indexB].q;
+    const scalar h = step.dt;
+    switch (j.type) {
+    case BX_JOINT_SPHERE:
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        j.linearMass = PointMass(j.mA, j.mB, j.iA, j.iB, j.rA, j.rB);
+        break;
+    case BX_JOINT_WELD: {
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        j.linearMass = PointMass(j.mA, j.mB, j.iA, j.iB, j.rA, j.rB);
+        Quat q = RotationError(qA, qB, j.referenceRotation);
+        Vec3 C = Mul(qA, -q.v);
+        if (j.frequencyHz > scalar(0)) {
+            Mat33 invM = j.iA + j.iB;
+            Mat33 m = Inverse(invM);
+            Vec3 gamma, bias;
+            SoftCoefficients(gamma.x, bias.x, j.frequencyHz, j.dampingRatio, m.x.x, C.x, h);
+            SoftCoefficients(gamma.y, bias.y, j.frequencyHz, j.dampingRatio, m.y.y, C.y, h);
+            SoftCoefficients(gamma.z, bias.z, j.frequencyHz, j.dampingRatio, m.z.z, C.z, h);
+            j.gamma3 = Diagonal(gamma.x, gamma.y, gamma.z);
+            j.bias3 = bias;
+            invM = invM + j.gamma3;
+            j.angularMass3 = Inverse(invM);
+        } else {
+            j.gamma3 = Diagonal(scalar(0));
+            j.angularMass3 = Inverse(j.iA + j.iB);
+            j.bias3 = j.correctionFactor * step.inv_dt * C;
+        }
+        break;
+    }
+    case BX_JOINT_SPRING: {
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        j.n = xB + j.rB - xA - j.rA;
+        scalar length = Length(j.n);
+        if (length > kLinearSlop) j.n /= length;
+        else j.n.SetZero();
+        Vec3 rnA = Cross(j.rA, j.n);
+        Vec3 rnB = Cross(j.rB, j.n);
+        scalar mass = j.mA + j.mB + Dot(j.iA * rnA, rnA) + Dot(j.iB * rnB, rnB);
+        j.mass1 = mass > scalar(0) ? scalar(1) / mass : scalar(0);
+        if (j.frequencyHz > scalar(0)) {
+            scalar C = length - j.length;
+            scalar omega = scalar(2) * kPi * j.frequencyHz;
+            scalar d = scalar(2) * j.mass1 * j.dampingRatio * omega;
+            scalar k = j.mass1 * omega * omega;
+            j.gamma1 = h * (d + h * k);
+            j.gamma1 = j.gamma1 != scalar(0) ? scalar(1) / j.gamma1 : scalar(0);
+            j.bias1 = j.gamma1 * h * k * C;
+            mass += j.gamma1;
+            j.mass1 = mass != scalar(0) ? scalar(1) / mass : scalar(0);
+        } else {
+            j.bias1 = scalar(0);
+            j.gamma1 = scalar(0);
+        }
+        break;
+    }
+    case BX_JOINT_MOUSE: {
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Mat33 M = Diagonal(j.mB);
+        Mat33 RB = Skew(j.rB);
+        Mat33 RBT = Transpose(RB);
+        Mat33 invM = M + RB * j.iB * RBT;
+        Mat33 m = Inverse(invM);
+        Vec3 C = xB + j.rB - j.target;
+        Vec3 gamma, bias;
+        SoftCoefficients(gamma.x, bias.x, j.frequencyHz, j.dampingRatio, m.x.x, C.x, h);
+        SoftCoefficients(gamma.y, bias.y, j.frequencyHz, j.dampingRatio, m.y.y, C.y, h);
+        SoftCoefficients(gamma.z, bias.z, j.frequencyHz, j.dampingRatio, m.z.z, C.z, h);
+        j.gamma3 = Diagonal(gamma.x, gamma.y, gamma.z);
+        j.bias3 = bias;
+        invM = invM + j.gamma3;
+        j.linearMass = Inverse(invM);
+        break;
+    }
+    case BX_JOINT_FRICTION:
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        j.linearMass = PointMass(j.mA, j.mB, j.iA, j.iB, j.rA, j.rB);
+        j.angularMass3 = j.iA + j.iB;
+        break;
+    case BX_JOINT_MOTOR: {
+        j.rA = Mul(qA, j.linearOffset - j.localCenterA);
+        j.rB = Mul(qB, -j.localCenterB);
+        j.linearMass = PointMass(j.mA, j.mB, j.iA, j.iB, j.rA, j.rB);
+        j.linearError = xB + j.rB - xA - j.rA;
+        Quat q1 = Conjugate(qA) * qB;
+        if (Dot(q1, j.angularOffset) < scalar(0)) q1 = -q1;
+        Quat qw = step.inv_dt * scalar(2) * (j.angularOffset - q1) * Conjugate(q1);
+        j.angularVelocity = Mul(qA, qw.v);
+        j.angularVelocity *= j.correctionFactor;
+        j.angularMass3 = j.iA + j.iB;
+        break;
+    }
+    default: break;
+    }
+    if (step.warmStarting) {
+        j.linearImpulse *= step.dtRatio;
+        j.angularImpulse3 *= step.dtRatio;
+        j.impulse1 *= step.dtRatio;
+    } else {
+        j.linearImpulse.SetZero();
+        j.angularImpulse3.SetZero();
+        j.impulse1 = scalar(0);
+    }
+}
+
+// Point impulse P at rA / rB: vA -= mA P, wA -= iA (rA x P), and the opposite on B.
+static void ApplyPoint(const Joint& j, const Vec3& P, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
+    vA -= j.mA * P;
+    wA -= j.iA * Cross(j.rA, P);
+    vB += j.mB * P;
+    wB += j.iB * Cross(j.rB, P);
+}
+
+// A point or angular impulse clamped to |impulse| <= maxImpulse (friction_joint.cpp:118-133).
+static Vec3 ClampedDelta(Vec3& accumulated, const Vec3& impulse, scalar maxImpulse) {
+    Vec3 old = accumulated;
+    accumulated += impulse;
+    if (LengthSquared(accumulated) > maxImpulse * maxImpulse) {
+        accumulated.Normalize();
+        accumulated *= maxImpulse;
+    }
+    return accumulated - old;
+}
+
+static void WarmStartOther(Joint& j, Velocity* vel) {
+    if (j.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:102-106
+        vel[j.indexB].v += j.mB * j.linearImpulse;
+        vel[j.indexB].w += j.iB * Cross(j.rB, j.linearImpulse);
+        return;
+    }
+    if (j.type == BX_JOINT_SPRING) {  // spring_joint.cpp:175-183: in place
+        Vec3 P = j.impulse1 * j.n;
+        vel[j.indexA].v -= j.mA * P;
+        vel[j.indexA].w -= j.iA * Cross(j.rA, P);
+        vel[j.indexB].v += j.mB * P;
+        vel[j.indexB].w += j.iB * Cross(j.rB, P);
+        return;
+    }
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w, vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    if (j.type == BX_JOINT_FRICTION) {  // friction_joint.cpp:88-107
+        Vec3 P = j.linearImpulse;
+        vA -= j.mA * P;
+        wA -= j.iA * (Cross(j.rA, P) + j.angularImpulse3);
+        vB += j.mB * P;
+        wB += j.iB * (Cross(j.rB, P) + j.angularImpulse3);
+    } else {  // sphere_joint.cpp:81-97; weld_joint.cpp:179-205 and motor_joint.cpp:115-141 add the angular part
+        ApplyPoint(j, j.linearImpulse, vA, wA, vB, wB);
+        if (j.type == BX_JOINT_WELD || j.type == BX_JOINT_MOTOR) {
+            wA -= j.iA * j.angularImpulse3;
+            wB += j.iB * j.angularImpulse3;
+        }
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+static void SolveOtherVelocity(const TimeStep& step, Joint& j, Velocity* vel) {
+    const scalar h = step.dt;
+    if (j.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:108-131
+        Vec3 vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+        Vec3 Cdot = vB + Cross(wB, j.rB);
+        Vec3 impulse = -j.linearMass * (Cdot + j.bias3 + j.gamma3 * j.linearImpulse);
+        Vec3 oldImpulse = j.linearImpulse;
+        j.linearImpulse += impulse;
+        scalar maxImpulse = h * j.maxForce;
+        if (Dot(j.linearImpulse, j.linearImpulse) > maxImpulse * maxImpulse)
+            j.linearImpulse *= maxImpulse / Length(j.linearImpulse);
+        impulse = j.linearImpulse - oldImpulse;
+        vB += j.mB * impulse;
+        wB += j.iB * Cross(j.rB, impulse);
+        vel[j.indexB].v = vB;
+        vel[j.indexB].w = wB;
+        return;
+    }
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w, vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    switch (j.type) {
+    case BX_JOINT_SPHERE: {  // sphere_joint.cpp:99-122
+        Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+        Vec3 impulse = j.linearMass.Solve(-Cdot);
+        j.linearImpulse += impulse;
+        ApplyPoint(j, impulse, vA, wA, vB, wB);
+        break;
+    }
+    case BX_JOINT_WELD: {  // weld_joint.cpp:207-246
+        {
+            Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+            Vec3 impulse = j.linearMass.Solve(-Cdot);
+            j.linearImpulse += impulse;
+            ApplyPoint(j, impulse, vA, wA, vB, wB);
+        }
+        {
+            Vec3 Cdot = wB - wA;
+            Vec3 impulse = -j.angularMass3 * (Cdot + j.bias3 + j.gamma3 * j.angularImpulse3);
+            j.angularImpulse3 += impulse;
+            wA -= j.iA * impulse;
+            wB += j.iB * impulse;
+        }
+        break;
+    }
+    case BX_JOINT_SPRING: {  // spring_joint.cpp:185-212
+        Vec3 dv = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+        scalar Cdot = Dot(j.n, dv);
+        scalar impulse = -j.mass1 * (Cdot + j.bias1 + j.gamma1 * j.impulse1);
+        j.impulse1 += impulse;
+        ApplyPoint(j, impulse * j.n, vA, wA, vB, wB);
+        break;
+    }
+    case BX_JOINT_FRICTION: {  // friction_joint.cpp:109-158: angular, then linear
+        {
+            Vec3 Cdot = wB - wA;
+            Vec3 impulse = ClampedDelta(j.angularImpulse3, j.angularMass3.Solve(-Cdot), h * j.maxTorque);
+            wA -= j.iA * impulse;
+            wB += j.iB * impulse;
+        }
+        {
+            Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+            Vec3 impulse = ClampedDelta(j.linearImpulse, j.linearMass.Solve(-Cdot), h * j.maxForce);
+            ApplyPoint(j, impulse, vA, wA, vB, wB);
+        }
+        break;
+    }
+    case BX_JOINT_MOTOR: {  // motor_joint.cpp:143-200: linear, then angular
+        {
+            Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA) + step.inv_dt * j.correctionFactor * j.linearError;
+            Vec3 impulse = ClampedDelta(j.linearImpulse, j.linearMass.Solve(-Cdot), h * j.maxForce);
+            ApplyPoint(j, impulse, vA, wA, vB, wB);
+        }
+        {
+            Vec3 Cdot = (wB - wA) - j.angularVelocity;
+            Vec3 impulse = ClampedDelta(j.angularImpulse3, j.angularMass3.Solve(-Cdot), h * j.maxTorque);
+            wA -= j.iA * impulse;
+            wB += j.iB * impulse;
+        }
+        break;
+    }
+    default: break;
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+// SolvePositionConstraints: the point constraint of the sphere and weld joints
+// (sphere_joint.cpp:124-165, weld_joint.cpp:248-296), the rigid spring (spring_joint.cpp:214-271,
+// whose orientations are not renormalized); the mouse, friction and motor joints and a soft spring
+// have none.
+static bool SolveOtherPosition(Joint& j, Position* pos) {
+    if (j.type == BX_JOINT_MOUSE || j.type == BX_JOINT_FRICTION || j.type == BX_JOINT_MOTOR ||
+        (j.type == BX_JOINT_SPRING && j.frequencyHz > scalar(0)))
+        return true;
+    Vec3 xA = pos[j.indexA].x, xB = pos[j.indexB].x;
+    Quat qA = pos[j.indexA].q, qB = pos[j.indexB].q;
+    Mat33 iA = pos[j.indexA].I, iB = pos[j.indexB].I;
+    bool solved;
+    if (j.type == BX_JOINT_SPRING) {
+        scalar mA = j.mA, mB = j.mB;
+        Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Vec3 n = xB + rB - xA - rA;
+        scalar length = Length(n);
+        scalar C = length - j.length;
+        C = Clamp(C, -kMaxLinearCorrection, kMaxLinearCorrection);
+        n.Normalize();
+        Vec3 rnA = Cross(rA, n);
+        Vec3 rnB = Cross(rB, n);
+        scalar kMass = mA + mB + Dot(iA * rnA, rnA) + Dot(iB * rnB, rnB);
+        scalar mass = kMass > scalar(0) ? scalar(1) / kMass : scalar(0);
+        scalar lambda = -mass * C;
+        Vec3 impulse = lambda * n;
+        xA -= mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        iA = RotateToFrame(j.localInvIA, qA);
+        xB += mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        iB = RotateToFrame(j.localInvIB, qB);
+        solved = Abs(C) < kLinearSlop;
+    } else {  // sphere, weld
+        Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Mat33 mass = PointMass(j.mA, j.mB, iA, iB, rA, rB);
+        Vec3 C = xB + rB - xA - rA;
+        Vec3 impulse = mass.Solve(-C);
+        xA -= j.mA * impulse;
+        qA -= Derivative(qA, iA * Cross(rA, impulse));
+        qA.Normalize();
+        iA = RotateToFrame(j.localInvIA, qA);
+        xB += j.mB * impulse;
+        qB += Derivative(qB, iB * Cross(rB, impulse));
+        qB.Normalize();
+        iB = RotateToFrame(j.localInvIB, qB);
+        solved = Length(C) <= kLinearSlop;
+    }
+    pos[j.indexA].x = xA;
+    pos[j.indexA].q = qA;
+    pos[j.indexB].x = xB;
+    pos[j.indexB].q = qB;
+    pos[j.indexA].I = iA;
+    pos[j.indexB].I = iB;
+    return solved;
+}
+
+static void InitJoint(World& w, const TimeStep& step, Joint& j, Position* pos) {
+    if (j.type == BX_JOINT_REVOLUTE) InitRevolute(w, step, j, pos);
+    else InitOther(w, step, j, pos);
+}
+static void WarmStartJoint(Joint& j, Velocity* vel) {
+    if (j.type == BX_JOINT_REVOLUTE) WarmStartRevolute(j, vel);
+    else WarmStartOther(j, vel);
+}
+static void SolveJointVelocity(const TimeStep& step, Joint& j, Velocity* vel) {
+    if (j.type == BX_JOINT_REVOLUTE) SolveRevoluteVelocity(step, j, vel);
+    else SolveOtherVelocity(step, j, vel);
+}
+static bool SolveJointPosition(Joint& j, Position* pos) {
+    return j.type == BX_JOINT_REVOLUTE ? SolveRevolutePosition(j, pos) : SolveOtherPosition(j, pos);
 }
 
 // b3JointSolver (joint_solver.cpp:31-64): the island's joints in island order.

@@ -179,6 +179,7 @@ inline Vec3 MulT(const Mat33& A, const Vec3& v) { return Vec3(Dot(A.x, v), Dot(A
 inline Mat33 Transpose(const Mat33& A) {
     return Mat33(Vec3(A.x.x, A.y.x, A.z.x), Vec3(A.x.y, A.y.y, A.z.y), Vec3(A.x.z, A.y.z, A.z.z));
 }
+inline Mat33 operator-(const Mat33& A) { return scalar(-1) * A; }  // mat33.h:123-126
 inline Mat33 Diagonal(scalar s) { return Mat33(Vec3(s, 0, 0), Vec3(0, s, 0), Vec3(0, 0, s)); }
 inline Mat33 Diagonal(scalar a, scalar b, scalar c) { return Mat33(Vec3(a, 0, 0), Vec3(0, b, 0), Vec3(0, 0, c)); }
 inline Mat33 Skew(const Vec3& v) {

@@ -100,6 +100,16 @@ struct Joint {
     Mat33 linearMass;
     Vec3 a1, a2;
     Mat22 angularMass;
+    // the other joint types (sphere, weld, spring, mouse, friction, motor): definitions, carried
+    // impulses and per-solve values
+    scalar frequencyHz = 0, dampingRatio = 0, length = 0, maxForce = 0, maxTorque = 0, correctionFactor = 0;
+    Vec3 target, linearOffset;
+    Quat angularOffset;
+    Vec3 angularImpulse3;       // weld, friction, motor
+    scalar impulse1 = 0;        // spring
+    Mat33 gamma3, angularMass3;
+    Vec3 bias3, n, linearError, angularVelocity;
+    scalar mass1 = 0, gamma1 = 0, bias1 = 0;
 };
 
 struct Position { Vec3 x; Quat q; Mat33 I; };

@@ -49,10 +49,6 @@ void (*g_blockEnd)() = nullptr;
 void (*g_hostAccessEnd)(void* p, size_t n) = nullptr;
 void (*g_onFree)(void* p) = nullptr;
 
-void WorldHook(const void* view, uint32_t w) {
-    if (g_worldHook) g_worldHook(view, w);
-}
-
 namespace {
 
 const size_t kStackBytes = size_t(128) << 10;  // per workgroup thread; the largest kernel frame is a few KiB
@@ -64,6 +60,10 @@ struct Fiber {
     char* stack = nullptr;
     int state = kDone;
     void* fake = nullptr;
+    // the world this thread works on (BX_WORLD_HOOK): lanes of one workgroup may work on different
+    // worlds (an island per lane group), so the range check follows the thread that runs
+    const void* view = nullptr;
+    uint32_t world = 0;
 };
 
 struct Block {
@@ -83,6 +83,20 @@ struct Block {
 };
 
 thread_local Block* tl_block = nullptr;
+
+}  // namespace
+
+void WorldHook(const void* view, uint32_t w) {
+    if (!g_worldHook) return;
+    if (tl_block && tl_block->cur >= 0) {
+        Fiber& f = tl_block->fibers[size_t(tl_block->cur)];
+        f.view = view;
+        f.world = w;
+    }
+    g_worldHook(view, w);
+}
+
+namespace {
 
 void ToScheduler(bool dying) {
     Block* b = tl_block;
@@ -131,6 +145,7 @@ void RunBlock(Block& b, const std::function<void()>& body, unsigned n) {
         makecontext(&f.ctx, FiberEntry, 0);
         f.state = kReady;
         f.fake = nullptr;
+        f.view = nullptr;
     }
     Block* prev = tl_block;
     tl_block = &b;
@@ -141,6 +156,7 @@ void RunBlock(Block& b, const std::function<void()>& body, unsigned n) {
             ran = true;
             b.cur = int(i);
             tl_ctx.threadIdx = dim3(i, 0, 0);
+            if (g_worldHook && b.fibers[i].view) g_worldHook(b.fibers[i].view, b.fibers[i].world);
 #ifdef BXCPU_ASAN
             __sanitizer_start_switch_fiber(&b.schedFake, b.fibers[i].stack, kStackBytes);
 #endif

@@ -175,6 +175,29 @@ print("time of impact and casts equal to the oracle", flush=True)
     assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
 
 
+def test_cpu_build_joints(tmp_path):
+    """The device tests of the sphere, weld, spring, mouse, friction and motor joints
+    (tests/test_gpu_joints.py) on the host build under AddressSanitizer with the per-world check."""
+    _make("asan")
+    script = tmp_path / "joints.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {os.path.dirname(HERE)!r})
+from bounce_amd import device
+device.load({os.path.join(BUILD, "_out", "libbxstep_asan.so")!r})
+from tests import test_gpu_joints as t
+ctx = device.Context(0)
+for mode in (1, 5):
+    t.test_joint_zoo_device_equals_oracle(ctx, mode)
+t.test_joints_created_and_destroyed_between_steps(ctx)
+t.test_unsupported_joint_types_are_refused(ctx)
+print("joints equal to the oracle", flush=True)
+""")
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "equal to the oracle" in r.stdout, r.stdout[-2000:] + r.stderr[-3000:]
+
+
 def test_cpu_build_body_and_world_edits(tmp_path):
     """The device test of body and world edits (tests/test_gpu_edits.py) on the host build under
     AddressSanitizer with the per-world check: every step of the scripted setters, forces, impulses,
