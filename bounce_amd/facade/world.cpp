@@ -716,40 +716,111 @@ void b3World::DestroyBody(b3Body* body) {
 }
 
 b3Joint* b3World::CreateJoint(const b3JointDef& def) {
-    if (def.type != e_revoluteJoint || !def.bodyA || !def.bodyB || def.bodyA->m_world != this ||
-        def.bodyB->m_world != this)
-        return nullptr;
-    const b3RevoluteJointDef& r = static_cast<const b3RevoluteJointDef&>(def);
+    if (!def.bodyA || !def.bodyB || def.bodyA->m_world != this || def.bodyB->m_world != this) return nullptr;
     bx_joint_desc d;
     std::memset(&d, 0, sizeof(d));
-    d.type = BX_JOINT_REVOLUTE;
-    d.body_a = r.bodyA->m_index;
-    d.body_b = r.bodyB->m_index;
-    d.collide_connected = r.collideConnected ? 1u : 0u;
-    d.use_initialize = r.m_initialize ? 1u : 0u;
-    Copy3(d.axis, r.m_axis);
-    Copy3(d.anchor, r.m_anchor);
-    Copy3(d.local_anchor_a, r.localAnchorA);
-    Copy4(d.local_rotation_a, r.localRotationA);
-    Copy3(d.local_anchor_b, r.localAnchorB);
-    Copy4(d.local_rotation_b, r.localRotationB);
-    Copy4(d.reference_rotation, r.referenceRotation);
-    d.enable_limit = r.enableLimit ? 1u : 0u;
-    d.lower_angle = r.lowerAngle;
-    d.upper_angle = r.upperAngle;
-    d.enable_motor = r.enableMotor ? 1u : 0u;
-    d.motor_speed = r.motorSpeed;
-    d.max_motor_torque = r.maxMotorTorque;
+    d.body_a = def.bodyA->m_index;
+    d.body_b = def.bodyB->m_index;
+    d.collide_connected = def.collideConnected ? 1u : 0u;
+    switch (def.type) {
+        case e_revoluteJoint: {
+            const b3RevoluteJointDef& r = static_cast<const b3RevoluteJointDef&>(def);
+            d.type = BX_JOINT_REVOLUTE;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.axis, r.m_axis);
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy4(d.local_rotation_a, r.localRotationA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            Copy4(d.local_rotation_b, r.localRotationB);
+            Copy4(d.reference_rotation, r.referenceRotation);
+            d.enable_limit = r.enableLimit ? 1u : 0u;
+            d.lower_angle = r.lowerAngle;
+            d.upper_angle = r.upperAngle;
+            d.enable_motor = r.enableMotor ? 1u : 0u;
+            d.motor_speed = r.motorSpeed;
+            d.max_motor_torque = r.maxMotorTorque;
+            break;
+        }
+        case e_sphereJoint: {
+            const b3SphereJointDef& r = static_cast<const b3SphereJointDef&>(def);
+            d.type = BX_JOINT_SPHERE;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            break;
+        }
+        case e_weldJoint: {
+            const b3WeldJointDef& r = static_cast<const b3WeldJointDef&>(def);
+            d.type = BX_JOINT_WELD;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            Copy4(d.reference_rotation, r.referenceRotation);
+            d.frequency_hz = r.frequencyHz;
+            d.damping_ratio = r.dampingRatio;
+            d.correction_factor = r.correctionFactor;
+            break;
+        }
+        case e_springJoint: {
+            const b3SpringJointDef& r = static_cast<const b3SpringJointDef&>(def);
+            d.type = BX_JOINT_SPRING;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchorA);
+            Copy3(d.anchor_b, r.m_anchorB);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            d.length = r.length;
+            d.frequency_hz = r.frequencyHz;
+            d.damping_ratio = r.dampingRatio;
+            break;
+        }
+        case e_mouseJoint: {
+            const b3MouseJointDef& r = static_cast<const b3MouseJointDef&>(def);
+            d.type = BX_JOINT_MOUSE;
+            Copy3(d.target, r.target);
+            d.max_force = r.maxForce;
+            d.frequency_hz = r.frequencyHz;
+            d.damping_ratio = r.dampingRatio;
+            break;
+        }
+        case e_frictionJoint: {
+            const b3FrictionJointDef& r = static_cast<const b3FrictionJointDef&>(def);
+            d.type = BX_JOINT_FRICTION;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            d.max_force = r.maxForce;
+            d.max_torque = r.maxTorque;
+            break;
+        }
+        case e_motorJoint: {
+            const b3MotorJointDef& r = static_cast<const b3MotorJointDef&>(def);
+            d.type = BX_JOINT_MOTOR;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.linear_offset, r.linearOffset);
+            Copy4(d.angular_offset, r.angularOffset);
+            d.max_force = r.maxForce;
+            d.max_torque = r.maxTorque;
+            d.correction_factor = r.correctionFactor;
+            break;
+        }
+        default:
+            return nullptr;  // cone, prismatic and wheel joints are not run on the device
+    }
     m_device->newJoints.push_back(d);
     QueueTopology(BX_TOPO_CREATE_JOINT, uint32(m_device->newJoints.size() - 1));
 
     b3Joint* j = new b3Joint();
-    j->m_def = r;
+    j->m_def = def;
     j->m_userData = def.userData;
     j->m_index = uint32(m_joints.size());
     j->m_next = m_jointList;  // joint_manager.cpp:30-73: world list and both bodies' lists prepend
     m_jointList = j;
-    b3Body* bodies[2] = {r.bodyA, r.bodyB};
+    b3Body* bodies[2] = {def.bodyA, def.bodyB};
     b3JointEdge* edges[2] = {&j->m_edgeA, &j->m_edgeB};
     for (int k = 0; k < 2; ++k) {
         b3JointEdge* e = edges[k];

@@ -293,6 +293,142 @@ struct b3RevoluteJointDef : public b3JointDef {
     b3Vec3 m_axis, m_anchor;
 };
 
+// The other joint definitions (sphere_joint.h, weld_joint.h, spring_joint.h, mouse_joint.h,
+// friction_joint.h, motor_joint.h): the same members and defaults. Initialize records its
+// arguments; the local frames are computed from the bodies' transforms on the device when the
+// joint is created there, as the reference's Initialize computes them from the current transforms.
+struct b3SphereJointDef : public b3JointDef {
+    b3SphereJointDef() : m_initialize(false) {
+        type = e_sphereJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        m_anchor.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchor) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchor = anchor;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    bool m_initialize;
+    b3Vec3 m_anchor;
+};
+
+struct b3WeldJointDef : public b3JointDef {
+    b3WeldJointDef() : m_initialize(false) {
+        type = e_weldJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        referenceRotation.SetIdentity();
+        frequencyHz = scalar(0);
+        dampingRatio = scalar(0);
+        correctionFactor = scalar(0.2);
+        m_anchor.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchor) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchor = anchor;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    b3Quat referenceRotation;
+    scalar frequencyHz;
+    scalar dampingRatio;
+    scalar correctionFactor;
+    bool m_initialize;
+    b3Vec3 m_anchor;
+};
+
+struct b3SpringJointDef : public b3JointDef {
+    b3SpringJointDef() : m_initialize(false) {
+        type = e_springJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        length = scalar(0);
+        frequencyHz = scalar(0);
+        dampingRatio = scalar(0);
+        m_anchorA.SetZero();
+        m_anchorB.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchorA, const b3Vec3& anchorB) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchorA = anchorA;
+        m_anchorB = anchorB;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    scalar length;
+    scalar frequencyHz;
+    scalar dampingRatio;
+    bool m_initialize;
+    b3Vec3 m_anchorA, m_anchorB;
+};
+
+struct b3MouseJointDef : public b3JointDef {
+    b3MouseJointDef() {
+        type = e_mouseJoint;
+        target.SetZero();
+        maxForce = scalar(0);
+        frequencyHz = scalar(6);
+        dampingRatio = scalar(0.8);
+    }
+    b3Vec3 target;
+    scalar maxForce;
+    scalar frequencyHz;
+    scalar dampingRatio;
+};
+
+struct b3FrictionJointDef : public b3JointDef {
+    b3FrictionJointDef() : m_initialize(false) {
+        type = e_frictionJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        maxForce = scalar(0);
+        maxTorque = scalar(0);
+        m_anchor.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchor) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchor = anchor;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    scalar maxForce;
+    scalar maxTorque;
+    bool m_initialize;
+    b3Vec3 m_anchor;
+};
+
+struct b3MotorJointDef : public b3JointDef {
+    b3MotorJointDef() : m_initialize(false) {
+        type = e_motorJoint;
+        linearOffset.SetZero();
+        angularOffset.SetIdentity();
+        maxForce = scalar(0);
+        maxTorque = scalar(0);
+        correctionFactor = scalar(0.2);
+    }
+    void Initialize(b3Body* bA, b3Body* bB) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+    }
+    b3Vec3 linearOffset;
+    b3Quat angularOffset;
+    scalar maxForce;
+    scalar maxTorque;
+    scalar correctionFactor;
+    bool m_initialize;
+};
+
 class b3Joint {
 public:
     b3JointType GetType() const { return m_def.type; }
@@ -309,14 +445,20 @@ public:
 private:
     friend class b3World;
     b3Joint() {}
-    b3RevoluteJointDef m_def;
-    uint32 m_index;  // creation index in the world
+    b3JointDef m_def;  // the definition's common members
+    uint32 m_index;    // creation index in the world
     b3JointEdge m_edgeA, m_edgeB;
     b3Joint* m_next;
     void* m_userData;
 };
 
 typedef b3Joint b3RevoluteJoint;
+typedef b3Joint b3SphereJoint;
+typedef b3Joint b3WeldJoint;
+typedef b3Joint b3SpringJoint;
+typedef b3Joint b3MouseJoint;
+typedef b3Joint b3FrictionJoint;
+typedef b3Joint b3MotorJoint;
 
 // A touching or merely overlapping pair of fixtures (contact.h); the world's list holds the pairs the
 // last step (or topology change) left, newest first. Valid until the next step or world change.

@@ -85,10 +85,11 @@ int main() {
 
     b3BoxHull groundHull(12.0f, 1.0f, 12.0f);
     b3BoxHull boxHull(0.5f, 0.5f, 0.5f);
+    b3Body* ground;
     {
         b3BodyDef gd;
         gd.userData = Tag(0);
-        b3Body* ground = world.CreateBody(gd);
+        ground = world.CreateBody(gd);
         b3HullShape gs;
         gs.m_hull = &groundHull;
         b3FixtureDef gf;
@@ -121,6 +122,7 @@ int main() {
     b3Body* compound = nullptr;
     b3Fixture* compoundSphere = nullptr;
     b3Joint* joint = nullptr;
+    b3Joint* mouse = nullptr;
 
     for (int step = 0; step < 180; ++step) {
         sphere->ApplyForceToCenter(b3Vec3(4.0f, 0.0f, 0.5f), true);
@@ -186,8 +188,30 @@ int main() {
         if (step == 150) {
             if (stack[4]) stack[4]->SetAwake(false);
             sphere->SetSleepingAllowed(false);
+            // a mouse joint lifts the capsule; a soft spring ties the spawned box to the compound;
+            // a weld joins the two lowest boxes
+            b3MouseJointDef md;
+            md.bodyA = ground;
+            md.bodyB = capsule;
+            md.target = capsule->GetPosition() + b3Vec3(0.0f, 2.0f, 0.0f);
+            md.maxForce = 200.0f;
+            md.frequencyHz = 5.0f;
+            md.dampingRatio = 0.7f;
+            mouse = world.CreateJoint(md);
+            b3SpringJointDef sd;
+            sd.Initialize(spawned, compound, spawned->GetPosition(), compound->GetWorldCenter());
+            sd.frequencyHz = 2.0f;
+            sd.dampingRatio = 0.3f;
+            world.CreateJoint(sd);
+            b3WeldJointDef wd;
+            wd.Initialize(stack[0], stack[1], scalar(0.5) * (stack[0]->GetPosition() + stack[1]->GetPosition()));
+            world.CreateJoint(wd);
         }
         if (step == 160) world.SetSleeping(false);
+        if (step == 170) {
+            world.DestroyJoint(mouse);
+            mouse = nullptr;
+        }
 
         world.Step(1.0f / 60.0f, 8, 2);
         if (world.GetLastError()[0]) {

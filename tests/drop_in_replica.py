@@ -78,12 +78,12 @@ class Replica:
         self.queue.append(("t", T.BX_TOPO_CREATE_FIXTURE, len(self.new_fixtures) - 1))
         self.fixtures.append((fid, bid))
 
-    def create_revolute(self, a, b, axis, anchor):
+    def create_joint(self, a, b, kind=T.BX_JOINT_REVOLUTE, **fields):
         j = scenes.joint_array(1)
+        j["type"] = kind
         j["body_a"], j["body_b"] = self.bodies.index(a), self.bodies.index(b)
-        j["use_initialize"] = 1
-        j["axis"] = axis
-        j["anchor"] = anchor
+        for k, v in fields.items():
+            j[k] = v
         self.new_joints.append(j)
         self.queue.append(("t", T.BX_TOPO_CREATE_JOINT, len(self.new_joints) - 1))
         self.joints.append((a, b))
@@ -220,7 +220,7 @@ def spawn_apply_lines(kind="oracle"):
         if step == 70:
             a = np.asarray(R.state(7)["position"], dtype=F)
             b = np.asarray(R.state(8)["position"], dtype=F)
-            R.create_revolute(7, 8, (0.0, 0.0, 1.0), F(0.5) * (a + b))
+            R.create_joint(7, 8, use_initialize=1, axis=(0.0, 0.0, 1.0), anchor=F(0.5) * (a + b))
             joint = True
         if step == 90:
             R.destroy_body(3)
@@ -245,8 +245,17 @@ def spawn_apply_lines(kind="oracle"):
             if stack[4]:
                 R.edit(5, T.BX_EDIT_AWAKE, flag=False)
             R.edit(6, T.BX_EDIT_SLEEPING_ALLOWED, flag=False)
+            target = np.asarray(R.state(8)["position"], dtype=F) + np.array([0.0, 2.0, 0.0], dtype=F)
+            R.create_joint(0, 8, T.BX_JOINT_MOUSE, target=target, max_force=200.0, frequency_hz=5.0,
+                           damping_ratio=0.7)
+            R.create_joint(7, 9, T.BX_JOINT_SPRING, use_initialize=1, anchor=R.state(7)["position"],
+                           anchor_b=R.state(9)["world_center"], frequency_hz=2.0, damping_ratio=0.3)
+            mid = F(0.5) * (np.asarray(R.state(1)["position"], dtype=F) + np.asarray(R.state(2)["position"], dtype=F))
+            R.create_joint(1, 2, T.BX_JOINT_WELD, use_initialize=1, anchor=mid, correction_factor=0.2)
         if step == 160:
             R.edit(None, T.BX_EDIT_WORLD_SLEEPING, flag=False)
+        if step == 170:
+            R.destroy_joint(0)
         R.step()
         st = R.world.bodies()
         for bid in reversed(R.bodies):
