@@ -80,7 +80,9 @@ class bx_joint_desc(C.Structure):
                 ("max_motor_torque", f32), ("anchor_b", _arr(f32, 3)), ("frequency_hz", f32),
                 ("damping_ratio", f32), ("length", f32), ("max_force", f32), ("max_torque", f32),
                 ("correction_factor", f32), ("target", _arr(f32, 3)), ("linear_offset", _arr(f32, 3)),
-                ("angular_offset", _arr(f32, 4))]
+                ("angular_offset", _arr(f32, 4)), ("cone_angle", f32), ("local_axis_a", _arr(f32, 3)),
+                ("local_axis_b", _arr(f32, 3)), ("lower_translation", f32), ("upper_translation", f32),
+                ("axis_b", _arr(f32, 3))]
 
 
 class bx_world_desc(C.Structure):

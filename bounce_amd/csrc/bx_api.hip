@@ -822,9 +822,7 @@ static int MakeFixture(const bx_fixture_desc& fd, const HostLibrary& L, Fixture&
 // b3Joint::Create (joint.cpp:33-100) between two of `bodies`: the joint's constructor, with the frames
 // of its def's Initialize taken from the bodies' current transforms when asked.
 static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, Joint& j) {
-    if (jd.type == BX_JOINT_CONE || jd.type == BX_JOINT_PRISMATIC || jd.type == BX_JOINT_WHEEL)
-        return Fail(BX_E_UNSUPPORTED, "cone, prismatic and wheel joints are not supported by the device path");
-    if (jd.type < BX_JOINT_MOUSE || jd.type > BX_JOINT_MOTOR || jd.body_a >= bodies.size() ||
+    if (jd.type < BX_JOINT_MOUSE || jd.type > BX_JOINT_WHEEL || jd.body_a >= bodies.size() ||
         jd.body_b >= bodies.size() || jd.body_a == jd.body_b)
         return Fail(BX_E_INVALID, "bad joint descriptor");
     std::memset(&j, 0, sizeof(j));
@@ -849,7 +847,48 @@ static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, J
         j.localRotationA = j.localRotationB = j.referenceRotation = Quat(Vec3(0, 0, 0), scalar(1));
         return BX_OK;
     }
-    if (jd.type != BX_JOINT_REVOLUTE) {
+    if (jd.type == BX_JOINT_PRISMATIC || jd.type == BX_JOINT_WHEEL) {
+        j.referenceRotation = Q4(jd.reference_rotation);
+        j.localRotationA = j.localRotationB = Quat(Vec3(0, 0, 0), scalar(1));
+        Vec3 axisA = V3(jd.local_axis_a), axisB = V3(jd.local_axis_b);
+        if (jd.use_initialize) {  // prismatic_joint.cpp:23-35, wheel_joint.cpp:22-31
+            const Vec3 anchor = V3(jd.anchor);
+            j.localAnchorA = MulT(xfA, anchor);
+            j.localAnchorB = MulT(xfB, anchor);
+            axisA = MulC(A0.q, V3(jd.axis));
+            if (jd.type == BX_JOINT_PRISMATIC) j.referenceRotation = Conjugate(A0.q) * B0.q;
+            else axisB = MulC(B0.q, V3(jd.axis_b));
+        } else {
+            j.localAnchorA = V3(jd.local_anchor_a);
+            j.localAnchorB = V3(jd.local_anchor_b);
+        }
+        // the constructors (prismatic_joint.cpp:37-61, wheel_joint.cpp:33-66)
+        j.localXAxisA = axisA;
+        j.localXAxisA.Normalize();
+        ComputeBasis(j.localXAxisA, j.localYAxisA, j.localZAxisA);
+        j.enableMotor = jd.enable_motor ? 1u : 0u;
+        j.motorSpeed = jd.motor_speed;
+        if (jd.type == BX_JOINT_PRISMATIC) {
+            j.enableLimit = jd.enable_limit ? 1u : 0u;
+            j.lowerTranslation = jd.lower_translation;
+            j.upperTranslation = jd.upper_translation;
+        } else {
+            j.maxMotorTorque = jd.max_motor_torque;
+            j.localXAxisB = axisB;
+            j.localXAxisB.Normalize();
+            const Vec3 wA = Mul(A0.q, j.localXAxisA), wB = Mul(B0.q, j.localXAxisB);
+            const Vec3 u = Cross(wA, wB);
+            const scalar x = Dot(wA, wB), y = Length(u);
+            j.referenceCosine = x;
+            j.referenceAngle = scalar(::atan2(double(y), double(x)));
+        }
+        return BX_OK;
+    }
+    if (jd.type == BX_JOINT_CONE) {
+        j.coneAngle = jd.cone_angle;
+        j.enableLimit = jd.enable_limit ? 1u : 0u;
+    }
+    if (jd.type != BX_JOINT_REVOLUTE && jd.type != BX_JOINT_CONE) {
         j.localRotationA = Q4(jd.local_rotation_a);
         j.localRotationB = Q4(jd.local_rotation_b);
         j.referenceRotation = Q4(jd.reference_rotation);
@@ -869,10 +908,16 @@ static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, J
         }
         return BX_OK;
     }
-    if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48)
+    if (jd.use_initialize) {  // b3RevoluteJointDef::Initialize (revolute_joint.cpp:22-48),
+                              // b3ConeJointDef::Initialize (cone_joint.cpp:35-55) about x
         Mat33 rotation;
-        rotation.z = V3(jd.axis);
-        ComputeBasis(rotation.z, rotation.x, rotation.y);
+        if (jd.type == BX_JOINT_CONE) {
+            rotation.x = V3(jd.axis);
+            ComputeBasis(rotation.x, rotation.y, rotation.z);
+        } else {
+            rotation.z = V3(jd.axis);
+            ComputeBasis(rotation.z, rotation.x, rotation.y);
+        }
         Quat q = Mat33Quat(rotation);
         const Body& A = bodies[jd.body_a];
         const Body& Bd = bodies[jd.body_b];
@@ -889,6 +934,7 @@ static int MakeJoint(const bx_joint_desc& jd, const std::vector<Body>& bodies, J
         j.localRotationB = Q4(jd.local_rotation_b);
         j.referenceRotation = Q4(jd.reference_rotation);
     }
+    if (jd.type == BX_JOINT_CONE) return BX_OK;
     j.enableMotor = jd.enable_motor ? 1u : 0u;
     j.enableLimit = jd.enable_limit ? 1u : 0u;
     j.lowerAngle = jd.lower_angle;

@@ -239,6 +239,8 @@ __global__ void __launch_bounds__(kBlock) k_mid(BatchView B, StepParams sp) {
         j.angularImpulse = jc.angularImpulse;
         j.angularImpulse3 = jc.angularImpulse3;
         j.impulse1 = jc.impulse1;
+        j.linearImpulse2 = jc.linearImpulse2;
+        j.springImpulse = jc.springImpulse;
     }
     const u32 n = Min(ws.islandBodyCount, wi.islandBodyCap);
     const u32* ib = B.islandBodies + wi.islandBodyBase;

@@ -269,19 +269,27 @@ __device__ __forceinline__ void QStep(QCon& cur, QCon& ahead, const VelCon* next
     if (kSolve) QStoreImpulses(cur, self);
 }
 
+// Constraints a sweep requests ahead of the one it solves: each lane holds kQuadAhead + 1 register
+// buffers, so a constraint's loads have kQuadAhead solves to arrive in.
+#ifndef BX_QUAD_AHEAD
+#define BX_QUAD_AHEAD 2
+#endif
+constexpr u32 kQuadAhead = BX_QUAD_AHEAD;
+
 template <bool kSolve>
 __device__ __forceinline__ void QSweep(VelCon* vcs, u32 nc, scalar* slab, const QuadLane& L) {
-    // three register buffers rotate: the constraint two contacts ahead is requested while one is solved
-    QCon a, b, c;
-    QLoad(a, vcs, L);
-    QLoad(b, vcs + Min(1u, nc - 1), L);
+    // the buffers rotate: the constraint kQuadAhead places ahead is requested into the buffer of the
+    // one solved last, while the current one is solved
+    constexpr u32 kBuf = kQuadAhead + 1;
+    QCon q[kBuf];
+#pragma unroll
+    for (u32 s = 0; s < kQuadAhead; ++s) QLoad(q[s], vcs + Min(s, nc - 1), L);
     for (u32 k = 0;;) {
-        QStep<kSolve>(a, c, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
-        if (++k == nc) break;
-        QStep<kSolve>(b, a, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
-        if (++k == nc) break;
-        QStep<kSolve>(c, b, vcs + Min(k + 2, nc - 1), vcs + k, slab, L);
-        if (++k == nc) break;
+#pragma unroll
+        for (u32 s = 0; s < kBuf; ++s) {
+            QStep<kSolve>(q[s], q[(s + kQuadAhead) % kBuf], vcs + Min(k + kQuadAhead, nc - 1), vcs + k, slab, L);
+            if (++k == nc) return;
+        }
     }
 }
 

@@ -196,7 +196,7 @@ struct JointEdge { u32 joint, other; };
 
 // b3JointType values of the joints the device solves (joint.h:33-46).
 constexpr u32 kJointMouse = 1, kJointSpring = 2, kJointWeld = 3, kJointRevolute = 4, kJointSphere = 5,
-              kJointFriction = 7, kJointMotor = 8;
+              kJointCone = 6, kJointFriction = 7, kJointMotor = 8, kJointPrismatic = 9, kJointWheel = 10;
 
 // A joint (joints/*_joint.h): definition and the warm-start state that persists between steps. The
 // revolute members come first; the other types use the anchors, the reference rotation, the linear
@@ -217,8 +217,13 @@ struct Joint {
     Vec3 target;                       // mouse: world target
     Vec3 linearOffset;                 // motor
     Quat angularOffset;                // motor
-    Vec3 angularImpulse3;              // weld, friction, motor
-    scalar impulse1;                   // spring
+    Vec3 angularImpulse3;              // weld, friction, motor, prismatic
+    scalar impulse1;                   // spring; wheel: angular impulse
+    // cone, prismatic and wheel joints
+    Vec3 localXAxisA, localYAxisA, localZAxisA, localXAxisB;
+    scalar coneAngle, lowerTranslation, upperTranslation, referenceCosine, referenceAngle;
+    Vec2 linearImpulse2;               // prismatic, wheel: the two perpendicular rows
+    scalar springImpulse;              // wheel
 };
 
 // Solver constants of one island joint (b3RevoluteJoint's per-solve members), built in parallel
@@ -249,6 +254,14 @@ struct JointCon {
     Mat33 gamma3, angularMass3;
     Vec3 bias3, n, linearError, angularVelocity, angularImpulse3;
     scalar mass1, gamma1, bias1, impulse1;
+    // cone, prismatic and wheel joints: motorAxis holds the cone axis, the prismatic axis or the
+    // wheel's axis on A; mass1 the cone or spring mass; angularMass3 and bias3 the prismatic
+    // angular row
+    Vec3 perp1, perp2, s1, s2, s3, s4, axisB, u;
+    Mat22 mass22;
+    scalar angularMass1, coneAngle, referenceCosine, referenceAngle, springImpulse;
+    Vec3 localXAxisA, localYAxisA, localZAxisA, localXAxisB;
+    Vec2 linearImpulse2;
 };
 
 

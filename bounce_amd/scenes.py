@@ -623,7 +623,9 @@ def joint_zoo(n_worlds=1, seed=21, sleeping=False):
     """Every joint type of the device path in one world, on a ground box: boxes hung from the ground
     and from each other by a sphere joint, a rigid and a soft weld, a rigid and a soft spring, a motor
     joint pulling a box to an offset, a friction joint holding a box sliding on the ground, a mouse
-    joint dragging a box to a target, and a revolute joint; some frames from Initialize, some given.
+    joint dragging a box to a target, a revolute joint, a swinging box held inside a cone, a box driven
+    along a rail by a prismatic motor against its limits with another box on an equal-limit prismatic
+    below it, and a chassis on a sprung, motorized wheel; some frames from Initialize, some given.
     Each world is jittered by its seed."""
     s = SceneSet("joint_zoo")
     g = s.add_hull(hull_lib.box_hull(30.0, 0.5, 30.0))
@@ -631,12 +633,13 @@ def joint_zoo(n_worlds=1, seed=21, sleeping=False):
     rng = np.random.default_rng(seed)
     J = capi
     for _ in range(n_worlds):
-        n = 13
+        n = 18
         bodies = body_array(n)
         fixtures = fixture_array(n)
         _ground(bodies, fixtures, g)
         pos = np.array([[-6, 6, 0], [-6, 4.8, 0.2], [-3, 6, 0], [-3, 4.9, 0.1], [0, 6, 0], [0, 4.6, 0],
-                        [3, 6, 0], [3, 4.4, 0.3], [6, 0.81, 0], [-8, 0.81, 3], [8, 3, -3], [9, 6, 0]],
+                        [3, 6, 0], [3, 4.4, 0.3], [6, 0.81, 0], [-8, 0.81, 3], [8, 3, -3], [9, 6, 0],
+                        [-6, 8, 4], [-3, 9, 4], [3, 2, 4], [3, 1.2, 4], [-3, 7.9, 4]],
                        dtype=F32) + rng.uniform(-0.05, 0.05, (n - 1, 3)).astype(F32)
         sl = slice(1, n)
         bodies["type"][sl] = J.BX_BODY_DYNAMIC
@@ -644,11 +647,12 @@ def joint_zoo(n_worlds=1, seed=21, sleeping=False):
         bodies["orientation"][sl] = random_orientations(rng, n - 1) if rng.random() < 0.0 else (0, 0, 0, 1)
         bodies["linear_velocity"][9] = (3.0, 0.0, 1.0)      # slides on the ground against its friction joint
         bodies["angular_velocity"][9] = (0.0, 4.0, 0.0)
+        bodies["linear_velocity"][13] = (4.0, 0.0, 1.0)     # swings out against its cone
         fixtures["body"][sl] = np.arange(1, n)
         fixtures["shape_type"][sl] = J.BX_SHAPE_HULL
         fixtures["shape_index"][sl] = b
         fixtures["density"][sl] = 1.0
-        joints = joint_array(11)
+        joints = joint_array(16)
         spec = [
             # (type, a, b, use_initialize, anchor)
             (J.BX_JOINT_SPHERE, 0, 1, 1, (-6, 6.6, 0)),
@@ -662,6 +666,11 @@ def joint_zoo(n_worlds=1, seed=21, sleeping=False):
             (J.BX_JOINT_FRICTION, 0, 10, 1, (-8, 0.5, 3)),
             (J.BX_JOINT_MOUSE, 0, 11, 0, None),
             (J.BX_JOINT_REVOLUTE, 11, 12, 1, (8.5, 3, -3)),
+            (J.BX_JOINT_CONE, 0, 13, 1, (-6, 9, 4)),
+            (J.BX_JOINT_PRISMATIC, 0, 14, 1, (-3, 9, 4)),
+            (J.BX_JOINT_WHEEL, 15, 16, 1, (3, 1.2, 4)),
+            (J.BX_JOINT_PRISMATIC, 14, 17, 0, None),
+            (J.BX_JOINT_CONE, 13, 0, 0, None),
         ]
         for k, (t, a, bb, init, anchor) in enumerate(spec):
             j = joints[k]
@@ -684,5 +693,18 @@ def joint_zoo(n_worlds=1, seed=21, sleeping=False):
         joints[9]["frequency_hz"], joints[9]["damping_ratio"] = F32(6.0), F32(0.8)
         joints[10]["axis"] = (0.0, 0.0, 1.0)
         joints[10]["collide_connected"] = 1
+        joints[11]["axis"], joints[11]["cone_angle"], joints[11]["enable_limit"] = (0.0, -1.0, 0.0), F32(0.6), 1
+        joints[12]["axis"] = (1.0, 0.0, 0.3)
+        joints[12]["enable_limit"], joints[12]["lower_translation"], joints[12]["upper_translation"] = 1, -1.0, 1.5
+        joints[12]["enable_motor"], joints[12]["motor_speed"], joints[12]["max_force"] = 1, F32(2.0), F32(50.0)
+        joints[13]["axis"], joints[13]["axis_b"] = (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)
+        joints[13]["enable_motor"], joints[13]["motor_speed"], joints[13]["max_motor_torque"] = 1, F32(5.0), F32(10.0)
+        joints[13]["frequency_hz"], joints[13]["damping_ratio"] = F32(4.0), F32(0.7)
+        joints[14]["local_anchor_a"], joints[14]["local_anchor_b"] = (0.0, -0.55, 0.0), (0.0, 0.55, 0.0)
+        joints[14]["local_axis_a"] = (0.0, 1.0, 0.0)
+        joints[14]["enable_limit"] = 1          # lower = upper = 0: equal limits
+        joints[15]["local_anchor_b"] = (-6.0, 9.0, 4.0)    # the cone 11's anchor, on body 13 too
+        joints[15]["local_anchor_a"] = np.array([-6.0, 9.0, 4.0], dtype=F32) - pos[12]
+        joints[15]["cone_angle"] = F32(1.2)     # given frames, limit off
         s.worlds.append(World(bodies, fixtures, joints, flags=capi.BX_WORLD_SLEEPING if sleeping else 0))
     return s

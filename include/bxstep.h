@@ -164,6 +164,9 @@ typedef struct bx_joint_desc {
          spring    b3SpringJointDef::Initialize(A, B, anchor, anchor_b) (spring_joint.cpp:23-30)
          friction  b3FrictionJointDef::Initialize(A, B, anchor)      (friction_joint.cpp:23-29)
          motor     b3MotorJointDef::Initialize(A, B)                  (motor_joint.cpp:23-31)
+         cone      b3ConeJointDef::Initialize(A, B, axis, anchor, cone_angle) (cone_joint.cpp:35-55)
+         prismatic b3PrismaticJointDef::Initialize(A, B, anchor, axis) (prismatic_joint.cpp:23-35)
+         wheel     b3WheelJointDef::Initialize(A, B, anchor, axis, axis_b) (wheel_joint.cpp:22-31)
        otherwise the local_*, reference_rotation, length and offsets are used as given. A mouse
        joint's body-B anchor is always the target in body B's frame at creation
        (mouse_joint.cpp:22-31). */
@@ -192,6 +195,13 @@ typedef struct bx_joint_desc {
     float target[3];               /* mouse: the world target */
     float linear_offset[3];        /* motor: position of B in A's frame */
     float angular_offset[4];       /* motor: orientation of B in A's frame (x y z w) */
+    /* cone, prismatic and wheel joints (cone_joint.h, prismatic_joint.h, wheel_joint.h) */
+    float cone_angle;              /* cone: the cone's full angle (enable_limit: enableConeLimit) */
+    float local_axis_a[3];         /* prismatic, wheel: the axis in A's frame */
+    float local_axis_b[3];         /* wheel: the axis in B's frame */
+    float lower_translation;       /* prismatic limit (enable_limit); max_force is maxMotorForce */
+    float upper_translation;
+    float axis_b[3];               /* wheel Initialize: the world axis on B (axis: on A) */
 } bx_joint_desc;
 
 typedef struct bx_world_desc {

@@ -308,6 +308,60 @@ static bool AddOtherJoint(RefWorld* w, const bx_joint_desc& jd) {
         j = w->world.CreateJoint(def);
         break;
     }
+    case BX_JOINT_CONE: {
+        b3ConeJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, V3(jd.axis), anchor, scalar(jd.cone_angle));
+        else {
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localRotationA = Q4(jd.local_rotation_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.localRotationB = Q4(jd.local_rotation_b);
+            def.referenceRotation = Q4(jd.reference_rotation);
+            def.coneAngle = scalar(jd.cone_angle);
+        }
+        def.enableConeLimit = jd.enable_limit != 0;
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_PRISMATIC: {
+        b3PrismaticJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor, V3(jd.axis));
+        else {
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.localAxisA = V3(jd.local_axis_a);
+            def.referenceRotation = Q4(jd.reference_rotation);
+        }
+        def.enableLimit = jd.enable_limit != 0;
+        def.lowerTranslation = scalar(jd.lower_translation);
+        def.upperTranslation = scalar(jd.upper_translation);
+        def.enableMotor = jd.enable_motor != 0;
+        def.maxMotorForce = scalar(jd.max_force);
+        def.motorSpeed = scalar(jd.motor_speed);
+        def.correctionFactor = scalar(jd.correction_factor);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
+    case BX_JOINT_WHEEL: {
+        b3WheelJointDef def;
+        if (jd.use_initialize) def.Initialize(a, b, anchor, V3(jd.axis), V3(jd.axis_b));
+        else {
+            def.localAnchorA = V3(jd.local_anchor_a);
+            def.localAnchorB = V3(jd.local_anchor_b);
+            def.localAxisA = V3(jd.local_axis_a);
+            def.localAxisB = V3(jd.local_axis_b);
+        }
+        def.enableMotor = jd.enable_motor != 0;
+        def.maxMotorTorque = scalar(jd.max_motor_torque);
+        def.motorSpeed = scalar(jd.motor_speed);
+        def.frequencyHz = scalar(jd.frequency_hz);
+        def.dampingRatio = scalar(jd.damping_ratio);
+        common(def);
+        j = w->world.CreateJoint(def);
+        break;
+    }
     default: return false;
     }
     w->joints.push_back(j);

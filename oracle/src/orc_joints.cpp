@@ -7,6 +7,8 @@
 //   joints/revolute_joint.cpp:369-531   SolvePositionConstraints
 //   joints/joint_solver.cpp:31-64       the joint solver loops (island order)
 //   joint_manager.cpp:30-72             joint edges prepended to both bodies' joint lists
+//   joints/cone_joint.cpp, prismatic_joint.cpp, wheel_joint.cpp: Initialize, constructors and the
+//   solver functions of the cone, prismatic and wheel joints (the axis joints below)
 #include <cmath>
 #include "orc_world.h"
 #include "../../include/bxstep.h"
@@ -74,11 +76,47 @@ void CreateJoint(World& w, const bx_joint_desc& d) {
     j.angularOffset = Q4(d.angular_offset);
     j.angularImpulse3.SetZero();
     j.impulse1 = 0;
+    j.linearImpulse2.SetZero();
+    j.springImpulse = 0;
+    for (Vec3* v : {&j.localXAxisA, &j.localYAxisA, &j.localZAxisA, &j.localXAxisB, &j.motorAxis, &j.a1, &j.a2,
+                    &j.perp1, &j.perp2, &j.s1, &j.s2, &j.s3, &j.s4, &j.axisB, &j.u})
+        v->SetZero();
     const Body& A0 = w.bodies[j.bodyA];
     const Body& B0 = w.bodies[j.bodyB];
-    if (d.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:22-31: the target is B's anchor at creation
+    if (d.type == BX_JOINT_PRISMATIC || d.type == BX_JOINT_WHEEL) {
+        j.referenceRotation = Q4(d.reference_rotation);
+        j.localRotationA = j.localRotationB = Quat(Vec3(0, 0, 0), scalar(1));
+        Vec3 axisA = V3(d.local_axis_a), axisB = V3(d.local_axis_b);
+        if (d.use_initialize) {  // prismatic_joint.cpp:23-35, wheel_joint.cpp:22-31
+            Vec3 anchor = V3(d.anchor);
+            j.localAnchorA = MulT(A0.xf, anchor);
+            j.localAnchorB = MulT(B0.xf, anchor);
+            axisA = MulC(A0.xf.rotation, V3(d.axis));
+            if (d.type == BX_JOINT_PRISMATIC) j.referenceRotation = Conjugate(A0.sweep.orientation) * B0.sweep.orientation;
+            else axisB = MulC(B0.xf.rotation, V3(d.axis_b));
+        } else {
+            j.localAnchorA = V3(d.local_anchor_a);
+            j.localAnchorB = V3(d.local_anchor_b);
+        }
+        j.localXAxisA = axisA;  // the constructors (prismatic_joint.cpp:37-61, wheel_joint.cpp:33-66)
+        j.localXAxisA.Normalize();
+        ComputeBasis(j.localXAxisA, j.localYAxisA, j.localZAxisA);
+        if (d.type == BX_JOINT_WHEEL) {
+            j.localXAxisB = axisB;
+            j.localXAxisB.Normalize();
+            Vec3 wA = Mul(A0.xf.rotation, j.localXAxisA);
+            Vec3 wB = Mul(B0.xf.rotation, j.localXAxisB);
+            Vec3 u = Cross(wA, wB);
+            scalar x = Dot(wA, wB);
+            scalar y = Length(u);
+            j.referenceCosine = x;
+            j.referenceAngle = scalar(::atan2(double(y), double(x)));  // the reference calls ::atan2(double, double)
+        }
+        j.lowerTranslation = d.lower_translation;
+        j.upperTranslation = d.upper_translation;
+    } else if (d.type == BX_JOINT_MOUSE) {  // mouse_joint.cpp:22-31: the target is B's anchor at creation
         j.localAnchorB = MulT(B0.xf, j.target);
-    } else if (d.type != BX_JOINT_REVOLUTE) {
+    } else if (d.type != BX_JOINT_REVOLUTE && d.type != BX_JOINT_CONE) {
         if (d.use_initialize) {
             const Vec3 anchor = V3(d.anchor);
             if (d.type == BX_JOINT_MOTOR) {  // motor_joint.cpp:23-31
@@ -98,10 +136,15 @@ void CreateJoint(World& w, const bx_joint_desc& d) {
         }
         j.localRotationA = Q4(d.local_rotation_a);
         j.localRotationB = Q4(d.local_rotation_b);
-    } else if (d.use_initialize) {  // revolute_joint.cpp:22-48
+    } else if (d.use_initialize) {  // revolute_joint.cpp:22-48; cone_joint.cpp:35-55 builds the frame about x
         Mat33 rotation;
-        rotation.z = V3(d.axis);
-        ComputeBasis(rotation.z, rotation.x, rotation.y);
+        if (d.type == BX_JOINT_CONE) {
+            rotation.x = V3(d.axis);
+            ComputeBasis(rotation.x, rotation.y, rotation.z);
+        } else {
+            rotation.z = V3(d.axis);
+            ComputeBasis(rotation.z, rotation.x, rotation.y);
+        }
         Quat q = Mat33Quat(rotation);
         const Body& A = w.bodies[j.bodyA];
         const Body& B = w.bodies[j.bodyB];
@@ -118,6 +161,7 @@ void CreateJoint(World& w, const bx_joint_desc& d) {
         j.localRotationB = Q4(d.local_rotation_b);
         j.referenceRotation = Q4(d.reference_rotation);
     }
+    j.coneAngle = d.cone_angle;
     j.lowerAngle = d.lower_angle;
     j.upperAngle = d.upper_angle;
     j.enableLimit = d.enable_limit != 0;
@@ -773,20 +817,480 @@ static bool SolveOtherPosition(Joint& j, Position* pos) {
     return solved;
 }
 
+
+// ------------------------------------------------------------------ cone, prismatic and wheel joints
+// k = [k11 k12; k12 k22] of the two rows perpendicular to the axis, inverted (prismatic_joint.cpp:113-136,
+// wheel_joint.cpp:103-126, and again in both position solvers).
+static Mat22 PerpMass(scalar mA, scalar mB, const Mat33& iA, const Mat33& iB, const Vec3& s1, const Vec3& s2,
+                      const Vec3& s3, const Vec3& s4) {
+    scalar k11 = mA + mB + Dot(s1, iA * s1) + Dot(s2, iB * s2);
+    scalar k12 = Dot(s1, iA * s3) + Dot(s2, iB * s4);
+    scalar k22 = mA + mB + Dot(s3, iA * s3) + Dot(s4, iB * s4);
+    Mat22 K;
+    K.x.x = k11;
+    K.x.y = k12;
+    K.y.x = k12;
+    K.y.y = k22;
+    return Inverse(K);
+}
+
+static void InitAxis(World& w, const TimeStep& step, Joint& j, Position* pos) {
+    const Body& A = w.bodies[j.bodyA];
+    const Body& B = w.bodies[j.bodyB];
+    j.indexA = A.islandIndex;
+    j.indexB = B.islandIndex;
+    j.mA = A.invMass;
+    j.mB = B.invMass;
+    j.localCenterA = A.sweep.localCenter;
+    j.localCenterB = B.sweep.localCenter;
+    j.localInvIA = A.invI;
+    j.localInvIB = B.invI;
+    j.iA = pos[j.indexA].I;
+    j.iB = pos[j.indexB].I;
+    Vec3 xA = pos[j.indexA].x, xB = pos[j.indexB].x;
+    Quat qA = pos[j.indexA].q, qB = pos[j.indexB].q;
+    scalar mA = j.mA, mB = j.mB;
+    Mat33 iA = j.iA, iB = j.iB;
+    if (j.type == BX_JOINT_CONE) {  // cone_joint.cpp:70-168
+        Quat fA = qA * j.localRotationA;
+        Quat fB = qB * j.localRotationB;
+        Mat33 RfA = fA.GetRotationMatrix();
+        Mat33 RfB = fB.GetRotationMatrix();
+        j.rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        j.rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        j.linearMass = PointMass(mA, mB, iA, iB, j.rA, j.rB);
+        if (j.enableLimit) {
+            Vec3 u1 = RfA.x;
+            Vec3 u2 = RfB.x;
+            j.motorAxis = Cross(u2, u1);
+            scalar mass = Dot((iA + iB) * j.motorAxis, j.motorAxis);
+            j.mass1 = mass > scalar(0) ? scalar(1) / mass : scalar(0);
+            scalar cosine = Dot(u2, u1);
+            scalar sine = Length(j.motorAxis);
+            scalar angle = scalar(::atan2(double(sine), double(cosine)));
+            if (scalar(0.5) * j.coneAngle < angle) {
+                if (j.limitState != e_atLowerLimit) {
+                    j.limitState = e_atLowerLimit;
+                    j.limitImpulse = 0;
+                }
+            } else {
+                j.limitState = e_inactiveLimit;
+                j.limitImpulse = 0;
+            }
+        } else {
+            j.limitState = e_inactiveLimit;
+        }
+        if (step.warmStarting) {
+            j.linearImpulse *= step.dtRatio;
+            j.limitImpulse *= step.dtRatio;
+        } else {
+            j.linearImpulse.SetZero();
+            j.limitImpulse = 0;
+        }
+        return;
+    }
+    Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+    Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+    Vec3 d = xB + rB - xA - rA;
+    if (j.type == BX_JOINT_PRISMATIC) {  // prismatic_joint.cpp:63-200
+        j.motorAxis = Mul(qA, j.localXAxisA);
+        j.a1 = Cross(d + rA, j.motorAxis);
+        j.a2 = Cross(rB, j.motorAxis);
+        j.motorMass = mA + mB + Dot(iA * j.a1, j.a1) + Dot(iB * j.a2, j.a2);
+        if (j.motorMass > scalar(0)) j.motorMass = scalar(1) / j.motorMass;
+    }
+    j.perp1 = Mul(qA, j.localYAxisA);
+    j.s1 = Cross(d + rA, j.perp1);
+    j.s2 = Cross(rB, j.perp1);
+    j.perp2 = Mul(qA, j.localZAxisA);
+    j.s3 = Cross(d + rA, j.perp2);
+    j.s4 = Cross(rB, j.perp2);
+    j.mass22 = PerpMass(mA, mB, iA, iB, j.s1, j.s2, j.s3, j.s4);
+    if (j.type == BX_JOINT_PRISMATIC) {
+        j.angularMass3 = Inverse(iA + iB);
+        Quat q1 = Conjugate(qA) * qB;
+        Quat q2 = j.referenceRotation;
+        if (Dot(q1, q2) < scalar(0)) q1 = -q1;
+        Quat q = scalar(2) * (q2 - q1) * Conjugate(q1);
+        Vec3 C = Mul(qA, -q.v);
+        j.bias3 = j.correctionFactor * step.inv_dt * C;
+        if (j.enableLimit) {
+            scalar translation = Dot(j.motorAxis, d);
+            if (Abs(j.upperTranslation - j.lowerTranslation) < scalar(2) * kLinearSlop) {
+                j.limitState = e_equalLimits;
+            } else if (translation <= j.lowerTranslation) {
+                if (j.limitState != e_atLowerLimit) {
+                    j.limitState = e_atLowerLimit;
+                    j.limitImpulse = 0;
+                }
+            } else if (translation >= j.upperTranslation) {
+                if (j.limitState != e_atUpperLimit) {
+                    j.limitState = e_atUpperLimit;
+                    j.limitImpulse = 0;
+                }
+            } else {
+                j.limitState = e_inactiveLimit;
+                j.limitImpulse = 0;
+            }
+        } else {
+            j.limitState = e_inactiveLimit;
+            j.limitImpulse = 0;
+        }
+        if (!j.enableMotor) j.motorImpulse = 0;
+        if (step.warmStarting) {
+            j.linearImpulse2 *= step.dtRatio;
+            j.angularImpulse3 *= step.dtRatio;
+            j.motorImpulse *= step.dtRatio;
+            j.limitImpulse *= step.dtRatio;
+        } else {
+            j.linearImpulse2.SetZero();
+            j.angularImpulse3.SetZero();
+            j.motorImpulse = 0;
+            j.limitImpulse = 0;
+        }
+        return;
+    }
+    // wheel_joint.cpp:62-190; the spring's axis and arms keep their values while the spring is off
+    j.mass1 = 0;
+    j.bias1 = 0;
+    j.gamma1 = 0;
+    if (j.frequencyHz > scalar(0)) {
+        j.motorAxis = Mul(qA, j.localXAxisA);
+        j.a1 = Cross(d + rA, j.motorAxis);
+        j.a2 = Cross(rB, j.motorAxis);
+        scalar invMass = mA + mB + Dot(iA * j.a1, j.a1) + Dot(iB * j.a2, j.a2);
+        if (invMass > scalar(0)) {
+            j.mass1 = scalar(1) / invMass;
+            scalar C = Dot(d, j.motorAxis);
+            scalar omega = scalar(2) * kPi * j.frequencyHz;
+            scalar damp = scalar(2) * j.mass1 * j.dampingRatio * omega;
+            scalar k = j.mass1 * omega * omega;
+            scalar h = step.dt;
+            j.gamma1 = h * (damp + h * k);
+            if (j.gamma1 > scalar(0)) j.gamma1 = scalar(1) / j.gamma1;
+            j.bias1 = C * h * k * j.gamma1;
+            j.mass1 = invMass + j.gamma1;
+            if (j.mass1 > scalar(0)) j.mass1 = scalar(1) / j.mass1;
+        }
+    } else {
+        j.springImpulse = 0;
+    }
+    if (j.enableMotor) {
+        j.axisB = Mul(qB, j.localXAxisB);
+        j.motorMass = Dot((iA + iB) * j.axisB, j.axisB);
+        if (j.motorMass > scalar(0)) j.motorMass = scalar(1) / j.motorMass;
+    } else {
+        j.motorImpulse = 0;
+    }
+    {
+        Vec3 axisA = Mul(qA, j.localXAxisA);
+        Vec3 axisB = Mul(qB, j.localXAxisB);
+        j.u = Cross(axisA, axisB);
+        j.angularMass1 = Dot((iA + iB) * j.u, j.u);
+        if (j.angularMass1 > scalar(0)) j.angularMass1 = scalar(1) / j.angularMass1;
+    }
+    if (step.warmStarting) {
+        j.linearImpulse2 *= step.dtRatio;
+        j.springImpulse *= step.dtRatio;
+        j.motorImpulse *= step.dtRatio;
+        j.impulse1 *= step.dtRatio;
+    } else {
+        j.linearImpulse2.SetZero();
+        j.springImpulse = 0;
+        j.motorImpulse = 0;
+        j.impulse1 = 0;
+    }
+}
+
+// vA -= mA P, wA -= iA LA, vB += mB P, wB += iB LB.
+static void ApplyRows(const Joint& j, const Vec3& P, const Vec3& LA, const Vec3& LB, Vec3& vA, Vec3& wA, Vec3& vB,
+                      Vec3& wB) {
+    vA -= j.mA * P;
+    wA -= j.iA * LA;
+    vB += j.mB * P;
+    wB += j.iB * LB;
+}
+
+static void WarmStartAxis(Joint& j, Velocity* vel) {
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w, vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    if (j.type == BX_JOINT_CONE) {  // cone_joint.cpp:170-196
+        ApplyPoint(j, j.linearImpulse, vA, wA, vB, wB);
+        if (j.enableLimit && j.limitState != e_inactiveLimit) {
+            Vec3 L = j.limitImpulse * j.motorAxis;
+            wA -= j.iA * L;
+            wB += j.iB * L;
+        }
+    } else if (j.type == BX_JOINT_PRISMATIC) {  // prismatic_joint.cpp:202-222
+        Vec3 P = j.linearImpulse2.x * j.perp1 + j.linearImpulse2.y * j.perp2 +
+                 (j.motorImpulse + j.limitImpulse) * j.motorAxis;
+        Vec3 LA = j.linearImpulse2.x * j.s1 + j.linearImpulse2.y * j.s3 + j.angularImpulse3 +
+                  (j.motorImpulse + j.limitImpulse) * j.a1;
+        Vec3 LB = j.linearImpulse2.x * j.s2 + j.linearImpulse2.y * j.s4 + j.angularImpulse3 +
+                  (j.motorImpulse + j.limitImpulse) * j.a2;
+        ApplyRows(j, P, LA, LB, vA, wA, vB, wB);
+    } else {  // wheel_joint.cpp:192-222
+        Vec3 P = j.linearImpulse2.x * j.perp1 + j.linearImpulse2.y * j.perp2 + j.springImpulse * j.motorAxis;
+        Vec3 LA = j.linearImpulse2.x * j.s1 + j.linearImpulse2.y * j.s3 + j.springImpulse * j.a1 +
+                  j.motorImpulse * j.axisB;
+        Vec3 LB = j.linearImpulse2.x * j.s2 + j.linearImpulse2.y * j.s4 + j.springImpulse * j.a2 +
+                  j.motorImpulse * j.axisB;
+        ApplyRows(j, P, LA, LB, vA, wA, vB, wB);
+        Vec3 L = j.impulse1 * j.u;
+        wA += j.iA * L;
+        wB -= j.iB * L;
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+static void SolvePerp(Joint& j, Vec3& vA, Vec3& wA, Vec3& vB, Vec3& wB) {
+    Vec2 Cdot;
+    Cdot.x = Dot(j.perp1, vB - vA) + Dot(j.s2, wB) - Dot(j.s1, wA);
+    Cdot.y = Dot(j.perp2, vB - vA) + Dot(j.s4, wB) - Dot(j.s3, wA);
+    Vec2 impulse = j.mass22 * (-Cdot);
+    j.linearImpulse2 += impulse;
+    Vec3 P = impulse.x * j.perp1 + impulse.y * j.perp2;
+    Vec3 LA = impulse.x * j.s1 + impulse.y * j.s3;
+    Vec3 LB = impulse.x * j.s2 + impulse.y * j.s4;
+    ApplyRows(j, P, LA, LB, vA, wA, vB, wB);
+}
+
+static void SolveAxisVelocity(const TimeStep& step, Joint& j, Velocity* vel) {
+    Vec3 vA = vel[j.indexA].v, wA = vel[j.indexA].w, vB = vel[j.indexB].v, wB = vel[j.indexB].w;
+    if (j.type == BX_JOINT_CONE) {  // cone_joint.cpp:198-240
+        Vec3 Cdot = vB + Cross(wB, j.rB) - vA - Cross(wA, j.rA);
+        Vec3 P = j.linearMass.Solve(-Cdot);
+        j.linearImpulse += P;
+        ApplyPoint(j, P, vA, wA, vB, wB);
+        if (j.enableLimit && j.limitState != e_inactiveLimit) {
+            scalar c = Dot(j.motorAxis, wB - wA);
+            scalar impulse = -j.mass1 * c;
+            scalar oldImpulse = j.limitImpulse;
+            j.limitImpulse = Max(j.limitImpulse + impulse, scalar(0));
+            impulse = j.limitImpulse - oldImpulse;
+            Vec3 L = impulse * j.motorAxis;
+            wA -= j.iA * L;
+            wB += j.iB * L;
+        }
+    } else if (j.type == BX_JOINT_PRISMATIC) {  // prismatic_joint.cpp:224-315
+        if (j.enableMotor && j.limitState != e_equalLimits) {
+            scalar Cdot = Dot(j.motorAxis, vB - vA) + Dot(j.a2, wB) - Dot(j.a1, wA);
+            scalar impulse = j.motorMass * (j.motorSpeed - Cdot);
+            scalar oldImpulse = j.motorImpulse;
+            scalar maxImpulse = step.dt * j.maxForce;
+            j.motorImpulse = Clamp(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            ApplyRows(j, impulse * j.motorAxis, impulse * j.a1, impulse * j.a2, vA, wA, vB, wB);
+        }
+        SolvePerp(j, vA, wA, vB, wB);
+        {
+            Vec3 Cdot = wB - wA;
+            Vec3 impulse = -j.angularMass3 * (Cdot + j.bias3);
+            j.angularImpulse3 += impulse;
+            wA -= j.iA * impulse;
+            wB += j.iB * impulse;
+        }
+        if (j.enableLimit && j.limitState != e_inactiveLimit) {
+            scalar Cdot = Dot(j.motorAxis, vB - vA) + Dot(j.a2, wB) - Dot(j.a1, wA);
+            scalar impulse = j.motorMass * (-Cdot);
+            scalar oldImpulse = j.limitImpulse;
+            if (j.limitState == e_atLowerLimit) j.limitImpulse = Max(j.limitImpulse + impulse, scalar(0));
+            else if (j.limitState == e_atUpperLimit) j.limitImpulse = Min(j.limitImpulse + impulse, scalar(0));
+            impulse = j.limitImpulse - oldImpulse;
+            ApplyRows(j, impulse * j.motorAxis, impulse * j.a1, impulse * j.a2, vA, wA, vB, wB);
+        }
+    } else {  // wheel_joint.cpp:224-299
+        {
+            scalar Cdot = Dot(j.motorAxis, vB - vA) + Dot(j.a2, wB) - Dot(j.a1, wA);
+            scalar impulse = -j.mass1 * (Cdot + j.bias1 + j.gamma1 * j.springImpulse);
+            j.springImpulse += impulse;
+            ApplyRows(j, impulse * j.motorAxis, impulse * j.a1, impulse * j.a2, vA, wA, vB, wB);
+        }
+        if (j.enableMotor) {
+            scalar dw = Dot(wB - wA, j.axisB);
+            scalar Cdot = dw - j.motorSpeed;
+            scalar impulse = -j.motorMass * Cdot;
+            scalar oldImpulse = j.motorImpulse;
+            scalar maxImpulse = step.dt * j.maxMotorTorque;
+            j.motorImpulse = Clamp(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            Vec3 P = j.axisB * impulse;
+            wA -= j.iA * P;
+            wB += j.iB * P;
+        }
+        SolvePerp(j, vA, wA, vB, wB);
+        {
+            scalar Cdot = Dot(wA - wB, j.u);
+            scalar impulse = -j.angularMass1 * Cdot;
+            j.impulse1 += impulse;
+            Vec3 L = impulse * j.u;
+            wA += j.iA * L;
+            wB -= j.iB * L;
+        }
+    }
+    vel[j.indexA].v = vA;
+    vel[j.indexA].w = wA;
+    vel[j.indexB].v = vB;
+    vel[j.indexB].w = wB;
+}
+
+// Moves both bodies by P (linear) and LA / LB (angular), renormalizing and rotating the inverse inertia.
+static void Move(const Joint& j, const Vec3& P, const Vec3& LA, const Vec3& LB, Vec3& xA, Quat& qA, Mat33& iA,
+                 Vec3& xB, Quat& qB, Mat33& iB) {
+    xA -= j.mA * P;
+    qA -= Derivative(qA, iA * LA);
+    qA.Normalize();
+    iA = RotateToFrame(j.localInvIA, qA);
+    xB += j.mB * P;
+    qB += Derivative(qB, iB * LB);
+    qB.Normalize();
+    iB = RotateToFrame(j.localInvIB, qB);
+}
+
+static bool SolveAxisPosition(Joint& j, Position* pos) {
+    Vec3 xA = pos[j.indexA].x, xB = pos[j.indexB].x;
+    Quat qA = pos[j.indexA].q, qB = pos[j.indexB].q;
+    Mat33 iA = pos[j.indexA].I, iB = pos[j.indexB].I;
+    scalar mA = j.mA, mB = j.mB;
+    bool solved;
+    if (j.type == BX_JOINT_CONE) {  // cone_joint.cpp:242-322
+        scalar limitError = 0;
+        if (j.enableLimit) {
+            Quat fA = qA * j.localRotationA;
+            Quat fB = qB * j.localRotationB;
+            Mat33 RfA = fA.GetRotationMatrix();
+            Mat33 RfB = fB.GetRotationMatrix();
+            Vec3 u1 = RfA.x;
+            Vec3 u2 = RfB.x;
+            Vec3 coneAxis = Cross(u2, u1);
+            scalar mass = Dot((iA + iB) * coneAxis, coneAxis);
+            scalar coneMass = mass > scalar(0) ? scalar(1) / mass : scalar(0);
+            scalar cosine = Dot(u2, u1);
+            scalar sine = Length(coneAxis);
+            scalar angle = scalar(::atan2(double(sine), double(cosine)));
+            if (scalar(0.5) * j.coneAngle < angle) {
+                scalar C = scalar(0.5) * j.coneAngle - angle;
+                limitError = -C;
+                C = Clamp(C + kAngularSlop, -kMaxAngularCorrection, scalar(0));
+                scalar coneImpulse = -C * coneMass;
+                Vec3 P = coneImpulse * coneAxis;
+                qA -= Derivative(qA, iA * P);
+                qA.Normalize();
+                iA = RotateToFrame(j.localInvIA, qA);
+                qB += Derivative(qB, iB * P);
+                qB.Normalize();
+                iB = RotateToFrame(j.localInvIB, qB);
+            }
+        }
+        Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Vec3 C = xB + rB - xA - rA;
+        scalar linearError = Length(C);
+        Mat33 mass = PointMass(mA, mB, iA, iB, rA, rB);
+        Vec3 P = mass.Solve(-C);
+        Move(j, P, Cross(rA, P), Cross(rB, P), xA, qA, iA, xB, qB, iB);
+        solved = linearError <= kLinearSlop && limitError <= kAngularSlop;
+    } else {  // prismatic_joint.cpp:317-441, wheel_joint.cpp:301-423
+        Vec3 rA = Mul(qA, j.localAnchorA - j.localCenterA);
+        Vec3 rB = Mul(qB, j.localAnchorB - j.localCenterB);
+        Vec3 d = xB + rB - xA - rA;
+        Vec3 axis = Mul(qA, j.localXAxisA);
+        Vec3 a1 = Cross(d + rA, axis);
+        Vec3 a2 = Cross(rB, axis);
+        Vec3 perp1 = Mul(qA, j.localYAxisA);
+        Vec3 s1 = Cross(d + rA, perp1);
+        Vec3 s2 = Cross(rB, perp1);
+        Vec3 perp2 = Mul(qA, j.localZAxisA);
+        Vec3 s3 = Cross(d + rA, perp2);
+        Vec3 s4 = Cross(rB, perp2);
+        Vec3 axisB = Mul(qB, j.localXAxisB);
+        scalar linearError = 0;
+        {
+            Vec2 C;
+            C.x = Dot(perp1, d);
+            C.y = Dot(perp2, d);
+            linearError += Abs(C.x) + Abs(C.y);
+            Mat22 linearMass = PerpMass(mA, mB, iA, iB, s1, s2, s3, s4);
+            Vec2 impulse = linearMass * (-C);
+            Vec3 P = impulse.x * perp1 + impulse.y * perp2;
+            Vec3 LA = impulse.x * s1 + impulse.y * s3;
+            Vec3 LB = impulse.x * s2 + impulse.y * s4;
+            Move(j, P, LA, LB, xA, qA, iA, xB, qB, iB);
+        }
+        if (j.type == BX_JOINT_PRISMATIC) {
+            if (j.enableLimit) {
+                scalar C = 0;
+                scalar translation = Dot(axis, d);
+                if (Abs(j.upperTranslation - j.lowerTranslation) < scalar(2) * kLinearSlop) {
+                    C = Clamp(translation, -kMaxLinearCorrection, kMaxLinearCorrection);
+                    linearError = Max(linearError, Abs(translation));
+                } else if (translation <= j.lowerTranslation) {
+                    C = Clamp(translation - j.lowerTranslation + kLinearSlop, -kMaxLinearCorrection, scalar(0));
+                    linearError = Max(linearError, j.lowerTranslation - translation);
+                } else if (translation >= j.upperTranslation) {
+                    C = Clamp(translation - j.upperTranslation - kLinearSlop, scalar(0), kMaxLinearCorrection);
+                    linearError = Max(linearError, translation - j.upperTranslation);
+                }
+                scalar limitMass = mA + mB + Dot(iA * a1, a1) + Dot(iB * a2, a2);
+                limitMass = limitMass > scalar(0) ? scalar(1) / limitMass : scalar(0);
+                scalar impulse = limitMass * -C;
+                Move(j, impulse * axis, impulse * a1, impulse * a2, xA, qA, iA, xB, qB, iB);
+            }
+            solved = linearError <= kLinearSlop;
+        } else {
+            scalar angularError = 0;
+            Vec3 u = Cross(axis, axisB);
+            scalar cosine = Dot(axis, axisB);
+            scalar C = cosine - j.referenceCosine;
+            scalar sine = Length(u);
+            scalar angle = scalar(::atan2(double(sine), double(cosine)));
+            scalar angleC = angle - j.referenceAngle;
+            angularError += Abs(angleC);
+            scalar mass = Dot((iA + iB) * u, u);
+            if (mass > scalar(0)) mass = scalar(1) / mass;
+            scalar impulse = -mass * C;
+            Vec3 P = impulse * u;
+            qA += Derivative(qA, iA * P);
+            qA.Normalize();
+            iA = RotateToFrame(j.localInvIA, qA);
+            qB -= Derivative(qB, iB * P);
+            qB.Normalize();
+            iB = RotateToFrame(j.localInvIB, qB);
+            solved = linearError <= kLinearSlop && angularError <= kAngularSlop;
+        }
+    }
+    pos[j.indexA].x = xA;
+    pos[j.indexA].q = qA;
+    pos[j.indexB].x = xB;
+    pos[j.indexB].q = qB;
+    pos[j.indexA].I = iA;
+    pos[j.indexB].I = iB;
+    return solved;
+}
+
+static bool IsAxis(u32 type) { return type == BX_JOINT_CONE || type == BX_JOINT_PRISMATIC || type == BX_JOINT_WHEEL; }
+
 static void InitJoint(World& w, const TimeStep& step, Joint& j, Position* pos) {
     if (j.type == BX_JOINT_REVOLUTE) InitRevolute(w, step, j, pos);
+    else if (IsAxis(j.type)) InitAxis(w, step, j, pos);
     else InitOther(w, step, j, pos);
 }
 static void WarmStartJoint(Joint& j, Velocity* vel) {
     if (j.type == BX_JOINT_REVOLUTE) WarmStartRevolute(j, vel);
+    else if (IsAxis(j.type)) WarmStartAxis(j, vel);
     else WarmStartOther(j, vel);
 }
 static void SolveJointVelocity(const TimeStep& step, Joint& j, Velocity* vel) {
     if (j.type == BX_JOINT_REVOLUTE) SolveRevoluteVelocity(step, j, vel);
+    else if (IsAxis(j.type)) SolveAxisVelocity(step, j, vel);
     else SolveOtherVelocity(step, j, vel);
 }
 static bool SolveJointPosition(Joint& j, Position* pos) {
-    return j.type == BX_JOINT_REVOLUTE ? SolveRevolutePosition(j, pos) : SolveOtherPosition(j, pos);
+    if (j.type == BX_JOINT_REVOLUTE) return SolveRevolutePosition(j, pos);
+    if (IsAxis(j.type)) return SolveAxisPosition(j, pos);
+    return SolveOtherPosition(j, pos);
 }
 
 // b3JointSolver (joint_solver.cpp:31-64): the island's joints in island order.

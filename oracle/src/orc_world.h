@@ -110,6 +110,16 @@ struct Joint {
     Mat33 gamma3, angularMass3;
     Vec3 bias3, n, linearError, angularVelocity;
     scalar mass1 = 0, gamma1 = 0, bias1 = 0;
+    // cone, prismatic and wheel joints (cone_joint.h, prismatic_joint.h, wheel_joint.h): motorAxis
+    // holds the cone axis, the prismatic axis or the wheel's axis on A; mass1 the cone or spring
+    // mass; limitImpulse/limitState the cone and prismatic limits; impulse1 the wheel's angular
+    // impulse; angularImpulse3, angularMass3 and bias3 the prismatic angular row
+    Vec3 localXAxisA, localYAxisA, localZAxisA, localXAxisB;
+    scalar coneAngle = 0, lowerTranslation = 0, upperTranslation = 0, referenceCosine = 0, referenceAngle = 0;
+    Vec2 linearImpulse2;
+    scalar springImpulse = 0, angularMass1 = 0;
+    Vec3 perp1, perp2, s1, s2, s3, s4, axisB, u;
+    Mat22 mass22;
 };
 
 struct Position { Vec3 x; Quat q; Mat33 I; };

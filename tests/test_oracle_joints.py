@@ -1,9 +1,13 @@
-"""The oracle's sphere, weld, spring, mouse, friction and motor joints (joints/*_joint.cpp) against the
+"""The oracle's sphere, weld, spring, mouse, friction, motor, cone, prismatic and wheel joints
+(joints/*_joint.cpp) against the
 reference build (oracle/_ref), bit for bit on every step of a world holding each of them (rigid and soft
 welds and springs, frames from Initialize and given), with sleeping off and on; and what each joint
 holds in the oracle's settled world: the sphere joint's anchors meet, the rigid weld keeps its
-reference rotation, the rigid spring keeps its length, and the mouse joint's anchor sits on its target
-up to the soft constraint's sag."""
+reference rotation, the rigid spring keeps its length, the mouse joint's anchor sits on its target
+up to the soft constraint's sag, the cone holds its body inside the cone, the prismatic joint moves
+its body along the axis inside the limits, and the wheel stays on the chassis's suspension axis.
+The cone, prismatic and wheel joints are in the same world and compared with the reference build
+the same way."""
 import numpy as np
 import pytest
 
@@ -65,3 +69,28 @@ def test_joints_hold_what_they_constrain():
     anchor0 = target - scene.worlds[0].bodies[11]["position"]
     omega = 2 * np.pi * float(j[9]["frequency_hz"])
     assert np.linalg.norm(_world_point(b[11], anchor0) - target) < 4 * 9.8 / omega ** 2
+    # cone 11 (ground and body 13, from Initialize about -y, cone 0.6): the anchors meet, and body 13's
+    # frame axis stays within half the cone angle of the ground's, up to the angular slop and the
+    # solver's residual
+    w0 = scene.worlds[0].bodies
+    anchor = np.asarray(j[11]["anchor"], np.float64)
+    assert np.linalg.norm(_world_point(b[13], anchor - w0[13]["position"]) - anchor) < 0.01
+    axis = np.asarray(j[11]["axis"], np.float64)
+    cosang = np.clip(_rot(q(13), axis) @ axis, -1.0, 1.0)
+    assert np.arccos(cosang) <= 0.5 * float(j[11]["cone_angle"]) + 0.05
+    # prismatic 12 (ground and body 14 along (1, 0, 0.3), limits [-1, 1.5]): the anchor moves along
+    # the axis only, inside the limits
+    anchor = np.asarray(j[12]["anchor"], np.float64)
+    axis = np.asarray(j[12]["axis"], np.float64)
+    axis /= np.linalg.norm(axis)
+    d = _world_point(b[14], anchor - w0[14]["position"]) - anchor
+    t = d @ axis
+    assert -1.0 - 0.02 <= t <= 1.5 + 0.02
+    assert np.linalg.norm(d - t * axis) < 0.05
+    # wheel 13 (chassis 15, wheel 16): the wheel's anchor stays on the chassis's suspension axis
+    anchor = np.asarray(j[13]["anchor"], np.float64)
+    pa = _world_point(b[15], anchor - w0[15]["position"])
+    pb = _world_point(b[16], anchor - w0[16]["position"])
+    up = _rot(q(15), np.asarray(j[13]["axis"], np.float64))
+    d = pb - pa
+    assert np.linalg.norm(d - (d @ up) * up) < 0.02
