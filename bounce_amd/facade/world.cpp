@@ -808,8 +808,60 @@ b3Joint* b3World::CreateJoint(const b3JointDef& def) {
             d.correction_factor = r.correctionFactor;
             break;
         }
+        case e_coneJoint: {
+            const b3ConeJointDef& r = static_cast<const b3ConeJointDef&>(def);
+            d.type = BX_JOINT_CONE;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.axis, r.m_axis);
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy4(d.local_rotation_a, r.localRotationA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            Copy4(d.local_rotation_b, r.localRotationB);
+            Copy4(d.reference_rotation, r.referenceRotation);
+            d.enable_limit = r.enableConeLimit ? 1u : 0u;
+            d.cone_angle = r.coneAngle;
+            break;
+        }
+        case e_prismaticJoint: {
+            const b3PrismaticJointDef& r = static_cast<const b3PrismaticJointDef&>(def);
+            d.type = BX_JOINT_PRISMATIC;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.axis, r.m_axis);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            Copy3(d.local_axis_a, r.localAxisA);
+            Copy4(d.reference_rotation, r.referenceRotation);
+            d.enable_limit = r.enableLimit ? 1u : 0u;
+            d.lower_translation = r.lowerTranslation;
+            d.upper_translation = r.upperTranslation;
+            d.enable_motor = r.enableMotor ? 1u : 0u;
+            d.max_force = r.maxMotorForce;
+            d.motor_speed = r.motorSpeed;
+            d.correction_factor = r.correctionFactor;
+            break;
+        }
+        case e_wheelJoint: {
+            const b3WheelJointDef& r = static_cast<const b3WheelJointDef&>(def);
+            d.type = BX_JOINT_WHEEL;
+            d.use_initialize = r.m_initialize ? 1u : 0u;
+            Copy3(d.anchor, r.m_anchor);
+            Copy3(d.axis, r.m_axisA);
+            Copy3(d.axis_b, r.m_axisB);
+            Copy3(d.local_anchor_a, r.localAnchorA);
+            Copy3(d.local_anchor_b, r.localAnchorB);
+            Copy3(d.local_axis_a, r.localAxisA);
+            Copy3(d.local_axis_b, r.localAxisB);
+            d.enable_motor = r.enableMotor ? 1u : 0u;
+            d.max_motor_torque = r.maxMotorTorque;
+            d.motor_speed = r.motorSpeed;
+            d.frequency_hz = r.frequencyHz;
+            d.damping_ratio = r.dampingRatio;
+            break;
+        }
         default:
-            return nullptr;  // cone, prismatic and wheel joints are not run on the device
+            return nullptr;
     }
     m_device->newJoints.push_back(d);
     QueueTopology(BX_TOPO_CREATE_JOINT, uint32(m_device->newJoints.size() - 1));

@@ -429,6 +429,114 @@ struct b3MotorJointDef : public b3JointDef {
     bool m_initialize;
 };
 
+struct b3ConeJointDef : public b3JointDef {
+    b3ConeJointDef() : m_initialize(false) {
+        type = e_coneJoint;
+        localAnchorA.SetZero();
+        localRotationA.SetIdentity();
+        localAnchorB.SetZero();
+        localRotationB.SetIdentity();
+        referenceRotation.SetIdentity();
+        enableConeLimit = false;
+        coneAngle = scalar(0);
+        m_axis.SetZero();
+        m_anchor.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& axis, const b3Vec3& anchor, scalar angle) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_axis = axis;
+        m_anchor = anchor;
+        coneAngle = angle;
+    }
+    b3Vec3 localAnchorA;
+    b3Quat localRotationA;
+    b3Vec3 localAnchorB;
+    b3Quat localRotationB;
+    b3Quat referenceRotation;
+    bool enableConeLimit;
+    scalar coneAngle;
+    bool m_initialize;
+    b3Vec3 m_axis, m_anchor;
+};
+
+struct b3PrismaticJointDef : public b3JointDef {
+    b3PrismaticJointDef() : m_initialize(false) {
+        type = e_prismaticJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        localAxisA.Set(scalar(1), scalar(0), scalar(0));
+        referenceRotation.SetIdentity();
+        enableLimit = false;
+        lowerTranslation = scalar(0);
+        upperTranslation = scalar(0);
+        enableMotor = false;
+        maxMotorForce = scalar(0);
+        motorSpeed = scalar(0);
+        correctionFactor = scalar(0.2);
+        m_anchor.SetZero();
+        m_axis.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchor, const b3Vec3& axis) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchor = anchor;
+        m_axis = axis;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    b3Vec3 localAxisA;
+    b3Quat referenceRotation;
+    bool enableLimit;
+    scalar lowerTranslation;
+    scalar upperTranslation;
+    bool enableMotor;
+    scalar maxMotorForce;
+    scalar motorSpeed;
+    scalar correctionFactor;
+    bool m_initialize;
+    b3Vec3 m_anchor, m_axis;
+};
+
+struct b3WheelJointDef : public b3JointDef {
+    b3WheelJointDef() : m_initialize(false) {
+        type = e_wheelJoint;
+        localAnchorA.SetZero();
+        localAnchorB.SetZero();
+        localAxisA.Set(scalar(0), scalar(1), scalar(0));
+        localAxisB.Set(scalar(1), scalar(0), scalar(0));
+        enableMotor = false;
+        maxMotorTorque = scalar(0);
+        motorSpeed = scalar(0);
+        frequencyHz = scalar(4);
+        dampingRatio = scalar(0.7);
+        m_anchor.SetZero();
+        m_axisA.SetZero();
+        m_axisB.SetZero();
+    }
+    void Initialize(b3Body* bA, b3Body* bB, const b3Vec3& anchor, const b3Vec3& axisA, const b3Vec3& axisB) {
+        bodyA = bA;
+        bodyB = bB;
+        m_initialize = true;
+        m_anchor = anchor;
+        m_axisA = axisA;
+        m_axisB = axisB;
+    }
+    b3Vec3 localAnchorA;
+    b3Vec3 localAnchorB;
+    b3Vec3 localAxisA;
+    b3Vec3 localAxisB;
+    bool enableMotor;
+    scalar maxMotorTorque;
+    scalar motorSpeed;
+    scalar frequencyHz;
+    scalar dampingRatio;
+    bool m_initialize;
+    b3Vec3 m_anchor, m_axisA, m_axisB;
+};
+
 class b3Joint {
 public:
     b3JointType GetType() const { return m_def.type; }
@@ -459,6 +567,9 @@ typedef b3Joint b3SpringJoint;
 typedef b3Joint b3MouseJoint;
 typedef b3Joint b3FrictionJoint;
 typedef b3Joint b3MotorJoint;
+typedef b3Joint b3ConeJoint;
+typedef b3Joint b3PrismaticJoint;
+typedef b3Joint b3WheelJoint;
 
 // A touching or merely overlapping pair of fixtures (contact.h); the world's list holds the pairs the
 // last step (or topology change) left, newest first. Valid until the next step or world change.

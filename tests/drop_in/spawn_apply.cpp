@@ -189,7 +189,7 @@ int main() {
             if (stack[4]) stack[4]->SetAwake(false);
             sphere->SetSleepingAllowed(false);
             // a mouse joint lifts the capsule; a soft spring ties the spawned box to the compound;
-            // a weld joins the two lowest boxes
+            // a weld joins the two lowest boxes;
             b3MouseJointDef md;
             md.bodyA = ground;
             md.bodyB = capsule;
@@ -206,6 +206,13 @@ int main() {
             b3WeldJointDef wd;
             wd.Initialize(stack[0], stack[1], scalar(0.5) * (stack[0]->GetPosition() + stack[1]->GetPosition()));
             world.CreateJoint(wd);
+            // and a prismatic rail with limits keeps the pushed sphere on a line
+            b3PrismaticJointDef pd;
+            pd.Initialize(ground, sphere, sphere->GetPosition(), b3Vec3(1.0f, 0.0f, 0.0f));
+            pd.enableLimit = true;
+            pd.lowerTranslation = -1.0f;
+            pd.upperTranslation = 2.0f;
+            world.CreateJoint(pd);
         }
         if (step == 160) world.SetSleeping(false);
         if (step == 170) {

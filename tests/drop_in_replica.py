@@ -252,6 +252,9 @@ def spawn_apply_lines(kind="oracle"):
                            anchor_b=R.state(9)["world_center"], frequency_hz=2.0, damping_ratio=0.3)
             mid = F(0.5) * (np.asarray(R.state(1)["position"], dtype=F) + np.asarray(R.state(2)["position"], dtype=F))
             R.create_joint(1, 2, T.BX_JOINT_WELD, use_initialize=1, anchor=mid, correction_factor=0.2)
+            R.create_joint(0, 6, T.BX_JOINT_PRISMATIC, use_initialize=1, anchor=R.state(6)["position"],
+                           axis=(1.0, 0.0, 0.0), enable_limit=1, lower_translation=-1.0, upper_translation=2.0,
+                           correction_factor=0.2)
         if step == 160:
             R.edit(None, T.BX_EDIT_WORLD_SLEEPING, flag=False)
         if step == 170:
