@@ -200,3 +200,24 @@ def test_batched_mixed_worlds_match_single(gpu_ctx):
         diffs = compare_worlds(DeviceWorld(batch, w), r)
         assert not diffs, f"world {w}: {diffs}"
         assert batch.counts(w).status == 0
+
+
+def test_thousand_dropped_boxes_full_600_steps(gpu_ctx):
+    """The reference's single-world case at its full size and length: 1,000 unit boxes dropped from a
+    20 x 50 x 20 m volume onto a 100 x 1 x 100 m ground, 600 steps at 60 Hz with 8/2 iterations; the
+    device equals the oracle every tenth step and at the end (bodies, contacts with manifolds, the pair
+    list, the island solve order)."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    scene = scenes.drop_boxes(1000)
+    print("[parity] 1000 dropped boxes: 600 steps", flush=True)
+    batch = device.Batch(gpu_ctx, scene)
+    ref = CpuWorld("oracle", scene)
+    dw = DeviceWorld(batch, 0)
+    for s in range(600):
+        batch.step()
+        ref.step()
+        if s % 10 == 9:
+            diffs = compare_worlds(dw, ref)
+            assert not diffs, f"step {s}: {diffs} (device status {batch.counts(0).status:#x})"
+    assert batch.counts(0).status == 0
