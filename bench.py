@@ -81,9 +81,10 @@ def parse():
     p.add_argument("--parity-worlds", type=int, default=8,
                    help="worlds of the benchmarked batch compared with the oracle after timing (0: none)")
     p.add_argument("--serve-worlds", type=int, default=512)
-    p.add_argument("--serve-rate", type=float, default=1000.0)
-    p.add_argument("--serve-seconds", type=float, default=0.0)
-    p.add_argument("--serve-budget-gb", type=float, default=1.0)
+    p.add_argument("--serve-rate", type=float, default=500.0)
+    p.add_argument("--serve-seconds", type=float, default=4.0,
+                   help="seconds of the serving leg after the batched timing (0: none)")
+    p.add_argument("--serve-budget-gb", type=float, default=16.0)
     p.add_argument("--serve-catalog", choices=("boxes", "mixed"), default="boxes")
     return p.parse_args()
 
