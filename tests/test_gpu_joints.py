@@ -81,3 +81,26 @@ def test_unsupported_joint_types_are_refused(gpu_ctx):
         scene.worlds[0].joints[0]["type"] = t
         with pytest.raises(device.BxError):
             device.Batch(gpu_ctx, scene, solver_mode=1)
+
+
+def test_joint_worlds_in_a_pool_equal_oracle(gpu_ctx):
+    """Worlds of every joint type admitted into a pool and stepped in changing mixes (one launch
+    sequence per tick over the listed slots) equal the oracle world stepped as often, at the end."""
+    from bounce_amd import device
+    from oracle.oracle import CpuWorld
+    scene = scenes.concat([scenes.joint_zoo(2, seed=31), scenes.joint_zoo(1, seed=32, sleeping=True)])
+    print("[joints] pool of joint worlds: 60 ticks", flush=True)
+    pool = device.Pool(gpu_ctx, scene, budget_bytes=64 << 20, max_worlds=3, solver_mode=5)
+    slots = [pool.admit(w) for w in scene.worlds]
+    steps = [0, 0, 0]
+    for t in range(60):
+        pick = [i for i in range(3) if (t + i) % 4 != 0]
+        pool.step_list([slots[i] for i in pick])
+        for i in pick:
+            steps[i] += 1
+    for i in range(3):
+        ref = CpuWorld("oracle", scene, i)
+        ref.step(n=steps[i])
+        d = compare_worlds(_W(pool, slots[i]), ref)
+        assert not d, f"world {i} after {steps[i]} steps: {d}"
+    pool.close()
