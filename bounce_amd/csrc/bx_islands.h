@@ -114,16 +114,28 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
         carry += total;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        for (u32 k = n; k-- > 0;) {
-            edges[cursor[cfa[k]]++] = k;
-            edges[cursor[cfb[k]]++] = k;
-        }
+    // every fixture's contacts, newest (highest order position) first: filled in any order, then
+    // each contact placed at its rank among its fixture's contacts in the pair search's copy
+    // (B.edgeSlots), which the island search reads back
+    for (u32 k = threadIdx.x; k < n; k += blockDim.x) {
+        edges[atomicAdd(&cursor[cfa[k]], 1u)] = k;
+        edges[atomicAdd(&cursor[cfb[k]], 1u)] = k;
     }
     __syncthreads();
-    // the pair search of this step reuses the adjacency
+    u32* slots = B.edgeSlots + 2 * wi.contactBase;
+    for (u32 k = threadIdx.x; k < n; k += blockDim.x) {
+        const u32 ends[2] = {cfa[k], cfb[k]};
+        for (u32 s = 0; s < 2; ++s) {
+            const u32 f = ends[s];
+            u32 rank = 0;
+            for (u32 e = off[f]; e < off[f + 1]; ++e) rank += edges[e] > k ? 1u : 0u;
+            slots[off[f] + rank] = k;
+        }
+    }
     for (u32 f = threadIdx.x; f <= nf; f += blockDim.x) B.edgeOffset[wi.edgeBase + f] = off[f];
-    for (u32 e = threadIdx.x; e < 2 * n; e += blockDim.x) B.edgeSlots[2 * wi.contactBase + e] = edges[e];
+    __syncthreads();
+    for (u32 e = threadIdx.x; e < 2 * n; e += blockDim.x) edges[e] = slots[e];
+    __syncthreads();
 
     // 3. depth-first search (one lane)
     if (threadIdx.x == 0) {
