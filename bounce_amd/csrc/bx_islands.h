@@ -183,11 +183,13 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
                         // one constraint entry per manifold (a mesh contact has up to three,
                         // solved one after the other as b3ContactSolver does; a touching convex
                         // contact has one)
+                        // (the order position k becomes the contact slot in the fix-up below, which
+                        // reads the order list in batches instead of once per edge here)
                         if (kMesh) {
                             for (u32 j = 0; j < (cf >> 2); ++j)
-                                icontacts[nc++] = make_uint4(order[k] | (j << 30), fbody[fa], fbody[fbb], 0u);
+                                icontacts[nc++] = make_uint4(k | (j << 30), fbody[fa], fbody[fbb], 0u);
                         } else {
-                            icontacts[nc++] = make_uint4(order[k], fbody[fa], fbody[fbb], 0u);
+                            icontacts[nc++] = make_uint4(k, fbody[fa], fbody[fbb], 0u);
                         }
                         cfl[k] = cf | 1u;
                         u32 other = (fa == f ? fbody[fbb] : fbody[fa]) & 0x7FFFFFFFu;
@@ -213,11 +215,30 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
             info.jointCount = nj - info.jointStart;
             // island-local indices of each contact's bodies (a static body keeps only its index in
             // the island being built, exactly what b3ContactSolver reads for this island)
-            for (u32 k = info.contactStart; k < nc; ++k) {
-                uint4 e = icontacts[k];
+            // (eight entries per batch: their global loads are issued together)
+            auto fixContact = [&](uint4& e, u32 slot) {
+                e.x = slot | (e.x & 0xC0000000u);
                 e.y = bidx[e.y & 0x7FFFFFFFu];
                 e.z = bidx[e.z & 0x7FFFFFFFu];
                 e.w = info.bodyStart;
+            };
+            u32 k = info.contactStart;
+            for (; k + 8 <= nc; k += 8) {
+                uint4 e[8];
+                u32 slot[8];
+#pragma unroll
+                for (u32 t = 0; t < 8; ++t) e[t] = icontacts[k + t];
+#pragma unroll
+                for (u32 t = 0; t < 8; ++t) slot[t] = order[e[t].x & 0x3FFFFFFFu];
+#pragma unroll
+                for (u32 t = 0; t < 8; ++t) {
+                    fixContact(e[t], slot[t]);
+                    icontacts[k + t] = e[t];
+                }
+            }
+            for (; k < nc; ++k) {
+                uint4 e = icontacts[k];
+                fixContact(e, order[e.x & 0x3FFFFFFFu]);
                 icontacts[k] = e;
             }
             for (u32 k = info.jointStart; k < nj; ++k) {  // same for the island's joints
@@ -228,9 +249,16 @@ __device__ __forceinline__ void BuildIslands(const BatchView& B, u32 w, const Wo
                 ijoints[k] = e;
             }
             islands[islandCount++] = info;
-            for (u32 k = 0; k < bodyCount; ++k) {  // world.cpp:411-419
-                u32 bi = ibodies[info.bodyStart + k];
-                if ((bflags[bi] >> 8) == kStatic) bflags[bi] &= ~kIsland;
+            for (u32 k0 = 0; k0 < bodyCount; k0 += 8) {  // world.cpp:411-419, eight loads per batch
+                u32 bis[8];
+#pragma unroll
+                for (u32 t = 0; t < 8; ++t) {
+                    const u32 at = info.bodyStart + k0 + t;
+                    bis[t] = k0 + t < bodyCount && at < wi.islandBodyCap ? ibodies[at] : kNull;
+                }
+#pragma unroll
+                for (u32 t = 0; t < 8; ++t)
+                    if (bis[t] != kNull && (bflags[bis[t]] >> 8) == kStatic) bflags[bis[t]] &= ~kIsland;
             }
         }
         ws.islandCount = islandCount;
